@@ -1,0 +1,157 @@
+/* include/vbh.h -- C ABI of the MI355X-native hip-pcg backend (libvbh.so).
+ *
+ * The library replaces, behind a plain C boundary (no C++ or torch types),
+ * the reference's solver plugin and its CSR stiffness assembly:
+ *   - vibestran::SolverBackend / CudaPCGSolverBackend
+ *       (reference include/solver/solver_backend.hpp:20-51,
+ *        include/solver/cuda_pcg_solver_backend.hpp:29-76)
+ *   - SparseMatrixBuilder::add_element_stiffness/build_csr driven per element
+ *       (reference src/solver/assembly_parallel.hpp:68-110,
+ *        src/core/sparse_matrix.cpp:16-150)
+ *   - ElementBase::stiffness_matrix for CHEXA8 (EAS default, SRI), CPENTA6
+ *     (EAS), CTETRA4 and CQUAD4 (MITC4+ default, Mindlin)
+ *       (reference include/elements/element_base.hpp:30-61)
+ *
+ * Conventions: all host pointers are caller-owned and only read/written for
+ * the duration of the call. Every entry point returns a vbh_status; on
+ * failure vbh_last_error() holds a thread-local message whose text follows
+ * the reference's SolverError messages. Indices are 0-based int32
+ * (reference EqIndex, include/core/dof_map.hpp:19). One context is used by
+ * one host thread at a time (the reference backend is not thread-safe
+ * either, solver_backend.hpp:20-51).
+ */
+#ifndef VBH_H
+#define VBH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vbh_ctx vbh_ctx;
+
+typedef enum {
+  VBH_OK = 0,
+  VBH_ERR_ARG = 1,        /* invalid argument (std::invalid_argument in the reference) */
+  VBH_ERR_NO_DEVICE = 2,  /* try_create() -> nullopt (cuda_pcg_solver_backend.hpp:45-47) */
+  VBH_ERR_SOLVER = 3,     /* SolverError (include/core/exceptions.hpp:14-17) */
+  VBH_ERR_HIP = 4,        /* HIP runtime failure */
+  VBH_ERR_STATE = 5       /* call order error (e.g. solve before a matrix is set) */
+} vbh_status;
+
+/* Precision modes for vbh_solve / vbh_solve_block. */
+typedef enum {
+  VBH_PREC_FP64 = 0,   /* plain FP64 Jacobi-PCG: cuda-pcg (cuda_pcg_backend_impl.hpp:2116-2132) */
+  VBH_PREC_MIXED = 1   /* FP32 SpMV + Jacobi, FP64 true residual and solution update
+                          (replaces cuda-pcg-mixed, cuda_pcg_backend_impl.hpp:2134-2262) */
+} vbh_precision;
+
+/* Element type codes for the batched assembly entry points. */
+typedef enum { VBH_CHEXA8 = 1, VBH_CPENTA6 = 2, VBH_CTETRA4 = 3, VBH_CQUAD4 = 4 } vbh_elem_type;
+
+/* Solid formulations (PSOLID ISOP, reference element_factory.cpp:31-61). */
+typedef enum { VBH_SOLID_EAS = 0, VBH_SOLID_SRI = 1 } vbh_solid_form;
+/* Shell formulations (PARAM,SHELLFORM, reference element_factory.cpp:13-24). */
+typedef enum { VBH_SHELL_MITC4 = 0, VBH_SHELL_MINDLIN = 1 } vbh_shell_form;
+
+/* Diagnostics of the most recent solve (reference last_iteration_count /
+ * last_estimated_error, solver_backend.hpp:36-42) plus device timings. */
+typedef struct {
+  int iterations;         /* PCG iterations (SpMV count of the inner solver) */
+  int refinements;        /* FP64 true-residual recomputes (mixed mode) */
+  double rel_res;         /* final relative residual ||F-Ku||/||F|| */
+  double solve_ms;        /* device time of the solve, hipEvents on the ctx stream */
+  double spmv_ms_avg;     /* mean duration of one inner SpMV launch (0 if unmeasured) */
+  int format;             /* device operator format: 0 scalar CSR, 3 BSR3, 6 BSR6 */
+} vbh_solve_stats;
+
+/* Counters of the most recent vbh_assemble call. */
+typedef struct {
+  long long nnz_topological;  /* lower pairs implied by element connectivity */
+  long long nnz;              /* stored lower nonzeros after exact-zero drops */
+  double pattern_ms;          /* device time: pair keys, radix sort, unique */
+  double element_ms;          /* device time: element matrices + scatter */
+} vbh_assembly_stats;
+
+/* ---- errors / devices ---------------------------------------------------- */
+const char* vbh_last_error(void);
+int vbh_device_count(void);
+
+/* ---- context: CudaPCGSolverBackend::try_create (cuda_pcg_backend_impl.hpp:2041-2065).
+ * tol <= 0 selects 1e-8, max_iters <= 0 selects 10000 (impl:2055).
+ * Returns VBH_ERR_NO_DEVICE when no GPU is usable. */
+int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out);
+void vbh_ctx_destroy(vbh_ctx* ctx);
+/* CudaPCGSolverBackend::device_name (cuda_pcg_solver_backend.hpp:72) */
+const char* vbh_device_name(const vbh_ctx* ctx);
+int vbh_set_precision(vbh_ctx* ctx, int precision);
+int vbh_set_tolerance(vbh_ctx* ctx, double tol, int max_iters);
+/* Operator format for the SpMV: -1 auto, 0 scalar CSR, 3 BSR 3x3, 6 BSR 6x6. */
+int vbh_set_format(vbh_ctx* ctx, int format);
+
+/* ---- matrix input at the SolverBackend boundary ------------------------
+ * K as SparseMatrixBuilder::CsrData (sparse_matrix.hpp:63-87): row_ptr[n+1],
+ * col_ind[nnz], values[nnz], rows sorted by column, duplicates summed;
+ * lower_only != 0 means symmetric lower storage (L + L^T - diag L).
+ * Validates like validate_pcg_inputs / build_jacobi_diagonal
+ * (cuda_pcg_backend_impl.hpp:2076-2114): empty matrix, zero or missing
+ * diagonal -> VBH_ERR_SOLVER. */
+int vbh_set_csr(vbh_ctx* ctx, int n, int nnz, const int* row_ptr, const int* col_ind,
+                const double* values, int lower_only);
+/* Size of the lower CSR currently held (after vbh_set_csr or assembly). */
+int vbh_get_csr_size(const vbh_ctx* ctx, int* n, int* nnz);
+/* Download the lower CSR held by the context (any pointer may be NULL). */
+int vbh_get_csr(const vbh_ctx* ctx, int* row_ptr, int* col_ind, double* values);
+
+/* ---- DOF numbering and assembly on the device ----------------------------
+ * vbh_dof_map: build_analysis_dof_map (reference src/solver/analysis_support.cpp:176-214
+ * with DofMap::build/constrain_batch, src/core/dof_map.cpp:10-97): 6 slots per
+ * node in sorted node-ID order; solids activate T1-T3, CQUAD4 all six
+ * (analysis_support.cpp:39-68); SPC'd DOFs removed. conn holds 0-based node
+ * indices; spc_masks bit d = DOF d+1. eq_out is nnodes*6 (-1 = constrained). */
+int vbh_dof_map(vbh_ctx* ctx, int nnodes, const int* node_ids, int nelem, const int* elem_type,
+                const int* elem_ptr, const int* conn, int nspc, const int* spc_nodes,
+                const int* spc_masks, int* eq_out, int* n_free);
+/* vbh_assemble: element stiffness for every element (CHEXA8 EAS/SRI, CPENTA6
+ * EAS, CTETRA4; CQUAD4 MITC4+/Mindlin) and the lower CSR of
+ * SparseMatrixBuilder::add_element_stiffness + build_csr (reference
+ * src/solver/assembly_parallel.hpp:68-110, src/core/sparse_matrix.cpp:16-150),
+ * built and kept on the device as the context's operator. eq is the nnodes*6
+ * map of vbh_dof_map; thickness may be NULL for solid-only meshes. */
+int vbh_assemble(vbh_ctx* ctx, int n, int nnodes, const double* xyz, const int* eq, int nelem,
+                 const int* elem_type, const int* elem_ptr, const int* conn, const double* E,
+                 const double* nu, const double* thickness, int solid_form, int shell_form);
+int vbh_last_assembly_stats(const vbh_ctx* ctx, vbh_assembly_stats* out);
+
+/* ---- solve: SolverBackend::solve(K, F) -> u (solver_backend.hpp:26-29) ----
+ * F and u are host arrays of length n. iters / rel_res may be NULL. */
+int vbh_solve(vbh_ctx* ctx, const double* F, double* u, int* iters, double* rel_res);
+/* Same with device-resident F and u (HBM pointers); no host copies. */
+int vbh_solve_device(vbh_ctx* ctx, const double* dF, double* du, int* iters, double* rel_res);
+/* Many load cases sharing K in one fused block-PCG (no reference counterpart:
+ * the reference re-assembles and re-solves per SUBCASE, linear_static.cpp:202-320).
+ * F and U are column-major n x k host arrays; iters/rel_res have length k. */
+int vbh_solve_block(vbh_ctx* ctx, int k, const double* F, double* U, int* iters,
+                    double* rel_res);
+int vbh_solve_block_device(vbh_ctx* ctx, int k, const double* dF, double* dU, int* iters,
+                           double* rel_res);
+int vbh_last_stats(const vbh_ctx* ctx, vbh_solve_stats* out);
+
+/* ---- operator application ------------------------------------------------
+ * y = K x in FP64 with the context's device operator (reference
+ * CsrData::multiply, sparse_matrix.cpp:152-181). Host arrays of length n. */
+int vbh_spmv(vbh_ctx* ctx, const double* x, double* y);
+/* Time `reps` back-to-back SpMV launches of the inner format in the given
+ * precision (0 FP64, 1 FP32) on device-resident vectors; returns the mean
+ * launch duration in ms measured with hipEvents on the context stream. */
+int vbh_bench_spmv(vbh_ctx* ctx, int precision, int reps, double* ms_per_launch);
+
+/* ---- device memory helpers for device-resident workflows ---------------- */
+int vbh_device_alloc(vbh_ctx* ctx, long long bytes, void** dptr);
+int vbh_device_free(vbh_ctx* ctx, void* dptr);
+int vbh_memcpy_h2d(vbh_ctx* ctx, void* dst, const void* src, long long bytes);
+int vbh_memcpy_d2h(vbh_ctx* ctx, void* dst, const void* src, long long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VBH_H */

@@ -1,0 +1,113 @@
+// oracle/src/oracle.hpp
+//
+// TEST INFRASTRUCTURE ONLY. This is the CPU FP64 restatement of the
+// reference's linear-static hot path (vibestran). It is the checker for the
+// HIP product path: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it. The product library never links or calls it.
+//
+// Every function cites the reference file:line it restates. The reference
+// itself cannot be compiled here (Eigen, spdlog, CHOLMOD and <format> are
+// absent), so these restatements are pinned by the reference's own unit-test
+// known answers, e2e decks and closed forms instead (see tests/).
+#pragma once
+#include <cstdint>
+
+extern "C" {
+
+// Status codes returned by every orc_* entry point.
+enum { ORC_OK = 0, ORC_ERR_ARG = 1, ORC_ERR_SOLVER = 3 };
+
+// Thread-local text of the last error (reference SolverError::what()).
+const char* orc_last_error(void);
+
+// ---- DofMap: src/core/dof_map.cpp:10-104 --------------------------------
+// Opaque map over the node set; slots are 6 per node in sorted-ID order.
+typedef struct OrcDofMap OrcDofMap;
+OrcDofMap* orc_dofmap_build(int nnodes, const int* node_ids, int dofs_per_node);
+void orc_dofmap_free(OrcDofMap* m);
+int  orc_dofmap_constrain(OrcDofMap* m, int node, int dof0);
+int  orc_dofmap_constrain_batch(OrcDofMap* m, int count, const int* nodes, const int* dofs);
+int  orc_dofmap_eq_index(const OrcDofMap* m, int node, int dof0);
+int  orc_dofmap_num_free(const OrcDofMap* m);
+int  orc_dofmap_num_total(const OrcDofMap* m);
+
+// Element type codes shared by the oracle's model-level helpers.
+enum { ORC_CHEXA8 = 1, ORC_CPENTA6 = 2, ORC_CTETRA4 = 3, ORC_CQUAD4 = 4 };
+
+// build_analysis_dof_map (src/solver/analysis_support.cpp:176-214): activate
+// DOFs per element family (39-68), constrain inactive then SPC DOFs.
+// spc_masks[i] bit d (0..5) set = DOF d+1 of spc_nodes[i] is constrained.
+// eq_out is nnodes*6, in the order of node_ids. Returns n_free or <0.
+int orc_analysis_dof_map(int nnodes, const int* node_ids,
+                         int nelem, const int* elem_type, const int* elem_ptr,
+                         const int* elem_nodes,
+                         int nspc, const int* spc_nodes, const int* spc_masks,
+                         int* eq_out);
+
+// ---- Elements ------------------------------------------------------------
+// All element matrices are written row-major, ndof x ndof.
+// CHexa8 SRI: src/elements/solid_elements.cpp:95-180
+int orc_hexa8_ke_sri(const double* xyz24, double E, double nu, double* ke576);
+// CHexa8Eas: src/elements/solid_elements.cpp:1054-1206
+int orc_hexa8_ke_eas(const double* xyz24, double E, double nu, double* ke576);
+// CTetra4: src/elements/solid_elements.cpp:312-384
+int orc_tetra4_ke(const double* xyz12, double E, double nu, double* ke144);
+// CPenta6Eas: src/elements/solid_elements.cpp:1333-1467
+int orc_penta6_ke_eas(const double* xyz18, double E, double nu, double* ke324);
+// Consistent mass (CHexa8 182-217 / CHexa8Eas 1208-1243), CTetra4 386-411.
+int orc_hexa8_me(const double* xyz24, double rho, double* me576);
+int orc_tetra4_me(const double* xyz12, double rho, double* me144);
+int orc_penta6_me(const double* xyz18, double rho, double* me324);
+
+// ---- SparseMatrixBuilder / CsrData: src/core/sparse_matrix.cpp -----------
+typedef struct OrcBuilder OrcBuilder;
+typedef struct OrcCsr OrcCsr;
+OrcBuilder* orc_builder_create(int n);
+void orc_builder_free(OrcBuilder* b);
+void orc_builder_add(OrcBuilder* b, int row, int col, double v);          // hpp:43-45
+int  orc_builder_add_element(OrcBuilder* b, int ndof, const int* gdofs,
+                             const double* ke_rowmajor);                  // cpp:16-35
+long long orc_builder_num_triplets(const OrcBuilder* b);
+OrcCsr* orc_builder_build_csr(OrcBuilder* b);                              // cpp:62-150
+void orc_csr_free(OrcCsr* c);
+int  orc_csr_n(const OrcCsr* c);
+int  orc_csr_nnz(const OrcCsr* c);
+int  orc_csr_is_lower(const OrcCsr* c);
+void orc_csr_copy(const OrcCsr* c, int* row_ptr, int* col_ind, double* values);
+OrcCsr* orc_csr_from_arrays(int n, int nnz, const int* row_ptr, const int* col_ind,
+                            const double* values, int lower_only);
+int  orc_csr_multiply(const OrcCsr* c, const double* x, double* y);          // cpp:152-181
+OrcCsr* orc_csr_expanded_symmetric(const OrcCsr* c);                         // cpp:183-235
+OrcCsr* orc_csr_lower_triangle(const OrcCsr* c);                             // cpp:237-274
+
+// Model-level assembly of a solid mesh (assembly_parallel.hpp:68-110 serial
+// branch, element_factory.cpp:31-61). conn uses the element's node count
+// (8 CHEXA8 / 6 CPENTA6 / 4 CTETRA4) with 0-based node indices into xyz;
+// eq is the nnodes*6 analysis DOF map. formulation: 0 = EAS (default),
+// 1 = SRI (PSOLID ISOP=SRI; CTETRA4 ignores it).
+OrcCsr* orc_assemble_solids(int n, int nelem, const int* elem_type, const int* elem_ptr,
+                            const int* conn, const double* xyz, const int* eq,
+                            const double* E, const double* nu, int formulation);
+
+// ---- Jacobi-PCG (semantics of cuda_pcg_backend_impl.hpp:1689-1999 Jacobi
+// branch with build_jacobi_diagonal 2076-2101 and validate 2103-2114).
+// Returns ORC_OK, or ORC_ERR_SOLVER with orc_last_error() text.
+int orc_pcg_jacobi(const OrcCsr* K, const double* F, double tol, int max_iters,
+                   int nthreads, double* u, int* iters, double* rel_res);
+// Same recurrence, timed for a fixed number of iterations (CPU baseline).
+int orc_pcg_jacobi_fixed(const OrcCsr* K, const double* F, int iters, int nthreads,
+                         double* u, double* seconds);
+// True relative residual ||F-Ku||/||F|| (cuda_pcg_backend_impl.hpp:90-114).
+double orc_true_relative_residual(const OrcCsr* K, const double* u, const double* F);
+
+// ---- Loads -----------------------------------------------------------------
+// PLOAD4 on a CHEXA8 face: face selection select_solid_face
+// (linear_static_pressure.cpp:430-517) and 2x2 Gauss integration
+// integrate_quad_surface_load (154-196) with normal_sign -1 (solids, 745-753).
+// g1_local/g34_local are element-local corner indices (or -1 for none);
+// pressures are the 4 corner pressures. forces24 receives the element
+// force vector [3 per element node].
+int orc_pload4_hexa8(const double* xyz24, int g1_local, int g34_local,
+                     const double* pressures4, double* forces24);
+
+}  // extern "C"

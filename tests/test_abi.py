@@ -1,0 +1,52 @@
+"""The C ABI library loads and exports every symbol include/vbh.h declares
+(no compute calls: this runs without a GPU)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from vibestran_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "vbh.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vbh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "vbh_solve" in syms and "vbh_assemble" in syms and len(syms) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    assert set(declared_symbols()) <= set(_lib.SIGNATURES)
+
+
+def test_dynamic_symbol_table():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\b(vbh_[a-z0-9_]+)\b", out))
+    assert set(declared_symbols()) <= exported
+
+
+def test_library_embeds_gfx950_code_object():
+    data = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_try_create_without_device_returns_none():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from vibestran_amd import HipPcgSolverBackend
+    assert HipPcgSolverBackend.try_create() is None

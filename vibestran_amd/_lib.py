@@ -1,0 +1,108 @@
+"""ctypes binding of libvbh.so (the C ABI declared in include/vbh.h).
+
+The HIP library is the product path: if it is missing this module raises
+instead of falling back to any CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvbh.so")
+
+VBH_OK, VBH_ERR_ARG, VBH_ERR_NO_DEVICE, VBH_ERR_SOLVER, VBH_ERR_HIP, VBH_ERR_STATE = range(6)
+VBH_PREC_FP64, VBH_PREC_MIXED = 0, 1
+
+
+class vbh_solve_stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("refinements", C.c_int),
+        ("rel_res", C.c_double),
+        ("solve_ms", C.c_double),
+        ("spmv_ms_avg", C.c_double),
+        ("format", C.c_int),
+    ]
+
+
+class vbh_assembly_stats(C.Structure):
+    _fields_ = [
+        ("nnz_topological", C.c_longlong),
+        ("nnz", C.c_longlong),
+        ("pattern_ms", C.c_double),
+        ("element_ms", C.c_double),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_LL = C.c_longlong
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol declared in include/vbh.h.
+SIGNATURES = {
+    "vbh_last_error": (C.c_char_p, []),
+    "vbh_device_count": (_I, []),
+    "vbh_ctx_create": (_I, [_I, _D, _I, C.POINTER(_P)]),
+    "vbh_ctx_destroy": (None, [_P]),
+    "vbh_device_name": (C.c_char_p, [_P]),
+    "vbh_set_precision": (_I, [_P, _I]),
+    "vbh_set_tolerance": (_I, [_P, _D, _I]),
+    "vbh_set_format": (_I, [_P, _I]),
+    "vbh_set_csr": (_I, [_P, _I, _I, _IP, _IP, _DP, _I]),
+    "vbh_get_csr_size": (_I, [_P, _IP, _IP]),
+    "vbh_get_csr": (_I, [_P, _IP, _IP, _DP]),
+    "vbh_dof_map": (_I, [_P, _I, _IP, _I, _IP, _IP, _IP, _I, _IP, _IP, _IP, _IP]),
+    "vbh_assemble": (_I, [_P, _I, _I, _DP, _IP, _I, _IP, _IP, _IP, _DP, _DP, _DP, _I, _I]),
+    "vbh_last_assembly_stats": (_I, [_P, C.POINTER(vbh_assembly_stats)]),
+    "vbh_solve": (_I, [_P, _DP, _DP, _IP, _DP]),
+    "vbh_solve_device": (_I, [_P, _P, _P, _IP, _DP]),
+    "vbh_solve_block": (_I, [_P, _I, _DP, _DP, _IP, _DP]),
+    "vbh_solve_block_device": (_I, [_P, _I, _P, _P, _IP, _DP]),
+    "vbh_last_stats": (_I, [_P, C.POINTER(vbh_solve_stats)]),
+    "vbh_spmv": (_I, [_P, _DP, _DP]),
+    "vbh_bench_spmv": (_I, [_P, _I, _I, _DP]),
+    "vbh_device_alloc": (_I, [_P, _LL, C.POINTER(_P)]),
+    "vbh_device_free": (_I, [_P, _P]),
+    "vbh_memcpy_h2d": (_I, [_P, _P, _P, _LL]),
+    "vbh_memcpy_d2h": (_I, [_P, _P, _P, _LL]),
+}
+
+_lib = None
+
+
+def load(required_symbols=True):
+    """Load libvbh.so. Raises RuntimeError when the extension is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libvbh.so not found at {LIB_PATH}: build the HIP extension first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        if not hasattr(lib, name):
+            if required_symbols:
+                raise RuntimeError(f"libvbh.so lacks symbol {name}")
+            continue
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().vbh_last_error().decode(errors="replace")
+
+
+def ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def dp(a):
+    return a.ctypes.data_as(_DP)

@@ -1,0 +1,83 @@
+"""Device DOF numbering and batched element assembly through libvbh.
+
+``assemble_model`` replaces the reference's host assembly loop
+(src/solver/assembly_parallel.hpp:68-110 + SparseMatrixBuilder::build_csr,
+src/core/sparse_matrix.cpp:16-150): element matrices are computed and
+scattered on the GPU, and the lower CSR operator stays resident in HBM as the
+backend's matrix. The returned CsrData is a handle to that device matrix; its
+host arrays are fetched lazily (``fetch()``) for inspection or parity tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .backend import CsrData, _check
+from .model import Model
+
+
+class DeviceCsr(CsrData):
+    """CsrData whose arrays live on the device inside a backend context."""
+
+    def __init__(self, backend, n, nnz):
+        super().__init__(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0), n, nnz, "Lower")
+        self._backend = backend
+        self._fetched = False
+
+    def fetch(self):
+        if not self._fetched:
+            lib = L.load()
+            rp = np.empty(self.n + 1, dtype=np.int32)
+            ci = np.empty(self.nnz, dtype=np.int32)
+            v = np.empty(self.nnz, dtype=np.float64)
+            _check(lib.vbh_get_csr(self._backend._h, L.ip(rp), L.ip(ci), L.dp(v)))
+            self.row_ptr, self.col_ind, self.values = rp, ci, v
+            self._fetched = True
+        return self
+
+
+def dof_map_device(backend, model: Model):
+    """GPU equation numbering; returns (eq int32[nn,6], n_free)."""
+    lib = L.load()
+    ids = np.ascontiguousarray(model.node_ids, dtype=np.int32)
+    et = np.ascontiguousarray(model.elem_type, dtype=np.int32)
+    ep = np.ascontiguousarray(model.elem_ptr, dtype=np.int32)
+    cn = np.ascontiguousarray(model.conn, dtype=np.int32)
+    sn = np.ascontiguousarray(model.spc_nodes, dtype=np.int32)
+    sm = np.ascontiguousarray(model.spc_masks, dtype=np.int32)
+    eq = np.empty((model.nn, 6), dtype=np.int32)
+    nf = C.c_int(0)
+    _check(lib.vbh_dof_map(backend._h, len(ids), L.ip(ids), len(et), L.ip(et), L.ip(ep), L.ip(cn),
+                           len(sn), L.ip(sn), L.ip(sm), L.ip(eq), C.byref(nf)))
+    return eq, nf.value
+
+
+def assemble_model(backend, model: Model, eq: np.ndarray, n: int) -> DeviceCsr:
+    """Assemble K on the device and install it as ``backend``'s operator."""
+    lib = L.load()
+    xyz = np.ascontiguousarray(model.xyz, dtype=np.float64)
+    eqc = np.ascontiguousarray(eq, dtype=np.int32)
+    et = np.ascontiguousarray(model.elem_type, dtype=np.int32)
+    ep = np.ascontiguousarray(model.elem_ptr, dtype=np.int32)
+    cn = np.ascontiguousarray(model.conn, dtype=np.int32)
+    E = np.ascontiguousarray(model.E, dtype=np.float64)
+    nu = np.ascontiguousarray(model.nu, dtype=np.float64)
+    th = None if model.thickness is None else np.ascontiguousarray(model.thickness, dtype=np.float64)
+    _check(lib.vbh_assemble(backend._h, n, model.nn, L.dp(xyz), L.ip(eqc), model.ne, L.ip(et),
+                            L.ip(ep), L.ip(cn), L.dp(E), L.dp(nu),
+                            L.dp(th) if th is not None else None, int(model.solid_form),
+                            int(model.shell_form)))
+    nn_ = C.c_int(0)
+    nnz = C.c_int(0)
+    _check(lib.vbh_get_csr_size(backend._h, C.byref(nn_), C.byref(nnz)))
+    K = DeviceCsr(backend, nn_.value, nnz.value)
+    backend._uploaded = K
+    return K
+
+
+def assembly_stats(backend) -> L.vbh_assembly_stats:
+    s = L.vbh_assembly_stats()
+    _check(L.load().vbh_last_assembly_stats(backend._h, C.byref(s)))
+    return s

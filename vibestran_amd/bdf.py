@@ -1,0 +1,205 @@
+"""Nastran BDF subset reader for the linear-static hot path.
+
+Cards: GRID, CHEXA, CPENTA, CTETRA, CQUAD4, MAT1, PSOLID, PSHELL, SPC1, SPC,
+FORCE, MOMENT, PLOAD4, GRAV, PARAM (SHELLFORM); case control SUBCASE/LOAD/SPC.
+Free field (commas) and small fixed field with '+'/'*'/blank continuations.
+Real parsing follows the reference's Nastran rules: implicit exponents
+("1.-3"), D exponents, trailing dots (reference src/io/bdf_parser.cpp:780-830).
+PSHELL defaults: 12I/T^3 = 1.0, TS/T = 0.833333 (bdf_parser.cpp:1265-1287);
+PSOLID ISOP=SRI selects the SRI CHEXA8, anything else EAS (1289-1306).
+"""
+from __future__ import annotations
+
+import re
+from collections import defaultdict
+
+import numpy as np
+
+from .model import CHEXA8, CPENTA6, CQUAD4, CTETRA4, LoadCase, Model, Pload4
+
+_IMPLICIT = re.compile(r"^([+-]?[0-9]*\.?[0-9]*)([+-][0-9]+)$")
+
+
+def nastran_real(s: str) -> float:
+    s = s.strip().upper().replace("D", "E")
+    if not s:
+        return 0.0
+    try:
+        return float(s)
+    except ValueError:
+        m = _IMPLICIT.match(s)
+        if m:
+            return float(f"{m.group(1)}E{m.group(2)}")
+        raise
+
+
+def _split_fixed(line: str):
+    line = line.rstrip("\n")
+    fields = [line[0:8]]
+    for k in range(8, 72, 8):
+        fields.append(line[k:k + 8])
+    return [f.strip() for f in fields]
+
+
+def _cards(lines):
+    """Yield (name, fields[1:]) with continuations merged."""
+    cur = None
+    for raw in lines:
+        line = raw.split("$", 1)[0].rstrip()
+        if not line.strip():
+            continue
+        is_cont = line[0] in "+*" or line[0] == " "
+        if "," in line:
+            fields = [f.strip() for f in line.split(",")]
+        else:
+            fields = _split_fixed(line)
+        if is_cont and cur is not None:
+            cur[1].extend(fields[1:] if "," in line else fields[1:9])
+            continue
+        if cur is not None:
+            yield cur
+        name = fields[0].upper().rstrip("*")
+        body = fields[1:] if "," in line else fields[1:9]
+        # a trailing continuation marker on a free-field line
+        if body and body[-1].startswith("+") and "," in line:
+            body = body[:-1]
+        cur = [name, body]
+    if cur is not None:
+        yield cur
+
+
+def _int(f, default=0):
+    f = f.strip()
+    return int(f) if f else default
+
+
+def read_bdf(path_or_text: str, is_text=False) -> Model:
+    text = path_or_text if is_text else open(path_or_text).read()
+    lines = text.splitlines()
+    # case control
+    subcases = []
+    bulk_at = 0
+    cur = None
+    for i, ln in enumerate(lines):
+        u = ln.strip().upper()
+        if u.startswith("BEGIN BULK"):
+            bulk_at = i + 1
+            break
+        if u.startswith("SUBCASE"):
+            cur = {"id": int(u.split()[1]), "load": 0, "spc": 0}
+            subcases.append(cur)
+        elif "=" in u and cur is not None:
+            k, v = [x.strip() for x in u.split("=", 1)]
+            if k == "LOAD":
+                cur["load"] = int(v)
+            elif k == "SPC":
+                cur["spc"] = int(v)
+    grids, elems, mats, props = {}, [], {}, {}
+    spcs = defaultdict(list)
+    forces = defaultdict(list)
+    ploads = defaultdict(list)
+    params = {}
+    for name, f in _cards(lines[bulk_at:]):
+        f = f + [""] * 16
+        if name == "GRID":
+            grids[int(f[0])] = (nastran_real(f[2]), nastran_real(f[3]), nastran_real(f[4]))
+        elif name in ("CHEXA", "CPENTA", "CTETRA", "CQUAD4"):
+            nn = {"CHEXA": 8, "CPENTA": 6, "CTETRA": 4, "CQUAD4": 4}[name]
+            nodes = [int(x) for x in f[2:2 + nn]]
+            t = {"CHEXA": CHEXA8, "CPENTA": CPENTA6, "CTETRA": CTETRA4, "CQUAD4": CQUAD4}[name]
+            elems.append((int(f[0]), t, int(f[1]), nodes))
+        elif name == "MAT1":
+            E = nastran_real(f[1])
+            G = nastran_real(f[2]) if f[2] else None
+            nu = nastran_real(f[3]) if f[3] else None
+            if nu is None and G is not None:
+                nu = E / (2 * G) - 1
+            rho = nastran_real(f[4]) if f[4] else 0.0
+            mats[int(f[0])] = (E, nu if nu is not None else 0.0, rho)
+        elif name == "PSOLID":
+            props[int(f[0])] = ("PSOLID", int(f[1]), f[5].upper() if len(f) > 5 else "")
+        elif name == "PSHELL":
+            props[int(f[0])] = ("PSHELL", int(f[1]), nastran_real(f[2]))
+        elif name == "SPC1":
+            sid, comp = int(f[0]), f[1]
+            mask = 0
+            for ch in comp:
+                mask |= 1 << (int(ch) - 1)
+            rest = [x for x in f[2:] if x]
+            if len(rest) >= 3 and rest[1].upper() == "THRU":
+                ids = range(int(rest[0]), int(rest[2]) + 1)
+            else:
+                ids = [int(x) for x in rest]
+            for g in ids:
+                spcs[sid].append((g, mask))
+        elif name == "SPC":
+            sid = int(f[0])
+            for k in (1, 4):
+                if f[k]:
+                    mask = 0
+                    for ch in f[k + 1]:
+                        mask |= 1 << (int(ch) - 1)
+                    spcs[sid].append((int(f[k]), mask))
+        elif name in ("FORCE", "MOMENT"):
+            sid, g = int(f[0]), int(f[1])
+            scale = nastran_real(f[3])
+            v = np.array([nastran_real(f[4]), nastran_real(f[5]), nastran_real(f[6])]) * scale
+            forces[sid].append((g, 0 if name == "FORCE" else 3, v))
+        elif name == "PLOAD4":
+            sid, eid = int(f[0]), int(f[1])
+            p = [nastran_real(x) if x else None for x in f[2:6]]
+            p1 = p[0] if p[0] is not None else 0.0
+            p = [x if x is not None else p1 for x in p]
+            g1 = _int(f[6], -1) if f[6] else -1
+            g34 = _int(f[7], -1) if f[7] else -1
+            ploads[sid].append((eid, g1, g34, p))
+        elif name == "PARAM":
+            params[f[0].upper()] = f[1].upper()
+    ids = np.array(sorted(grids), dtype=np.int32)
+    index = {g: i for i, g in enumerate(ids)}
+    xyz = np.array([grids[g] for g in ids], dtype=np.float64)
+    etype, eptr, conn, E, nu, rho, th = [], [0], [], [], [], [], []
+    solid_form = 0
+    eid_index = {}
+    for k, (eid, t, pid, nodes) in enumerate(elems):
+        eid_index[eid] = k
+        pr = props[pid]
+        mat = mats[pr[1]]
+        etype.append(t)
+        conn.extend(index[g] for g in nodes)
+        eptr.append(len(conn))
+        E.append(mat[0])
+        nu.append(mat[1])
+        rho.append(mat[2])
+        th.append(pr[2] if pr[0] == "PSHELL" else 0.0)
+        if pr[0] == "PSOLID" and pr[2] == "SRI":
+            solid_form = 1
+    sc = subcases[0] if subcases else {"load": 0, "spc": 0}
+    spc_list = spcs.get(sc["spc"], [])
+    m = Model(node_ids=ids, xyz=xyz, elem_type=np.array(etype, np.int32),
+              elem_ptr=np.array(eptr, np.int32), conn=np.array(conn, np.int32),
+              E=np.array(E), nu=np.array(nu), rho=np.array(rho),
+              thickness=np.array(th) if any(t == CQUAD4 for t in etype) else None,
+              solid_form=solid_form,
+              shell_form=1 if params.get("SHELLFORM", "MITC4").startswith("MINDLIN") else 0,
+              spc_nodes=np.array([index[g] for g, _ in spc_list], np.int32),
+              spc_masks=np.array([mk for _, mk in spc_list], np.int32))
+    for s in subcases or [sc]:
+        nodal = np.zeros((m.nn, 6))
+        for g, off, v in forces.get(s["load"], []):
+            nodal[index[g], off:off + 3] += v
+        pl = None
+        if ploads.get(s["load"]):
+            rows = ploads[s["load"]]
+            elist = np.array([eid_index[e] for e, *_ in rows])
+            c2 = [m.conn[m.elem_ptr[k]:m.elem_ptr[k + 1]] for k in elist]
+            def local(k, g):
+                if g < 0:
+                    return -1
+                return int(np.nonzero(c2[k] == index[g])[0][0])
+            pl = Pload4(elem=elist, g1=np.array([local(k, r[1]) for k, r in enumerate(rows)]),
+                        g34=np.array([local(k, r[2]) for k, r in enumerate(rows)]),
+                        p=np.array([r[3] for r in rows]))
+        m.loads.append(LoadCase(nodal=nodal, pload4=pl))
+    m.meta["params"] = params
+    return m

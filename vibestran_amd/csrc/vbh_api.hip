@@ -1,0 +1,675 @@
+// vibestran_amd/csrc/vbh_api.hip -- C ABI of libvbh: context, operator upload,
+// Jacobi-PCG drivers (FP64 and FP32 + FP64 refresh), diagnostics.
+//
+// Drop-in for the reference's SolverBackend plugin (reference
+// include/solver/solver_backend.hpp:20-51); error texts follow the reference's
+// SolverError messages (cuda_pcg_backend_impl.hpp:1964-1970, 2076-2114).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "vbh_ctx.hpp"
+
+namespace vbh {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void throw_error(int code, const std::string& msg) { throw Error{code, msg}; }
+
+std::string fmt_g(double v) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%g", v);
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// Operator construction on the host from the reference's CsrData layout.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct HostFull {
+  int n = 0;
+  std::vector<int> rp, ci;
+  std::vector<double> v;
+};
+
+// Lower (or full) CSR -> full symmetric CSR with sorted rows; same result as
+// CsrData::expanded_symmetric (sparse_matrix.cpp:183-235).
+HostFull expand_full(int n, int nnz, const int* rp, const int* ci, const double* v, bool lower) {
+  HostFull f;
+  f.n = n;
+  if (!lower) {
+    f.rp.assign(rp, rp + n + 1);
+    f.ci.assign(ci, ci + nnz);
+    f.v.assign(v, v + nnz);
+    return f;
+  }
+  f.rp.assign(static_cast<size_t>(n) + 1, 0);
+  for (int r = 0; r < n; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      ++f.rp[static_cast<size_t>(r) + 1];
+      if (ci[k] != r) ++f.rp[static_cast<size_t>(ci[k]) + 1];
+    }
+  std::partial_sum(f.rp.begin(), f.rp.end(), f.rp.begin());
+  const size_t tot = static_cast<size_t>(f.rp[static_cast<size_t>(n)]);
+  f.ci.resize(tot);
+  f.v.resize(tot);
+  std::vector<int> cur(f.rp.begin(), f.rp.begin() + n);
+  for (int r = 0; r < n; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int c = ci[k];
+      int o = cur[static_cast<size_t>(r)]++;
+      f.ci[static_cast<size_t>(o)] = c;
+      f.v[static_cast<size_t>(o)] = v[k];
+      if (c != r) {
+        o = cur[static_cast<size_t>(c)]++;
+        f.ci[static_cast<size_t>(o)] = r;
+        f.v[static_cast<size_t>(o)] = v[k];
+      }
+    }
+  return f;
+}
+
+struct HostBsr {
+  int B = 0, nbr = 0;
+  std::vector<int> brp, bci;
+  std::vector<double> v;  // entry-major per block row
+};
+
+HostBsr to_bsr(const HostFull& f, int B) {
+  HostBsr b;
+  b.B = B;
+  b.nbr = (f.n + B - 1) / B;
+  b.brp.assign(static_cast<size_t>(b.nbr) + 1, 0);
+  std::vector<int> cols;
+  std::vector<std::vector<int>> rows_cols(1);
+  // pass 1: block pattern
+  for (int I = 0; I < b.nbr; ++I) {
+    cols.clear();
+    for (int r = B * I; r < std::min(f.n, B * I + B); ++r)
+      for (int k = f.rp[static_cast<size_t>(r)]; k < f.rp[static_cast<size_t>(r) + 1]; ++k)
+        cols.push_back(f.ci[static_cast<size_t>(k)] / B);
+    cols.push_back(I);
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    b.bci.insert(b.bci.end(), cols.begin(), cols.end());
+    b.brp[static_cast<size_t>(I) + 1] = static_cast<int>(b.bci.size());
+  }
+  b.v.assign(b.bci.size() * static_cast<size_t>(B * B), 0.0);
+  // pass 2: values
+  for (int I = 0; I < b.nbr; ++I) {
+    const int o = b.brp[static_cast<size_t>(I)], nb = b.brp[static_cast<size_t>(I) + 1] - o;
+    const int* seg = b.bci.data() + o;
+    for (int r = B * I; r < B * I + B; ++r) {
+      if (r >= f.n) {  // padding row: identity diagonal
+        const int k = static_cast<int>(std::lower_bound(seg, seg + nb, I) - seg);
+        const int e = (r % B) * B + (r % B);
+        b.v[static_cast<size_t>(B * B) * o + static_cast<size_t>(e) * nb + k] = 1.0;
+        continue;
+      }
+      for (int q = f.rp[static_cast<size_t>(r)]; q < f.rp[static_cast<size_t>(r) + 1]; ++q) {
+        const int c = f.ci[static_cast<size_t>(q)];
+        const int k = static_cast<int>(std::lower_bound(seg, seg + nb, c / B) - seg);
+        const int e = (r % B) * B + (c % B);
+        b.v[static_cast<size_t>(B * B) * o + static_cast<size_t>(e) * nb + k] += f.v[static_cast<size_t>(q)];
+      }
+    }
+  }
+  return b;
+}
+
+}  // namespace
+
+void Ctx::select_stream_device() { VBH_HIP(hipSetDevice(device)); }
+
+void Ctx::upload_operator(int n_, int nnz, const int* rp, const int* ci, const double* v,
+                          bool lower) {
+  select_stream_device();
+  if (n_ <= 0) throw_error(VBH_ERR_SOLVER, "HIP PCG: stiffness matrix is empty -- no free DOFs");
+  if (!rp || !ci || (!v && nnz > 0)) throw_error(VBH_ERR_ARG, "vbh_set_csr: null array");
+  if (rp[0] != 0 || rp[n_] != nnz) throw_error(VBH_ERR_ARG, "vbh_set_csr: row_ptr inconsistent with nnz");
+  // Jacobi diagonal with the reference's checks (build_jacobi_diagonal).
+  std::vector<double> diag(static_cast<size_t>(n_), 0.0);
+  for (int i = 0; i < n_; ++i) {
+    bool found = false;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      if (ci[k] < 0 || ci[k] >= n_) throw_error(VBH_ERR_ARG, "vbh_set_csr: column index out of range");
+      if (lower && ci[k] > i) throw_error(VBH_ERR_ARG, "vbh_set_csr: upper entry in lower storage");
+      if (ci[k] == i && !found) {
+        if (v[k] == 0.0)
+          throw_error(VBH_ERR_SOLVER, "HIP PCG: zero diagonal at row " + std::to_string(i) +
+                                          " -- matrix is singular. Check boundary conditions.");
+        diag[static_cast<size_t>(i)] = v[k];
+        found = true;
+      }
+    }
+    if (!found)
+      throw_error(VBH_ERR_SOLVER, "HIP PCG: missing diagonal at row " + std::to_string(i) +
+                                      " -- matrix is singular. Check boundary conditions.");
+  }
+  n = n_;
+  // Keep the caller's lower CSR (reference CsrData) for vbh_get_csr.
+  lower_on_host = true;
+  if (lower) {
+    lower_rp.assign(rp, rp + n_ + 1);
+    lower_ci.assign(ci, ci + nnz);
+    lower_v.assign(v, v + nnz);
+  } else {
+    lower_rp.assign(static_cast<size_t>(n_) + 1, 0);
+    lower_ci.clear();
+    lower_v.clear();
+    for (int r = 0; r < n_; ++r) {
+      for (int k = rp[r]; k < rp[r + 1]; ++k)
+        if (ci[k] <= r) {
+          lower_ci.push_back(ci[k]);
+          lower_v.push_back(v[k]);
+        }
+      lower_rp[static_cast<size_t>(r) + 1] = static_cast<int>(lower_ci.size());
+    }
+  }
+  HostFull full = expand_full(n_, nnz, rp, ci, v, lower);
+  // Format choice: BSR when the block fill stays low.
+  int fmt = format_req;
+  HostBsr bsr;
+  if (fmt < 0) {
+    fmt = 0;
+    for (int B : {6, 3}) {
+      if (n_ < 4 * B) continue;
+      HostBsr cand = to_bsr(full, B);
+      const double fill = static_cast<double>(cand.v.size()) / static_cast<double>(full.v.size());
+      if (fill <= (B == 6 ? 1.12 : 1.25)) {
+        bsr = std::move(cand);
+        fmt = B;
+        break;
+      }
+    }
+  } else if (fmt == 3 || fmt == 6) {
+    bsr = to_bsr(full, fmt);
+  } else {
+    fmt = 0;
+  }
+  format = fmt;
+  n_pad = (fmt == 0) ? n_ : bsr.nbr * fmt;
+  hipStream_t s = stream;
+  // scalar full CSR (always kept: generic path and FP64 reference operator)
+  csr_rp.upload(full.rp.data(), full.rp.size(), s);
+  csr_ci.upload(full.ci.data(), full.ci.size(), s);
+  csr_v64.upload(full.v.data(), full.v.size(), s);
+  {
+    std::vector<float> f32(full.v.begin(), full.v.end());
+    csr_v32.upload(f32.data(), f32.size(), s);
+    VBH_HIP(hipStreamSynchronize(s));
+  }
+  csr.n = n_;
+  csr.nnz = static_cast<int>(full.v.size());
+  csr.rp = csr_rp.p;
+  csr.ci = csr_ci.p;
+  csr.v64 = csr_v64.p;
+  csr.v32 = csr_v32.p;
+  if (fmt != 0) {
+    bsr_rp.upload(bsr.brp.data(), bsr.brp.size(), s);
+    bsr_ci.upload(bsr.bci.data(), bsr.bci.size(), s);
+    bsr_v64.upload(bsr.v.data(), bsr.v.size(), s);
+    std::vector<float> f32(bsr.v.begin(), bsr.v.end());
+    bsr_v32.upload(f32.data(), f32.size(), s);
+    VBH_HIP(hipStreamSynchronize(s));
+    bsrd.B = fmt;
+    bsrd.nbr = bsr.nbr;
+    bsrd.nnzb = static_cast<int>(bsr.bci.size());
+    bsrd.brp = bsr_rp.p;
+    bsrd.bci = bsr_ci.p;
+    bsrd.v64 = bsr_v64.p;
+    bsrd.v32 = bsr_v32.p;
+  } else {
+    bsr_rp.release();
+    bsr_ci.release();
+    bsr_v64.release();
+    bsr_v32.release();
+    bsrd = BsrDev{};
+  }
+  diag.resize(static_cast<size_t>(n_pad), 1.0);
+  diag64.upload(diag.data(), diag.size(), s);
+  {
+    std::vector<float> d32(diag.begin(), diag.end());
+    diag32.upload(d32.data(), d32.size(), s);
+    VBH_HIP(hipStreamSynchronize(s));
+  }
+  l_nnz = static_cast<int>(lower_ci.size());
+  operator_version += 1;
+  have_matrix = true;
+}
+
+// ---------------------------------------------------------------------------
+// Solve driver
+// ---------------------------------------------------------------------------
+void Ctx::ensure_work(int nvec_len) {
+  const size_t N = static_cast<size_t>(nvec_len);
+  if (work_len == N && work_version == operator_version) return;
+  w_F.resize(N);
+  w_u.resize(N);
+  w_r64.resize(N);
+  w_r.resize(N);
+  w_z.resize(N);
+  w_p.resize(N);
+  w_Ap.resize(N);
+  w_r32.resize(N);
+  w_z32.resize(N);
+  w_p32.resize(N);
+  w_Ap32.resize(N);
+  if (!red_partials.p) {
+    red_partials.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
+    red_counter.resize(4);
+    VBH_HIP(hipMemsetAsync(red_counter.p, 0, 4 * sizeof(unsigned), stream));
+  }
+  if (!st_dev.p) st_dev.resize(1);
+  if (!st_host) VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PcgScalars), 0));
+  // zero the padding tails once (F/u beyond n stay zero)
+  launch_fill_f64(static_cast<long long>(N), w_F.p, 0.0, stream);
+  drop_graphs();
+  work_len = N;
+  work_version = operator_version;
+}
+
+void Ctx::drop_graphs() {
+  for (auto& g : graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
+}
+
+PcgParams Ctx::params() const {
+  PcgParams p;
+  p.tol = tol;
+  p.max_iters = max_iters;
+  p.refresh_delta = refresh_delta;
+  return p;
+}
+
+void Ctx::enqueue_iterations(int precision_mode, int count) {
+  RedWs ws{red_partials.p, red_counter.p};
+  const int N = static_cast<int>(work_len);
+  const PcgParams prm = params();
+  for (int i = 0; i < count; ++i) {
+    if (precision_mode == VBH_PREC_FP64) {
+      launch_pcg_spmv_alpha<double>(format, csr, bsrd, N, w_p.p, w_Ap.p, st_dev.p, ws, stream);
+      launch_pcg_update<double>(N, 0, diag64.p, w_p.p, w_Ap.p, w_u.p, w_r.p, w_z.p, st_dev.p, prm,
+                                ws, stream);
+      launch_pcg_pupdate<double>(N, w_z.p, w_p.p, st_dev.p, stream);
+    } else {
+      launch_pcg_spmv_alpha<float>(format, csr, bsrd, N, w_p32.p, w_Ap32.p, st_dev.p, ws, stream);
+      launch_pcg_update<float>(N, 1, diag32.p, w_p32.p, w_Ap32.p, w_u.p, w_r32.p, w_z32.p, st_dev.p,
+                               prm, ws, stream);
+      launch_pcg_pupdate<float>(N, w_z32.p, w_p32.p, st_dev.p, stream);
+    }
+  }
+}
+
+hipGraphExec_t Ctx::iteration_graph(int precision_mode) {
+  GraphSlot& g = graphs[precision_mode];
+  if (g.exec && g.iters == iters_per_graph && g.tol == tol && g.max_iters == max_iters)
+    return g.exec;
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  hipGraph_t graph = nullptr;
+  VBH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  enqueue_iterations(precision_mode, iters_per_graph);
+  VBH_HIP(hipStreamEndCapture(stream, &graph));
+  VBH_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(graph);
+  g.iters = iters_per_graph;
+  g.tol = tol;
+  g.max_iters = max_iters;
+  return g.exec;
+}
+
+void Ctx::read_state() {
+  VBH_HIP(hipMemcpyAsync(st_host, st_dev.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+  VBH_HIP(hipStreamSynchronize(stream));
+}
+
+// Runs one solve on device-resident F (w_F) into w_u. Returns the scalars.
+PcgScalars Ctx::run_solve(int precision_mode) {
+  RedWs ws{red_partials.p, red_counter.p};
+  const int N = static_cast<int>(work_len);
+  const PcgParams prm = params();
+  VBH_HIP(hipEventRecord(ev_start, stream));
+  if (precision_mode == VBH_PREC_FP64)
+    launch_pcg_init<double>(N, w_F.p, diag64.p, w_u.p, w_r.p, w_z.p, w_p.p, st_dev.p, ws, stream);
+  else
+    launch_pcg_init<float>(N, w_F.p, diag32.p, w_u.p, w_r32.p, w_z32.p, w_p32.p, st_dev.p, ws,
+                           stream);
+  read_state();
+  if (!st_host->done) {
+    hipGraphExec_t ge = iteration_graph(precision_mode);
+    for (;;) {
+      VBH_HIP(hipGraphLaunch(ge, stream));
+      read_state();
+      if (!st_host->done) continue;
+      if (precision_mode == VBH_PREC_MIXED && (st_host->flags & kFlagRefresh)) {
+        launch_refresh_true_residual(format, csr, bsrd, N, w_F.p, w_u.p, w_r64.p, st_dev.p, prm, ws,
+                                     stream);
+        launch_refresh_reload(N, w_r64.p, diag32.p, w_r32.p, w_z32.p, w_p32.p, st_dev.p, prm, ws,
+                              stream);
+        read_state();
+        if (!st_host->done) continue;
+      }
+      break;
+    }
+  }
+  VBH_HIP(hipEventRecord(ev_stop, stream));
+  VBH_HIP(hipEventSynchronize(ev_stop));
+  float ms = 0.0f;
+  VBH_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  PcgScalars s = *st_host;
+  stats.iterations = s.iter;
+  stats.refinements = s.refreshes;
+  stats.rel_res = (precision_mode == VBH_PREC_MIXED && (s.flags & (kFlagConverged | kFlagStall)))
+                      ? s.rel_true
+                      : s.rel;
+  if (s.flags & kFlagZeroRhs) stats.rel_res = 0.0;
+  stats.solve_ms = ms;
+  stats.format = format;
+  return s;
+}
+
+// Maps the device flags onto the reference's outcomes.
+void Ctx::check_outcome(const PcgScalars& s, int precision_mode) {
+  if (s.flags & kFlagRz0)
+    throw_error(VBH_ERR_SOLVER, "HIP PCG: non-positive initial r*z=" + fmt_g(s.rz) +
+                                    " -- preconditioner is not SPD or matrix is singular");
+  if (s.flags & (kFlagBreakdown | kFlagMaxIter)) {
+    // Breakdown is reported as non-convergence (impl:1934-1942, 1964-1970).
+    const double rel = (s.flags & kFlagBreakdown) ? 1e300 : s.rel;
+    last_iters = (s.flags & kFlagBreakdown) ? max_iters : s.iter;
+    last_rel = rel;
+    throw_error(VBH_ERR_SOLVER,
+                "HIP PCG: did not converge after " + std::to_string(max_iters) +
+                    " iterations (relative residual " + fmt_g(rel) + " > tolerance " + fmt_g(tol) +
+                    "). Consider increasing max iterations or using a direct solver.");
+  }
+  (void)precision_mode;
+  last_iters = s.iter;
+  last_rel = stats.rel_res;
+}
+
+}  // namespace vbh
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+using vbh::Ctx;
+using vbh::guarded;
+
+extern "C" {
+
+const char* vbh_last_error(void) { return vbh::g_last_error.c_str(); }
+
+int vbh_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
+  if (!out) {
+    vbh::set_error("vbh_ctx_create: null out");
+    return VBH_ERR_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    vbh::set_error("no HIP device available");
+    return VBH_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    vbh::set_error("vbh_ctx_create: device index out of range");
+    return VBH_ERR_NO_DEVICE;
+  }
+  return guarded([&] {
+    auto c = std::make_unique<Ctx>();
+    c->device = device;
+    VBH_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    VBH_HIP(hipGetDeviceProperties(&prop, device));
+    c->name = prop.name;
+    c->arch = prop.gcnArchName;
+    c->tol = tol > 0.0 ? tol : 1e-8;
+    c->max_iters = max_iters > 0 ? max_iters : 10000;
+    VBH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    VBH_HIP(hipEventCreate(&c->ev_start));
+    VBH_HIP(hipEventCreate(&c->ev_stop));
+    *out = reinterpret_cast<vbh_ctx*>(c.release());
+  });
+}
+
+void vbh_ctx_destroy(vbh_ctx* h) {
+  if (!h) return;
+  delete reinterpret_cast<Ctx*>(h);
+}
+
+const char* vbh_device_name(const vbh_ctx* h) {
+  return h ? reinterpret_cast<const Ctx*>(h)->name.c_str() : "";
+}
+
+int vbh_set_precision(vbh_ctx* h, int precision) {
+  if (!h || (precision != VBH_PREC_FP64 && precision != VBH_PREC_MIXED)) {
+    vbh::set_error("vbh_set_precision: bad argument");
+    return VBH_ERR_ARG;
+  }
+  reinterpret_cast<Ctx*>(h)->precision = precision;
+  return VBH_OK;
+}
+
+int vbh_set_tolerance(vbh_ctx* h, double tol, int max_iters) {
+  if (!h) {
+    vbh::set_error("vbh_set_tolerance: null ctx");
+    return VBH_ERR_ARG;
+  }
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  c->tol = tol > 0.0 ? tol : 1e-8;
+  c->max_iters = max_iters > 0 ? max_iters : 10000;
+  return VBH_OK;
+}
+
+int vbh_set_format(vbh_ctx* h, int format) {
+  if (!h || !(format == -1 || format == 0 || format == 3 || format == 6)) {
+    vbh::set_error("vbh_set_format: format must be -1, 0, 3 or 6");
+    return VBH_ERR_ARG;
+  }
+  reinterpret_cast<Ctx*>(h)->format_req = format;
+  return VBH_OK;
+}
+
+int vbh_set_csr(vbh_ctx* h, int n, int nnz, const int* row_ptr, const int* col_ind,
+                const double* values, int lower_only) {
+  if (!h) {
+    vbh::set_error("vbh_set_csr: null ctx");
+    return VBH_ERR_ARG;
+  }
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded([&] { c->upload_operator(n, nnz, row_ptr, col_ind, values, lower_only != 0); });
+}
+
+int vbh_get_csr_size(const vbh_ctx* h, int* n, int* nnz) {
+  if (!h) {
+    vbh::set_error("vbh_get_csr_size: null ctx");
+    return VBH_ERR_ARG;
+  }
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c->have_matrix) {
+    vbh::set_error("no matrix set");
+    return VBH_ERR_STATE;
+  }
+  if (n) *n = c->n;
+  if (nnz) *nnz = c->l_nnz;
+  return VBH_OK;
+}
+
+int vbh_get_csr(const vbh_ctx* h, int* row_ptr, int* col_ind, double* values) {
+  if (!h) {
+    vbh::set_error("vbh_get_csr: null ctx");
+    return VBH_ERR_ARG;
+  }
+  Ctx* c = const_cast<Ctx*>(reinterpret_cast<const Ctx*>(h));
+  if (!c->have_matrix) {
+    vbh::set_error("no matrix set");
+    return VBH_ERR_STATE;
+  }
+  return guarded([&] {
+    c->fetch_lower_to_host();
+    if (row_ptr) std::copy(c->lower_rp.begin(), c->lower_rp.end(), row_ptr);
+    if (col_ind) std::copy(c->lower_ci.begin(), c->lower_ci.end(), col_ind);
+    if (values) std::copy(c->lower_v.begin(), c->lower_v.end(), values);
+  });
+}
+
+static int solve_impl(Ctx* c, const double* F, double* u, bool device_ptrs, int* iters,
+                      double* rel_res) {
+  return guarded([&] {
+    if (!c->have_matrix) vbh::throw_error(VBH_ERR_STATE, "vbh_solve: no matrix set");
+    if (!F || !u) vbh::throw_error(VBH_ERR_ARG, "vbh_solve: null vector");
+    c->select_stream_device();
+    c->ensure_work(c->n_pad);
+    const size_t bytes = static_cast<size_t>(c->n) * sizeof(double);
+    VBH_HIP(hipMemcpyAsync(c->w_F.p, F, bytes,
+                           device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           c->stream));
+    vbh::PcgScalars s = c->run_solve(c->precision);
+    VBH_HIP(hipMemcpyAsync(u, c->w_u.p, bytes,
+                           device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           c->stream));
+    VBH_HIP(hipStreamSynchronize(c->stream));
+    if (iters) *iters = s.iter;
+    if (rel_res) *rel_res = c->stats.rel_res;
+    c->check_outcome(s, c->precision);
+  });
+}
+
+int vbh_solve(vbh_ctx* h, const double* F, double* u, int* iters, double* rel_res) {
+  if (!h) {
+    vbh::set_error("vbh_solve: null ctx");
+    return VBH_ERR_ARG;
+  }
+  return solve_impl(reinterpret_cast<Ctx*>(h), F, u, false, iters, rel_res);
+}
+
+int vbh_solve_device(vbh_ctx* h, const double* dF, double* du, int* iters, double* rel_res) {
+  if (!h) {
+    vbh::set_error("vbh_solve_device: null ctx");
+    return VBH_ERR_ARG;
+  }
+  return solve_impl(reinterpret_cast<Ctx*>(h), dF, du, true, iters, rel_res);
+}
+
+int vbh_last_stats(const vbh_ctx* h, vbh_solve_stats* out) {
+  if (!h || !out) {
+    vbh::set_error("vbh_last_stats: null argument");
+    return VBH_ERR_ARG;
+  }
+  *out = reinterpret_cast<const Ctx*>(h)->stats;
+  return VBH_OK;
+}
+
+int vbh_spmv(vbh_ctx* h, const double* x, double* y) {
+  if (!h) {
+    vbh::set_error("vbh_spmv: null ctx");
+    return VBH_ERR_ARG;
+  }
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded([&] {
+    if (!c->have_matrix) vbh::throw_error(VBH_ERR_STATE, "vbh_spmv: no matrix set");
+    c->select_stream_device();
+    c->ensure_work(c->n_pad);
+    const size_t bytes = static_cast<size_t>(c->n) * sizeof(double);
+    VBH_HIP(hipMemcpyAsync(c->w_r64.p, x, bytes, hipMemcpyHostToDevice, c->stream));
+    if (c->n_pad > c->n)
+      vbh::launch_fill_f64(c->n_pad - c->n, c->w_r64.p + c->n, 0.0, c->stream);
+    if (c->format == 0)
+      vbh::launch_spmv_csr<double>(c->csr, c->w_r64.p, c->w_Ap.p, c->stream);
+    else
+      vbh::launch_spmv_bsr<double>(c->bsrd, c->w_r64.p, c->w_Ap.p, c->stream);
+    VBH_HIP(hipMemcpyAsync(y, c->w_Ap.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    VBH_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
+  if (!h || reps <= 0 || !ms_per_launch) {
+    vbh::set_error("vbh_bench_spmv: bad argument");
+    return VBH_ERR_ARG;
+  }
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded([&] {
+    if (!c->have_matrix) vbh::throw_error(VBH_ERR_STATE, "vbh_bench_spmv: no matrix set");
+    c->select_stream_device();
+    c->ensure_work(c->n_pad);
+    vbh::launch_fill_f64(c->n_pad, c->w_r64.p, 1.0, c->stream);
+    vbh::launch_cast_f64_to_f32(c->n_pad, c->w_r64.p, c->w_p32.p, c->stream);
+    auto one = [&] {
+      if (precision == 0) {
+        if (c->format == 0) vbh::launch_spmv_csr<double>(c->csr, c->w_r64.p, c->w_Ap.p, c->stream);
+        else vbh::launch_spmv_bsr<double>(c->bsrd, c->w_r64.p, c->w_Ap.p, c->stream);
+      } else {
+        if (c->format == 0) vbh::launch_spmv_csr<float>(c->csr, c->w_p32.p, c->w_Ap32.p, c->stream);
+        else vbh::launch_spmv_bsr<float>(c->bsrd, c->w_p32.p, c->w_Ap32.p, c->stream);
+      }
+    };
+    one();
+    VBH_HIP(hipEventRecord(c->ev_start, c->stream));
+    for (int i = 0; i < reps; ++i) one();
+    VBH_HIP(hipEventRecord(c->ev_stop, c->stream));
+    VBH_HIP(hipEventSynchronize(c->ev_stop));
+    float ms = 0.0f;
+    VBH_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    *ms_per_launch = static_cast<double>(ms) / reps;
+  });
+}
+
+int vbh_device_alloc(vbh_ctx* h, long long bytes, void** dptr) {
+  if (!h || !dptr || bytes < 0) {
+    vbh::set_error("vbh_device_alloc: bad argument");
+    return VBH_ERR_ARG;
+  }
+  return guarded([&] {
+    reinterpret_cast<Ctx*>(h)->select_stream_device();
+    VBH_HIP(hipMalloc(dptr, static_cast<size_t>(bytes > 0 ? bytes : 1)));
+  });
+}
+
+int vbh_device_free(vbh_ctx* h, void* dptr) {
+  (void)h;
+  return guarded([&] { VBH_HIP(hipFree(dptr)); });
+}
+
+int vbh_memcpy_h2d(vbh_ctx* h, void* dst, const void* src, long long bytes) {
+  return guarded([&] {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    VBH_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice, c->stream));
+    VBH_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int vbh_memcpy_d2h(vbh_ctx* h, void* dst, const void* src, long long bytes) {
+  return guarded([&] {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    VBH_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost, c->stream));
+    VBH_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+// Block-PCG entry points: implemented in vbh_block.hip.
+}

@@ -1,0 +1,1106 @@
+// vibestran_amd/csrc/vbh_assemble.hip -- DOF numbering, batched element
+// stiffness and lower-CSR assembly on the device (gfx950).
+//
+// Replaces the reference's per-element host loop
+//   make_element -> global_dof_indices -> stiffness_matrix -> to_row_major ->
+//   MpcHandler fast path -> SparseMatrixBuilder::add_element_stiffness ->
+//   build_csr
+// (reference src/solver/assembly_parallel.hpp:68-110, src/core/sparse_matrix.cpp:16-150)
+// with four device passes:
+//   1. element DOF lists           (solid_global_dof_indices, solid_elements.cpp:29-42)
+//   2. topological lower pattern   every (max(gi,gj), min(gi,gj)) pair of every
+//                                  element, radix-sorted and de-duplicated
+//   3. element matrices + scatter  one wave64 per element; each lane owns a few
+//                                  lower entries, accumulated with the closed
+//                                  isotropic form lam*a_p*b_q + mu*a_q*b_p +
+//                                  mu*delta_pq*(a.b) (== B^T D B for the
+//                                  isotropic D of solid_elements.cpp:17-27);
+//                                  EAS modes condensed through a 9x9 Cholesky in
+//                                  LDS (solid_elements.cpp:1201-1204). Entries
+//                                  that are exactly 0.0 are not added, and each
+//                                  CSR slot records whether any element gave it
+//                                  a nonzero (sparse_matrix.cpp:30-33).
+//   4. compaction                  slots nobody touched with a nonzero vanish,
+//                                  so the pattern equals the reference's.
+// Summation order inside a slot follows atomic arrival (the reference's TBB
+// merge order is also thread-dependent, assembly_parallel.hpp:84-100).
+#include <cstring>
+#include <numeric>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <vector>
+
+#include "vbh_ctx.hpp"
+
+namespace vbh {
+namespace {
+
+// ---- lower-triangle pair tables ---------------------------------------------------
+template <int N>
+struct PairTable {
+  unsigned short v[N * (N + 1) / 2];
+  constexpr PairTable() : v{} {
+    int t = 0;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j <= i; ++j) v[t++] = static_cast<unsigned short>((i << 8) | j);
+  }
+};
+__constant__ PairTable<24> c_pairs24;
+__constant__ PairTable<18> c_pairs18;
+__constant__ PairTable<12> c_pairs12;
+__constant__ PairTable<9> c_pairs9;
+
+template <int N>
+__device__ __forceinline__ unsigned short pair_at(int t);
+template <>
+__device__ __forceinline__ unsigned short pair_at<24>(int t) { return c_pairs24.v[t]; }
+template <>
+__device__ __forceinline__ unsigned short pair_at<18>(int t) { return c_pairs18.v[t]; }
+template <>
+__device__ __forceinline__ unsigned short pair_at<12>(int t) { return c_pairs12.v[t]; }
+template <>
+__device__ __forceinline__ unsigned short pair_at<9>(int t) { return c_pairs9.v[t]; }
+
+enum ElemKind : int { kHexEas = 0, kHexSri = 1, kPentaEas = 2, kTet4 = 3 };
+
+template <int K>
+struct Kind {
+  static constexpr int NN = (K == kTet4) ? 4 : (K == kPentaEas ? 6 : 8);
+  static constexpr int ND = 3 * NN;
+  static constexpr int NL = ND * (ND + 1) / 2;
+  static constexpr bool EAS = (K == kHexEas || K == kPentaEas);
+  static constexpr int SU = (NL + 63) / 64;
+  static constexpr int SA = (ND * 9 + 63) / 64;
+};
+
+constexpr int kGdStride = 24;  // element DOF list stride (max DOFs per element)
+
+// Error word: bit 0 non-positive Jacobian, bit 1 Kaa not SPD, bit 2 index error.
+// err[1] holds the lowest offending element index (atomicMin).
+
+// ---- 1. element DOF lists ------------------------------------------------------------
+__global__ void k_gdofs(int nelem, const int* __restrict__ etype, const int* __restrict__ eptr,
+                        const int* __restrict__ conn, const int* __restrict__ eq,
+                        int* __restrict__ gd) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int t = etype[e];
+  const int o = eptr[e], nn = eptr[e + 1] - o;
+  int* g = gd + static_cast<size_t>(e) * kGdStride;
+  int k = 0;
+  if (t == VBH_CQUAD4) {
+    for (int a = 0; a < nn; ++a)
+      for (int d = 0; d < 6; ++d) g[k++] = eq[6 * conn[o + a] + d];
+  } else {
+    for (int a = 0; a < nn; ++a)
+      for (int d = 0; d < 3; ++d) g[k++] = eq[6 * conn[o + a] + d];
+  }
+  for (; k < kGdStride; ++k) g[k] = -1;
+}
+
+// ---- 2. topological pattern keys ----------------------------------------------------------
+template <int ND>
+__global__ void k_pair_keys(int nsel, const int* __restrict__ sel, const int* __restrict__ gd,
+                            unsigned long long* __restrict__ keys, int bits) {
+  constexpr int NL = ND * (ND + 1) / 2;
+  const long long idx = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (idx >= static_cast<long long>(nsel) * NL) return;
+  const int s = static_cast<int>(idx / NL), t = static_cast<int>(idx % NL);
+  const unsigned short pr = pair_at<ND>(t);
+  const int e = sel[s];
+  const int gi = gd[static_cast<size_t>(e) * kGdStride + (pr >> 8)];
+  const int gj = gd[static_cast<size_t>(e) * kGdStride + (pr & 0xff)];
+  unsigned long long key = (1ull << (2 * bits)) - 1ull;  // sentinel sorts last
+  if (gi >= 0 && gj >= 0) {
+    const unsigned long long r = static_cast<unsigned long long>(gi > gj ? gi : gj);
+    const unsigned long long c = static_cast<unsigned long long>(gi > gj ? gj : gi);
+    key = (r << bits) | c;
+  }
+  keys[idx] = key;
+}
+
+__global__ void k_mark_unique(long long m, const unsigned long long* __restrict__ keys,
+                              unsigned long long sentinel, int* __restrict__ head) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= m) return;
+  const unsigned long long k = keys[i];
+  head[i] = (k != sentinel && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+}
+
+__global__ void k_compact_keys(long long m, const unsigned long long* __restrict__ keys,
+                               const int* __restrict__ head, const int* __restrict__ pos,
+                               int bits, int* __restrict__ row_of, int* __restrict__ col) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= m || !head[i]) return;
+  const unsigned long long k = keys[i];
+  const int p = pos[i];
+  row_of[p] = static_cast<int>(k >> bits);
+  col[p] = static_cast<int>(k & ((1ull << bits) - 1ull));
+}
+
+// row_ptr from a row-sorted entry list (handles empty rows).
+__global__ void k_row_ptr(int nnz, int n, const int* __restrict__ row_of, int* __restrict__ rp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nnz) return;
+  const int prev = (i == 0) ? -1 : row_of[i - 1];
+  const int cur = (i == nnz) ? n : row_of[i];
+  for (int r = prev + 1; r <= cur; ++r) rp[r] = i;
+}
+
+__device__ __forceinline__ int find_slot(const int* __restrict__ rp, const int* __restrict__ ci,
+                                         int row, int col) {
+  int lo = rp[row], hi = rp[row + 1] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = ci[mid];
+    if (c == col) return mid;
+    if (c < col) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// ---- 3. element matrices -------------------------------------------------------------------
+struct M3d {
+  double a[3][3];
+};
+
+__device__ __forceinline__ double det3(const M3d& m) {
+  return m.a[0][0] * (m.a[1][1] * m.a[2][2] - m.a[1][2] * m.a[2][1]) -
+         m.a[1][0] * (m.a[0][1] * m.a[2][2] - m.a[0][2] * m.a[2][1]) +
+         m.a[2][0] * (m.a[0][1] * m.a[1][2] - m.a[0][2] * m.a[1][1]);
+}
+
+__device__ __forceinline__ M3d inv3(const M3d& m, double det) {
+  const double id = 1.0 / det;
+  M3d r;
+  r.a[0][0] = (m.a[1][1] * m.a[2][2] - m.a[1][2] * m.a[2][1]) * id;
+  r.a[1][0] = (m.a[1][2] * m.a[2][0] - m.a[1][0] * m.a[2][2]) * id;
+  r.a[2][0] = (m.a[1][0] * m.a[2][1] - m.a[1][1] * m.a[2][0]) * id;
+  r.a[0][1] = (m.a[0][2] * m.a[2][1] - m.a[0][1] * m.a[2][2]) * id;
+  r.a[1][1] = (m.a[0][0] * m.a[2][2] - m.a[0][2] * m.a[2][0]) * id;
+  r.a[2][1] = (m.a[0][1] * m.a[2][0] - m.a[0][0] * m.a[2][1]) * id;
+  r.a[0][2] = (m.a[0][1] * m.a[1][2] - m.a[0][2] * m.a[1][1]) * id;
+  r.a[1][2] = (m.a[0][2] * m.a[1][0] - m.a[0][0] * m.a[1][2]) * id;
+  r.a[2][2] = (m.a[0][0] * m.a[1][1] - m.a[0][1] * m.a[1][0]) * id;
+  return r;
+}
+
+// Parametric shape derivatives (reference CHexa8::shape_functions
+// solid_elements.cpp:76-93 and CPenta6::shape_functions 510-548).
+template <int K>
+__device__ __forceinline__ void shape_d(double r0, double r1, double r2, double (&d)[Kind<K>::NN][3]) {
+  if constexpr (Kind<K>::NN == 8) {
+    constexpr double xn[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    constexpr double en[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    constexpr double zn[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      d[i][0] = 0.125 * xn[i] * (1 + en[i] * r1) * (1 + zn[i] * r2);
+      d[i][1] = 0.125 * en[i] * (1 + xn[i] * r0) * (1 + zn[i] * r2);
+      d[i][2] = 0.125 * zn[i] * (1 + xn[i] * r0) * (1 + en[i] * r1);
+    }
+  } else if constexpr (Kind<K>::NN == 6) {
+    const double L3 = 1.0 - r0 - r1, zm = (1.0 - r2) * 0.5, zp = (1.0 + r2) * 0.5;
+    const double a[6] = {zm, 0.0, -zm, zp, 0.0, -zp};
+    const double b[6] = {0.0, zm, -zm, 0.0, zp, -zp};
+    const double c[6] = {-r0 * 0.5, -r1 * 0.5, -L3 * 0.5, r0 * 0.5, r1 * 0.5, L3 * 0.5};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      d[i][0] = a[i];
+      d[i][1] = b[i];
+      d[i][2] = c[i];
+    }
+  }
+}
+
+template <int NN>
+__device__ __forceinline__ M3d jac(const double (&X)[NN][3], const double (&d)[NN][3]) {
+  M3d J;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) J.a[i][j] = 0.0;
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) J.a[i][j] += d[n][i] * X[n][j];
+  return J;
+}
+
+// Isotropic bilinear form of two "gradient vectors" a (mode direction p) and
+// b (mode direction q): lam*a_p*b_q + mu*a_q*b_p + mu*delta_pq*(a.b).
+__device__ __forceinline__ double iso(double lam, double mu, const double* a, int p,
+                                      const double* b, int q) {
+  double v = lam * a[p] * b[q] + mu * a[q] * b[p];
+  if (p == q) v += mu * (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+  return v;
+}
+
+constexpr int kElemWaves = 4;
+
+template <int K>
+struct ElemLds {
+  static constexpr int ND = Kind<K>::ND;
+  static constexpr int WORDS = Kind<K>::EAS ? (ND * 9 + 81 + 9 * ND) : 1;
+};
+
+template <int K>
+__global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
+    int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
+    const int* __restrict__ conn, const double* __restrict__ xyz, const double* __restrict__ Ev,
+    const double* __restrict__ nuv, const int* __restrict__ gd, const int* __restrict__ rp,
+    const int* __restrict__ ci, double* __restrict__ val, int* __restrict__ touched,
+    int* __restrict__ err) {
+  using KD = Kind<K>;
+  constexpr int NN = KD::NN, ND = KD::ND, NL = KD::NL;
+  __shared__ double lds_all[kElemWaves][ElemLds<K>::WORDS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* lds = lds_all[wave];
+  for (int base = blockIdx.x * kElemWaves; base < nsel; base += gridDim.x * kElemWaves) {
+    const int s = base + wave;
+    const bool active = s < nsel;
+    const int e = active ? sel[s] : sel[base];
+    double X[NN][3];
+    const int o = eptr[e];
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) X[a][d] = xyz[3 * conn[o + a] + d];
+    const double E = Ev[e], nu = nuv[e];
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    const double mu = E / (2 * (1 + nu));
+    double kuu[KD::SU];
+#pragma unroll
+    for (int q = 0; q < KD::SU; ++q) kuu[q] = 0.0;
+    double kua[KD::SA];
+#pragma unroll
+    for (int q = 0; q < KD::SA; ++q) kua[q] = 0.0;
+    double kaa = 0.0;
+    bool bad = false;
+
+    if constexpr (K == kTet4) {
+      // CST tetrahedron (solid_elements.cpp:312-384): gradients from the
+      // cofactors of [1 x y z]; Ke = V * B^T D B.
+      const double x10 = X[1][0] - X[0][0], y10 = X[1][1] - X[0][1], z10 = X[1][2] - X[0][2];
+      const double x20 = X[2][0] - X[0][0], y20 = X[2][1] - X[0][1], z20 = X[2][2] - X[0][2];
+      const double x30 = X[3][0] - X[0][0], y30 = X[3][1] - X[0][1], z30 = X[3][2] - X[0][2];
+      M3d J;
+      J.a[0][0] = x10; J.a[0][1] = y10; J.a[0][2] = z10;
+      J.a[1][0] = x20; J.a[1][1] = y20; J.a[1][2] = z20;
+      J.a[2][0] = x30; J.a[2][1] = y30; J.a[2][2] = z30;
+      const double V6 = det3(J);
+      const double V = V6 / 6.0;
+      if (!(V > 0)) bad = true;
+      const M3d Ji = inv3(J, bad ? 1.0 : V6);
+      double dN[4][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        dN[1][c] = Ji.a[c][0];
+        dN[2][c] = Ji.a[c][1];
+        dN[3][c] = Ji.a[c][2];
+        dN[0][c] = -(dN[1][c] + dN[2][c] + dN[3][c]);
+      }
+#pragma unroll
+      for (int q = 0; q < KD::SU; ++q) {
+        const int t = lane + 64 * q;
+        if (t < NL) {
+          const unsigned short pr = pair_at<ND>(t);
+          const int i = pr >> 8, j = pr & 0xff;
+          kuu[q] = V * iso(lam, mu, dN[i / 3], i % 3, dN[j / 3], j % 3);
+        }
+      }
+    } else {
+      // Gauss rule: hexa 2x2x2 (weight 1), penta 3 triangle x 2 axial (1/6).
+      constexpr int NG = (NN == 8) ? 8 : 6;
+      const double gp = 0.57735026918962584;  // 1/sqrt(3)
+      // centroidal Jacobian (EAS mapping / SRI volumetric point)
+      double d0[NN][3];
+      if constexpr (NN == 8) shape_d<K>(0.0, 0.0, 0.0, d0);
+      else shape_d<K>(1.0 / 3.0, 1.0 / 3.0, 0.0, d0);
+      const M3d J0 = jac<NN>(X, d0);
+      const double detJ0 = det3(J0);
+      if (!(detJ0 > 0)) bad = true;
+      const M3d J0i = inv3(J0, bad ? 1.0 : detJ0);
+#pragma unroll 1
+      for (int g = 0; g < NG; ++g) {
+        double r0, r1, r2, wq;
+        if constexpr (NN == 8) {
+          r0 = (g & 4) ? gp : -gp;
+          r1 = (g & 2) ? gp : -gp;
+          r2 = (g & 1) ? gp : -gp;
+          wq = 1.0;
+        } else {
+          const int ti = g >> 1;
+          r0 = (ti == 0) ? 2.0 / 3.0 : 1.0 / 6.0;
+          r1 = (ti == 1) ? 2.0 / 3.0 : 1.0 / 6.0;
+          r2 = (g & 1) ? gp : -gp;
+          wq = 1.0 / 6.0;
+        }
+        double d[NN][3];
+        shape_d<K>(r0, r1, r2, d);
+        const M3d J = jac<NN>(X, d);
+        const double detJ = det3(J);
+        if (!(detJ > 0)) bad = true;
+        const M3d Ji = inv3(J, bad ? 1.0 : detJ);
+        double dN[NN][3];
+#pragma unroll
+        for (int n = 0; n < NN; ++n)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            dN[n][c] = Ji.a[c][0] * d[n][0] + Ji.a[c][1] * d[n][1] + Ji.a[c][2] * d[n][2];
+        const double w = detJ * wq;
+        const double lamg = (K == kHexSri) ? 0.0 : lam;  // SRI: deviatoric part here
+#pragma unroll
+        for (int q = 0; q < KD::SU; ++q) {
+          const int t = lane + 64 * q;
+          if (t < NL) {
+            const unsigned short pr = pair_at<ND>(t);
+            const int i = pr >> 8, j = pr & 0xff;
+            kuu[q] += w * iso(lamg, mu, dN[i / 3], i % 3, dN[j / 3], j % 3);
+          }
+        }
+        if constexpr (KD::EAS) {
+          // enhanced-mode gradients g_k (scaled by detJ0/detJ):
+          //   hexa: rows of J0^{-1} times (xi, eta, zeta)   (solid_elements.cpp:1131-1133)
+          //   penta: columns of J0^{-1} times (L1-1/3, L2-1/3, zeta) (1399-1410)
+          const double sc = detJ0 / detJ;
+          double gk[3][3];
+          if constexpr (NN == 8) {
+            const double prm[3] = {r0, r1, r2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) gk[k][c] = J0i.a[k][c] * prm[k] * sc;
+          } else {
+            const double f[3] = {r0 - 1.0 / 3.0, r1 - 1.0 / 3.0, r2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) gk[k][c] = J0i.a[c][k] * f[k] * sc;
+          }
+#pragma unroll
+          for (int q = 0; q < KD::SA; ++q) {
+            const int t = lane + 64 * q;
+            if (t < ND * 9) {
+              const int i = t / 9, m = t % 9;
+              kua[q] += w * iso(lam, mu, dN[i / 3], i % 3, gk[m / 3], m % 3);
+            }
+          }
+          if (lane < 45) {
+            const unsigned short pr = pair_at<9>(lane);
+            const int m1 = pr >> 8, m2 = pr & 0xff;
+            kaa += w * iso(lam, mu, gk[m1 / 3], m1 % 3, gk[m2 / 3], m2 % 3);
+          }
+        }
+      }
+      if constexpr (K == kHexSri) {
+        // volumetric part at the centroid, weight 8 (solid_elements.cpp:153-177)
+        double dN[NN][3];
+#pragma unroll
+        for (int n = 0; n < NN; ++n)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            dN[n][c] = J0i.a[c][0] * d0[n][0] + J0i.a[c][1] * d0[n][1] + J0i.a[c][2] * d0[n][2];
+        const double w = detJ0 * 8.0;
+#pragma unroll
+        for (int q = 0; q < KD::SU; ++q) {
+          const int t = lane + 64 * q;
+          if (t < NL) {
+            const unsigned short pr = pair_at<ND>(t);
+            const int i = pr >> 8, j = pr & 0xff;
+            kuu[q] += w * lam * dN[i / 3][i % 3] * dN[j / 3][j % 3];
+          }
+        }
+      }
+    }
+
+    if constexpr (KD::EAS) {
+      // static condensation Ke = Kuu - Kua Kaa^{-1} Kua^T, Kaa factored in LDS
+      double* sKua = lds;               // [ND][9]
+      double* sKaa = lds + ND * 9;      // [9][9] -> L in place
+      double* sX = sKaa + 81;           // [9][ND]
+#pragma unroll
+      for (int q = 0; q < KD::SA; ++q) {
+        const int t = lane + 64 * q;
+        if (t < ND * 9) sKua[t] = kua[q];
+      }
+      if (lane < 45) {
+        const unsigned short pr = pair_at<9>(lane);
+        const int m1 = pr >> 8, m2 = pr & 0xff;
+        sKaa[m1 * 9 + m2] = kaa;
+        sKaa[m2 * 9 + m1] = kaa;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        for (int j = 0; j < 9; ++j) {
+          double dj = sKaa[j * 9 + j];
+          for (int k = 0; k < j; ++k) dj -= sKaa[j * 9 + k] * sKaa[j * 9 + k];
+          if (!(dj > 0.0)) {
+            dj = 1.0;
+            if (active) atomicOr(&err[0], 2);
+          }
+          dj = sqrt(dj);
+          sKaa[j * 9 + j] = dj;
+          for (int i = j + 1; i < 9; ++i) {
+            double sv = sKaa[i * 9 + j];
+            for (int k = 0; k < j; ++k) sv -= sKaa[i * 9 + k] * sKaa[j * 9 + k];
+            sKaa[i * 9 + j] = sv / dj;
+          }
+        }
+      }
+      __syncthreads();
+      if (lane < ND) {
+        double y[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+          double sv = sKua[lane * 9 + i];
+          for (int k = 0; k < i; ++k) sv -= sKaa[i * 9 + k] * y[k];
+          y[i] = sv / sKaa[i * 9 + i];
+        }
+#pragma unroll
+        for (int i = 8; i >= 0; --i) {
+          double sv = y[i];
+          for (int k = i + 1; k < 9; ++k) sv -= sKaa[k * 9 + i] * y[k];
+          y[i] = sv / sKaa[i * 9 + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sX[i * ND + lane] = y[i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < KD::SU; ++q) {
+        const int t = lane + 64 * q;
+        if (t < NL) {
+          const unsigned short pr = pair_at<ND>(t);
+          const int i = pr >> 8, j = pr & 0xff;
+          double sv = 0.0;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) sv += sKua[i * 9 + k] * sX[k * ND + j];
+          kuu[q] -= sv;
+        }
+      }
+      __syncthreads();
+    }
+
+    if (active) {
+      if (bad && lane == 0) {
+        atomicOr(&err[0], 1);
+        atomicMin(&err[1], e);
+      }
+      const int* g = gd + static_cast<size_t>(e) * kGdStride;
+#pragma unroll
+      for (int q = 0; q < KD::SU; ++q) {
+        const int t = lane + 64 * q;
+        if (t < NL) {
+          const unsigned short pr = pair_at<ND>(t);
+          const int gi = g[pr >> 8], gj = g[pr & 0xff];
+          const double v = kuu[q];
+          if (gi >= 0 && gj >= 0 && v != 0.0) {
+            const int row = gi > gj ? gi : gj, col = gi > gj ? gj : gi;
+            const int slot = find_slot(rp, ci, row, col);
+            if (slot < 0) {
+              atomicOr(&err[0], 4);
+            } else {
+              atomicAdd(&val[slot], v);
+              touched[slot] = 1;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---- 4. compaction -------------------------------------------------------------------------------
+__global__ void k_compact_csr(int nnz, const int* __restrict__ touched, const int* __restrict__ pos,
+                              const int* __restrict__ ci, const double* __restrict__ v,
+                              const int* __restrict__ row_of, int* __restrict__ ci2,
+                              double* __restrict__ v2, int* __restrict__ row2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz || !touched[i]) return;
+  const int p = pos[i];
+  ci2[p] = ci[i];
+  v2[p] = v[i];
+  row2[p] = row_of[i];
+}
+
+// Jacobi diagonal from the lower CSR (diagonal is the last entry of each row);
+// err bit 8 zero diagonal, bit 16 missing diagonal, err[2] = first bad row.
+__global__ void k_diag_from_lower(int n, int n_pad, const int* __restrict__ rp,
+                                  const int* __restrict__ ci, const double* __restrict__ v,
+                                  double* __restrict__ d64, float* __restrict__ d32,
+                                  int* __restrict__ err) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_pad) return;
+  double d = 1.0;
+  if (r < n) {
+    const int e = rp[r + 1] - 1;
+    if (e < rp[r] || ci[e] != r) {
+      atomicOr(&err[0], 16);
+      atomicMin(&err[2], r);
+    } else {
+      d = v[e];
+      if (d == 0.0) {
+        atomicOr(&err[0], 8);
+        atomicMin(&err[2], r);
+      }
+    }
+  }
+  d64[r] = d;
+  d32[r] = static_cast<float>(d);
+}
+
+// ---- BSR (full symmetric) from the lower CSR ----------------------------------------------------
+__global__ void k_block_keys(int nnz, int B, const int* __restrict__ row_of,
+                             const int* __restrict__ ci, unsigned long long* __restrict__ keys,
+                             int bits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const unsigned long long I = static_cast<unsigned long long>(row_of[i] / B);
+  const unsigned long long J = static_cast<unsigned long long>(ci[i] / B);
+  keys[2 * static_cast<size_t>(i)] = (I << bits) | J;
+  keys[2 * static_cast<size_t>(i) + 1] = (J << bits) | I;
+}
+
+template <int B>
+__global__ void k_bsr_fill(int nnz, int n, const int* __restrict__ row_of, const int* __restrict__ ci,
+                           const double* __restrict__ v, const int* __restrict__ brp,
+                           const int* __restrict__ bci, double* __restrict__ bv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const int r = row_of[i], c = ci[i];
+  const double x = v[i];
+  {
+    const int I = r / B, J = c / B;
+    const int o = brp[I], nb = brp[I + 1] - o;
+    const int k = find_slot(brp, bci, I, J) - o;
+    bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((r % B) * B + (c % B)) * nb + k] = x;
+  }
+  if (r != c) {
+    const int I = c / B, J = r / B;
+    const int o = brp[I], nb = brp[I + 1] - o;
+    const int k = find_slot(brp, bci, I, J) - o;
+    bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((c % B) * B + (r % B)) * nb + k] = x;
+  }
+}
+
+template <int B>
+__global__ void k_bsr_pad_diag(int n, int nbr, const int* __restrict__ brp,
+                               const int* __restrict__ bci, double* __restrict__ bv) {
+  const int r = n + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nbr * B) return;
+  const int I = r / B;
+  const int o = brp[I], nb = brp[I + 1] - o;
+  const int k = find_slot(brp, bci, I, I) - o;
+  bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((r % B) * B + (r % B)) * nb + k] = 1.0;
+}
+
+__global__ void k_cast(long long m, const double* __restrict__ a, float* __restrict__ b) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    b[i] = static_cast<float>(a[i]);
+}
+
+__global__ void k_zero_i32(long long m, int* __restrict__ a) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    a[i] = 0;
+}
+
+__global__ void k_zero_f64(long long m, double* __restrict__ a) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    a[i] = 0.0;
+}
+
+__global__ void k_init_err(int* err) {
+  err[0] = 0;
+  err[1] = 0x7fffffff;
+  err[2] = 0x7fffffff;
+  err[3] = 0;
+}
+
+// ---- DOF numbering -------------------------------------------------------------------------------
+__global__ void k_dof_active(int nelem, const int* __restrict__ etype, const int* __restrict__ eptr,
+                             const int* __restrict__ conn, int* __restrict__ mask) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int bits = (etype[e] == VBH_CQUAD4) ? 0x3f : 0x07;  // mark_element_dofs 39-68
+  for (int k = eptr[e]; k < eptr[e + 1]; ++k) atomicOr(&mask[conn[k]], bits);
+}
+
+__global__ void k_dof_spc(int nspc, const int* __restrict__ sn, const int* __restrict__ sm,
+                          int* __restrict__ spc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nspc) return;
+  atomicOr(&spc[sn[i]], sm[i] & 0x3f);
+}
+
+__global__ void k_dof_count(int nn, const int* __restrict__ order, const int* __restrict__ mask,
+                            const int* __restrict__ spc, int* __restrict__ cnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nn) return;
+  const int i = order[s];
+  cnt[s] = __popc(mask[i] & ~spc[i]);
+}
+
+__global__ void k_dof_assign(int nn, const int* __restrict__ order, const int* __restrict__ mask,
+                             const int* __restrict__ spc, const int* __restrict__ base,
+                             int* __restrict__ eq) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nn) return;
+  const int i = order[s];
+  const int f = mask[i] & ~spc[i];
+  int k = base[s];
+  for (int d = 0; d < 6; ++d) eq[6 * i + d] = ((f >> d) & 1) ? k++ : -1;
+}
+
+int bits_for(long long n) {
+  int b = 1;
+  while ((1ll << b) <= n) ++b;
+  return b;
+}
+
+template <typename T>
+void dzero(DBuf<T>& b, hipStream_t s);
+template <>
+void dzero<int>(DBuf<int>& b, hipStream_t s) {
+  if (b.n) hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(b.n, 1024)), dim3(256), 0, s,
+                              static_cast<long long>(b.n), b.p);
+}
+template <>
+void dzero<double>(DBuf<double>& b, hipStream_t s) {
+  if (b.n) hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(b.n, 1024)), dim3(256), 0, s,
+                              static_cast<long long>(b.n), b.p);
+}
+
+// rocPRIM helpers ----------------------------------------------------------------------------------
+void sort_keys(DBuf<unsigned long long>& in, DBuf<unsigned long long>& out, size_t m, int end_bit,
+               hipStream_t s) {
+  size_t tmp = 0;
+  VBH_HIP(rocprim::radix_sort_keys(nullptr, tmp, in.p, out.p, m, 0u, static_cast<unsigned>(end_bit), s));
+  DBuf<char> t;
+  t.resize(std::max<size_t>(tmp, 1));
+  VBH_HIP(rocprim::radix_sort_keys(t.p, tmp, in.p, out.p, m, 0u, static_cast<unsigned>(end_bit), s));
+  VBH_HIP(hipStreamSynchronize(s));
+}
+
+void excl_scan(const int* in, int* out, size_t m, hipStream_t s) {
+  size_t tmp = 0;
+  VBH_HIP(rocprim::exclusive_scan(nullptr, tmp, in, out, 0, m, rocprim::plus<int>(), s));
+  DBuf<char> t;
+  t.resize(std::max<size_t>(tmp, 1));
+  VBH_HIP(rocprim::exclusive_scan(t.p, tmp, in, out, 0, m, rocprim::plus<int>(), s));
+  VBH_HIP(hipStreamSynchronize(s));
+}
+
+int read_last_plus(const int* scan, const int* flags, size_t m, hipStream_t s) {
+  if (m == 0) return 0;
+  int a = 0, b = 0;
+  VBH_HIP(hipMemcpyAsync(&a, scan + m - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipMemcpyAsync(&b, flags + m - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  return a + b;
+}
+
+}  // namespace
+
+// Unique sorted (row,col) list from sorted keys: fills row_of/col, returns count.
+static int unique_pairs(DBuf<unsigned long long>& keys, size_t m, int bits, DBuf<int>& row_of,
+                        DBuf<int>& col, hipStream_t s) {
+  DBuf<int> head, pos;
+  head.resize(std::max<size_t>(m, 1));
+  pos.resize(std::max<size_t>(m, 1));
+  const unsigned long long sentinel = (1ull << (2 * bits)) - 1ull;
+  hipLaunchKernelGGL(k_mark_unique, dim3((m + 255) / 256), dim3(256), 0, s,
+                     static_cast<long long>(m), keys.p, sentinel, head.p);
+  excl_scan(head.p, pos.p, m, s);
+  const int cnt = read_last_plus(pos.p, head.p, m, s);
+  row_of.resize(std::max(cnt, 1));
+  col.resize(std::max(cnt, 1));
+  hipLaunchKernelGGL(k_compact_keys, dim3((m + 255) / 256), dim3(256), 0, s,
+                     static_cast<long long>(m), keys.p, head.p, pos.p, bits, row_of.p, col.p);
+  VBH_HIP(hipStreamSynchronize(s));
+  return cnt;
+}
+
+// Build the BSR(B) operator on the device from the device lower CSR held in
+// ctx (l_rp/l_ci/l_v/l_row). Returns false if the block fill is too high.
+bool build_device_bsr(Ctx& c, int B) {
+  hipStream_t s = c.stream;
+  const int n = c.n, nnz = c.l_nnz;
+  const int nbr = (n + B - 1) / B;
+  const int bits = bits_for(nbr);
+  DBuf<unsigned long long> keys, sorted;
+  const size_t m = 2 * static_cast<size_t>(nnz);
+  keys.resize(m);
+  sorted.resize(m);
+  hipLaunchKernelGGL(k_block_keys, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, B, c.l_row.p,
+                     c.l_ci.p, keys.p, bits);
+  sort_keys(keys, sorted, m, 2 * bits, s);
+  keys.release();
+  DBuf<int> brow, bcol;
+  const int nnzb = unique_pairs(sorted, m, bits, brow, bcol, s);
+  sorted.release();
+  const long long full_nnz = 2ll * nnz - n;
+  const double fill = static_cast<double>(nnzb) * B * B / static_cast<double>(full_nnz);
+  if (fill > (B == 6 ? 1.12 : 1.25)) return false;
+  c.bsr_rp.resize(static_cast<size_t>(nbr) + 1);
+  hipLaunchKernelGGL(k_row_ptr, dim3((nnzb + 1 + 255) / 256), dim3(256), 0, s, nnzb, nbr, brow.p,
+                     c.bsr_rp.p);
+  c.bsr_ci.resize(static_cast<size_t>(nnzb));
+  VBH_HIP(hipMemcpyAsync(c.bsr_ci.p, bcol.p, sizeof(int) * nnzb, hipMemcpyDeviceToDevice, s));
+  c.bsr_v64.resize(static_cast<size_t>(nnzb) * B * B);
+  dzero(c.bsr_v64, s);
+  if (B == 3) {
+    hipLaunchKernelGGL(k_bsr_fill<3>, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, n, c.l_row.p,
+                       c.l_ci.p, c.l_v.p, c.bsr_rp.p, c.bsr_ci.p, c.bsr_v64.p);
+    if (nbr * 3 > n)
+      hipLaunchKernelGGL(k_bsr_pad_diag<3>, dim3(1), dim3(64), 0, s, n, nbr, c.bsr_rp.p, c.bsr_ci.p,
+                         c.bsr_v64.p);
+  } else {
+    hipLaunchKernelGGL(k_bsr_fill<6>, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, n, c.l_row.p,
+                       c.l_ci.p, c.l_v.p, c.bsr_rp.p, c.bsr_ci.p, c.bsr_v64.p);
+    if (nbr * 6 > n)
+      hipLaunchKernelGGL(k_bsr_pad_diag<6>, dim3(1), dim3(64), 0, s, n, nbr, c.bsr_rp.p, c.bsr_ci.p,
+                         c.bsr_v64.p);
+  }
+  c.bsr_v32.resize(c.bsr_v64.n);
+  hipLaunchKernelGGL(k_cast, dim3(grid_for(c.bsr_v64.n, 1024)), dim3(256), 0, s,
+                     static_cast<long long>(c.bsr_v64.n), c.bsr_v64.p, c.bsr_v32.p);
+  VBH_HIP(hipStreamSynchronize(s));
+  c.bsrd.B = B;
+  c.bsrd.nbr = nbr;
+  c.bsrd.nnzb = nnzb;
+  c.bsrd.brp = c.bsr_rp.p;
+  c.bsrd.bci = c.bsr_ci.p;
+  c.bsrd.v64 = c.bsr_v64.p;
+  c.bsrd.v32 = c.bsr_v32.p;
+  return true;
+}
+
+void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int nelem,
+                   const int* etype_h, const int* eptr_h, const int* conn_h, const double* E_h,
+                   const double* nu_h, const double* thick_h, int solid_form, int shell_form) {
+  (void)thick_h;
+  (void)shell_form;
+  select_stream_device();
+  hipStream_t s = stream;
+  if (n_ <= 0) throw_error(VBH_ERR_SOLVER, "HIP PCG: stiffness matrix is empty -- no free DOFs");
+  if (nelem <= 0 || nnodes <= 0) throw_error(VBH_ERR_ARG, "vbh_assemble: empty mesh");
+  // host-side grouping by element kind
+  std::vector<int> groups[4];
+  for (int e = 0; e < nelem; ++e) {
+    const int t = etype_h[e], nn = eptr_h[e + 1] - eptr_h[e];
+    if (t == VBH_CHEXA8 && nn == 8) groups[solid_form == VBH_SOLID_SRI ? kHexSri : kHexEas].push_back(e);
+    else if (t == VBH_CPENTA6 && nn == 6) groups[kPentaEas].push_back(e);
+    else if (t == VBH_CTETRA4 && nn == 4) groups[kTet4].push_back(e);
+    else throw_error(VBH_ERR_ARG, "vbh_assemble: unsupported element type or node count at element " +
+                                      std::to_string(e));
+  }
+  const int nconn = eptr_h[nelem];
+  for (int k = 0; k < nconn; ++k)
+    if (conn_h[k] < 0 || conn_h[k] >= nnodes)
+      throw_error(VBH_ERR_ARG, "vbh_assemble: connectivity index out of range");
+  for (int k = 0; k < 6 * nnodes; ++k)
+    if (eq_h[k] < -1 || eq_h[k] >= n_) throw_error(VBH_ERR_ARG, "vbh_assemble: eq index out of range");
+
+  DBuf<double> d_xyz, d_E, d_nu;
+  DBuf<int> d_eq, d_et, d_ep, d_conn, d_gd, d_err;
+  d_xyz.upload(xyz_h, static_cast<size_t>(nnodes) * 3, s);
+  d_eq.upload(eq_h, static_cast<size_t>(nnodes) * 6, s);
+  d_et.upload(etype_h, static_cast<size_t>(nelem), s);
+  d_ep.upload(eptr_h, static_cast<size_t>(nelem) + 1, s);
+  d_conn.upload(conn_h, static_cast<size_t>(nconn), s);
+  d_E.upload(E_h, static_cast<size_t>(nelem), s);
+  d_nu.upload(nu_h, static_cast<size_t>(nelem), s);
+  d_gd.resize(static_cast<size_t>(nelem) * kGdStride);
+  d_err.resize(4);
+  hipLaunchKernelGGL(k_init_err, dim3(1), dim3(1), 0, s, d_err.p);
+  hipLaunchKernelGGL(k_gdofs, dim3((nelem + 255) / 256), dim3(256), 0, s, nelem, d_et.p, d_ep.p,
+                     d_conn.p, d_eq.p, d_gd.p);
+  DBuf<int> d_sel[4];
+  for (int k = 0; k < 4; ++k)
+    if (!groups[k].empty()) d_sel[k].upload(groups[k].data(), groups[k].size(), s);
+
+  hipEvent_t e0, e1, e2;
+  VBH_HIP(hipEventCreate(&e0));
+  VBH_HIP(hipEventCreate(&e1));
+  VBH_HIP(hipEventCreate(&e2));
+  VBH_HIP(hipEventRecord(e0, s));
+
+  // 2. topological pattern
+  const int bits = bits_for(n_);
+  size_t m = 0;
+  const int ndk[4] = {24, 24, 18, 12};
+  for (int k = 0; k < 4; ++k) m += groups[k].size() * static_cast<size_t>(ndk[k] * (ndk[k] + 1) / 2);
+  DBuf<int> t_row, t_col;
+  int nnz_topo = 0;
+  {
+    DBuf<unsigned long long> keys, sorted;
+    keys.resize(m);
+    sorted.resize(m);
+    size_t off = 0;
+    for (int k = 0; k < 4; ++k) {
+      const int ns = static_cast<int>(groups[k].size());
+      if (!ns) continue;
+      const int NL = ndk[k] * (ndk[k] + 1) / 2;
+      const long long cnt = static_cast<long long>(ns) * NL;
+      const dim3 g(static_cast<unsigned>((cnt + 255) / 256));
+      if (ndk[k] == 24)
+        hipLaunchKernelGGL(k_pair_keys<24>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p, keys.p + off, bits);
+      else if (ndk[k] == 18)
+        hipLaunchKernelGGL(k_pair_keys<18>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p, keys.p + off, bits);
+      else
+        hipLaunchKernelGGL(k_pair_keys<12>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p, keys.p + off, bits);
+      off += static_cast<size_t>(cnt);
+    }
+    sort_keys(keys, sorted, m, 2 * bits, s);
+    keys.release();
+    nnz_topo = unique_pairs(sorted, m, bits, t_row, t_col, s);
+  }
+  DBuf<int> t_rp;
+  t_rp.resize(static_cast<size_t>(n_) + 1);
+  hipLaunchKernelGGL(k_row_ptr, dim3((nnz_topo + 1 + 255) / 256), dim3(256), 0, s, nnz_topo, n_,
+                     t_row.p, t_rp.p);
+  VBH_HIP(hipEventRecord(e1, s));
+
+  // 3. element matrices + scatter
+  DBuf<double> t_val;
+  DBuf<int> touched;
+  t_val.resize(static_cast<size_t>(nnz_topo));
+  touched.resize(static_cast<size_t>(nnz_topo));
+  dzero(t_val, s);
+  dzero(touched, s);
+  for (int k = 0; k < 4; ++k) {
+    const int ns = static_cast<int>(groups[k].size());
+    if (!ns) continue;
+    const dim3 g(static_cast<unsigned>(std::min((ns + kElemWaves - 1) / kElemWaves, 65535)));
+    const dim3 b(64 * kElemWaves);
+    if (k == kHexEas)
+      hipLaunchKernelGGL(k_element_stiffness<kHexEas>, g, b, 0, s, ns, d_sel[k].p, d_ep.p, d_conn.p,
+                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p);
+    else if (k == kHexSri)
+      hipLaunchKernelGGL(k_element_stiffness<kHexSri>, g, b, 0, s, ns, d_sel[k].p, d_ep.p, d_conn.p,
+                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p);
+    else if (k == kPentaEas)
+      hipLaunchKernelGGL(k_element_stiffness<kPentaEas>, g, b, 0, s, ns, d_sel[k].p, d_ep.p,
+                         d_conn.p, d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p,
+                         touched.p, d_err.p);
+    else
+      hipLaunchKernelGGL(k_element_stiffness<kTet4>, g, b, 0, s, ns, d_sel[k].p, d_ep.p, d_conn.p,
+                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p);
+  }
+  VBH_HIP(hipEventRecord(e2, s));
+  int herr[4];
+  VBH_HIP(hipMemcpyAsync(herr, d_err.p, sizeof herr, hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  float ms_pat = 0, ms_elem = 0;
+  VBH_HIP(hipEventElapsedTime(&ms_pat, e0, e1));
+  VBH_HIP(hipEventElapsedTime(&ms_elem, e1, e2));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  if (herr[0] & 1)
+    throw_error(VBH_ERR_SOLVER, "element " + std::to_string(herr[1]) + ": non-positive Jacobian");
+  if (herr[0] & 2) throw_error(VBH_ERR_SOLVER, "EAS element: Kaa factorization failed");
+  if (herr[0] & 4) throw_error(VBH_ERR_HIP, "assembly: CSR slot not found (internal error)");
+
+  // 4. compaction of slots without a nonzero contribution
+  DBuf<int> pos;
+  pos.resize(std::max(nnz_topo, 1));
+  excl_scan(touched.p, pos.p, static_cast<size_t>(nnz_topo), s);
+  const int nnz = read_last_plus(pos.p, touched.p, static_cast<size_t>(nnz_topo), s);
+  l_ci.resize(static_cast<size_t>(std::max(nnz, 1)));
+  l_v.resize(static_cast<size_t>(std::max(nnz, 1)));
+  l_row.resize(static_cast<size_t>(std::max(nnz, 1)));
+  hipLaunchKernelGGL(k_compact_csr, dim3((nnz_topo + 255) / 256), dim3(256), 0, s, nnz_topo,
+                     touched.p, pos.p, t_col.p, t_val.p, t_row.p, l_ci.p, l_v.p, l_row.p);
+  l_rp.resize(static_cast<size_t>(n_) + 1);
+  hipLaunchKernelGGL(k_row_ptr, dim3((nnz + 1 + 255) / 256), dim3(256), 0, s, nnz, n_, l_row.p,
+                     l_rp.p);
+  VBH_HIP(hipStreamSynchronize(s));
+  n = n_;
+  l_nnz = nnz;
+  asm_stats.nnz_topological = nnz_topo;
+  asm_stats.nnz = nnz;
+  asm_stats.pattern_ms = ms_pat;
+  asm_stats.element_ms = ms_elem;
+  lower_on_host = false;
+  {
+    bool shells = false, solids = false;
+    for (int e = 0; e < nelem; ++e) (etype_h[e] == VBH_CQUAD4 ? shells : solids) = true;
+    block_hint = (shells && !solids) ? 6 : (solids && !shells ? 3 : 0);
+  }
+  finish_device_operator();
+}
+
+// Device operator from the device lower CSR: diagonal checks, BSR build (or a
+// host round trip for matrices without block structure).
+void Ctx::finish_device_operator() {
+  hipStream_t s = stream;
+  // Block size: 6 when all rows come in node groups of six, else 3.
+  int B = format_req;
+  if (B < 0) B = block_hint;
+  bool ok = false;
+  if (B == 3 || B == 6) ok = build_device_bsr(*this, B);
+  if (!ok) {
+    // no usable block structure: go through the host conversion path
+    fetch_lower_to_host();
+    const std::vector<int> rp = lower_rp, ci = lower_ci;
+    const std::vector<double> v = lower_v;
+    upload_operator(n, static_cast<int>(ci.size()), rp.data(), ci.data(), v.data(), true);
+    return;
+  }
+  format = B;
+  n_pad = bsrd.nbr * B;
+  DBuf<int> d_err;
+  d_err.resize(4);
+  hipLaunchKernelGGL(k_init_err, dim3(1), dim3(1), 0, s, d_err.p);
+  diag64.resize(static_cast<size_t>(n_pad));
+  diag32.resize(static_cast<size_t>(n_pad));
+  hipLaunchKernelGGL(k_diag_from_lower, dim3((n_pad + 255) / 256), dim3(256), 0, s, n, n_pad,
+                     l_rp.p, l_ci.p, l_v.p, diag64.p, diag32.p, d_err.p);
+  int herr[4];
+  VBH_HIP(hipMemcpyAsync(herr, d_err.p, sizeof herr, hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  if (herr[0] & 16)
+    throw_error(VBH_ERR_SOLVER, "HIP PCG: missing diagonal at row " + std::to_string(herr[2]) +
+                                    " -- matrix is singular. Check boundary conditions.");
+  if (herr[0] & 8)
+    throw_error(VBH_ERR_SOLVER, "HIP PCG: zero diagonal at row " + std::to_string(herr[2]) +
+                                    " -- matrix is singular. Check boundary conditions.");
+  // scalar CSR is not built for device-assembled operators
+  csr = CsrDev{};
+  csr.n = n;
+  csr_rp.release();
+  csr_ci.release();
+  csr_v64.release();
+  csr_v32.release();
+  operator_version += 1;
+  have_matrix = true;
+}
+
+void Ctx::fetch_lower_to_host() {
+  if (lower_on_host) return;
+  lower_rp.resize(static_cast<size_t>(n) + 1);
+  lower_ci.resize(static_cast<size_t>(l_nnz));
+  lower_v.resize(static_cast<size_t>(l_nnz));
+  VBH_HIP(hipMemcpyAsync(lower_rp.data(), l_rp.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, stream));
+  if (l_nnz) {
+    VBH_HIP(hipMemcpyAsync(lower_ci.data(), l_ci.p, sizeof(int) * l_nnz, hipMemcpyDeviceToHost, stream));
+    VBH_HIP(hipMemcpyAsync(lower_v.data(), l_v.p, sizeof(double) * l_nnz, hipMemcpyDeviceToHost, stream));
+  }
+  VBH_HIP(hipStreamSynchronize(stream));
+  lower_on_host = true;
+}
+
+// GPU DOF numbering (closed form of build_analysis_dof_map).
+int Ctx::dof_map(int nnodes, const int* ids_h, int nelem, const int* etype_h, const int* eptr_h,
+                 const int* conn_h, int nspc, const int* spc_nodes_h, const int* spc_masks_h,
+                 int* eq_out) {
+  select_stream_device();
+  hipStream_t s = stream;
+  if (nnodes <= 0) throw_error(VBH_ERR_ARG, "vbh_dof_map: no nodes");
+  const int nconn = nelem > 0 ? eptr_h[nelem] : 0;
+  for (int k = 0; k < nconn; ++k)
+    if (conn_h[k] < 0 || conn_h[k] >= nnodes) throw_error(VBH_ERR_ARG, "vbh_dof_map: bad connectivity");
+  for (int k = 0; k < nspc; ++k)
+    if (spc_nodes_h[k] < 0 || spc_nodes_h[k] >= nnodes) throw_error(VBH_ERR_ARG, "vbh_dof_map: bad SPC node");
+  DBuf<int> ids, idx, ids_s, order, et, ep, cn, mask, spc, sn, sm, cnt, base, eq;
+  ids.upload(ids_h, static_cast<size_t>(nnodes), s);
+  std::vector<int> iota(static_cast<size_t>(nnodes));
+  std::iota(iota.begin(), iota.end(), 0);
+  idx.upload(iota.data(), iota.size(), s);
+  ids_s.resize(static_cast<size_t>(nnodes));
+  order.resize(static_cast<size_t>(nnodes));
+  {
+    size_t tmp = 0;
+    VBH_HIP(rocprim::radix_sort_pairs(nullptr, tmp, ids.p, ids_s.p, idx.p, order.p,
+                                      static_cast<size_t>(nnodes), 0u, 32u, s));
+    DBuf<char> t;
+    t.resize(std::max<size_t>(tmp, 1));
+    VBH_HIP(rocprim::radix_sort_pairs(t.p, tmp, ids.p, ids_s.p, idx.p, order.p,
+                                      static_cast<size_t>(nnodes), 0u, 32u, s));
+    VBH_HIP(hipStreamSynchronize(s));
+  }
+  mask.resize(static_cast<size_t>(nnodes));
+  spc.resize(static_cast<size_t>(nnodes));
+  dzero(mask, s);
+  dzero(spc, s);
+  if (nelem > 0) {
+    et.upload(etype_h, static_cast<size_t>(nelem), s);
+    ep.upload(eptr_h, static_cast<size_t>(nelem) + 1, s);
+    cn.upload(conn_h, static_cast<size_t>(std::max(nconn, 1)), s);
+    hipLaunchKernelGGL(k_dof_active, dim3((nelem + 255) / 256), dim3(256), 0, s, nelem, et.p, ep.p,
+                       cn.p, mask.p);
+  }
+  if (nspc > 0) {
+    sn.upload(spc_nodes_h, static_cast<size_t>(nspc), s);
+    sm.upload(spc_masks_h, static_cast<size_t>(nspc), s);
+    hipLaunchKernelGGL(k_dof_spc, dim3((nspc + 255) / 256), dim3(256), 0, s, nspc, sn.p, sm.p, spc.p);
+  }
+  cnt.resize(static_cast<size_t>(nnodes));
+  base.resize(static_cast<size_t>(nnodes));
+  hipLaunchKernelGGL(k_dof_count, dim3((nnodes + 255) / 256), dim3(256), 0, s, nnodes, order.p,
+                     mask.p, spc.p, cnt.p);
+  excl_scan(cnt.p, base.p, static_cast<size_t>(nnodes), s);
+  const int nfree = read_last_plus(base.p, cnt.p, static_cast<size_t>(nnodes), s);
+  eq.resize(static_cast<size_t>(nnodes) * 6);
+  hipLaunchKernelGGL(k_dof_assign, dim3((nnodes + 255) / 256), dim3(256), 0, s, nnodes, order.p,
+                     mask.p, spc.p, base.p, eq.p);
+  VBH_HIP(hipMemcpyAsync(eq_out, eq.p, sizeof(int) * 6 * static_cast<size_t>(nnodes),
+                         hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  return nfree;
+}
+
+}  // namespace vbh
+
+extern "C" {
+
+int vbh_dof_map(vbh_ctx* h, int nnodes, const int* node_ids, int nelem, const int* elem_type,
+                const int* elem_ptr, const int* conn, int nspc, const int* spc_nodes,
+                const int* spc_masks, int* eq_out, int* n_free) {
+  if (!h || !node_ids || !eq_out) {
+    vbh::set_error("vbh_dof_map: null argument");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] {
+    const int nf = c->dof_map(nnodes, node_ids, nelem, elem_type, elem_ptr, conn, nspc, spc_nodes,
+                              spc_masks, eq_out);
+    if (n_free) *n_free = nf;
+  });
+}
+
+int vbh_assemble(vbh_ctx* h, int n, int nnodes, const double* xyz, const int* eq, int nelem,
+                 const int* elem_type, const int* elem_ptr, const int* conn, const double* E,
+                 const double* nu, const double* thickness, int solid_form, int shell_form) {
+  if (!h || !xyz || !eq || !elem_type || !elem_ptr || !conn || !E || !nu) {
+    vbh::set_error("vbh_assemble: null argument");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] {
+    c->assemble(n, nnodes, xyz, eq, nelem, elem_type, elem_ptr, conn, E, nu, thickness, solid_form,
+                shell_form);
+  });
+}
+
+int vbh_last_assembly_stats(const vbh_ctx* h, vbh_assembly_stats* out) {
+  if (!h || !out) {
+    vbh::set_error("vbh_last_assembly_stats: null argument");
+    return VBH_ERR_ARG;
+  }
+  *out = reinterpret_cast<const vbh::Ctx*>(h)->asm_stats;
+  return VBH_OK;
+}
+
+}  // extern "C"

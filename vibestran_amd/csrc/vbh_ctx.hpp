@@ -1,0 +1,99 @@
+// vibestran_amd/csrc/vbh_ctx.hpp -- the context behind vbh_ctx*.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "vbh_internal.hpp"
+
+namespace vbh {
+
+struct GraphSlot {
+  hipGraphExec_t exec = nullptr;
+  int iters = 0;
+  double tol = 0.0;
+  int max_iters = 0;
+};
+
+struct Ctx {
+  // device / stream
+  int device = 0;
+  std::string name, arch;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+
+  // solver configuration (CudaPCGContext equivalent, impl:2030-2040)
+  double tol = 1e-8;
+  int max_iters = 10000;
+  int precision = VBH_PREC_FP64;
+  int format_req = -1;
+  double refresh_delta = 0.05;
+  int iters_per_graph = 32;
+
+  // operator
+  bool have_matrix = false;
+  int n = 0, n_pad = 0, format = 0;
+  long long operator_version = 0;
+  std::vector<int> lower_rp, lower_ci;    // reference CsrData (lower) on the host
+  std::vector<double> lower_v;
+  bool lower_on_host = false;
+  int l_nnz = 0;
+  int block_hint = 3;
+  DBuf<int> l_rp, l_ci, l_row;            // device lower CSR (assembled operators)
+  DBuf<double> l_v;
+  vbh_assembly_stats asm_stats{};
+  DBuf<int> csr_rp, csr_ci;
+  DBuf<double> csr_v64;
+  DBuf<float> csr_v32;
+  DBuf<int> bsr_rp, bsr_ci;
+  DBuf<double> bsr_v64;
+  DBuf<float> bsr_v32;
+  DBuf<double> diag64;
+  DBuf<float> diag32;
+  CsrDev csr;
+  BsrDev bsrd;
+
+  // solve work space
+  size_t work_len = 0;
+  long long work_version = -1;
+  DBuf<double> w_F, w_u, w_r64, w_r, w_z, w_p, w_Ap;
+  DBuf<float> w_r32, w_z32, w_p32, w_Ap32;
+  DBuf<double> red_partials;
+  DBuf<unsigned> red_counter;
+  DBuf<PcgScalars> st_dev;
+  PcgScalars* st_host = nullptr;
+  GraphSlot graphs[2];
+
+  // diagnostics (SolverBackend::last_iteration_count / last_estimated_error)
+  int last_iters = 0;
+  double last_rel = 0.0;
+  vbh_solve_stats stats{};
+
+  ~Ctx() {
+    if (device >= 0) (void)hipSetDevice(device);
+    drop_graphs();
+    if (st_host) (void)hipHostFree(st_host);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void select_stream_device();
+  void upload_operator(int n, int nnz, const int* rp, const int* ci, const double* v, bool lower);
+  void ensure_work(int nvec_len);
+  void assemble(int n, int nnodes, const double* xyz, const int* eq, int nelem, const int* etype,
+                const int* eptr, const int* conn, const double* E, const double* nu,
+                const double* thick, int solid_form, int shell_form);
+  void finish_device_operator();
+  void fetch_lower_to_host();
+  int dof_map(int nnodes, const int* ids, int nelem, const int* etype, const int* eptr,
+              const int* conn, int nspc, const int* spc_nodes, const int* spc_masks, int* eq_out);
+  void drop_graphs();
+  PcgParams params() const;
+  void enqueue_iterations(int precision_mode, int count);
+  hipGraphExec_t iteration_graph(int precision_mode);
+  void read_state();
+  PcgScalars run_solve(int precision_mode);
+  void check_outcome(const PcgScalars& s, int precision_mode);
+};
+
+}  // namespace vbh

@@ -1,0 +1,147 @@
+"""Seeded synthetic decks for the linear-static workloads.
+
+Node and element numbering follow the reference's deck generators:
+CHEXA8 blocks use id = i*(ny+1)*(nz+1) + j*(nz+1) + k + 1 with corners
+(-,-,-),(+,-,-),(+,+,-),(-,+,-) then the z+ face
+(reference tools/generate_chexa_mesh.py:93-125); CQUAD4 tubes use
+G1=(ix,ic), G2=(ix,ic+1), G3=(ix+1,ic+1), G4=(ix+1,ic)
+(reference tools/generate_cquad4_tube.py docstring 18-31).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .model import CHEXA8, CQUAD4, LoadCase, Model, Pload4
+
+
+def _hex_grid(nx, ny, nz, h=1.0):
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    ijk = np.stack([i.reshape(-1), j.reshape(-1), k.reshape(-1)], axis=1)
+    xyz = ijk.astype(np.float64) * h
+    ids = (ijk[:, 0] * (ny + 1) * (nz + 1) + ijk[:, 1] * (nz + 1) + ijk[:, 2] + 1).astype(np.int32)
+    return ijk, xyz, ids
+
+
+def _hex_conn(nx, ny, nz):
+    ie, je, ke = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ie, je, ke = ie.reshape(-1), je.reshape(-1), ke.reshape(-1)
+
+    def n(di, dj, dk):
+        return (ie + di) * (ny + 1) * (nz + 1) + (je + dj) * (nz + 1) + (ke + dk)
+
+    conn = np.stack([n(0, 0, 0), n(1, 0, 0), n(1, 1, 0), n(0, 1, 0),
+                     n(0, 0, 1), n(1, 0, 1), n(1, 1, 1), n(0, 1, 1)], axis=1)
+    return conn.astype(np.int32)
+
+
+def hex_block(nx, ny, nz, h=1.0, E=200e3, nu=0.3, rho=7.85e-9, jitter=0.0, jitter_seed=1,
+              E_sigma=0.0, E_seed=2, solid_form=0):
+    """Structured CHEXA8 block on [0,nx*h]x[0,ny*h]x[0,nz*h]. Interior nodes get
+    uniform jitter of +-jitter*h per axis; E is lognormal with median E when
+    E_sigma > 0. The x=0 face is clamped (SPC1 123)."""
+    ijk, xyz, ids = _hex_grid(nx, ny, nz, h)
+    if jitter > 0.0:
+        rng = np.random.default_rng(jitter_seed)
+        interior = ((ijk[:, 0] > 0) & (ijk[:, 0] < nx) & (ijk[:, 1] > 0) & (ijk[:, 1] < ny)
+                    & (ijk[:, 2] > 0) & (ijk[:, 2] < nz))
+        d = rng.uniform(-jitter * h, jitter * h, size=xyz.shape)
+        xyz = xyz + d * interior[:, None]
+    conn = _hex_conn(nx, ny, nz)
+    ne = conn.shape[0]
+    if E_sigma > 0.0:
+        rng = np.random.default_rng(E_seed)
+        Ee = E * np.exp(E_sigma * rng.standard_normal(ne))
+    else:
+        Ee = np.full(ne, float(E))
+    spc_nodes = np.nonzero(ijk[:, 0] == 0)[0].astype(np.int32)
+    m = Model(node_ids=ids, xyz=xyz, elem_type=np.full(ne, CHEXA8, np.int32),
+              elem_ptr=(np.arange(ne + 1) * 8).astype(np.int32), conn=conn.reshape(-1),
+              E=Ee, nu=np.full(ne, float(nu)), rho=np.full(ne, float(rho)),
+              solid_form=solid_form, spc_nodes=spc_nodes,
+              spc_masks=np.full(len(spc_nodes), 0b000111, np.int32))
+    m.meta.update(dict(nx=nx, ny=ny, nz=nz, h=h, ijk=ijk))
+    return m
+
+
+def config1_hex_cantilever():
+    """Plain SOL 101, CPU-runnable: CHEXA8 20x10x10 unit cubes, E=200e3,
+    nu=0.3, rho=7.85e-9, SPC123 on x=0, FORCE -1000 in z spread evenly over
+    the x=20 face nodes."""
+    m = hex_block(20, 10, 10)
+    ijk = m.meta["ijk"]
+    tip = np.nonzero(ijk[:, 0] == 20)[0]
+    nodal = np.zeros((m.nn, 6))
+    nodal[tip, 2] = -1000.0 / len(tip)
+    m.loads.append(LoadCase(nodal=nodal))
+    m.meta["name"] = "hex20x10x10-cantilever"
+    return m
+
+
+def config2_hex_jitter(nx=120, ny=40, nz=40, seed=2024):
+    """Mixed-precision workload: CHEXA8 120x40x40 with +-10% interior jitter,
+    lognormal E (median 200e3, sigma 0.3), nu=0.3, clamped x=0, uniform
+    PLOAD4 p=1.0 on the top (z max) faces."""
+    m = hex_block(nx, ny, nz, jitter=0.1, jitter_seed=seed, E_sigma=0.3, E_seed=seed + 1)
+    ne = m.ne
+    ke = np.arange(ne) % nz
+    top = np.nonzero(ke == nz - 1)[0]
+    # G1 = corner 5 (local 4), G34 = corner 7 (local 6): the z+ face.
+    pl = Pload4(elem=top, g1=np.full(len(top), 4), g34=np.full(len(top), 6),
+                p=np.ones((len(top), 4)))
+    m.loads.append(LoadCase(nodal=np.zeros((m.nn, 6)), pload4=pl))
+    m.meta["name"] = f"hex{nx}x{ny}x{nz}-jitter-lognormalE-pload4"
+    return m
+
+
+def config4_simp(nx=80, ny=40, nz=20, seed=7, E0=200e3, Emin=1e-3):
+    """Adjoint workload: CHEXA8 80x40x20, rho_e ~ U[0.01,1], SIMP modulus
+    E_e = Emin + rho_e^3 (E0 - Emin), nu=0.3, clamped x=0, point FORCE -z at
+    the mid-edge node (nx, ny/2, 0) of the free end."""
+    m = hex_block(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    dens = rng.uniform(0.01, 1.0, size=m.ne)
+    m.E = Emin + dens ** 3 * (E0 - Emin)
+    ijk = m.meta["ijk"]
+    tip = np.nonzero((ijk[:, 0] == nx) & (ijk[:, 1] == ny // 2) & (ijk[:, 2] == 0))[0]
+    nodal = np.zeros((m.nn, 6))
+    nodal[tip, 2] = -1000.0
+    m.loads.append(LoadCase(nodal=nodal))
+    m.meta.update(dict(name=f"hex{nx}x{ny}x{nz}-simp", density=dens, E0=E0, Emin=Emin,
+                       tip_node=int(tip[0])))
+    return m
+
+
+def cquad4_tube(n_circ=400, n_axial=800, radius=50.0, length=400.0, t=2.0, E=70e3, nu=0.33,
+                rho=2.7e-9, n_cases=32, seed=11, load_scale=1e3):
+    """Batched-RHS workload: CQUAD4 cylindrical tube along x, node id =
+    ix*n_circ + ic + 1, one end (x=0) clamped 123456, n_cases load cases each
+    a seeded resultant force+moment in U[-1,1]*load_scale spread evenly over
+    the free-end ring nodes."""
+    ix, ic = np.meshgrid(np.arange(n_axial + 1), np.arange(n_circ), indexing="ij")
+    ix, ic = ix.reshape(-1), ic.reshape(-1)
+    th = 2.0 * np.pi * ic / n_circ
+    xyz = np.stack([length * ix / n_axial, radius * np.cos(th), radius * np.sin(th)], axis=1)
+    ids = (ix * n_circ + ic + 1).astype(np.int32)
+    ex, ec = np.meshgrid(np.arange(n_axial), np.arange(n_circ), indexing="ij")
+    ex, ec = ex.reshape(-1), ec.reshape(-1)
+
+    def node(a, c):
+        return a * n_circ + (c % n_circ)
+
+    conn = np.stack([node(ex, ec), node(ex, ec + 1), node(ex + 1, ec + 1), node(ex + 1, ec)], axis=1)
+    ne = conn.shape[0]
+    spc_nodes = np.nonzero(ix == 0)[0].astype(np.int32)
+    m = Model(node_ids=ids, xyz=xyz, elem_type=np.full(ne, CQUAD4, np.int32),
+              elem_ptr=(np.arange(ne + 1) * 4).astype(np.int32), conn=conn.reshape(-1).astype(np.int32),
+              E=np.full(ne, float(E)), nu=np.full(ne, float(nu)), rho=np.full(ne, float(rho)),
+              thickness=np.full(ne, float(t)), spc_nodes=spc_nodes,
+              spc_masks=np.full(len(spc_nodes), 0b111111, np.int32))
+    tip = np.nonzero(ix == n_axial)[0]
+    rng = np.random.default_rng(seed)
+    for _ in range(n_cases):
+        fm = rng.uniform(-1.0, 1.0, size=6) * load_scale
+        nodal = np.zeros((m.nn, 6))
+        nodal[tip, :] = fm / len(tip)
+        m.loads.append(LoadCase(nodal=nodal))
+    m.meta.update(dict(name=f"cquad4-tube-{n_circ}x{n_axial}", n_circ=n_circ, n_axial=n_axial))
+    return m
