@@ -352,7 +352,13 @@ PcgScalars Ctx::run_solve(int precision_mode) {
   read_state();
   if (!st_host->done) {
     hipGraphExec_t ge = iteration_graph(precision_mode);
+    // Every graph launch that is not paused advances the device iteration
+    // count, so this cap is never reached by a healthy solve.
+    const long long cap = 2ll * (max_iters / iters_per_graph + 2) + 64;
+    long long launches = 0;
     for (;;) {
+      if (++launches > cap)
+        throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
       VBH_HIP(hipGraphLaunch(ge, stream));
       read_state();
       if (!st_host->done) continue;

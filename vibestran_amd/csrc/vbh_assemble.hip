@@ -579,14 +579,16 @@ __global__ void k_bsr_fill(int nnz, int n, const int* __restrict__ row_of, const
   {
     const int I = r / B, J = c / B;
     const int o = brp[I], nb = brp[I + 1] - o;
-    const int k = find_slot(brp, bci, I, J) - o;
-    bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((r % B) * B + (c % B)) * nb + k] = x;
+    const int slot = find_slot(brp, bci, I, J);
+    if (slot >= 0)
+      bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((r % B) * B + (c % B)) * nb + (slot - o)] = x;
   }
   if (r != c) {
     const int I = c / B, J = r / B;
     const int o = brp[I], nb = brp[I + 1] - o;
-    const int k = find_slot(brp, bci, I, J) - o;
-    bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((c % B) * B + (r % B)) * nb + k] = x;
+    const int slot = find_slot(brp, bci, I, J);
+    if (slot >= 0)
+      bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((c % B) * B + (r % B)) * nb + (slot - o)] = x;
   }
 }
 
@@ -597,8 +599,9 @@ __global__ void k_bsr_pad_diag(int n, int nbr, const int* __restrict__ brp,
   if (r >= nbr * B) return;
   const int I = r / B;
   const int o = brp[I], nb = brp[I + 1] - o;
-  const int k = find_slot(brp, bci, I, I) - o;
-  bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((r % B) * B + (r % B)) * nb + k] = 1.0;
+  const int slot = find_slot(brp, bci, I, I);
+  if (slot >= 0)
+    bv[static_cast<size_t>(B * B) * o + static_cast<size_t>((r % B) * B + (r % B)) * nb + (slot - o)] = 1.0;
 }
 
 __global__ void k_cast(long long m, const double* __restrict__ a, float* __restrict__ b) {
