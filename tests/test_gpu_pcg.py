@@ -110,6 +110,10 @@ def test_error_paths(gpu_backend_factory):
         be.solve(csr_of(tridiag(50, 2.0, -1.0)), np.ones(50))
     be2 = gpu_backend_factory()
     with pytest.raises(SolverError, match="did not converge"):  # p.Ap <= 0 breakdown
+        # indefinite [[1,2],[2,1]]: r.z = 2 > 0 but p.Ap = -2
+        be2.solve(CsrData.from_arrays([0, 1, 3], [0, 0, 1], [1.0, 2.0, 1.0], lower_only=True),
+                  np.array([1.0, -1.0]))
+    with pytest.raises(SolverError, match="non-positive initial r"):  # impl:1859-1862
         be2.solve(CsrData.from_arrays([0, 1, 2], [0, 1], [1.0, -1.0], lower_only=True),
                   np.array([0.0, 1.0]))
 

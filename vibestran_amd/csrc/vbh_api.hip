@@ -449,6 +449,11 @@ int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
     hipDeviceProp_t prop;
     VBH_HIP(hipGetDeviceProperties(&prop, device));
     c->name = prop.name;
+    if (c->name.empty()) {
+      std::string arch = prop.gcnArchName;
+      c->name = "AMD Instinct " + arch.substr(0, arch.find(':')) + " (" +
+                std::to_string(prop.multiProcessorCount) + " CUs)";
+    }
     c->arch = prop.gcnArchName;
     c->tol = tol > 0.0 ? tol : 1e-8;
     c->max_iters = max_iters > 0 ? max_iters : 10000;

@@ -242,6 +242,91 @@ __device__ __forceinline__ double iso(double lam, double mu, const double* a, in
   return v;
 }
 
+__device__ __forceinline__ const int* g_of(const int* gd, int e) {
+  return gd + static_cast<size_t>(e) * kGdStride;
+}
+
+// det3 with the operation order of the oracle (expansion along column 0).
+__device__ double det3_nc(double m00, double m01, double m02, double m10, double m11, double m12,
+                          double m20, double m21, double m22) {
+#pragma clang fp contract(off)
+  const double h0 = m00 * (m11 * m22 - m12 * m21);
+  const double h1 = m10 * (m01 * m22 - m02 * m21);
+  const double h2 = m20 * (m01 * m12 - m02 * m11);
+  return h0 - h1 + h2;
+}
+
+template <int SU>
+__device__ void tet_entries(const double (&X)[4][3], double E, double nu, int lane, const int* g,
+                            double (&kuu)[SU], bool& bad) {
+#pragma clang fp contract(off)
+  double A[4][4];
+  for (int i = 0; i < 4; ++i) {
+    A[i][0] = 1.0;
+    for (int d = 0; d < 3; ++d) A[i][d + 1] = X[i][d];
+  }
+  auto minor_det = [&](int sr, int sc) {
+    double m[3][3];
+    int ri = 0;
+    for (int r = 0; r < 4; ++r) {
+      if (r == sr) continue;
+      int cj = 0;
+      for (int c = 0; c < 4; ++c) {
+        if (c == sc) continue;
+        m[ri][cj++] = A[r][c];
+      }
+      ++ri;
+    }
+    return det3_nc(m[0][0], m[0][1], m[0][2], m[1][0], m[1][1], m[1][2], m[2][0], m[2][1], m[2][2]);
+  };
+  double V6 = 0.0;
+  for (int c = 0; c < 4; ++c) V6 += ((c % 2) ? -1.0 : 1.0) * A[0][c] * minor_det(0, c);
+  const double V = V6 / 6.0;
+  if (!(V > 0)) bad = true;
+  double dn[4][3];
+  for (int n = 0; n < 4; ++n)
+    for (int d = 0; d < 3; ++d) {
+      const int i = d + 1;  // cof(i, n) = (-1)^(i+n) minor(n, i)
+      dn[n][d] = ((((i + n) % 2) ? -1.0 : 1.0) * minor_det(n, i)) / V6;
+    }
+  const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+  const double mu = E / (2 * (1 + nu));
+  double D[6][6];
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) D[a][b] = 0.0;
+  D[0][0] = lam + 2 * mu; D[0][1] = lam; D[0][2] = lam;
+  D[1][0] = lam; D[1][1] = lam + 2 * mu; D[1][2] = lam;
+  D[2][0] = lam; D[2][1] = lam; D[2][2] = lam + 2 * mu;
+  D[3][3] = mu; D[4][4] = mu; D[5][5] = mu;
+  auto bcol = [&](int j, double (&b)[6]) {
+    const int n = j / 3, c = j % 3;
+    for (int k = 0; k < 6; ++k) b[k] = 0.0;
+    const double x = dn[n][0], y = dn[n][1], z = dn[n][2];
+    if (c == 0) { b[0] = x; b[3] = y; b[5] = z; }
+    else if (c == 1) { b[1] = y; b[3] = x; b[4] = z; }
+    else { b[2] = z; b[4] = y; b[5] = x; }
+  };
+#pragma unroll
+  for (int q = 0; q < SU; ++q) {
+    const int t = lane + 64 * q;
+    if (t < 78) {
+      const unsigned short pr = pair_at<12>(t);
+      int i = pr >> 8, j = pr & 0xff;
+      if (g[i] < g[j]) { const int tmp = i; i = j; j = tmp; }  // row = larger global index
+      double bi[6], bj[6];
+      bcol(i, bi);
+      bcol(j, bj);
+      double sum = 0.0;
+      for (int k = 0; k < 6; ++k) {
+        double db = 0.0;
+        for (int l = 0; l < 6; ++l) db += D[k][l] * bj[l];
+        sum += bi[k] * db;
+      }
+      kuu[q] = sum * V;
+    }
+  }
+}
+
 constexpr int kElemWaves = 4;
 
 template <int K>
@@ -286,35 +371,11 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
 
     if constexpr (K == kTet4) {
       // CST tetrahedron (solid_elements.cpp:312-384): gradients from the
-      // cofactors of [1 x y z]; Ke = V * B^T D B.
-      const double x10 = X[1][0] - X[0][0], y10 = X[1][1] - X[0][1], z10 = X[1][2] - X[0][2];
-      const double x20 = X[2][0] - X[0][0], y20 = X[2][1] - X[0][1], z20 = X[2][2] - X[0][2];
-      const double x30 = X[3][0] - X[0][0], y30 = X[3][1] - X[0][1], z30 = X[3][2] - X[0][2];
-      M3d J;
-      J.a[0][0] = x10; J.a[0][1] = y10; J.a[0][2] = z10;
-      J.a[1][0] = x20; J.a[1][1] = y20; J.a[1][2] = z20;
-      J.a[2][0] = x30; J.a[2][1] = y30; J.a[2][2] = z30;
-      const double V6 = det3(J);
-      const double V = V6 / 6.0;
-      if (!(V > 0)) bad = true;
-      const M3d Ji = inv3(J, bad ? 1.0 : V6);
-      double dN[4][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        dN[1][c] = Ji.a[c][0];
-        dN[2][c] = Ji.a[c][1];
-        dN[3][c] = Ji.a[c][2];
-        dN[0][c] = -(dN[1][c] + dN[2][c] + dN[3][c]);
-      }
-#pragma unroll
-      for (int q = 0; q < KD::SU; ++q) {
-        const int t = lane + 64 * q;
-        if (t < NL) {
-          const unsigned short pr = pair_at<ND>(t);
-          const int i = pr >> 8, j = pr & 0xff;
-          kuu[q] = V * iso(lam, mu, dN[i / 3], i % 3, dN[j / 3], j % 3);
-        }
-      }
+      // cofactors of A4 = [1 x y z], Ke = V * B^T D B, evaluated in the same
+      // operation order as the CPU oracle without FMA contraction, so that
+      // entries which cancel to exactly 0.0 there do so here too (the CSR
+      // pattern drops exact zeros).
+      tet_entries<KD::SU>(X, E, nu, lane, g_of(gd, e), kuu, bad);
     } else {
       // Gauss rule: hexa 2x2x2 (weight 1), penta 3 triangle x 2 axial (1/6).
       constexpr int NG = (NN == 8) ? 8 : 6;
