@@ -141,21 +141,33 @@ def main():
         fp64 = {"time_s": t64, "iters": it64, "iters_per_s": it64 / t64, "rel_res": rel64}
         log(f"[bench] fp64 solve: {it64} iters, rel {rel64:.3e}, {t64*1e3:.1f} ms")
 
-    # dominant kernel: the inner FP32 SpMV, timed with hipEvents on the ctx stream
-    ms32 = C.c_double(0.0)
-    ms64 = C.c_double(0.0)
-    lib.vbh_bench_spmv(h, 1, args.spmv_reps, C.byref(ms32))
-    lib.vbh_bench_spmv(h, 0, args.spmv_reps, C.byref(ms64))
+    # dominant kernel: the fused FP32 PCG SpMV (search direction + K p + p.Ap),
+    # timed with hipEvents on the context stream; plus the update kernel
+    def kt(which, reps=args.spmv_reps):
+        ms = C.c_double(0.0)
+        if lib.vbh_bench_spmv(h, which, reps, C.byref(ms)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return ms.value
+    lib.vbh_memcpy_h2d(h, dF, F.ctypes.data_as(C.c_void_p), nbytes)
+    ms32, ms64 = kt(3), kt(2)
+    ms_upd32, ms_upd64 = kt(5), kt(4)
+    ms_plain32 = kt(1)
     nnz_full = 2 * K.nnz - n
-    # algorithmic bytes of one SpMV launch, full-CSR-equivalent: values + int32
-    # column per nonzero, row pointers, read x, write y
-    spmv32_bytes = nnz_full * (4 + 4) + 4 * (n + 1) + 2 * 4 * n
-    spmv64_bytes = nnz_full * (8 + 4) + 4 * (n + 1) + 2 * 8 * n
-    ach32 = spmv32_bytes / (ms32.value * 1e-3) / 1e9
-    ach64 = spmv64_bytes / (ms64.value * 1e-3) / 1e9
+    # algorithmic bytes per launch, full-CSR-equivalent matrix (value + int32
+    # column per nonzero, row pointers) plus the kernel's vector streams:
+    #   cg_spmv  reads r, diag, p_old and writes p_new, Ap   (5 streams)
+    #   cg_update reads p, Ap, r, diag, u(f64), writes u(f64), r
+    spmv32_bytes = nnz_full * (4 + 4) + 4 * (n + 1) + 5 * 4 * n
+    spmv64_bytes = nnz_full * (8 + 4) + 4 * (n + 1) + 5 * 8 * n
+    upd32_bytes = (4 * 4 + 8 + 8 + 4) * n
+    ach32 = spmv32_bytes / (ms32 * 1e-3) / 1e9
+    ach64 = spmv64_bytes / (ms64 * 1e-3) / 1e9
     # whole-iteration algorithmic bytes (13 vector streams, FP32 inner)
     b_iter32 = nnz_full * (4 + 4) + 4 * (n + 1) + 13 * n * 4
     iter_gbs = b_iter32 * iters_mixed / (devs[-1]) / 1e9
+    log(f"[bench] cg_spmv fp32 {ms32*1e3:.1f} us, fp64 {ms64*1e3:.1f} us; cg_update fp32 "
+        f"{ms_upd32*1e3:.1f} us, fp64 {ms_upd64*1e3:.1f} us; plain spmv fp32 {ms_plain32*1e3:.1f} us; "
+        f"iteration {devs[-1]/iters_mixed*1e6:.1f} us")
 
     cpu = None
     if not args.no_cpu:
@@ -189,12 +201,17 @@ def main():
         "speedup_vs_fp64": round(fp64["time_s"] / t_mixed, 3) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "FP32 BSR SpMV",
-                     "bytes_per_launch": int(spmv32_bytes), "ms_per_launch": ms32.value},
-        "roofline_fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                               "ms_per_launch": ms64.value},
+                     "traffic": None, "kernel": "k_cg_spmv_bsr<float,3> (fused FP32 PCG SpMV)",
+                     "bytes_per_launch": int(spmv32_bytes), "us_per_launch": round(ms32 * 1e3, 2)},
+        "kernels_us": {"cg_spmv_f32": round(ms32 * 1e3, 2), "cg_spmv_f64": round(ms64 * 1e3, 2),
+                       "cg_update_f32": round(ms_upd32 * 1e3, 2),
+                       "cg_update_f64": round(ms_upd64 * 1e3, 2),
+                       "spmv_plain_f32": round(ms_plain32 * 1e3, 2),
+                       "cg_update_f32_GBs": round(upd32_bytes / (ms_upd32 * 1e-3) / 1e9, 1),
+                       "cg_spmv_f64_GBs": round(ach64, 1)},
         "iteration_roofline": {"bytes_per_iter": int(b_iter32), "achieved": round(iter_gbs, 1),
-                               "frac": round(iter_gbs / HBM_PEAK_GBS, 4)},
+                               "frac": round(iter_gbs / HBM_PEAK_GBS, 4),
+                               "us_per_iter": round(devs[-1] / iters_mixed * 1e6, 2)},
         "assembly_ms": {"pattern": ast.pattern_ms, "elements": ast.element_ms},
         "cpu_baseline": cpu,
     }

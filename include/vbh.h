@@ -140,10 +140,12 @@ int vbh_last_stats(const vbh_ctx* ctx, vbh_solve_stats* out);
  * y = K x in FP64 with the context's device operator (reference
  * CsrData::multiply, sparse_matrix.cpp:152-181). Host arrays of length n. */
 int vbh_spmv(vbh_ctx* ctx, const double* x, double* y);
-/* Time `reps` back-to-back SpMV launches of the inner format in the given
- * precision (0 FP64, 1 FP32) on device-resident vectors; returns the mean
- * launch duration in ms measured with hipEvents on the context stream. */
-int vbh_bench_spmv(vbh_ctx* ctx, int precision, int reps, double* ms_per_launch);
+/* Time `reps` back-to-back launches of one solver kernel on device-resident
+ * vectors; returns the mean launch duration in ms measured with hipEvents on
+ * the context stream. which: 0/1 plain SpMV FP64/FP32, 2/3 the fused PCG
+ * SpMV kernel (search direction + K p + p.Ap) FP64/FP32, 4/5 the fused PCG
+ * update kernel FP64/FP32. 2..5 use the F of the last solve. */
+int vbh_bench_spmv(vbh_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* ---- device memory helpers for device-resident workflows ---------------- */
 int vbh_device_alloc(vbh_ctx* ctx, long long bytes, void** dptr);

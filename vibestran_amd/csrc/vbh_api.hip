@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -267,15 +268,18 @@ void Ctx::ensure_work(int nvec_len) {
   w_z32.resize(N);
   w_p32.resize(N);
   w_Ap32.resize(N);
-  if (!red_partials.p) {
-    red_partials.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
-    red_counter.resize(4);
-    VBH_HIP(hipMemsetAsync(red_counter.p, 0, 4 * sizeof(unsigned), stream));
+  if (!part_a.p) {
+    part_a.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
+    part_b.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
+    part_r.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
   }
   if (!st_dev.p) st_dev.resize(1);
   if (!st_host) VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PcgScalars), 0));
-  // zero the padding tails once (F/u beyond n stay zero)
+  // zero the padding tails once (F beyond n stays zero)
   launch_fill_f64(static_cast<long long>(N), w_F.p, 0.0, stream);
+  const int rows = (format == 0) ? n : bsrd.nbr;
+  red_n_a = grid_for(rows, rows_per_block(format));
+  red_n_b = grid_for(static_cast<long long>(N), kBlock * 4);
   drop_graphs();
   work_len = N;
   work_version = operator_version;
@@ -296,21 +300,38 @@ PcgParams Ctx::params() const {
   return p;
 }
 
+CgRed Ctx::red() const {
+  CgRed r;
+  r.partA = part_a.p;
+  r.partB = part_b.p;
+  r.partR = part_r.p;
+  r.nA = red_n_a;
+  r.nB = red_n_b;
+  r.nR = red_n_a;
+  return r;
+}
+
+CgBufs<double> Ctx::bufs64() {
+  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_p.p, w_z.p, w_Ap.p, w_u.p};
+}
+
+CgBufs<float> Ctx::bufs32() {
+  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_p32.p, w_z32.p, w_Ap32.p, w_u.p};
+}
+
 void Ctx::enqueue_iterations(int precision_mode, int count) {
-  RedWs ws{red_partials.p, red_counter.p};
   const int N = static_cast<int>(work_len);
   const PcgParams prm = params();
+  const CgRed rd = red();
   for (int i = 0; i < count; ++i) {
     if (precision_mode == VBH_PREC_FP64) {
-      launch_pcg_spmv_alpha<double>(format, csr, bsrd, N, w_p.p, w_Ap.p, st_dev.p, ws, stream);
-      launch_pcg_update<double>(N, 0, diag64.p, w_p.p, w_Ap.p, w_u.p, w_r.p, w_z.p, st_dev.p, prm,
-                                ws, stream);
-      launch_pcg_pupdate<double>(N, w_z.p, w_p.p, st_dev.p, stream);
+      const CgBufs<double> b = bufs64();
+      launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, prm, 0, stream);
+      launch_cg_update<double>(N, b, st_dev.p, rd, stream);
     } else {
-      launch_pcg_spmv_alpha<float>(format, csr, bsrd, N, w_p32.p, w_Ap32.p, st_dev.p, ws, stream);
-      launch_pcg_update<float>(N, 1, diag32.p, w_p32.p, w_Ap32.p, w_u.p, w_r32.p, w_z32.p, st_dev.p,
-                               prm, ws, stream);
-      launch_pcg_pupdate<float>(N, w_z32.p, w_p32.p, st_dev.p, stream);
+      const CgBufs<float> b = bufs32();
+      launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, prm, 1, stream);
+      launch_cg_update<float>(N, b, st_dev.p, rd, stream);
     }
   }
 }
@@ -340,45 +361,42 @@ void Ctx::read_state() {
 
 // Runs one solve on device-resident F (w_F) into w_u. Returns the scalars.
 PcgScalars Ctx::run_solve(int precision_mode) {
-  RedWs ws{red_partials.p, red_counter.p};
   const int N = static_cast<int>(work_len);
   const PcgParams prm = params();
+  const CgRed rd = red();
   VBH_HIP(hipEventRecord(ev_start, stream));
   if (precision_mode == VBH_PREC_FP64)
-    launch_pcg_init<double>(N, w_F.p, diag64.p, w_u.p, w_r.p, w_z.p, w_p.p, st_dev.p, ws, stream);
+    launch_cg_init<double>(N, bufs64(), st_dev.p, rd, stream);
   else
-    launch_pcg_init<float>(N, w_F.p, diag32.p, w_u.p, w_r32.p, w_z32.p, w_p32.p, st_dev.p, ws,
-                           stream);
-  read_state();
-  if (!st_host->done) {
-    hipGraphExec_t ge = iteration_graph(precision_mode);
-    // Every graph launch that is not paused advances the device iteration
-    // count, so this cap is never reached by a healthy solve.
-    const long long cap = 2ll * (max_iters / iters_per_graph + 2) + 64;
-    long long launches = 0;
-    for (;;) {
-      if (++launches > cap)
-        throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
-      VBH_HIP(hipGraphLaunch(ge, stream));
+    launch_cg_init<float>(N, bufs32(), st_dev.p, rd, stream);
+  const char* ge_env = std::getenv("VBH_USE_GRAPH");
+  const bool use_graph = !(ge_env && ge_env[0] == '0');
+  hipGraphExec_t ge = use_graph ? iteration_graph(precision_mode) : nullptr;
+  // Each graph launch that does not end paused or finished completes
+  // iters_per_graph iterations, so this cap is never reached by a healthy solve.
+  const long long cap = 2ll * (max_iters / iters_per_graph + 2) + 64;
+  long long launches = 0;
+  for (;;) {
+    if (++launches > cap)
+      throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
+    if (use_graph) VBH_HIP(hipGraphLaunch(ge, stream));
+    else enqueue_iterations(precision_mode, iters_per_graph);
+    read_state();
+    if (!st_host->done) continue;
+    if (precision_mode == VBH_PREC_MIXED && (st_host->flags & kFlagRefresh)) {
+      launch_refresh_residual(format, csr, bsrd, w_F.p, w_u.p, w_r64.p, st_dev.p, rd, stream);
+      launch_refresh_reload(N, w_r64.p, diag32.p, w_r32.p, st_dev.p, rd, prm, stream);
       read_state();
       if (!st_host->done) continue;
-      if (precision_mode == VBH_PREC_MIXED && (st_host->flags & kFlagRefresh)) {
-        launch_refresh_true_residual(format, csr, bsrd, N, w_F.p, w_u.p, w_r64.p, st_dev.p, prm, ws,
-                                     stream);
-        launch_refresh_reload(N, w_r64.p, diag32.p, w_r32.p, w_z32.p, w_p32.p, st_dev.p, prm, ws,
-                              stream);
-        read_state();
-        if (!st_host->done) continue;
-      }
-      break;
     }
+    break;
   }
   VBH_HIP(hipEventRecord(ev_stop, stream));
   VBH_HIP(hipEventSynchronize(ev_stop));
   float ms = 0.0f;
   VBH_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
   PcgScalars s = *st_host;
-  stats.iterations = s.iter;
+  stats.iterations = s.iter_done;
   stats.refinements = s.refreshes;
   stats.rel_res = (precision_mode == VBH_PREC_MIXED && (s.flags & (kFlagConverged | kFlagStall)))
                       ? s.rel_true
@@ -392,12 +410,12 @@ PcgScalars Ctx::run_solve(int precision_mode) {
 // Maps the device flags onto the reference's outcomes.
 void Ctx::check_outcome(const PcgScalars& s, int precision_mode) {
   if (s.flags & kFlagRz0)
-    throw_error(VBH_ERR_SOLVER, "HIP PCG: non-positive initial r*z=" + fmt_g(s.rz) +
+    throw_error(VBH_ERR_SOLVER, "HIP PCG: non-positive initial r*z=" + fmt_g(s.rz[0]) +
                                     " -- preconditioner is not SPD or matrix is singular");
   if (s.flags & (kFlagBreakdown | kFlagMaxIter)) {
     // Breakdown is reported as non-convergence (impl:1934-1942, 1964-1970).
     const double rel = (s.flags & kFlagBreakdown) ? 1e300 : s.rel;
-    last_iters = (s.flags & kFlagBreakdown) ? max_iters : s.iter;
+    last_iters = (s.flags & kFlagBreakdown) ? max_iters : s.iter_done;
     last_rel = rel;
     throw_error(VBH_ERR_SOLVER,
                 "HIP PCG: did not converge after " + std::to_string(max_iters) +
@@ -405,7 +423,7 @@ void Ctx::check_outcome(const PcgScalars& s, int precision_mode) {
                     "). Consider increasing max iterations or using a direct solver.");
   }
   (void)precision_mode;
-  last_iters = s.iter;
+  last_iters = s.iter_done;
   last_rel = stats.rel_res;
 }
 
@@ -561,7 +579,7 @@ static int solve_impl(Ctx* c, const double* F, double* u, bool device_ptrs, int*
                            device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                            c->stream));
     VBH_HIP(hipStreamSynchronize(c->stream));
-    if (iters) *iters = s.iter;
+    if (iters) *iters = s.iter_done;
     if (rel_res) *rel_res = c->stats.rel_res;
     c->check_outcome(s, c->precision);
   });
@@ -627,8 +645,24 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
     c->ensure_work(c->n_pad);
     vbh::launch_fill_f64(c->n_pad, c->w_r64.p, 1.0, c->stream);
     vbh::launch_cast_f64_to_f32(c->n_pad, c->w_r64.p, c->w_p32.p, c->stream);
+    const vbh::CgRed rd = c->red();
+    const vbh::PcgParams prm = c->params();
+    if (precision >= 2) {  // fused solver kernels need an initialised state (F in w_F)
+      if (precision % 2 == 0) vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
+      else vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
+    }
     auto one = [&] {
-      if (precision == 0) {
+      if (precision == 2) {
+        vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd, prm, 0,
+                                    c->stream);
+      } else if (precision == 3) {
+        vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd, prm, 1,
+                                   c->stream);
+      } else if (precision == 4) {
+        vbh::launch_cg_update<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
+      } else if (precision == 5) {
+        vbh::launch_cg_update<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
+      } else if (precision == 0) {
         if (c->format == 0) vbh::launch_spmv_csr<double>(c->csr, c->w_r64.p, c->w_Ap.p, c->stream);
         else vbh::launch_spmv_bsr<double>(c->bsrd, c->w_r64.p, c->w_Ap.p, c->stream);
       } else {

@@ -57,8 +57,8 @@ struct Ctx {
   long long work_version = -1;
   DBuf<double> w_F, w_u, w_r64, w_r, w_z, w_p, w_Ap;
   DBuf<float> w_r32, w_z32, w_p32, w_Ap32;
-  DBuf<double> red_partials;
-  DBuf<unsigned> red_counter;
+  DBuf<double> part_a, part_b, part_r;
+  int red_n_a = 1, red_n_b = 1;
   DBuf<PcgScalars> st_dev;
   PcgScalars* st_host = nullptr;
   GraphSlot graphs[2];
@@ -89,6 +89,9 @@ struct Ctx {
               const int* conn, int nspc, const int* spc_nodes, const int* spc_masks, int* eq_out);
   void drop_graphs();
   PcgParams params() const;
+  CgRed red() const;
+  CgBufs<double> bufs64();
+  CgBufs<float> bufs32();
   void enqueue_iterations(int precision_mode, int count);
   hipGraphExec_t iteration_graph(int precision_mode);
   void read_state();

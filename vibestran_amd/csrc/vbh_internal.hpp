@@ -77,14 +77,18 @@ constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kMaxGrid = 2048;         // grid-stride cap: bounds the partial sums
 constexpr int kMaxRed = 4;             // values per grid reduction
 
-// Device-side PCG scalars. One instance per right-hand side.
+// Device-side PCG scalars (one instance per right-hand side). Written only by
+// block 0 of a kernel, in fields no other block of that kernel reads.
 struct PcgScalars {
-  double rz, rz_new, pAp, rr, alpha, beta, rel, norm_b;
-  double rr_refresh;   // recursive rr at the last true-residual refresh (mixed)
-  double rel_true;     // last FP64 true relative residual (mixed)
-  double rel_prev;     // previous true relative residual (stall detection)
-  int iter, done, flags, refreshes;
-  int stalls, pad0, pad1, pad2;
+  double rz[2];          // r.z of iterate k in slot k&1
+  double alpha, beta, pAp, rr, rel, norm_b;
+  double rr_refresh;     // recursive r.r at the last true-residual refresh (mixed)
+  double rel_true;       // last FP64 true relative residual (mixed)
+  double hist_rel[2];    // refresh history, slot = refresh count & 1
+  int hist_stalls[2];
+  int iter_done;         // completed iterations (written by cg_update)
+  int iter_cur;          // iteration in progress (written by cg_spmv)
+  int done, flags, refreshes, pad0;
 };
 
 enum : int {
@@ -94,7 +98,8 @@ enum : int {
   kFlagRz0 = 8,          // r.z <= 0 at start (impl:1859-1862)
   kFlagZeroRhs = 16,     // ||F|| < 1e-300 (impl:1822-1830)
   kFlagRefresh = 32,     // mixed: paused, FP64 true residual requested
-  kFlagStall = 64        // mixed: true residual stopped improving
+  kFlagStall = 64,       // mixed: true residual stopped improving
+  kFlagResumed = 128     // mixed: first cg_spmv after a refresh
 };
 
 struct PcgParams {
@@ -103,10 +108,24 @@ struct PcgParams {
   double refresh_delta;  // mixed: refresh when rr < delta^2 * rr_refresh
 };
 
-// Reduction workspace: per-value partials and one arrival counter per use.
-struct RedWs {
-  double* partials;   // kMaxRed * kMaxGrid
-  unsigned* counter;  // one counter
+// Per-block partial sums passed from one kernel to the next. nA/nB/nR are the
+// grid sizes of the producing kernels (<= kMaxGrid).
+struct CgRed {
+  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap
+  double* partB;  // from cg_init / cg_update / refresh_reload: r.z, r.r, (F.F)
+  double* partR;  // from refresh_residual: |F - K u|^2
+  int nA, nB, nR;
+};
+
+template <typename T>
+struct CgBufs {
+  const double* F;
+  const T* diag;
+  T* r;
+  T* p0;
+  T* p1;
+  T* Ap;
+  double* u64;
 };
 
 // ---- operator formats ---------------------------------------------------------
@@ -133,33 +152,27 @@ struct BsrDev {
 // ---- kernel launchers (vbh_kernels.hip) ------------------------------------------
 int grid_for(long long work_items, int items_per_block);
 
-// y = A x ; optionally dot partial of x.y reduced into red (finalize via mode).
-enum class SpmvEpilogue : int { None = 0, PcgAlpha = 1, TrueResidual = 2 };
-
 template <typename T>
 void launch_spmv_csr(const CsrDev& A, const T* x, T* y, hipStream_t s);
 template <typename T>
 void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s);
 
-// PCG building blocks (vector type T: double or float; dots in double).
+// Jacobi-PCG building blocks (vector type T: double or float; dots in double).
+int rows_per_block(int format);
 template <typename T>
-void launch_pcg_init(int n, const double* F, const T* diag, double* u64, T* r, T* z, T* p,
-                     PcgScalars* st, RedWs ws, hipStream_t s);
+void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
 template <typename T>
-void launch_pcg_spmv_alpha(int format, const CsrDev& A, const BsrDev& Ab, int n, const T* p,
-                           T* Ap, PcgScalars* st, RedWs ws, hipStream_t s);
+void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
+                    PcgScalars* st, const CgRed& red, const PcgParams& prm, int mixed,
+                    hipStream_t s);
 template <typename T>
-void launch_pcg_update(int n, int mixed, const T* diag, const T* p, const T* Ap, double* u64,
-                       T* r, T* z, PcgScalars* st, PcgParams prm, RedWs ws, hipStream_t s);
-template <typename T>
-void launch_pcg_pupdate(int n, const T* z, T* p, PcgScalars* st, hipStream_t s);
-// Mixed refresh: r64 = F - K64 u64 (FP64), decide convergence, reload r32/z32
-// and continue the recurrence with the refreshed residual.
-void launch_refresh_true_residual(int format, const CsrDev& A, const BsrDev& Ab, int n,
-                                  const double* F, const double* u64, double* r64, PcgScalars* st,
-                                  PcgParams prm, RedWs ws, hipStream_t s);
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
-                           float* p, PcgScalars* st, PcgParams prm, RedWs ws, hipStream_t s);
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
+// Mixed refresh: r64 = F - K64 u64 (FP64), then decide and reload r32.
+void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
+                             const double* u64, double* r64, PcgScalars* st, const CgRed& red,
+                             hipStream_t s);
+void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
+                           const CgRed& red, const PcgParams& prm, hipStream_t s);
 
 // Plain elementwise helpers.
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s);

@@ -7,11 +7,12 @@
 //  * BSR (B x B blocks): 32 lanes per block row for B=3 (27 blocks per node in
 //    a hex mesh), 16 for B=6; each lane owns one block, values are stored
 //    entry-major per block row so the lanes of a row load consecutive words.
-//  * PCG dot products are fused into the producing kernel and reduced in a
-//    fixed order (per-block partials + last-arriving block), so results are
-//    deterministic for a given grid. All scalars (alpha, beta, convergence)
-//    stay on the device; kernels exit at entry once `done` is set, which lets
-//    the host enqueue many iterations (hipGraph) and poll only occasionally.
+//  * PCG dot products are fused into the producing kernel as per-block
+//    partial sums that the next kernel reduces in a fixed order, so results
+//    are deterministic for a given grid. All scalars (alpha, beta,
+//    convergence) stay on the device; kernels exit at entry once `done` is
+//    set, which lets the host enqueue many iterations (hipGraph) and poll
+//    only occasionally.
 // Reference semantics (not code) followed: Jacobi-PCG of
 // cuda_pcg_backend_impl.hpp:1795-1999, kernels 629-751.
 #include "vbh_internal.hpp"
@@ -64,38 +65,6 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
     }
   }
   __syncthreads();
-}
-
-// Sums v over the whole grid. Returns true in exactly one block (the last to
-// arrive); there thread 0 holds the totals. The partials are summed in block
-// index order, independent of arrival order.
-template <int NV>
-__device__ bool grid_sum(double (&v)[NV], RedWs ws, double* lds, int* s_last) {
-  block_sum<NV>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) ws.partials[i * kMaxGrid + blockIdx.x] = v[i];
-    __threadfence();
-    const unsigned prev = atomicAdd(ws.counter, 1u);
-    *s_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!*s_last) return false;
-  __threadfence();
-  double t[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) t[i] = 0.0;
-  for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < NV; ++i)
-      t[i] += __builtin_nontemporal_load(&ws.partials[i * kMaxGrid + b]);
-  block_sum<NV>(t, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = t[i];
-    atomicExch(ws.counter, 0u);
-  }
-  return true;
 }
 
 // ---- row kernels ---------------------------------------------------------------
@@ -170,208 +139,314 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
   }
 }
 
-// ---- PCG: SpMV + p.Ap + alpha -------------------------------------------------------
-__device__ __forceinline__ void finalize_alpha(PcgScalars* st, double pAp) {
-  st->pAp = pAp;
-  if (!(pAp > 0.0)) {  // breakdown (cuda_pcg_backend_impl.hpp:691-696)
-    st->alpha = 0.0;
-    st->flags |= kFlagBreakdown;
-    st->rel = 1e300;
-    st->done = 1;
-    return;
+// ---- Jacobi-PCG, two kernels per iteration ------------------------------------------
+// Every kernel that needs a global sum reduces the per-block partial sums its
+// predecessor wrote (fixed order, every block redundantly), so no atomics or
+// in-kernel fences are needed and the scalars are bit-identical in all blocks.
+// Only block 0 writes the scalar state, and never a field that other blocks
+// of the same kernel read.
+//
+//   init      r = F, u = 0, p_{-1} = 0; partials (r.z, r.r, F.F)       -> partB
+//   cg_spmv   reduce partB -> rz_k, rr_k; convergence / pause decision;
+//             p_k = r_k/diag + beta_k p_{k-1} formed in the gather;
+//             Ap_k = K p_k; partial p_k.Ap_k                           -> partA
+//   cg_update reduce partA -> alpha_k; u += alpha p_k; r -= alpha Ap_k;
+//             partials (r.z, r.r)                                       -> partB
+// Mixed precision pauses in cg_spmv when the recursive residual fell by
+// refresh_delta since the last refresh (or below tol); the host then runs
+//   refresh_residual  r64 = F - K64 u (FP64)                            -> partR
+//   refresh_reload    reduce partR: converged / stalled / continue with
+//                     r = (float) r64                                    -> partB
+// and resumes: the next cg_spmv continues the recurrence (beta from the
+// refreshed r.z), as in reliable-update CG.
+
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nprod,
+                                                double (&out)[NV], double* lds) {
+  double t[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) t[i] = 0.0;
+  for (int b = threadIdx.x; b < nprod; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) t[i] += part[i * kMaxGrid + b];
+  block_sum<NV>(t, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lds[4 * kMaxRed + i] = t[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) out[i] = lds[4 * kMaxRed + i];
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* __restrict__ part,
+                                               double* lds) {
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) part[i * kMaxGrid + blockIdx.x] = v[i];
+}
+
+constexpr int kLds = 5 * kMaxRed;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScalars* st,
+                                                    CgRed red) {
+  __shared__ double lds[kLds];
+  double v[3] = {0.0, 0.0, 0.0};  // r.z, r.r (precision T), F.F
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double f = b.F[i];
+    const T ri = static_cast<T>(f);
+    const T zi = ri / b.diag[i];
+    b.u64[i] = 0.0;
+    b.r[i] = ri;
+    b.p1[i] = T(0);  // p_{-1}
+    v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+    v[2] += f * f;
   }
-  st->alpha = st->rz / pAp;
+  write_partials<3>(v, red.partB, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter_done = 0;
+    st->iter_cur = 0;
+    st->flags = 0;
+    st->done = 0;
+    st->refreshes = 0;
+    st->hist_stalls[0] = st->hist_stalls[1] = 0;
+    st->hist_rel[0] = st->hist_rel[1] = 1e300;
+    st->alpha = st->beta = st->pAp = 0.0;
+    st->rel = 1.0;
+    st->rel_true = 1.0;
+    st->rz[0] = st->rz[1] = 0.0;
+  }
+}
+
+// Prologue shared by the CSR and BSR variants of cg_spmv. Returns false if the
+// kernel must exit; otherwise beta_k for the on-the-fly search direction.
+__device__ __forceinline__ bool cg_spmv_prologue(PcgScalars* st, const CgRed& red,
+                                                 const PcgParams& prm, int mixed, double* lds,
+                                                 int* s_i, int& k, double& beta) {
+  if (threadIdx.x == 0) {
+    s_i[0] = st->done;
+    s_i[1] = st->iter_done;
+    s_i[2] = st->flags;
+  }
+  __syncthreads();
+  if (s_i[0]) return false;
+  k = s_i[1];
+  const bool resumed = (s_i[2] & kFlagResumed) != 0;
+  double v[3];
+  reduce_partials<3>(red.partB, red.nB, v, lds);
+  const double rz = v[0], rr = v[1];
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  if (k == 0 && !resumed) {
+    const double norm_b = sqrt(fabs(v[2]));
+    if (lead) {
+      st->norm_b = norm_b;
+      st->rr_refresh = rr;
+      st->rr = rr;
+      st->rz[0] = rz;
+      st->iter_cur = 0;
+    }
+    if (norm_b < 1e-300) {  // zero load: u = 0 after 0 iterations (impl:1822-1830)
+      if (lead) {
+        st->rel = 0.0;
+        st->flags |= kFlagZeroRhs;
+        st->done = 1;
+      }
+      return false;
+    }
+    if (rz <= 0.0) {  // impl:1859-1862
+      if (lead) {
+        st->rz[0] = rz;
+        st->flags |= kFlagRz0;
+        st->done = 1;
+      }
+      return false;
+    }
+    if (prm.max_iters <= 0) {
+      if (lead) {
+        st->flags |= kFlagMaxIter;
+        st->done = 1;
+      }
+      return false;
+    }
+    beta = 0.0;
+    return true;
+  }
+  const double rel = sqrt(fabs(rr)) / st->norm_b;
+  int stop = 0;
+  if (!mixed) {
+    if (rel < prm.tol) stop = kFlagConverged;  // finalize_iteration, impl:704-738
+  } else if (!resumed &&
+             (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh)) {
+    stop = kFlagRefresh;  // pause for an FP64 true-residual refresh
+  }
+  if (!stop && k >= prm.max_iters) stop = kFlagMaxIter;
+  if (stop) {
+    if (lead) {
+      st->rel = rel;
+      st->rr = rr;
+      st->iter_cur = k;
+      st->flags |= stop;
+      st->done = 1;
+    }
+    return false;
+  }
+  beta = rz / st->rz[(k - 1) & 1];
+  if (lead) {
+    st->rz[k & 1] = rz;
+    st->rr = rr;
+    st->rel = rel;
+    st->beta = beta;
+    st->iter_cur = k;
+    if (resumed) st->rr_refresh = rr;
+  }
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ T pdir(const CgBufs<T>& b, const T* __restrict__ pold, T beta, int j) {
+  return b.r[j] / b.diag[j] + beta * pold[j];
 }
 
 template <typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv_csr(CsrDev A, const T* __restrict__ p,
-                                                         T* __restrict__ Ap, PcgScalars* st,
-                                                         RedWs ws) {
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  if (st->done) return;
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
+                                                        CgRed red, PcgParams prm, int mixed) {
+  __shared__ double lds[kLds];
+  __shared__ int s_i[4];
+  int k;
+  double betad;
+  if (!cg_spmv_prologue(st, red, prm, mixed, lds, s_i, k, betad)) return;
+  const T beta = static_cast<T>(betad);
+  const T* pold = (k & 1) ? b.p0 : b.p1;
+  T* pnew = (k & 1) ? b.p1 : b.p0;
   const T* val = vals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
   double d[1] = {0.0};
   for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
-    const T s = csr_row<T, G>(A, val, p, r, lg);
+    T s = T(0);
+    const int e = A.rp[r + 1];
+    for (int q = A.rp[r] + lg; q < e; q += G) s += val[q] * pdir(b, pold, beta, A.ci[q]);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
     if (lg == 0) {
-      Ap[r] = s;
-      d[0] += static_cast<double>(p[r]) * static_cast<double>(s);
+      const T pr = pdir(b, pold, beta, r);
+      pnew[r] = pr;
+      b.Ap[r] = s;
+      d[0] += static_cast<double>(pr) * static_cast<double>(s);
     }
   }
-  if (grid_sum<1>(d, ws, lds, &s_last) && threadIdx.x == 0) finalize_alpha(st, d[0]);
+  write_partials<1>(d, red.partA, lds);
 }
 
 template <typename T, int B>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv_bsr(BsrDev A, const T* __restrict__ p,
-                                                         T* __restrict__ Ap, PcgScalars* st,
-                                                         RedWs ws) {
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
+                                                        CgRed red, PcgParams prm, int mixed) {
   constexpr int G = BsrLanes<B>::G;
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  if (st->done) return;
+  __shared__ double lds[kLds];
+  __shared__ int s_i[4];
+  int k;
+  double betad;
+  if (!cg_spmv_prologue(st, red, prm, mixed, lds, s_i, k, betad)) return;
+  const T beta = static_cast<T>(betad);
+  const T* pold = (k & 1) ? b.p0 : b.p1;
+  T* pnew = (k & 1) ? b.p1 : b.p0;
   const T* val = bvals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
   double d[1] = {0.0};
   for (int I = (blockIdx.x * blockDim.x + threadIdx.x) / G; I < A.nbr; I += stride) {
     T acc[B];
-    bsr_row<T, B, G>(A, val, p, I, lg, acc);
+#pragma unroll
+    for (int r = 0; r < B; ++r) acc[r] = T(0);
+    const int o = A.brp[I];
+    const int nb = A.brp[I + 1] - o;
+    const T* vb = val + static_cast<size_t>(B * B) * o;
+    for (int q = lg; q < nb; q += G) {
+      const int J = A.bci[o + q];
+      T xj[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) xj[c] = pdir(b, pold, beta, B * J + c);
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) acc[r] += vb[(r * B + c) * nb + q] * xj[c];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int r = 0; r < B; ++r) acc[r] += __shfl_xor(acc[r], off, G);
     if (lg == 0) {
 #pragma unroll
       for (int r = 0; r < B; ++r) {
-        Ap[B * I + r] = acc[r];
-        d[0] += static_cast<double>(p[B * I + r]) * static_cast<double>(acc[r]);
+        const T pr = pdir(b, pold, beta, B * I + r);
+        pnew[B * I + r] = pr;
+        b.Ap[B * I + r] = acc[r];
+        d[0] += static_cast<double>(pr) * static_cast<double>(acc[r]);
       }
     }
   }
-  if (grid_sum<1>(d, ws, lds, &s_last) && threadIdx.x == 0) finalize_alpha(st, d[0]);
+  write_partials<1>(d, red.partA, lds);
 }
 
-// ---- PCG: init ---------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __restrict__ F,
-                                                     const T* __restrict__ diag,
-                                                     double* __restrict__ u64, T* __restrict__ r,
-                                                     T* __restrict__ z, T* __restrict__ p,
-                                                     PcgScalars* st, RedWs ws) {
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  double v[3] = {0.0, 0.0, 0.0};  // ||F||^2, r.z, r.r (precision T)
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double f = F[i];
-    const T ri = static_cast<T>(f);
-    const T zi = ri / diag[i];
-    u64[i] = 0.0;
-    r[i] = ri;
-    z[i] = zi;
-    p[i] = zi;
-    v[0] += f * f;
-    v[1] += static_cast<double>(ri) * static_cast<double>(zi);
-    v[2] += static_cast<double>(ri) * static_cast<double>(ri);
+__global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgScalars* st,
+                                                      CgRed red) {
+  __shared__ double lds[kLds];
+  __shared__ int s_i[2];
+  if (threadIdx.x == 0) {
+    s_i[0] = st->done;
+    s_i[1] = st->iter_cur;
   }
-  if (grid_sum<3>(v, ws, lds, &s_last) && threadIdx.x == 0) {
-    st->iter = 0;
-    st->flags = 0;
-    st->done = 0;
-    st->refreshes = 0;
-    st->stalls = 0;
-    st->alpha = st->beta = 0.0;
-    st->pAp = 0.0;
-    st->rel = 1.0;
-    st->rel_true = 1.0;
-    st->rel_prev = 1e300;
-    const double norm_b = sqrt(fabs(v[0]));
-    st->norm_b = norm_b;
-    st->rz = v[1];
-    st->rr = v[2];
-    st->rr_refresh = v[2];
-    if (norm_b < 1e-300) {  // impl:1822-1830: zero displacement, 0 iterations
-      st->flags |= kFlagZeroRhs;
-      st->rel = 0.0;
-      st->done = 1;
-    } else if (v[1] <= 0.0) {  // impl:1859-1862
-      st->flags |= kFlagRz0;
+  __syncthreads();
+  if (s_i[0]) return;
+  const int k = s_i[1];
+  double v1[1];
+  reduce_partials<1>(red.partA, red.nA, v1, lds);
+  const double pAp = v1[0];
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  if (!(pAp > 0.0)) {  // breakdown (cuda_pcg_backend_impl.hpp:691-696)
+    if (lead) {
+      st->pAp = pAp;
+      st->alpha = 0.0;
+      st->rel = 1e300;
+      st->flags |= kFlagBreakdown;
       st->done = 1;
     }
+    return;
   }
-}
-
-// ---- PCG: u += alpha p, r -= alpha Ap, z = r/diag, dots, beta -----------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int mixed, const T* __restrict__ diag,
-                                                       const T* __restrict__ p,
-                                                       const T* __restrict__ Ap,
-                                                       double* __restrict__ u64,
-                                                       T* __restrict__ r, T* __restrict__ z,
-                                                       PcgScalars* st, PcgParams prm, RedWs ws) {
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  if (st->done) return;
-  const double alpha = st->alpha;
+  const double alpha = st->rz[k & 1] / pAp;
   const T a = static_cast<T>(alpha);
+  const T* p = (k & 1) ? b.p1 : b.p0;
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const T pi = p[i];
-    u64[i] += alpha * static_cast<double>(pi);
-    const T ri = r[i] - a * Ap[i];
-    const T zi = ri / diag[i];
-    r[i] = ri;
-    z[i] = zi;
+    b.u64[i] += alpha * static_cast<double>(pi);
+    const T ri = b.r[i] - a * b.Ap[i];
+    const T zi = ri / b.diag[i];
+    b.r[i] = ri;
     v[0] += static_cast<double>(ri) * static_cast<double>(zi);
     v[1] += static_cast<double>(ri) * static_cast<double>(ri);
   }
-  if (grid_sum<2>(v, ws, lds, &s_last) && threadIdx.x == 0) {
-    const double rz_new = v[0], rr = v[1];
-    st->rz_new = rz_new;
-    st->rr = rr;
-    const double rel = sqrt(fabs(rr)) / st->norm_b;
-    st->rel = rel;
-    st->iter += 1;
-    if (!mixed) {
-      if (rel < prm.tol) {  // finalize_iteration_kernel, impl:704-738
-        st->flags |= kFlagConverged;
-        st->beta = 0.0;
-        st->rz = rz_new;
-        st->done = 1;
-        return;
-      }
-    } else if (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh) {
-      // Pause for an FP64 true-residual refresh (the recurrence resumes with
-      // the refreshed residual; beta is computed then).
-      st->flags |= kFlagRefresh;
-      st->done = 1;
-      return;
-    }
-    st->beta = rz_new / st->rz;
-    st->rz = rz_new;
-    if (st->iter >= prm.max_iters) {
-      st->flags |= kFlagMaxIter;
-      st->done = 1;
-    }
+  write_partials<2>(v, red.partB, lds);
+  if (lead) {
+    st->pAp = pAp;
+    st->alpha = alpha;
+    st->iter_done = k + 1;
+    st->flags &= ~kFlagResumed;
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, const T* __restrict__ z,
-                                                        T* __restrict__ p, PcgScalars* st) {
-  if (st->done) return;
-  const T b = static_cast<T>(st->beta);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    p[i] = z[i] + b * p[i];
-}
-
-// ---- mixed precision refresh --------------------------------------------------------
-// r64 = F - K u64 in FP64 with the true-residual norm; decides convergence.
-__device__ __forceinline__ void finalize_true_residual(PcgScalars* st, double rr, PcgParams prm) {
-  const double rel = sqrt(fabs(rr)) / st->norm_b;
-  st->rel_true = rel;
-  st->refreshes += 1;
-  if (rel < prm.tol) {
-    st->rel = rel;
-    st->flags = (st->flags & ~kFlagRefresh) | kFlagConverged;
-    return;  // done stays set
-  }
-  // Stall: the true residual no longer follows the recursive one.
-  if (rel > 0.9 * st->rel_prev) st->stalls += 1; else st->stalls = 0;
-  st->rel_prev = rel;
-  if (st->stalls >= 4) {
-    st->rel = rel;
-    st->flags = (st->flags & ~kFlagRefresh) | kFlagStall;
-  }
-}
-
+// ---- mixed precision refresh ----------------------------------------------------------
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_true_residual_csr(CsrDev A, const double* __restrict__ F,
-                                                              const double* __restrict__ u,
-                                                              double* __restrict__ r64,
-                                                              PcgScalars* st, PcgParams prm,
-                                                              RedWs ws) {
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  if (!(st->flags & kFlagRefresh)) return;
+__global__ __launch_bounds__(kBlock) void k_refresh_residual_csr(CsrDev A, const double* __restrict__ F,
+                                                                 const double* __restrict__ u,
+                                                                 double* __restrict__ r64,
+                                                                 PcgScalars* st, CgRed red) {
+  __shared__ double lds[kLds];
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
   double v[1] = {0.0};
@@ -383,19 +458,17 @@ __global__ __launch_bounds__(kBlock) void k_true_residual_csr(CsrDev A, const do
       v[0] += ri * ri;
     }
   }
-  if (grid_sum<1>(v, ws, lds, &s_last) && threadIdx.x == 0) finalize_true_residual(st, v[0], prm);
+  write_partials<1>(v, red.partR, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
 template <int B>
-__global__ __launch_bounds__(kBlock) void k_true_residual_bsr(BsrDev A, const double* __restrict__ F,
-                                                              const double* __restrict__ u,
-                                                              double* __restrict__ r64,
-                                                              PcgScalars* st, PcgParams prm,
-                                                              RedWs ws) {
+__global__ __launch_bounds__(kBlock) void k_refresh_residual_bsr(BsrDev A, const double* __restrict__ F,
+                                                                 const double* __restrict__ u,
+                                                                 double* __restrict__ r64,
+                                                                 PcgScalars* st, CgRed red) {
   constexpr int G = BsrLanes<B>::G;
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  if (!(st->flags & kFlagRefresh)) return;
+  __shared__ double lds[kLds];
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
   double v[1] = {0.0};
@@ -410,38 +483,55 @@ __global__ __launch_bounds__(kBlock) void k_true_residual_bsr(BsrDev A, const do
         v[0] += ri * ri;
       }
   }
-  if (grid_sum<1>(v, ws, lds, &s_last) && threadIdx.x == 0) finalize_true_residual(st, v[0], prm);
+  write_partials<1>(v, red.partR, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
-// r32 = (float) r64, z = r32/diag, then beta from the refreshed r.z.
+// Decide from the FP64 true residual; on "continue" reload r = (float) r64.
 __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* __restrict__ r64,
                                                            const float* __restrict__ diag,
-                                                           float* __restrict__ r,
-                                                           float* __restrict__ z, PcgScalars* st,
-                                                           PcgParams prm, RedWs ws) {
-  __shared__ double lds[4 * kMaxRed];
-  __shared__ int s_last;
-  if (!(st->flags & kFlagRefresh)) return;
-  double v[2] = {0.0, 0.0};
+                                                           float* __restrict__ r, PcgScalars* st,
+                                                           CgRed red, PcgParams prm) {
+  __shared__ double lds[kLds];
+  double v1[1];
+  reduce_partials<1>(red.partR, red.nR, v1, lds);
+  const double rel = sqrt(fabs(v1[0])) / st->norm_b;
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  // stall history is double-buffered by refresh count (refresh_residual
+  // advanced the count), so no block reads what block 0 writes here
+  const int rc = st->refreshes;
+  const double rel_prev = st->hist_rel[(rc - 1) & 1];
+  int stalls = st->hist_stalls[(rc - 1) & 1];
+  int stop = 0;
+  if (rel < prm.tol) {
+    stop = kFlagConverged;
+  } else {
+    stalls = (rel > 0.9 * rel_prev) ? stalls + 1 : 0;  // true residual stopped following
+    if (stalls >= 4) stop = kFlagStall;
+  }
+  if (lead) {
+    st->rel_true = rel;
+    st->hist_rel[rc & 1] = rel;
+    st->hist_stalls[rc & 1] = stalls;
+  }
+  if (stop) {
+    if (lead) {
+      st->rel = rel;
+      st->flags = (st->flags & ~kFlagRefresh) | stop;
+    }
+    return;  // done stays set
+  }
+  double v[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float ri = static_cast<float>(r64[i]);
     const float zi = ri / diag[i];
     r[i] = ri;
-    z[i] = zi;
     v[0] += static_cast<double>(ri) * static_cast<double>(zi);
     v[1] += static_cast<double>(ri) * static_cast<double>(ri);
   }
-  if (grid_sum<2>(v, ws, lds, &s_last) && threadIdx.x == 0) {
-    st->rz_new = v[0];
-    st->rr = v[1];
-    st->rr_refresh = v[1];
-    st->beta = v[0] / st->rz;
-    st->rz = v[0];
-    st->flags &= ~kFlagRefresh;
-    if (st->iter >= prm.max_iters) {
-      st->flags |= kFlagMaxIter;
-      return;  // done stays set
-    }
+  write_partials<3>(v, red.partB, lds);
+  if (lead) {
+    st->flags = (st->flags & ~kFlagRefresh) | kFlagResumed;
     st->done = 0;
   }
 }
@@ -479,62 +569,52 @@ void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s) {
 }
 
 template <typename T>
-void launch_pcg_init(int n, const double* F, const T* diag, double* u64, T* r, T* z, T* p,
-                     PcgScalars* st, RedWs ws, hipStream_t s) {
-  hipLaunchKernelGGL((k_pcg_init<T>), dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, n, F,
-                     diag, u64, r, z, p, st, ws);
+void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_init<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red);
 }
 
 template <typename T>
-void launch_pcg_spmv_alpha(int format, const CsrDev& A, const BsrDev& Ab, int n, const T* p, T* Ap,
-                           PcgScalars* st, RedWs ws, hipStream_t s) {
-  (void)n;
+void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
+                    PcgScalars* st, const CgRed& red, const PcgParams& prm, int mixed,
+                    hipStream_t s) {
   if (format == 0)
-    hipLaunchKernelGGL((k_pcg_spmv_csr<T, kCsrG>), dim3(grid_for(A.n, kBlock / kCsrG)),
-                       dim3(kBlock), 0, s, A, p, Ap, st, ws);
+    hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red,
+                       prm, mixed);
   else if (format == 3)
-    hipLaunchKernelGGL((k_pcg_spmv_bsr<T, 3>), dim3(grid_for(Ab.nbr, kBlock / BsrLanes<3>::G)),
-                       dim3(kBlock), 0, s, Ab, p, Ap, st, ws);
+    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red, prm,
+                       mixed);
   else
-    hipLaunchKernelGGL((k_pcg_spmv_bsr<T, 6>), dim3(grid_for(Ab.nbr, kBlock / BsrLanes<6>::G)),
-                       dim3(kBlock), 0, s, Ab, p, Ap, st, ws);
+    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red, prm,
+                       mixed);
 }
 
 template <typename T>
-void launch_pcg_update(int n, int mixed, const T* diag, const T* p, const T* Ap, double* u64, T* r,
-                       T* z, PcgScalars* st, PcgParams prm, RedWs ws, hipStream_t s) {
-  hipLaunchKernelGGL((k_pcg_update<T>), dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, n, mixed,
-                     diag, p, Ap, u64, r, z, st, prm, ws);
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red);
 }
 
-template <typename T>
-void launch_pcg_pupdate(int n, const T* z, T* p, PcgScalars* st, hipStream_t s) {
-  hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, n, z, p,
-                     st);
-}
-
-void launch_refresh_true_residual(int format, const CsrDev& A, const BsrDev& Ab, int n,
-                                  const double* F, const double* u64, double* r64, PcgScalars* st,
-                                  PcgParams prm, RedWs ws, hipStream_t s) {
-  (void)n;
+void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
+                             const double* u64, double* r64, PcgScalars* st, const CgRed& red,
+                             hipStream_t s) {
   if (format == 0)
-    hipLaunchKernelGGL((k_true_residual_csr<kCsrG>), dim3(grid_for(A.n, kBlock / kCsrG)),
-                       dim3(kBlock), 0, s, A, F, u64, r64, st, prm, ws);
+    hipLaunchKernelGGL((k_refresh_residual_csr<kCsrG>), dim3(red.nR), dim3(kBlock), 0, s, A, F, u64,
+                       r64, st, red);
   else if (format == 3)
-    hipLaunchKernelGGL((k_true_residual_bsr<3>), dim3(grid_for(Ab.nbr, kBlock / BsrLanes<3>::G)),
-                       dim3(kBlock), 0, s, Ab, F, u64, r64, st, prm, ws);
+    hipLaunchKernelGGL((k_refresh_residual_bsr<3>), dim3(red.nR), dim3(kBlock), 0, s, Ab, F, u64,
+                       r64, st, red);
   else
-    hipLaunchKernelGGL((k_true_residual_bsr<6>), dim3(grid_for(Ab.nbr, kBlock / BsrLanes<6>::G)),
-                       dim3(kBlock), 0, s, Ab, F, u64, r64, st, prm, ws);
+    hipLaunchKernelGGL((k_refresh_residual_bsr<6>), dim3(red.nR), dim3(kBlock), 0, s, Ab, F, u64,
+                       r64, st, red);
 }
 
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
-                           float* p, PcgScalars* st, PcgParams prm, RedWs ws, hipStream_t s) {
-  hipLaunchKernelGGL(k_refresh_reload, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, n, r64,
-                     diag, r, z, st, prm, ws);
-  // p = z + beta p runs only when the reload resumed the iteration.
-  hipLaunchKernelGGL((k_pcg_pupdate<float>), dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, n,
-                     z, p, st);
+void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
+                           const CgRed& red, const PcgParams& prm, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_reload, dim3(red.nB), dim3(kBlock), 0, s, n, r64, diag, r, st, red,
+                     prm);
+}
+
+int rows_per_block(int format) {
+  return format == 0 ? kBlock / kCsrG : (format == 3 ? kBlock / BsrLanes<3>::G : kBlock / BsrLanes<6>::G);
 }
 
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s) {
@@ -549,20 +629,13 @@ template void launch_spmv_csr<double>(const CsrDev&, const double*, double*, hip
 template void launch_spmv_csr<float>(const CsrDev&, const float*, float*, hipStream_t);
 template void launch_spmv_bsr<double>(const BsrDev&, const double*, double*, hipStream_t);
 template void launch_spmv_bsr<float>(const BsrDev&, const float*, float*, hipStream_t);
-template void launch_pcg_init<double>(int, const double*, const double*, double*, double*, double*,
-                                      double*, PcgScalars*, RedWs, hipStream_t);
-template void launch_pcg_init<float>(int, const double*, const float*, double*, float*, float*,
-                                     float*, PcgScalars*, RedWs, hipStream_t);
-template void launch_pcg_spmv_alpha<double>(int, const CsrDev&, const BsrDev&, int, const double*,
-                                            double*, PcgScalars*, RedWs, hipStream_t);
-template void launch_pcg_spmv_alpha<float>(int, const CsrDev&, const BsrDev&, int, const float*,
-                                           float*, PcgScalars*, RedWs, hipStream_t);
-template void launch_pcg_update<double>(int, int, const double*, const double*, const double*,
-                                        double*, double*, double*, PcgScalars*, PcgParams, RedWs,
-                                        hipStream_t);
-template void launch_pcg_update<float>(int, int, const float*, const float*, const float*, double*,
-                                       float*, float*, PcgScalars*, PcgParams, RedWs, hipStream_t);
-template void launch_pcg_pupdate<double>(int, const double*, double*, PcgScalars*, hipStream_t);
-template void launch_pcg_pupdate<float>(int, const float*, float*, PcgScalars*, hipStream_t);
+template void launch_cg_init<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
+template void launch_cg_init<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
+template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const CgBufs<double>&,
+                                     PcgScalars*, const CgRed&, const PcgParams&, int, hipStream_t);
+template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
+                                    PcgScalars*, const CgRed&, const PcgParams&, int, hipStream_t);
+template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
+template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
 
 }  // namespace vbh
