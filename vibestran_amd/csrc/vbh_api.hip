@@ -648,8 +648,15 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
     const vbh::CgRed rd = c->red();
     const vbh::PcgParams prm = c->params();
     if (precision >= 2) {  // fused solver kernels need an initialised state (F in w_F)
-      if (precision % 2 == 0) vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
-      else vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
+      if (precision % 2 == 0) {
+        vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
+        vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd, prm, 0,
+                                    c->stream);
+      } else {
+        vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
+        vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd, prm, 1,
+                                   c->stream);
+      }
     }
     auto one = [&] {
       if (precision == 2) {
