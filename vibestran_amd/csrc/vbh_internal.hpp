@@ -80,15 +80,17 @@ constexpr int kMaxRed = 4;             // values per grid reduction
 // Device-side PCG scalars (one instance per right-hand side). Written only by
 // block 0 of a kernel, in fields no other block of that kernel reads.
 struct PcgScalars {
-  double rz[2];          // r.z of iterate k in slot k&1
-  double alpha, beta, pAp, rr, rel, norm_b;
+  double gamma[2];       // r.z of iteration i in slot i&1
+  double alpha[2];       // step length of iteration i in slot i&1
+  double beta, pAp, rr, rel, norm_b;
   double rr_refresh;     // recursive r.r at the last true-residual refresh (mixed)
   double rel_true;       // last FP64 true relative residual (mixed)
   double hist_rel[2];    // refresh history, slot = refresh count & 1
   int hist_stalls[2];
   int iter_done;         // completed iterations (written by cg_update)
   int iter_cur;          // iteration in progress (written by cg_spmv)
-  int done, flags, refreshes, pad0;
+  int resume_iter;       // iteration resumed after a refresh (written by refresh_reload)
+  int done, flags, refreshes;
 };
 
 enum : int {
@@ -98,8 +100,7 @@ enum : int {
   kFlagRz0 = 8,          // r.z <= 0 at start (impl:1859-1862)
   kFlagZeroRhs = 16,     // ||F|| < 1e-300 (impl:1822-1830)
   kFlagRefresh = 32,     // mixed: paused, FP64 true residual requested
-  kFlagStall = 64,       // mixed: true residual stopped improving
-  kFlagResumed = 128     // mixed: first cg_spmv after a refresh
+  kFlagStall = 64        // mixed: true residual stopped improving
 };
 
 struct PcgParams {
@@ -111,8 +112,8 @@ struct PcgParams {
 // Per-block partial sums passed from one kernel to the next. nA/nB/nR are the
 // grid sizes of the producing kernels (<= kMaxGrid).
 struct CgRed {
-  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap
-  double* partB;  // from cg_init / cg_update / refresh_reload: r.z, r.r, (F.F)
+  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: r.z, w.z, r.r
+  double* partB;  // from cg_init: F.F
   double* partR;  // from refresh_residual: |F - K u|^2
   int nA, nB, nR;
 };
@@ -122,9 +123,10 @@ struct CgBufs {
   const double* F;
   const T* diag;
   T* r;
-  T* p0;
-  T* p1;
-  T* Ap;
+  T* z;
+  T* p;
+  T* s;
+  T* w;
   double* u64;
 };
 
@@ -163,16 +165,16 @@ template <typename T>
 void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, const PcgParams& prm, int mixed,
-                    hipStream_t s);
+                    PcgScalars* st, const CgRed& red, hipStream_t s);
 template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
+                      const PcgParams& prm, int mixed, hipStream_t s);
 // Mixed refresh: r64 = F - K64 u64 (FP64), then decide and reload r32.
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
                              const double* u64, double* r64, PcgScalars* st, const CgRed& red,
                              hipStream_t s);
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
-                           const CgRed& red, const PcgParams& prm, hipStream_t s);
+void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
+                           PcgScalars* st, const CgRed& red, const PcgParams& prm, hipStream_t s);
 
 // Plain elementwise helpers.
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s);

@@ -139,26 +139,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
   }
 }
 
-// ---- Jacobi-PCG, two kernels per iteration ------------------------------------------
-// Every kernel that needs a global sum reduces the per-block partial sums its
-// predecessor wrote (fixed order, every block redundantly), so no atomics or
-// in-kernel fences are needed and the scalars are bit-identical in all blocks.
-// Only block 0 writes the scalar state, and never a field that other blocks
-// of the same kernel read.
+// ---- Jacobi-PCG, Chronopoulos-Gear form: two kernels per iteration -----------------
+// One global reduction per iteration. cg_spmv is a plain SpMV on the stored
+// z = r/diag with the three dots fused into it; cg_update reduces them (every
+// block redundantly, fixed order) and applies all vector updates:
 //
-//   init      r = F, u = 0, p_{-1} = 0; partials (r.z, r.r, F.F)       -> partB
-//   cg_spmv   reduce partB -> rz_k, rr_k; convergence / pause decision;
-//             p_k = r_k/diag + beta_k p_{k-1} formed in the gather;
-//             Ap_k = K p_k; partial p_k.Ap_k                           -> partA
-//   cg_update reduce partA -> alpha_k; u += alpha p_k; r -= alpha Ap_k;
-//             partials (r.z, r.r)                                       -> partB
-// Mixed precision pauses in cg_spmv when the recursive residual fell by
+//   init       u = 0, r = F, z = r/diag, p = s = 0; partial F.F       -> partB
+//   cg_spmv(i) w = K z; partials gamma = r.z, delta = w.z, rr = r.r   -> partA
+//   cg_update(i) reduce partA; convergence / pause / breakdown decision;
+//              beta = gamma/gamma_{i-1}; alpha = gamma/(delta - beta*gamma/alpha_{i-1})
+//              p = z + beta p; s = w + beta s; u += alpha p; r -= alpha s; z = r/diag
+//
+// (delta - beta*gamma/alpha_{i-1} equals p.Kp, so its sign test is the
+// reference's p.Ap <= 0 breakdown test.) The standard recurrence of the
+// reference (cuda_pcg_backend_impl.hpp:1867-1954) and this one produce the
+// same iterates in exact arithmetic; the convergence test is applied to the
+// same r_i after the same number of updates.
+// Mixed precision pauses in cg_update when the recursive residual fell by
 // refresh_delta since the last refresh (or below tol); the host then runs
 //   refresh_residual  r64 = F - K64 u (FP64)                            -> partR
 //   refresh_reload    reduce partR: converged / stalled / continue with
-//                     r = (float) r64                                    -> partB
-// and resumes: the next cg_spmv continues the recurrence (beta from the
-// refreshed r.z), as in reliable-update CG.
+//                     r = (float) r64, z = r/diag
+// and the recurrence continues from the refreshed residual (reliable-update CG).
+//
+// Only block 0 writes the scalar state, and never a field that other blocks
+// of the same kernel read: iteration counters are split (iter_cur written by
+// cg_spmv, iter_done by cg_update) and alpha/gamma are double-buffered.
 
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nprod,
@@ -194,249 +200,186 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScalars* st,
                                                     CgRed red) {
   __shared__ double lds[kLds];
-  double v[3] = {0.0, 0.0, 0.0};  // r.z, r.r (precision T), F.F
+  double v[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double f = b.F[i];
     const T ri = static_cast<T>(f);
-    const T zi = ri / b.diag[i];
     b.u64[i] = 0.0;
     b.r[i] = ri;
-    b.p1[i] = T(0);  // p_{-1}
-    v[0] += static_cast<double>(ri) * static_cast<double>(zi);
-    v[1] += static_cast<double>(ri) * static_cast<double>(ri);
-    v[2] += f * f;
+    b.z[i] = ri / b.diag[i];
+    b.p[i] = T(0);
+    b.s[i] = T(0);
+    v[0] += f * f;
   }
-  write_partials<3>(v, red.partB, lds);
+  write_partials<1>(v, red.partB, lds);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->iter_done = 0;
     st->iter_cur = 0;
+    st->resume_iter = -1;
     st->flags = 0;
     st->done = 0;
     st->refreshes = 0;
     st->hist_stalls[0] = st->hist_stalls[1] = 0;
     st->hist_rel[0] = st->hist_rel[1] = 1e300;
-    st->alpha = st->beta = st->pAp = 0.0;
+    st->gamma[0] = st->gamma[1] = 0.0;
+    st->alpha[0] = st->alpha[1] = 0.0;
+    st->beta = st->pAp = 0.0;
     st->rel = 1.0;
     st->rel_true = 1.0;
-    st->rz[0] = st->rz[1] = 0.0;
   }
 }
 
-// Prologue shared by the CSR and BSR variants of cg_spmv. Returns false if the
-// kernel must exit; otherwise beta_k for the on-the-fly search direction.
-__device__ __forceinline__ bool cg_spmv_prologue(PcgScalars* st, const CgRed& red,
-                                                 const PcgParams& prm, int mixed, double* lds,
-                                                 int* s_i, int& k, double& beta) {
+// Broadcast of state words: thread 0 reads, the block agrees.
+__device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
   if (threadIdx.x == 0) {
     s_i[0] = st->done;
     s_i[1] = st->iter_done;
-    s_i[2] = st->flags;
+    s_i[2] = st->iter_cur;
+    s_i[3] = st->resume_iter;
   }
   __syncthreads();
-  if (s_i[0]) return false;
-  k = s_i[1];
-  const bool resumed = (s_i[2] & kFlagResumed) != 0;
-  double v[3];
-  reduce_partials<3>(red.partB, red.nB, v, lds);
-  const double rz = v[0], rr = v[1];
-  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
-  if (k == 0 && !resumed) {
-    const double norm_b = sqrt(fabs(v[2]));
-    if (lead) {
-      st->norm_b = norm_b;
-      st->rr_refresh = rr;
-      st->rr = rr;
-      st->rz[0] = rz;
-      st->iter_cur = 0;
-    }
-    if (norm_b < 1e-300) {  // zero load: u = 0 after 0 iterations (impl:1822-1830)
-      if (lead) {
-        st->rel = 0.0;
-        st->flags |= kFlagZeroRhs;
-        st->done = 1;
-      }
-      return false;
-    }
-    if (rz <= 0.0) {  // impl:1859-1862
-      if (lead) {
-        st->rz[0] = rz;
-        st->flags |= kFlagRz0;
-        st->done = 1;
-      }
-      return false;
-    }
-    if (prm.max_iters <= 0) {
-      if (lead) {
-        st->flags |= kFlagMaxIter;
-        st->done = 1;
-      }
-      return false;
-    }
-    beta = 0.0;
-    return true;
-  }
-  const double rel = sqrt(fabs(rr)) / st->norm_b;
-  int stop = 0;
-  if (!mixed) {
-    if (rel < prm.tol) stop = kFlagConverged;  // finalize_iteration, impl:704-738
-  } else if (!resumed &&
-             (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh)) {
-    stop = kFlagRefresh;  // pause for an FP64 true-residual refresh
-  }
-  if (!stop && k >= prm.max_iters) stop = kFlagMaxIter;
-  if (stop) {
-    if (lead) {
-      st->rel = rel;
-      st->rr = rr;
-      st->iter_cur = k;
-      st->flags |= stop;
-      st->done = 1;
-    }
-    return false;
-  }
-  beta = rz / st->rz[(k - 1) & 1];
-  if (lead) {
-    st->rz[k & 1] = rz;
-    st->rr = rr;
-    st->rel = rel;
-    st->beta = beta;
-    st->iter_cur = k;
-    if (resumed) st->rr_refresh = rr;
-  }
-  return true;
-}
-
-template <typename T>
-__device__ __forceinline__ T pdir(const CgBufs<T>& b, const T* __restrict__ pold, T beta, int j) {
-  return b.r[j] / b.diag[j] + beta * pold[j];
 }
 
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red, PcgParams prm, int mixed) {
+                                                        CgRed red) {
   __shared__ double lds[kLds];
   __shared__ int s_i[4];
-  int k;
-  double betad;
-  if (!cg_spmv_prologue(st, red, prm, mixed, lds, s_i, k, betad)) return;
-  const T beta = static_cast<T>(betad);
-  const T* pold = (k & 1) ? b.p0 : b.p1;
-  T* pnew = (k & 1) ? b.p1 : b.p0;
+  read_words(st, s_i);
+  if (s_i[0]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = s_i[1];
   const T* val = vals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
-  double d[1] = {0.0};
+  double d[3] = {0.0, 0.0, 0.0};
   for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
-    T s = T(0);
-    const int e = A.rp[r + 1];
-    for (int q = A.rp[r] + lg; q < e; q += G) s += val[q] * pdir(b, pold, beta, A.ci[q]);
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+    const T w = csr_row<T, G>(A, val, b.z, r, lg);
     if (lg == 0) {
-      const T pr = pdir(b, pold, beta, r);
-      pnew[r] = pr;
-      b.Ap[r] = s;
-      d[0] += static_cast<double>(pr) * static_cast<double>(s);
+      b.w[r] = w;
+      const double rr = b.r[r], zz = b.z[r];
+      d[0] += rr * zz;
+      d[1] += static_cast<double>(w) * zz;
+      d[2] += rr * rr;
     }
   }
-  write_partials<1>(d, red.partA, lds);
+  write_partials<3>(d, red.partA, lds);
 }
 
 template <typename T, int B>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red, PcgParams prm, int mixed) {
+                                                        CgRed red) {
   constexpr int G = BsrLanes<B>::G;
   __shared__ double lds[kLds];
   __shared__ int s_i[4];
-  int k;
-  double betad;
-  if (!cg_spmv_prologue(st, red, prm, mixed, lds, s_i, k, betad)) return;
-  const T beta = static_cast<T>(betad);
-  const T* pold = (k & 1) ? b.p0 : b.p1;
-  T* pnew = (k & 1) ? b.p1 : b.p0;
+  read_words(st, s_i);
+  if (s_i[0]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = s_i[1];
   const T* val = bvals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
-  double d[1] = {0.0};
+  double d[3] = {0.0, 0.0, 0.0};
   for (int I = (blockIdx.x * blockDim.x + threadIdx.x) / G; I < A.nbr; I += stride) {
     T acc[B];
-#pragma unroll
-    for (int r = 0; r < B; ++r) acc[r] = T(0);
-    const int o = A.brp[I];
-    const int nb = A.brp[I + 1] - o;
-    const T* vb = val + static_cast<size_t>(B * B) * o;
-    for (int q = lg; q < nb; q += G) {
-      const int J = A.bci[o + q];
-      T xj[B];
-#pragma unroll
-      for (int c = 0; c < B; ++c) xj[c] = pdir(b, pold, beta, B * J + c);
-#pragma unroll
-      for (int r = 0; r < B; ++r)
-#pragma unroll
-        for (int c = 0; c < B; ++c) acc[r] += vb[(r * B + c) * nb + q] * xj[c];
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int r = 0; r < B; ++r) acc[r] += __shfl_xor(acc[r], off, G);
+    bsr_row<T, B, G>(A, val, b.z, I, lg, acc);
     if (lg == 0) {
 #pragma unroll
-      for (int r = 0; r < B; ++r) {
-        const T pr = pdir(b, pold, beta, B * I + r);
-        pnew[B * I + r] = pr;
-        b.Ap[B * I + r] = acc[r];
-        d[0] += static_cast<double>(pr) * static_cast<double>(acc[r]);
+      for (int c = 0; c < B; ++c) {
+        b.w[B * I + c] = acc[c];
+        const double rr = b.r[B * I + c], zz = b.z[B * I + c];
+        d[0] += rr * zz;
+        d[1] += static_cast<double>(acc[c]) * zz;
+        d[2] += rr * rr;
       }
     }
   }
-  write_partials<1>(d, red.partA, lds);
+  write_partials<3>(d, red.partA, lds);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgScalars* st,
-                                                      CgRed red) {
+                                                      CgRed red, PcgParams prm, int mixed) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[2];
-  if (threadIdx.x == 0) {
-    s_i[0] = st->done;
-    s_i[1] = st->iter_cur;
-  }
-  __syncthreads();
+  __shared__ int s_i[4];
+  read_words(st, s_i);
   if (s_i[0]) return;
-  const int k = s_i[1];
-  double v1[1];
-  reduce_partials<1>(red.partA, red.nA, v1, lds);
-  const double pAp = v1[0];
+  const int i = s_i[2];
+  const bool resumed = (s_i[3] == i);
   const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
-  if (!(pAp > 0.0)) {  // breakdown (cuda_pcg_backend_impl.hpp:691-696)
+  double v[3];
+  reduce_partials<3>(red.partA, red.nA, v, lds);
+  const double gamma = v[0], delta = v[1], rr = v[2];
+  double norm_b;
+  if (i == 0) {
+    double f[1];
+    reduce_partials<1>(red.partB, red.nB, f, lds);
+    norm_b = sqrt(fabs(f[0]));
+  } else {
+    norm_b = st->norm_b;
+  }
+  int stop = 0;
+  double rel = 0.0;
+  if (i == 0 && !resumed) {
+    if (norm_b < 1e-300) stop = kFlagZeroRhs;           // impl:1822-1830
+    else if (gamma <= 0.0) stop = kFlagRz0;              // impl:1859-1862
+    else if (prm.max_iters <= 0) stop = kFlagMaxIter;
+    rel = (stop == kFlagZeroRhs) ? 0.0 : 1.0;
+  } else {
+    rel = sqrt(fabs(rr)) / norm_b;
+    if (!mixed) {
+      if (rel < prm.tol) stop = kFlagConverged;          // finalize_iteration, impl:704-738
+    } else if (!resumed &&
+               (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh)) {
+      stop = kFlagRefresh;                               // pause for an FP64 refresh
+    }
+    if (!stop && i >= prm.max_iters) stop = kFlagMaxIter;
+  }
+  double beta = 0.0, alpha = 0.0, pAp = 0.0;
+  if (!stop) {
+    if (i == 0) {
+      pAp = delta;
+    } else {
+      beta = gamma / st->gamma[(i - 1) & 1];
+      pAp = delta - beta * gamma / st->alpha[(i - 1) & 1];
+    }
+    if (!(pAp > 0.0)) stop = kFlagBreakdown;             // impl:691-696
+    else alpha = gamma / pAp;
+  }
+  if (stop) {
     if (lead) {
+      if (i == 0) st->norm_b = norm_b;
+      st->rel = (stop == kFlagBreakdown) ? 1e300 : rel;
+      st->rr = rr;
       st->pAp = pAp;
-      st->alpha = 0.0;
-      st->rel = 1e300;
-      st->flags |= kFlagBreakdown;
+      if (stop == kFlagRz0) st->gamma[0] = gamma;
+      st->flags |= stop;
       st->done = 1;
     }
     return;
   }
-  const double alpha = st->rz[k & 1] / pAp;
-  const T a = static_cast<T>(alpha);
-  const T* p = (k & 1) ? b.p1 : b.p0;
-  double v[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const T pi = p[i];
-    b.u64[i] += alpha * static_cast<double>(pi);
-    const T ri = b.r[i] - a * b.Ap[i];
-    const T zi = ri / b.diag[i];
-    b.r[i] = ri;
-    v[0] += static_cast<double>(ri) * static_cast<double>(zi);
-    v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+  const T a = static_cast<T>(alpha), bt = static_cast<T>(beta);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const T p = b.z[k] + bt * b.p[k];
+    const T s = b.w[k] + bt * b.s[k];
+    b.p[k] = p;
+    b.s[k] = s;
+    b.u64[k] += alpha * static_cast<double>(p);
+    const T r = b.r[k] - a * s;
+    b.r[k] = r;
+    b.z[k] = r / b.diag[k];
   }
-  write_partials<2>(v, red.partB, lds);
   if (lead) {
+    if (i == 0) {
+      st->norm_b = norm_b;
+      st->rr_refresh = rr;
+    }
+    if (resumed) st->rr_refresh = rr;
+    st->gamma[i & 1] = gamma;
+    st->alpha[i & 1] = alpha;
+    st->beta = beta;
     st->pAp = pAp;
-    st->alpha = alpha;
-    st->iter_done = k + 1;
-    st->flags &= ~kFlagResumed;
+    st->rr = rr;
+    st->rel = rel;
+    st->iter_done = i + 1;
   }
 }
 
@@ -487,10 +430,12 @@ __global__ __launch_bounds__(kBlock) void k_refresh_residual_bsr(BsrDev A, const
   if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
-// Decide from the FP64 true residual; on "continue" reload r = (float) r64.
+// Decide from the FP64 true residual; on "continue" reload r = (float) r64 and
+// z = r/diag, and mark the paused iteration as resumed.
 __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* __restrict__ r64,
                                                            const float* __restrict__ diag,
-                                                           float* __restrict__ r, PcgScalars* st,
+                                                           float* __restrict__ r,
+                                                           float* __restrict__ z, PcgScalars* st,
                                                            CgRed red, PcgParams prm) {
   __shared__ double lds[kLds];
   double v1[1];
@@ -521,17 +466,14 @@ __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* 
     }
     return;  // done stays set
   }
-  double v[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float ri = static_cast<float>(r64[i]);
-    const float zi = ri / diag[i];
-    r[i] = ri;
-    v[0] += static_cast<double>(ri) * static_cast<double>(zi);
-    v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const float rk = static_cast<float>(r64[k]);
+    r[k] = rk;
+    z[k] = rk / diag[k];
   }
-  write_partials<3>(v, red.partB, lds);
   if (lead) {
-    st->flags = (st->flags & ~kFlagRefresh) | kFlagResumed;
+    st->resume_iter = st->iter_done;
+    st->flags &= ~kFlagRefresh;
     st->done = 0;
   }
 }
@@ -575,22 +517,19 @@ void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
 
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, const PcgParams& prm, int mixed,
-                    hipStream_t s) {
+                    PcgScalars* st, const CgRed& red, hipStream_t s) {
   if (format == 0)
-    hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red,
-                       prm, mixed);
+    hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red);
   else if (format == 3)
-    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red, prm,
-                       mixed);
+    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
   else
-    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red, prm,
-                       mixed);
+    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
 }
 
 template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_update<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red);
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
+                      const PcgParams& prm, int mixed, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red, prm, mixed);
 }
 
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
@@ -607,9 +546,9 @@ void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, cons
                        r64, st, red);
 }
 
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
-                           const CgRed& red, const PcgParams& prm, hipStream_t s) {
-  hipLaunchKernelGGL(k_refresh_reload, dim3(red.nB), dim3(kBlock), 0, s, n, r64, diag, r, st, red,
+void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
+                           PcgScalars* st, const CgRed& red, const PcgParams& prm, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_reload, dim3(red.nB), dim3(kBlock), 0, s, n, r64, diag, r, z, st, red,
                      prm);
 }
 
@@ -632,10 +571,12 @@ template void launch_spmv_bsr<float>(const BsrDev&, const float*, float*, hipStr
 template void launch_cg_init<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_init<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const CgBufs<double>&,
-                                     PcgScalars*, const CgRed&, const PcgParams&, int, hipStream_t);
+                                     PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
-                                    PcgScalars*, const CgRed&, const PcgParams&, int, hipStream_t);
-template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
-template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
+                                    PcgScalars*, const CgRed&, hipStream_t);
+template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
+                                       const PcgParams&, int, hipStream_t);
+template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
+                                      const PcgParams&, int, hipStream_t);
 
 }  // namespace vbh
