@@ -264,8 +264,6 @@ void Ctx::ensure_work(int nvec_len) {
   w_z.resize(N);
   w_p.resize(N);
   w_Ap.resize(N);
-  w_s.resize(N);
-  w_s32.resize(N);
   w_r32.resize(N);
   w_z32.resize(N);
   w_p32.resize(N);
@@ -314,11 +312,11 @@ CgRed Ctx::red() const {
 }
 
 CgBufs<double> Ctx::bufs64() {
-  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_z.p, w_p.p, w_s.p, w_Ap.p, w_u.p};
+  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_z.p, w_p.p, w_Ap.p, w_u.p};
 }
 
 CgBufs<float> Ctx::bufs32() {
-  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_z32.p, w_p32.p, w_s32.p, w_Ap32.p, w_u.p};
+  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_z32.p, w_p32.p, w_Ap32.p, w_u.p};
 }
 
 void Ctx::enqueue_iterations(int precision_mode, int count) {
@@ -328,12 +326,14 @@ void Ctx::enqueue_iterations(int precision_mode, int count) {
   for (int i = 0; i < count; ++i) {
     if (precision_mode == VBH_PREC_FP64) {
       const CgBufs<double> b = bufs64();
+      launch_cg_direction<double>(N, b, st_dev.p, rd, prm, 0, stream);
       launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, stream);
-      launch_cg_update<double>(N, b, st_dev.p, rd, prm, 0, stream);
+      launch_cg_update<double>(N, b, st_dev.p, rd, stream);
     } else {
       const CgBufs<float> b = bufs32();
+      launch_cg_direction<float>(N, b, st_dev.p, rd, prm, 1, stream);
       launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, stream);
-      launch_cg_update<float>(N, b, st_dev.p, rd, prm, 1, stream);
+      launch_cg_update<float>(N, b, st_dev.p, rd, stream);
     }
   }
 }
@@ -412,7 +412,7 @@ PcgScalars Ctx::run_solve(int precision_mode) {
 // Maps the device flags onto the reference's outcomes.
 void Ctx::check_outcome(const PcgScalars& s, int precision_mode) {
   if (s.flags & kFlagRz0)
-    throw_error(VBH_ERR_SOLVER, "HIP PCG: non-positive initial r*z=" + fmt_g(s.gamma[0]) +
+    throw_error(VBH_ERR_SOLVER, "HIP PCG: non-positive initial r*z=" + fmt_g(s.rz[0]) +
                                     " -- preconditioner is not SPD or matrix is singular");
   if (s.flags & (kFlagBreakdown | kFlagMaxIter)) {
     // Breakdown is reported as non-convergence (impl:1934-1942, 1964-1970).
@@ -652,10 +652,12 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
     if (precision >= 2) {  // fused solver kernels need an initialised state (F in w_F)
       if (precision % 2 == 0) {
         vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
+        vbh::launch_cg_direction<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, prm, 0, c->stream);
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
                                     c->stream);
       } else {
         vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
+        vbh::launch_cg_direction<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, prm, 1, c->stream);
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
                                    c->stream);
       }
@@ -668,9 +670,9 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
                                    c->stream);
       } else if (precision == 4) {
-        vbh::launch_cg_update<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, prm, 0, c->stream);
+        vbh::launch_cg_update<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
       } else if (precision == 5) {
-        vbh::launch_cg_update<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, prm, 1, c->stream);
+        vbh::launch_cg_update<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
       } else if (precision == 0) {
         if (c->format == 0) vbh::launch_spmv_csr<double>(c->csr, c->w_r64.p, c->w_Ap.p, c->stream);
         else vbh::launch_spmv_bsr<double>(c->bsrd, c->w_r64.p, c->w_Ap.p, c->stream);

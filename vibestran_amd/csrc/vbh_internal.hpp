@@ -80,15 +80,14 @@ constexpr int kMaxRed = 4;             // values per grid reduction
 // Device-side PCG scalars (one instance per right-hand side). Written only by
 // block 0 of a kernel, in fields no other block of that kernel reads.
 struct PcgScalars {
-  double gamma[2];       // r.z of iteration i in slot i&1
-  double alpha[2];       // step length of iteration i in slot i&1
-  double beta, pAp, rr, rel, norm_b;
+  double rz[2];          // r.z of iteration i in slot i&1
+  double alpha, beta, pAp, rr, rel, norm_b;
   double rr_refresh;     // recursive r.r at the last true-residual refresh (mixed)
   double rel_true;       // last FP64 true relative residual (mixed)
   double hist_rel[2];    // refresh history, slot = refresh count & 1
   int hist_stalls[2];
   int iter_done;         // completed iterations (written by cg_update)
-  int iter_cur;          // iteration in progress (written by cg_spmv)
+  int iter_cur;          // iteration in progress (written by cg_direction)
   int resume_iter;       // iteration resumed after a refresh (written by refresh_reload)
   int done, flags, refreshes;
 };
@@ -112,8 +111,8 @@ struct PcgParams {
 // Per-block partial sums passed from one kernel to the next. nA/nB/nR are the
 // grid sizes of the producing kernels (<= kMaxGrid).
 struct CgRed {
-  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: r.z, w.z, r.r
-  double* partB;  // from cg_init: F.F
+  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap
+  double* partB;  // from cg_init / cg_update / refresh_reload: r.z, r.r, (F.F)
   double* partR;  // from refresh_residual: |F - K u|^2
   int nA, nB, nR;
 };
@@ -125,8 +124,7 @@ struct CgBufs {
   T* r;
   T* z;
   T* p;
-  T* s;
-  T* w;
+  T* Ap;
   double* u64;
 };
 
@@ -167,8 +165,10 @@ template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
                     PcgScalars* st, const CgRed& red, hipStream_t s);
 template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
-                      const PcgParams& prm, int mixed, hipStream_t s);
+void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
+                         const PcgParams& prm, int mixed, hipStream_t s);
+template <typename T>
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
 // Mixed refresh: r64 = F - K64 u64 (FP64), then decide and reload r32.
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
                              const double* u64, double* r64, PcgScalars* st, const CgRed& red,

@@ -139,32 +139,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
   }
 }
 
-// ---- Jacobi-PCG, Chronopoulos-Gear form: two kernels per iteration -----------------
-// One global reduction per iteration. cg_spmv is a plain SpMV on the stored
-// z = r/diag with the three dots fused into it; cg_update reduces them (every
-// block redundantly, fixed order) and applies all vector updates:
+// ---- Jacobi-PCG: three kernels per iteration -------------------------------------------
+// The reference's recurrence (cuda_pcg_backend_impl.hpp:1867-1954) with every
+// global sum produced as per-block partials and reduced by the NEXT kernel
+// (every block redundantly, fixed order): deterministic, no atomics, no
+// in-kernel fences, scalars bit-identical across blocks.
 //
-//   init       u = 0, r = F, z = r/diag, p = s = 0; partial F.F       -> partB
-//   cg_spmv(i) w = K z; partials gamma = r.z, delta = w.z, rr = r.r   -> partA
-//   cg_update(i) reduce partA; convergence / pause / breakdown decision;
-//              beta = gamma/gamma_{i-1}; alpha = gamma/(delta - beta*gamma/alpha_{i-1})
-//              p = z + beta p; s = w + beta s; u += alpha p; r -= alpha s; z = r/diag
+//   init          u = 0, r = F, z = r/diag, p = 0; partials (r.z, r.r, F.F)   -> partB
+//   cg_direction  reduce partB -> rz_i, rr_i; convergence / pause / limit
+//                 decision (the reference's finalize step, impl:704-738);
+//                 beta = rz_i / rz_{i-1};  p = z + beta p
+//   cg_spmv       Ap = K p (plain SpMV); partial p.Ap                          -> partA
+//   cg_update     reduce partA -> alpha = rz_i / p.Ap (breakdown if p.Ap <= 0);
+//                 u += alpha p (FP64); r -= alpha Ap; z = r/diag;
+//                 partials (r.z, r.r)                                          -> partB
 //
-// (delta - beta*gamma/alpha_{i-1} equals p.Kp, so its sign test is the
-// reference's p.Ap <= 0 breakdown test.) The standard recurrence of the
-// reference (cuda_pcg_backend_impl.hpp:1867-1954) and this one produce the
-// same iterates in exact arithmetic; the convergence test is applied to the
-// same r_i after the same number of updates.
-// Mixed precision pauses in cg_update when the recursive residual fell by
+// Mixed precision pauses in cg_direction when the recursive residual fell by
 // refresh_delta since the last refresh (or below tol); the host then runs
-//   refresh_residual  r64 = F - K64 u (FP64)                            -> partR
+//   refresh_residual  r64 = F - K64 u (FP64)                                   -> partR
 //   refresh_reload    reduce partR: converged / stalled / continue with
-//                     r = (float) r64, z = r/diag
-// and the recurrence continues from the refreshed residual (reliable-update CG).
+//                     r = (float) r64, z = r/diag; partials (r.z, r.r)        -> partB
+// and cg_direction resumes the paused iteration with the refreshed residual
+// (beta from the refreshed r.z: reliable-update CG).
 //
 // Only block 0 writes the scalar state, and never a field that other blocks
 // of the same kernel read: iteration counters are split (iter_cur written by
-// cg_spmv, iter_done by cg_update) and alpha/gamma are double-buffered.
+// cg_direction, iter_done by cg_update) and rz is double-buffered.
 
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nprod,
@@ -196,39 +196,6 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restri
 
 constexpr int kLds = 5 * kMaxRed;
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScalars* st,
-                                                    CgRed red) {
-  __shared__ double lds[kLds];
-  double v[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double f = b.F[i];
-    const T ri = static_cast<T>(f);
-    b.u64[i] = 0.0;
-    b.r[i] = ri;
-    b.z[i] = ri / b.diag[i];
-    b.p[i] = T(0);
-    b.s[i] = T(0);
-    v[0] += f * f;
-  }
-  write_partials<1>(v, red.partB, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->iter_done = 0;
-    st->iter_cur = 0;
-    st->resume_iter = -1;
-    st->flags = 0;
-    st->done = 0;
-    st->refreshes = 0;
-    st->hist_stalls[0] = st->hist_stalls[1] = 0;
-    st->hist_rel[0] = st->hist_rel[1] = 1e300;
-    st->gamma[0] = st->gamma[1] = 0.0;
-    st->alpha[0] = st->alpha[1] = 0.0;
-    st->beta = st->pAp = 0.0;
-    st->rel = 1.0;
-    st->rel_true = 1.0;
-  }
-}
-
 // Broadcast of state words: thread 0 reads, the block agrees.
 __device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
   if (threadIdx.x == 0) {
@@ -240,6 +207,106 @@ __device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
   __syncthreads();
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScalars* st,
+                                                    CgRed red) {
+  __shared__ double lds[kLds];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double f = b.F[i];
+    const T ri = static_cast<T>(f);
+    const T zi = ri / b.diag[i];
+    b.u64[i] = 0.0;
+    b.r[i] = ri;
+    b.z[i] = zi;
+    b.p[i] = T(0);
+    v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+    v[2] += f * f;
+  }
+  write_partials<3>(v, red.partB, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter_done = 0;
+    st->iter_cur = 0;
+    st->resume_iter = -1;
+    st->flags = 0;
+    st->done = 0;
+    st->refreshes = 0;
+    st->hist_stalls[0] = st->hist_stalls[1] = 0;
+    st->hist_rel[0] = st->hist_rel[1] = 1e300;
+    st->rz[0] = st->rz[1] = 0.0;
+    st->alpha = st->beta = st->pAp = 0.0;
+    st->rel = 1.0;
+    st->rel_true = 1.0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, PcgScalars* st,
+                                                         CgRed red, PcgParams prm, int mixed) {
+  __shared__ double lds[kLds];
+  __shared__ int s_i[4];
+  read_words(st, s_i);
+  if (s_i[0]) return;
+  const int i = s_i[1];
+  const bool resumed = (s_i[3] == i);
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  double v[3];
+  reduce_partials<3>(red.partB, red.nB, v, lds);
+  const double rz = v[0], rr = v[1];
+  double norm_b, beta = 0.0, rel;
+  int stop = 0;
+  if (i == 0 && !resumed) {
+    norm_b = sqrt(fabs(v[2]));
+    rel = 1.0;
+    if (norm_b < 1e-300) {  // zero load: u = 0 after 0 iterations (impl:1822-1830)
+      stop = kFlagZeroRhs;
+      rel = 0.0;
+    } else if (rz <= 0.0) {  // impl:1859-1862
+      stop = kFlagRz0;
+    } else if (prm.max_iters <= 0) {
+      stop = kFlagMaxIter;
+    }
+  } else {
+    norm_b = st->norm_b;
+    rel = sqrt(fabs(rr)) / norm_b;
+    if (!mixed) {
+      if (rel < prm.tol) stop = kFlagConverged;
+    } else if (!resumed &&
+               (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh)) {
+      stop = kFlagRefresh;  // pause for an FP64 true-residual refresh
+    }
+    if (!stop && i >= prm.max_iters) stop = kFlagMaxIter;
+    if (!stop) beta = rz / st->rz[(i - 1) & 1];
+  }
+  if (stop) {
+    if (lead) {
+      if (i == 0 && !resumed) st->norm_b = norm_b;
+      st->rz[i & 1] = rz;
+      st->rr = rr;
+      st->rel = rel;
+      st->flags |= stop;
+      st->done = 1;
+    }
+    return;
+  }
+  const T bt = static_cast<T>(beta);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    b.p[k] = b.z[k] + bt * b.p[k];
+  if (lead) {
+    if (i == 0 && !resumed) {
+      st->norm_b = norm_b;
+      st->rr_refresh = rr;
+    }
+    if (resumed) st->rr_refresh = rr;
+    st->rz[i & 1] = rz;
+    st->rr = rr;
+    st->rel = rel;
+    st->beta = beta;
+    st->iter_cur = i;
+  }
+}
+
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
@@ -247,22 +314,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
   __shared__ int s_i[4];
   read_words(st, s_i);
   if (s_i[0]) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = s_i[1];
   const T* val = vals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
-  double d[3] = {0.0, 0.0, 0.0};
+  double d[1] = {0.0};
   for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
-    const T w = csr_row<T, G>(A, val, b.z, r, lg);
+    const T s = csr_row<T, G>(A, val, b.p, r, lg);
     if (lg == 0) {
-      b.w[r] = w;
-      const double rr = b.r[r], zz = b.z[r];
-      d[0] += rr * zz;
-      d[1] += static_cast<double>(w) * zz;
-      d[2] += rr * rr;
+      b.Ap[r] = s;
+      d[0] += static_cast<double>(b.p[r]) * static_cast<double>(s);
     }
   }
-  write_partials<3>(d, red.partA, lds);
+  write_partials<1>(d, red.partA, lds);
 }
 
 template <typename T, int B>
@@ -273,112 +336,62 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   __shared__ int s_i[4];
   read_words(st, s_i);
   if (s_i[0]) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = s_i[1];
   const T* val = bvals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
-  double d[3] = {0.0, 0.0, 0.0};
+  double d[1] = {0.0};
   for (int I = (blockIdx.x * blockDim.x + threadIdx.x) / G; I < A.nbr; I += stride) {
     T acc[B];
-    bsr_row<T, B, G>(A, val, b.z, I, lg, acc);
+    bsr_row<T, B, G>(A, val, b.p, I, lg, acc);
     if (lg == 0) {
 #pragma unroll
       for (int c = 0; c < B; ++c) {
-        b.w[B * I + c] = acc[c];
-        const double rr = b.r[B * I + c], zz = b.z[B * I + c];
-        d[0] += rr * zz;
-        d[1] += static_cast<double>(acc[c]) * zz;
-        d[2] += rr * rr;
+        b.Ap[B * I + c] = acc[c];
+        d[0] += static_cast<double>(b.p[B * I + c]) * static_cast<double>(acc[c]);
       }
     }
   }
-  write_partials<3>(d, red.partA, lds);
+  write_partials<1>(d, red.partA, lds);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgScalars* st,
-                                                      CgRed red, PcgParams prm, int mixed) {
+                                                      CgRed red) {
   __shared__ double lds[kLds];
   __shared__ int s_i[4];
   read_words(st, s_i);
   if (s_i[0]) return;
   const int i = s_i[2];
-  const bool resumed = (s_i[3] == i);
   const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
-  double v[3];
-  reduce_partials<3>(red.partA, red.nA, v, lds);
-  const double gamma = v[0], delta = v[1], rr = v[2];
-  double norm_b;
-  if (i == 0) {
-    double f[1];
-    reduce_partials<1>(red.partB, red.nB, f, lds);
-    norm_b = sqrt(fabs(f[0]));
-  } else {
-    norm_b = st->norm_b;
-  }
-  int stop = 0;
-  double rel = 0.0;
-  if (i == 0 && !resumed) {
-    if (norm_b < 1e-300) stop = kFlagZeroRhs;           // impl:1822-1830
-    else if (gamma <= 0.0) stop = kFlagRz0;              // impl:1859-1862
-    else if (prm.max_iters <= 0) stop = kFlagMaxIter;
-    rel = (stop == kFlagZeroRhs) ? 0.0 : 1.0;
-  } else {
-    rel = sqrt(fabs(rr)) / norm_b;
-    if (!mixed) {
-      if (rel < prm.tol) stop = kFlagConverged;          // finalize_iteration, impl:704-738
-    } else if (!resumed &&
-               (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh)) {
-      stop = kFlagRefresh;                               // pause for an FP64 refresh
-    }
-    if (!stop && i >= prm.max_iters) stop = kFlagMaxIter;
-  }
-  double beta = 0.0, alpha = 0.0, pAp = 0.0;
-  if (!stop) {
-    if (i == 0) {
-      pAp = delta;
-    } else {
-      beta = gamma / st->gamma[(i - 1) & 1];
-      pAp = delta - beta * gamma / st->alpha[(i - 1) & 1];
-    }
-    if (!(pAp > 0.0)) stop = kFlagBreakdown;             // impl:691-696
-    else alpha = gamma / pAp;
-  }
-  if (stop) {
+  double v1[1];
+  reduce_partials<1>(red.partA, red.nA, v1, lds);
+  const double pAp = v1[0];
+  if (!(pAp > 0.0)) {  // breakdown (impl:691-696)
     if (lead) {
-      if (i == 0) st->norm_b = norm_b;
-      st->rel = (stop == kFlagBreakdown) ? 1e300 : rel;
-      st->rr = rr;
       st->pAp = pAp;
-      if (stop == kFlagRz0) st->gamma[0] = gamma;
-      st->flags |= stop;
+      st->alpha = 0.0;
+      st->rel = 1e300;
+      st->flags |= kFlagBreakdown;
       st->done = 1;
     }
     return;
   }
-  const T a = static_cast<T>(alpha), bt = static_cast<T>(beta);
+  const double alpha = st->rz[i & 1] / pAp;
+  const T a = static_cast<T>(alpha);
+  double v[3] = {0.0, 0.0, 0.0};
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const T p = b.z[k] + bt * b.p[k];
-    const T s = b.w[k] + bt * b.s[k];
-    b.p[k] = p;
-    b.s[k] = s;
-    b.u64[k] += alpha * static_cast<double>(p);
-    const T r = b.r[k] - a * s;
+    b.u64[k] += alpha * static_cast<double>(b.p[k]);
+    const T r = b.r[k] - a * b.Ap[k];
+    const T z = r / b.diag[k];
     b.r[k] = r;
-    b.z[k] = r / b.diag[k];
+    b.z[k] = z;
+    v[0] += static_cast<double>(r) * static_cast<double>(z);
+    v[1] += static_cast<double>(r) * static_cast<double>(r);
   }
+  write_partials<3>(v, red.partB, lds);
   if (lead) {
-    if (i == 0) {
-      st->norm_b = norm_b;
-      st->rr_refresh = rr;
-    }
-    if (resumed) st->rr_refresh = rr;
-    st->gamma[i & 1] = gamma;
-    st->alpha[i & 1] = alpha;
-    st->beta = beta;
     st->pAp = pAp;
-    st->rr = rr;
-    st->rel = rel;
+    st->alpha = alpha;
     st->iter_done = i + 1;
   }
 }
@@ -430,8 +443,8 @@ __global__ __launch_bounds__(kBlock) void k_refresh_residual_bsr(BsrDev A, const
   if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
-// Decide from the FP64 true residual; on "continue" reload r = (float) r64 and
-// z = r/diag, and mark the paused iteration as resumed.
+// Decide from the FP64 true residual; on "continue" reload r = (float) r64,
+// z = r/diag (partials for cg_direction) and mark the paused iteration resumed.
 __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* __restrict__ r64,
                                                            const float* __restrict__ diag,
                                                            float* __restrict__ r,
@@ -466,11 +479,16 @@ __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* 
     }
     return;  // done stays set
   }
+  double v[3] = {0.0, 0.0, 0.0};
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const float rk = static_cast<float>(r64[k]);
+    const float zk = rk / diag[k];
     r[k] = rk;
-    z[k] = rk / diag[k];
+    z[k] = zk;
+    v[0] += static_cast<double>(rk) * static_cast<double>(zk);
+    v[1] += static_cast<double>(rk) * static_cast<double>(rk);
   }
+  write_partials<3>(v, red.partB, lds);
   if (lead) {
     st->resume_iter = st->iter_done;
     st->flags &= ~kFlagRefresh;
@@ -527,9 +545,15 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
 }
 
 template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
-                      const PcgParams& prm, int mixed, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_update<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red, prm, mixed);
+void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
+                         const PcgParams& prm, int mixed, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_direction<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red, prm,
+                     mixed);
+}
+
+template <typename T>
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red);
 }
 
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
@@ -574,9 +598,11 @@ template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const Cg
                                      PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
                                     PcgScalars*, const CgRed&, hipStream_t);
-template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
-                                       const PcgParams&, int, hipStream_t);
-template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
-                                      const PcgParams&, int, hipStream_t);
+template void launch_cg_direction<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
+                                          const PcgParams&, int, hipStream_t);
+template void launch_cg_direction<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
+                                         const PcgParams&, int, hipStream_t);
+template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
+template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
 
 }  // namespace vbh
