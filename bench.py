@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for profilers: one solve per precision, few kernel reps")
     return ap.parse_args()
 
 
@@ -73,6 +75,8 @@ def cpu_baseline(model, eq, n, F, seconds):
 
 def main():
     args = parse()
+    if args.profile:
+        args.steps, args.warmup, args.no_cpu, args.spmv_reps = 1, 0, True, 5
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
               "sharding is described in DESIGN.md but not built", file=sys.stderr)
@@ -152,6 +156,8 @@ def main():
     ms32, ms64 = kt(3), kt(2)
     ms_upd32, ms_upd64 = kt(5), kt(4)
     ms_plain32 = kt(1)
+    ms_copy = kt(6, 5)
+    copy_gbs = 2 * (256 << 20) / (ms_copy * 1e-3) / 1e9
     nnz_full = 2 * K.nnz - n
     # algorithmic bytes per launch, full-CSR-equivalent matrix (value + int32
     # column per nonzero, row pointers) plus the kernel's vector streams:
@@ -208,7 +214,9 @@ def main():
                        "cg_update_f64": round(ms_upd64 * 1e3, 2),
                        "spmv_plain_f32": round(ms_plain32 * 1e3, 2),
                        "cg_update_f32_GBs": round(upd32_bytes / (ms_upd32 * 1e-3) / 1e9, 1),
-                       "cg_spmv_f64_GBs": round(ach64, 1)},
+                       "cg_spmv_f64_GBs": round(ach64, 1),
+                       "stream_copy_256MiB_us": round(ms_copy * 1e3, 2),
+                       "stream_copy_GBs": round(copy_gbs, 1)},
         "iteration_roofline": {"bytes_per_iter": int(b_iter32), "achieved": round(iter_gbs, 1),
                                "frac": round(iter_gbs / HBM_PEAK_GBS, 4),
                                "us_per_iter": round(devs[-1] / iters_mixed * 1e6, 2)},

@@ -144,7 +144,8 @@ int vbh_spmv(vbh_ctx* ctx, const double* x, double* y);
  * vectors; returns the mean launch duration in ms measured with hipEvents on
  * the context stream. which: 0/1 plain SpMV FP64/FP32, 2/3 the fused PCG
  * SpMV kernel (search direction + K p + p.Ap) FP64/FP32, 4/5 the fused PCG
- * update kernel FP64/FP32. 2..5 use the F of the last solve. */
+ * update kernel FP64/FP32, 6 a 256 MiB device-to-device streaming copy
+ * (calibration of byte counters). 2..5 use the F of the last solve. */
 int vbh_bench_spmv(vbh_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* ---- device memory helpers for device-resident workflows ---------------- */

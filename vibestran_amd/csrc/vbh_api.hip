@@ -436,6 +436,7 @@ void Ctx::check_outcome(const PcgScalars& s, int precision_mode) {
 // ===========================================================================
 using vbh::Ctx;
 using vbh::guarded;
+using vbh::DBuf;
 
 extern "C" {
 
@@ -662,8 +663,17 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
                                    c->stream);
       }
     }
+    DBuf<char> cp_a, cp_b;
+    const long long cp_bytes = 256ll << 20;
+    if (precision == 6) {
+      cp_a.resize(static_cast<size_t>(cp_bytes));
+      cp_b.resize(static_cast<size_t>(cp_bytes));
+      VBH_HIP(hipMemsetAsync(cp_a.p, 0, static_cast<size_t>(cp_bytes), c->stream));
+    }
     auto one = [&] {
-      if (precision == 2) {
+      if (precision == 6) {
+        vbh::launch_stream_copy(cp_bytes, cp_a.p, cp_b.p, c->stream);
+      } else if (precision == 2) {
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
                                     c->stream);
       } else if (precision == 3) {

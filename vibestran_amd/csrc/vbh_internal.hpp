@@ -179,5 +179,6 @@ void launch_refresh_reload(int n, const double* r64, const float* diag, float* r
 // Plain elementwise helpers.
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s);
 void launch_fill_f64(long long n, double* a, double v, hipStream_t s);
+void launch_stream_copy(long long bytes, const void* a, void* b, hipStream_t s);
 
 }  // namespace vbh

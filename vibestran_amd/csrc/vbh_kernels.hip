@@ -501,6 +501,14 @@ __global__ void k_cast_f64_f32(int n, const double* __restrict__ a, float* __res
     b[i] = static_cast<float>(a[i]);
 }
 
+// Streaming copy used to calibrate the HBM byte counters (known traffic).
+__global__ __launch_bounds__(kBlock) void k_stream_copy(long long n, const double4* __restrict__ a,
+                                                        double4* __restrict__ b) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
 __global__ void k_fill_f64(long long n, double* __restrict__ a, double v) {
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -582,6 +590,12 @@ int rows_per_block(int format) {
 
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s) {
   hipLaunchKernelGGL(k_cast_f64_f32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, n, a, b);
+}
+
+void launch_stream_copy(long long bytes, const void* a, void* b, hipStream_t s) {
+  const long long n = bytes / static_cast<long long>(sizeof(double4));
+  hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(kBlock), 0, s, n,
+                     static_cast<const double4*>(a), static_cast<double4*>(b));
 }
 
 void launch_fill_f64(long long n, double* a, double v, hipStream_t s) {
