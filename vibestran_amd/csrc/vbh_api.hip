@@ -232,6 +232,7 @@ void Ctx::upload_operator(int n_, int nnz, const int* rp, const int* ci, const d
     bsrd.bci = bsr_ci.p;
     bsrd.v64 = bsr_v64.p;
     bsrd.v32 = bsr_v32.p;
+    build_sell(*this);
   } else {
     bsr_rp.release();
     bsr_ci.release();
@@ -277,7 +278,7 @@ void Ctx::ensure_work(int nvec_len) {
   if (!st_host) VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PcgScalars), 0));
   // zero the padding tails once (F beyond n stays zero)
   launch_fill_f64(static_cast<long long>(N), w_F.p, 0.0, stream);
-  const int rows = (format == 0) ? n : bsrd.nbr;
+  const int rows = (format == 0) ? n : bsrd.nslices;
   red_n_a = grid_for(rows, rows_per_block(format));
   red_n_b = grid_for(static_cast<long long>(N), kBlock * 4);
   drop_graphs();

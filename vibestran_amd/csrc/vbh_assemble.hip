@@ -846,6 +846,7 @@ bool build_device_bsr(Ctx& c, int B) {
   c.bsrd.bci = c.bsr_ci.p;
   c.bsrd.v64 = c.bsr_v64.p;
   c.bsrd.v32 = c.bsr_v32.p;
+  build_sell(c);
   return true;
 }
 

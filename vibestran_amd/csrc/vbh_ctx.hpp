@@ -47,6 +47,10 @@ struct Ctx {
   DBuf<int> bsr_rp, bsr_ci;
   DBuf<double> bsr_v64;
   DBuf<float> bsr_v32;
+  DBuf<int> sell_koff, sell_idx;
+  DBuf<double> sell_v64;
+  DBuf<float> sell_v32;
+  long long sell_padded_blocks = 0;
   DBuf<double> diag64;
   DBuf<float> diag32;
   CsrDev csr;

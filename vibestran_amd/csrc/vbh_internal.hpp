@@ -147,7 +147,21 @@ struct BsrDev {
   const int* bci = nullptr;
   const double* v64 = nullptr;
   const float* v32 = nullptr;
+  // Sliced-ELL form of the same blocks, the layout the solver kernels read:
+  // slice s = rows [64 s, 64 s + 64), one lane per scalar row. kmax_s =
+  // koff[s+1] - koff[s] block slots per row (padded with zero blocks); the B
+  // values of row g's k-th block are at ((koff[s] + k) * 64 + (g - 64 s)) * B,
+  // and its block column at sidx[(koff[s] + k) * sell_nps(B) + node - node0(s)]
+  // with node = g / B, node0(s) = 64 s / B.
+  int nslices = 0;
+  const int* koff = nullptr;
+  const int* sidx = nullptr;
+  const double* s64 = nullptr;
+  const float* s32 = nullptr;
 };
+
+constexpr int kSliceRows = 64;
+__host__ __device__ constexpr int sell_nps(int B) { return kSliceRows / B + 2; }
 
 // ---- kernel launchers (vbh_kernels.hip) ------------------------------------------
 int grid_for(long long work_items, int items_per_block);
@@ -175,6 +189,10 @@ void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, cons
                              hipStream_t s);
 void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
                            PcgScalars* st, const CgRed& red, const PcgParams& prm, hipStream_t s);
+
+// Builds the sliced-ELL arrays of a device BSR (vbh_sell.hip).
+struct Ctx;
+void build_sell(Ctx& c);
 
 // Plain elementwise helpers.
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s);

@@ -110,6 +110,34 @@ struct BsrLanes {
   static constexpr int G = (B == 3) ? 32 : 16;
 };
 
+template <typename T>
+__device__ __forceinline__ const T* svals(const BsrDev& A);
+template <>
+__device__ __forceinline__ const double* svals<double>(const BsrDev& A) { return A.s64; }
+template <>
+__device__ __forceinline__ const float* svals<float>(const BsrDev& A) { return A.s32; }
+
+// Row g = 64 s + l of the sliced-ELL operator times x (one lane, no reduction).
+template <typename T, int B>
+__device__ __forceinline__ T sell_row(const BsrDev& A, const T* __restrict__ val,
+                                      const T* __restrict__ x, int s, int l) {
+  constexpr int NPS = sell_nps(B);
+  const int k0 = A.koff[s], km = A.koff[s + 1] - k0;
+  const int node = (kSliceRows * s + l) / B;
+  const int node0 = (kSliceRows * s) / B;
+  const T* v = val + (static_cast<size_t>(k0) * kSliceRows + l) * B;
+  const int* ix = A.sidx + static_cast<size_t>(k0) * NPS + (node - node0);
+  T acc = T(0);
+#pragma unroll 4
+  for (int k = 0; k < km; ++k) {
+    const int J = ix[k * NPS];
+    const T* vk = v + static_cast<size_t>(k) * kSliceRows * B;
+#pragma unroll
+    for (int c = 0; c < B; ++c) acc += vk[c] * x[B * J + c];
+  }
+  return acc;
+}
+
 // ---- plain SpMV -------------------------------------------------------------------
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_csr(CsrDev A, const T* __restrict__ x,
@@ -126,16 +154,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_csr(CsrDev A, const T* __restri
 template <typename T, int B>
 __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restrict__ x,
                                                      T* __restrict__ y) {
-  constexpr int G = BsrLanes<B>::G;
-  const T* val = bvals<T>(A);
-  const int lg = threadIdx.x % G;
-  const int stride = gridDim.x * (blockDim.x / G);
-  for (int I = (blockIdx.x * blockDim.x + threadIdx.x) / G; I < A.nbr; I += stride) {
-    T acc[B];
-    bsr_row<T, B, G>(A, val, x, I, lg, acc);
-    if (lg == 0)
-#pragma unroll
-      for (int r = 0; r < B; ++r) y[B * I + r] = acc[r];
+  const T* val = svals<T>(A);
+  const int l = threadIdx.x & (kSliceRows - 1);
+  const int nrows = A.nbr * B;
+  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
+       s += gridDim.x * (blockDim.x / kSliceRows)) {
+    const T v = sell_row<T, B>(A, val, x, s, l);
+    const int g = kSliceRows * s + l;
+    if (g < nrows) y[g] = v;
   }
 }
 
@@ -331,24 +357,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
 template <typename T, int B>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
-  constexpr int G = BsrLanes<B>::G;
   __shared__ double lds[kLds];
   __shared__ int s_i[4];
   read_words(st, s_i);
   if (s_i[0]) return;
-  const T* val = bvals<T>(A);
-  const int lg = threadIdx.x % G;
-  const int stride = gridDim.x * (blockDim.x / G);
+  const T* val = svals<T>(A);
+  const int l = threadIdx.x & (kSliceRows - 1);
+  const int nrows = A.nbr * B;
   double d[1] = {0.0};
-  for (int I = (blockIdx.x * blockDim.x + threadIdx.x) / G; I < A.nbr; I += stride) {
-    T acc[B];
-    bsr_row<T, B, G>(A, val, b.p, I, lg, acc);
-    if (lg == 0) {
-#pragma unroll
-      for (int c = 0; c < B; ++c) {
-        b.Ap[B * I + c] = acc[c];
-        d[0] += static_cast<double>(b.p[B * I + c]) * static_cast<double>(acc[c]);
-      }
+  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
+       s += gridDim.x * (blockDim.x / kSliceRows)) {
+    const T v = sell_row<T, B>(A, val, b.p, s, l);
+    const int g = kSliceRows * s + l;
+    if (g < nrows) {
+      b.Ap[g] = v;
+      d[0] += static_cast<double>(b.p[g]) * static_cast<double>(v);
     }
   }
   write_partials<1>(d, red.partA, lds);
@@ -423,21 +446,19 @@ __global__ __launch_bounds__(kBlock) void k_refresh_residual_bsr(BsrDev A, const
                                                                  const double* __restrict__ u,
                                                                  double* __restrict__ r64,
                                                                  PcgScalars* st, CgRed red) {
-  constexpr int G = BsrLanes<B>::G;
   __shared__ double lds[kLds];
-  const int lg = threadIdx.x % G;
-  const int stride = gridDim.x * (blockDim.x / G);
+  const int l = threadIdx.x & (kSliceRows - 1);
+  const int nrows = A.nbr * B;
   double v[1] = {0.0};
-  for (int I = (blockIdx.x * blockDim.x + threadIdx.x) / G; I < A.nbr; I += stride) {
-    double acc[B];
-    bsr_row<double, B, G>(A, A.v64, u, I, lg, acc);
-    if (lg == 0)
-#pragma unroll
-      for (int r = 0; r < B; ++r) {
-        const double ri = F[B * I + r] - acc[r];
-        r64[B * I + r] = ri;
-        v[0] += ri * ri;
-      }
+  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
+       s += gridDim.x * (blockDim.x / kSliceRows)) {
+    const double ku = sell_row<double, B>(A, A.s64, u, s, l);
+    const int g = kSliceRows * s + l;
+    if (g < nrows) {
+      const double ri = F[g] - ku;
+      r64[g] = ri;
+      v[0] += ri * ri;
+    }
   }
   write_partials<1>(v, red.partR, lds);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
@@ -529,10 +550,10 @@ void launch_spmv_csr(const CsrDev& A, const T* x, T* y, hipStream_t s) {
 template <typename T>
 void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s) {
   if (A.B == 3)
-    hipLaunchKernelGGL((k_spmv_bsr<T, 3>), dim3(grid_for(A.nbr, kBlock / BsrLanes<3>::G)),
+    hipLaunchKernelGGL((k_spmv_bsr<T, 3>), dim3(grid_for(A.nslices, kBlock / kSliceRows)),
                        dim3(kBlock), 0, s, A, x, y);
   else
-    hipLaunchKernelGGL((k_spmv_bsr<T, 6>), dim3(grid_for(A.nbr, kBlock / BsrLanes<6>::G)),
+    hipLaunchKernelGGL((k_spmv_bsr<T, 6>), dim3(grid_for(A.nslices, kBlock / kSliceRows)),
                        dim3(kBlock), 0, s, A, x, y);
 }
 
@@ -585,7 +606,7 @@ void launch_refresh_reload(int n, const double* r64, const float* diag, float* r
 }
 
 int rows_per_block(int format) {
-  return format == 0 ? kBlock / kCsrG : (format == 3 ? kBlock / BsrLanes<3>::G : kBlock / BsrLanes<6>::G);
+  return format == 0 ? kBlock / kCsrG : kBlock / kSliceRows;  // block formats: slices per block
 }
 
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s) {

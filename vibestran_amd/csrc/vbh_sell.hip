@@ -1,0 +1,117 @@
+// vibestran_amd/csrc/vbh_sell.hip -- sliced-ELL layout of the block operator.
+//
+// The solver's SpMV assigns one lane to each scalar row and walks the row's
+// blocks; with 64-row slices the B values a lane needs per block are
+// contiguous and the lanes of a wave read one contiguous span per block slot,
+// with no cross-lane reduction. Built on the device from the BSR operator.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "vbh_ctx.hpp"
+
+namespace vbh {
+namespace {
+
+__global__ void k_sell_kmax(int nslices, int nbr, int B, const int* __restrict__ brp,
+                            int* __restrict__ kcnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  const int n0 = (kSliceRows * s) / B;
+  int n1 = (kSliceRows * s + kSliceRows - 1) / B;
+  if (n1 > nbr - 1) n1 = nbr - 1;
+  int km = 0;
+  for (int I = n0; I <= n1; ++I) {
+    const int nb = brp[I + 1] - brp[I];
+    km = nb > km ? nb : km;
+  }
+  kcnt[s] = km;
+}
+
+template <int B>
+__global__ void k_sell_fill(int nslices, int nbr, const int* __restrict__ brp,
+                            const int* __restrict__ bci, const double* __restrict__ bv,
+                            const int* __restrict__ koff, double* __restrict__ sv,
+                            int* __restrict__ sidx) {
+  const int s = blockIdx.x;
+  const int l = threadIdx.x;  // 0..63
+  if (s >= nslices) return;
+  const int k0 = koff[s], km = koff[s + 1] - k0;
+  const int g = kSliceRows * s + l;
+  const int node0 = (kSliceRows * s) / B;
+  if (g < nbr * B) {
+    const int I = g / B, r = g % B;
+    const int o = brp[I], nb = brp[I + 1] - o;
+    for (int k = 0; k < km; ++k)
+      for (int c = 0; c < B; ++c)
+        sv[(static_cast<size_t>(k0 + k) * kSliceRows + l) * B + c] =
+            (k < nb) ? bv[static_cast<size_t>(B * B) * o + static_cast<size_t>(r * B + c) * nb + k] : 0.0;
+  } else {
+    for (int k = 0; k < km; ++k)
+      for (int c = 0; c < B; ++c) sv[(static_cast<size_t>(k0 + k) * kSliceRows + l) * B + c] = 0.0;
+  }
+  constexpr int NPS = sell_nps(B);
+  if (l < NPS) {
+    const int I = node0 + l;
+    const int o = (I < nbr) ? brp[I] : 0;
+    const int nb = (I < nbr) ? brp[I + 1] - o : 0;
+    for (int k = 0; k < km; ++k)
+      sidx[static_cast<size_t>(k0 + k) * NPS + l] = (k < nb) ? bci[o + k] : (I < nbr ? I : 0);
+  }
+}
+
+__global__ void k_to_f32(long long m, const double* __restrict__ a, float* __restrict__ b) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    b[i] = static_cast<float>(a[i]);
+}
+
+}  // namespace
+
+void build_sell(Ctx& c) {
+  hipStream_t s = c.stream;
+  BsrDev& A = c.bsrd;
+  const int B = A.B;
+  const int nrows = A.nbr * B;
+  const int nslices = (nrows + kSliceRows - 1) / kSliceRows;
+  DBuf<int> kcnt;
+  kcnt.resize(static_cast<size_t>(nslices) + 1);
+  c.sell_koff.resize(static_cast<size_t>(nslices) + 1);
+  hipLaunchKernelGGL(k_sell_kmax, dim3((nslices + 255) / 256), dim3(256), 0, s, nslices, A.nbr, B,
+                     A.brp, kcnt.p);
+  // exclusive scan of kcnt (length nslices+1; the last slot is a zero pad)
+  VBH_HIP(hipMemsetAsync(kcnt.p + nslices, 0, sizeof(int), s));
+  {
+    size_t tmp = 0;
+    VBH_HIP(rocprim::exclusive_scan(nullptr, tmp, kcnt.p, c.sell_koff.p, 0,
+                                    static_cast<size_t>(nslices) + 1, rocprim::plus<int>(), s));
+    DBuf<char> t;
+    t.resize(tmp > 0 ? tmp : 1);
+    VBH_HIP(rocprim::exclusive_scan(t.p, tmp, kcnt.p, c.sell_koff.p, 0,
+                                    static_cast<size_t>(nslices) + 1, rocprim::plus<int>(), s));
+  }
+  int ktot = 0;
+  VBH_HIP(hipMemcpyAsync(&ktot, c.sell_koff.p + nslices, sizeof(int), hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  const size_t nval = static_cast<size_t>(ktot) * kSliceRows * B;
+  c.sell_v64.resize(nval > 0 ? nval : 1);
+  c.sell_v32.resize(nval > 0 ? nval : 1);
+  c.sell_idx.resize(static_cast<size_t>(ktot) * sell_nps(B) + 1);
+  if (B == 3)
+    hipLaunchKernelGGL(k_sell_fill<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr, A.brp,
+                       A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
+  else
+    hipLaunchKernelGGL(k_sell_fill<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr, A.brp,
+                       A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(static_cast<long long>(nval), 1024)), dim3(256), 0, s,
+                     static_cast<long long>(nval), c.sell_v64.p, c.sell_v32.p);
+  VBH_HIP(hipStreamSynchronize(s));
+  A.nslices = nslices;
+  A.koff = c.sell_koff.p;
+  A.sidx = c.sell_idx.p;
+  A.s64 = c.sell_v64.p;
+  A.s32 = c.sell_v32.p;
+  c.sell_padded_blocks = static_cast<long long>(ktot) * kSliceRows / B;
+}
+
+}  // namespace vbh
