@@ -328,12 +328,12 @@ void Ctx::enqueue_iterations(int precision_mode, int count) {
     if (precision_mode == VBH_PREC_FP64) {
       const CgBufs<double> b = bufs64();
       launch_cg_direction<double>(N, b, st_dev.p, rd, prm, 0, stream);
-      launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, stream);
+      launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, stream, 0);
       launch_cg_update<double>(N, b, st_dev.p, rd, stream);
     } else {
       const CgBufs<float> b = bufs32();
       launch_cg_direction<float>(N, b, st_dev.p, rd, prm, 1, stream);
-      launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, stream);
+      launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, stream, spmv_acc64);
       launch_cg_update<float>(N, b, st_dev.p, rd, stream);
     }
   }
@@ -479,6 +479,8 @@ int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
     c->arch = prop.gcnArchName;
     c->tol = tol > 0.0 ? tol : 1e-8;
     c->max_iters = max_iters > 0 ? max_iters : 10000;
+    if (const char* e = std::getenv("VBH_REFRESH_DELTA")) c->refresh_delta = std::atof(e);
+    if (const char* e = std::getenv("VBH_SPMV_ACC64")) c->spmv_acc64 = std::atoi(e);
     VBH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     VBH_HIP(hipEventCreate(&c->ev_start));
     VBH_HIP(hipEventCreate(&c->ev_stop));
@@ -656,12 +658,12 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
         vbh::launch_cg_direction<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, prm, 0, c->stream);
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
-                                    c->stream);
+                                    c->stream, 0);
       } else {
         vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
         vbh::launch_cg_direction<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, prm, 1, c->stream);
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
-                                   c->stream);
+                                   c->stream, c->spmv_acc64);
       }
     }
     DBuf<char> cp_a, cp_b;
@@ -676,10 +678,10 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         vbh::launch_stream_copy(cp_bytes, cp_a.p, cp_b.p, c->stream);
       } else if (precision == 2) {
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
-                                    c->stream);
+                                    c->stream, 0);
       } else if (precision == 3) {
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
-                                   c->stream);
+                                   c->stream, c->spmv_acc64);
       } else if (precision == 4) {
         vbh::launch_cg_update<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
       } else if (precision == 5) {

@@ -175,9 +175,10 @@ void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s);
 int rows_per_block(int format);
 template <typename T>
 void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
+// acc64: accumulate the SpMV rows in FP64 (products of the stored values).
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, hipStream_t s);
+                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64);
 template <typename T>
 void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
                          const PcgParams& prm, int mixed, hipStream_t s);

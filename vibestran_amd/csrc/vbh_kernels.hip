@@ -117,23 +117,24 @@ __device__ __forceinline__ const double* svals<double>(const BsrDev& A) { return
 template <>
 __device__ __forceinline__ const float* svals<float>(const BsrDev& A) { return A.s32; }
 
-// Row g = 64 s + l of the sliced-ELL operator times x (one lane, no reduction).
-template <typename T, int B>
-__device__ __forceinline__ T sell_row(const BsrDev& A, const T* __restrict__ val,
-                                      const T* __restrict__ x, int s, int l) {
+// Row g = 64 s + l of the sliced-ELL operator times x (one lane, no reduction),
+// accumulated in Acc.
+template <typename T, int B, typename Acc = T>
+__device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ val,
+                                        const T* __restrict__ x, int s, int l) {
   constexpr int NPS = sell_nps(B);
   const int k0 = A.koff[s], km = A.koff[s + 1] - k0;
   const int node = (kSliceRows * s + l) / B;
   const int node0 = (kSliceRows * s) / B;
   const T* v = val + (static_cast<size_t>(k0) * kSliceRows + l) * B;
   const int* ix = A.sidx + static_cast<size_t>(k0) * NPS + (node - node0);
-  T acc = T(0);
+  Acc acc = Acc(0);
 #pragma unroll 4
   for (int k = 0; k < km; ++k) {
     const int J = ix[k * NPS];
     const T* vk = v + static_cast<size_t>(k) * kSliceRows * B;
 #pragma unroll
-    for (int c = 0; c < B; ++c) acc += vk[c] * x[B * J + c];
+    for (int c = 0; c < B; ++c) acc += static_cast<Acc>(vk[c]) * static_cast<Acc>(x[B * J + c]);
   }
   return acc;
 }
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
   write_partials<1>(d, red.partA, lds);
 }
 
-template <typename T, int B>
+template <typename T, int B, typename Acc = T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
   __shared__ double lds[kLds];
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   double d[1] = {0.0};
   for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
        s += gridDim.x * (blockDim.x / kSliceRows)) {
-    const T v = sell_row<T, B>(A, val, b.p, s, l);
+    const T v = static_cast<T>(sell_row<T, B, Acc>(A, val, b.p, s, l));
     const int g = kSliceRows * s + l;
     if (g < nrows) {
       b.Ap[g] = v;
@@ -564,11 +565,15 @@ void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
 
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, hipStream_t s) {
+                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64) {
   if (format == 0)
     hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red);
+  else if (format == 3 && acc64)
+    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3, double>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
   else if (format == 3)
     hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
+  else if (acc64)
+    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6, double>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
   else
     hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
 }
@@ -630,9 +635,9 @@ template void launch_spmv_bsr<float>(const BsrDev&, const float*, float*, hipStr
 template void launch_cg_init<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_init<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const CgBufs<double>&,
-                                     PcgScalars*, const CgRed&, hipStream_t);
+                                     PcgScalars*, const CgRed&, hipStream_t, int);
 template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
-                                    PcgScalars*, const CgRed&, hipStream_t);
+                                    PcgScalars*, const CgRed&, hipStream_t, int);
 template void launch_cg_direction<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
                                           const PcgParams&, int, hipStream_t);
 template void launch_cg_direction<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
