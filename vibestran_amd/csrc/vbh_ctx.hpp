@@ -26,7 +26,7 @@ struct Ctx {
   int max_iters = 10000;
   int precision = VBH_PREC_FP64;
   int format_req = -1;
-  double refresh_delta = 0.05;
+  double refresh_delta = 0.2;
   int spmv_acc64 = 0;  // FP32 SpMV rows accumulated in FP64
   int iters_per_graph = 32;
 
