@@ -76,6 +76,9 @@ def _load():
         "orc_pcg_jacobi_fixed": (I, [P, DP, I, I, DP, DP]),
         "orc_true_relative_residual": (D, [P, DP, DP]),
         "orc_pload4_hexa8": (I, [DP, I, I, DP, DP]),
+        "orc_quad4_ke": (I, [DP, D, D, D, D, D, D, I, DP]),
+        "orc_quad4_me": (I, [DP, D, D, DP]),
+        "orc_assemble_model": (P, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -197,6 +200,15 @@ def penta6_me(xyz, rho):
     return _elem(lib().orc_penta6_me, xyz, rho, nd=18)
 
 
+def quad4_ke(xyz, E, nu, t, formulation="MITC4", G=0.0, twelveI_t3=1.0, tst=0.833333):
+    return _elem(lib().orc_quad4_ke, xyz, E, nu, G, t, twelveI_t3, tst,
+                 1 if formulation == "MINDLIN" else 0, nd=24)
+
+
+def quad4_me(xyz, rho, t):
+    return _elem(lib().orc_quad4_me, xyz, rho, t, nd=24)
+
+
 def pload4_hexa8(xyz, g1_local=-1, g34_local=-1, pressures=(1.0, 1.0, 1.0, 1.0)):
     x = _f64(xyz).reshape(-1)
     p = _f64(pressures)
@@ -303,6 +315,19 @@ def assemble_solids(n, elem_type, elem_ptr, conn, xyz, eq, E, nu, formulation="E
     e, v = _f64(E), _f64(nu)
     h = lib().orc_assemble_solids(n, len(et), _ip(et), _ip(ep), _ip(cn), _dp(x), _ip(q), _dp(e),
                                   _dp(v), 1 if formulation == "SRI" else 0)
+    if not h:
+        raise OracleError(_err())
+    return Csr(h)
+
+
+def assemble_model(n, model, eq):
+    """Assemble a vibestran_amd Model (solids and CQUAD4) through the oracle."""
+    et, ep, cn = _i32(model.elem_type), _i32(model.elem_ptr), _i32(model.conn)
+    x, q = _f64(model.xyz).reshape(-1), _i32(eq).reshape(-1)
+    e, v = _f64(model.E), _f64(model.nu)
+    th = _f64(model.thickness if model.thickness is not None else np.zeros(len(et)))
+    h = lib().orc_assemble_model(n, len(et), _ip(et), _ip(ep), _ip(cn), _dp(x), _ip(q), _dp(e),
+                                 _dp(v), _dp(th), int(model.solid_form), int(model.shell_form))
     if not h:
         raise OracleError(_err())
     return Csr(h)

@@ -217,6 +217,7 @@ bool condense(int nd, const double* Kuu, const double* Kua, const double* Kaa, d
 extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
+void orc_internal_set_error(const char* msg) { g_err = msg; }
 
 // ===========================================================================
 // DofMap (src/core/dof_map.cpp)
@@ -764,38 +765,47 @@ OrcCsr* orc_csr_lower_triangle(const OrcCsr* c) {
   return l;
 }
 
-// Solid mesh assembly through the element factory's default formulations
-// (element_factory.cpp:31-61) and assemble_element_matrix's serial branch
-// (assembly_parallel.hpp:101-108), with solid_global_dof_indices (29-42).
-OrcCsr* orc_assemble_solids(int n, int nelem, const int* et, const int* ep, const int* conn,
-                            const double* xyz, const int* eq, const double* E, const double* nu,
-                            int formulation) {
+// Mesh assembly through the element factory's default formulations
+// (element_factory.cpp:11-104) and assemble_element_matrix's serial branch
+// (assembly_parallel.hpp:101-108), with solid_global_dof_indices
+// (solid_elements.cpp:29-42) and the shell's six DOFs per node (cquad4.cpp:855-864).
+OrcCsr* orc_assemble_model(int n, int nelem, const int* et, const int* ep, const int* conn,
+                           const double* xyz, const int* eq, const double* E, const double* nu,
+                           const double* thick, int solid_form, int shell_form) {
   OrcBuilder b{n, {}};
   std::vector<double> ke(576), xe(24);
   std::vector<int> gd(24);
   for (int e = 0; e < nelem; ++e) {
     const int nn = ep[e + 1] - ep[e];
+    const int dpn = (et[e] == ORC_CQUAD4) ? 6 : 3;
     for (int a = 0; a < nn; ++a) {
       const int node = conn[ep[e] + a];
-      for (int d = 0; d < 3; ++d) {
-        xe[static_cast<size_t>(3 * a + d)] = xyz[3 * node + d];
-        gd[static_cast<size_t>(3 * a + d)] = eq[6 * node + d];
-      }
+      for (int d = 0; d < 3; ++d) xe[static_cast<size_t>(3 * a + d)] = xyz[3 * node + d];
+      for (int d = 0; d < dpn; ++d) gd[static_cast<size_t>(dpn * a + d)] = eq[6 * node + d];
     }
     int rc;
     if (et[e] == ORC_CHEXA8)
-      rc = formulation == 1 ? orc_hexa8_ke_sri(xe.data(), E[e], nu[e], ke.data())
-                            : orc_hexa8_ke_eas(xe.data(), E[e], nu[e], ke.data());
+      rc = solid_form == 1 ? orc_hexa8_ke_sri(xe.data(), E[e], nu[e], ke.data())
+                           : orc_hexa8_ke_eas(xe.data(), E[e], nu[e], ke.data());
     else if (et[e] == ORC_CTETRA4)
       rc = orc_tetra4_ke(xe.data(), E[e], nu[e], ke.data());
     else if (et[e] == ORC_CPENTA6)
       rc = orc_penta6_ke_eas(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CQUAD4)
+      rc = orc_quad4_ke(xe.data(), E[e], nu[e], 0.0, thick[e], 1.0, 0.833333, shell_form,
+                        ke.data());
     else
-      rc = fail(ORC_ERR_ARG, "orc_assemble_solids: unsupported element type");
+      rc = fail(ORC_ERR_ARG, "orc_assemble_model: unsupported element type");
     if (rc != ORC_OK) return nullptr;
-    orc_builder_add_element(&b, 3 * nn, gd.data(), ke.data());
+    orc_builder_add_element(&b, dpn * nn, gd.data(), ke.data());
   }
   return orc_builder_build_csr(&b);
+}
+
+OrcCsr* orc_assemble_solids(int n, int nelem, const int* et, const int* ep, const int* conn,
+                            const double* xyz, const int* eq, const double* E, const double* nu,
+                            int formulation) {
+  return orc_assemble_model(n, nelem, et, ep, conn, xyz, eq, E, nu, nullptr, formulation, 0);
 }
 
 // ===========================================================================

@@ -111,3 +111,23 @@ int orc_pload4_hexa8(const double* xyz24, int g1_local, int g34_local,
                      const double* pressures4, double* forces24);
 
 }  // extern "C"
+
+extern "C" {
+// ---- CQUAD4 shell: compute_shell_stiffness (src/elements/cquad4.cpp:580-670)
+// with compute_shell_frame 131-156, build_shell_section 189-218 (MID2/MID3 =
+// MID1, no MID4 coupling), evaluate_quad_point 220-252, MITC4+ membrane
+// 311-415 and shear 417-489, Mindlin membrane/shear 254-285 + one-point
+// in-plane shear 647-655, drilling penalty 109-129. formulation: 0 MITC4+
+// (default), 1 Mindlin. G <= 0 selects E/(2(1+nu)). Output row-major 24x24.
+int orc_quad4_ke(const double* xyz12, double E, double nu, double G, double t,
+                 double twelveI_t3, double tst, int formulation, double* ke576);
+// compute_shell_mass (cquad4.cpp:672-708): rho*t translational, rho*t^3/12
+// rotary, tiny drilling inertia.
+int orc_quad4_me(const double* xyz12, double rho, double t, double* me576);
+// Mixed solid/shell assembly: like orc_assemble_solids plus CQUAD4 elements
+// (6 DOFs per node, thickness[e]).
+OrcCsr* orc_assemble_model(int n, int nelem, const int* elem_type, const int* elem_ptr,
+                           const int* conn, const double* xyz, const int* eq, const double* E,
+                           const double* nu, const double* thickness, int solid_form,
+                           int shell_form);
+}
