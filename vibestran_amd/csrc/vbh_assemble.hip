@@ -35,20 +35,12 @@
 #include <vector>
 
 #include "vbh_ctx.hpp"
+#include "vbh_elem.hpp"
 
 namespace vbh {
 namespace {
 
 // ---- lower-triangle pair tables ---------------------------------------------------
-template <int N>
-struct PairTable {
-  unsigned short v[N * (N + 1) / 2];
-  constexpr PairTable() : v{} {
-    int t = 0;
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j <= i; ++j) v[t++] = static_cast<unsigned short>((i << 8) | j);
-  }
-};
 __constant__ PairTable<24> c_pairs24;
 __constant__ PairTable<18> c_pairs18;
 __constant__ PairTable<12> c_pairs12;
@@ -77,10 +69,6 @@ struct Kind {
   static constexpr int SA = (ND * 9 + 63) / 64;
 };
 
-constexpr int kGdStride = 24;  // element DOF list stride (max DOFs per element)
-
-// Error word: bit 0 non-positive Jacobian, bit 1 Kaa not SPD, bit 2 index error.
-// err[1] holds the lowest offending element index (atomicMin).
 
 // ---- 1. element DOF lists ------------------------------------------------------------
 __global__ void k_gdofs(int nelem, const int* __restrict__ etype, const int* __restrict__ eptr,
@@ -149,18 +137,6 @@ __global__ void k_row_ptr(int nnz, int n, const int* __restrict__ row_of, int* _
   const int prev = (i == 0) ? -1 : row_of[i - 1];
   const int cur = (i == nnz) ? n : row_of[i];
   for (int r = prev + 1; r <= cur; ++r) rp[r] = i;
-}
-
-__device__ __forceinline__ int find_slot(const int* __restrict__ rp, const int* __restrict__ ci,
-                                         int row, int col) {
-  int lo = rp[row], hi = rp[row + 1] - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int c = ci[mid];
-    if (c == col) return mid;
-    if (c < col) lo = mid + 1; else hi = mid - 1;
-  }
-  return -1;
 }
 
 // ---- 3. element matrices -------------------------------------------------------------------
@@ -326,8 +302,6 @@ __device__ void tet_entries(const double (&X)[4][3], double E, double nu, int la
     }
   }
 }
-
-constexpr int kElemWaves = 4;
 
 template <int K>
 struct ElemLds {
@@ -504,7 +478,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
           for (int k = 0; k < j; ++k) dj -= sKaa[j * 9 + k] * sKaa[j * 9 + k];
           if (!(dj > 0.0)) {
             dj = 1.0;
-            if (active) atomicOr(&err[0], 2);
+            if (active) atomicOr(&err[0], kErrKaa);
           }
           dj = sqrt(dj);
           sKaa[j * 9 + j] = dj;
@@ -551,7 +525,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
 
     if (active) {
       if (bad && lane == 0) {
-        atomicOr(&err[0], 1);
+        atomicOr(&err[0], kErrJacobian);
         atomicMin(&err[1], e);
       }
       const int* g = gd + static_cast<size_t>(e) * kGdStride;
@@ -566,7 +540,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
             const int row = gi > gj ? gi : gj, col = gi > gj ? gj : gi;
             const int slot = find_slot(rp, ci, row, col);
             if (slot < 0) {
-              atomicOr(&err[0], 4);
+              atomicOr(&err[0], kErrSlot);
             } else {
               atomicAdd(&val[slot], v);
               touched[slot] = 1;
@@ -853,21 +827,30 @@ bool build_device_bsr(Ctx& c, int B) {
 void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int nelem,
                    const int* etype_h, const int* eptr_h, const int* conn_h, const double* E_h,
                    const double* nu_h, const double* thick_h, int solid_form, int shell_form) {
-  (void)thick_h;
-  (void)shell_form;
   select_stream_device();
   hipStream_t s = stream;
   if (n_ <= 0) throw_error(VBH_ERR_SOLVER, "HIP PCG: stiffness matrix is empty -- no free DOFs");
   if (nelem <= 0 || nnodes <= 0) throw_error(VBH_ERR_ARG, "vbh_assemble: empty mesh");
   // host-side grouping by element kind
-  std::vector<int> groups[4];
+  constexpr int kQuad4 = 4;  // group index of CQUAD4 shells
+  std::vector<int> groups[5];
   for (int e = 0; e < nelem; ++e) {
     const int t = etype_h[e], nn = eptr_h[e + 1] - eptr_h[e];
     if (t == VBH_CHEXA8 && nn == 8) groups[solid_form == VBH_SOLID_SRI ? kHexSri : kHexEas].push_back(e);
     else if (t == VBH_CPENTA6 && nn == 6) groups[kPentaEas].push_back(e);
     else if (t == VBH_CTETRA4 && nn == 4) groups[kTet4].push_back(e);
+    else if (t == VBH_CQUAD4 && nn == 4) groups[kQuad4].push_back(e);
     else throw_error(VBH_ERR_ARG, "vbh_assemble: unsupported element type or node count at element " +
                                       std::to_string(e));
+  }
+  if (!groups[kQuad4].empty()) {
+    if (!thick_h) throw_error(VBH_ERR_ARG, "vbh_assemble: CQUAD4 elements need a thickness array");
+    if (shell_form != VBH_SHELL_MITC4 && shell_form != VBH_SHELL_MINDLIN)
+      throw_error(VBH_ERR_ARG, "vbh_assemble: unknown shell formulation");
+    for (int e : groups[kQuad4])
+      if (!(thick_h[e] > 0.0))
+        throw_error(VBH_ERR_ARG, "vbh_assemble: CQUAD4 element " + std::to_string(e) +
+                                     " has non-positive thickness");
   }
   const int nconn = eptr_h[nelem];
   for (int k = 0; k < nconn; ++k)
@@ -876,7 +859,7 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   for (int k = 0; k < 6 * nnodes; ++k)
     if (eq_h[k] < -1 || eq_h[k] >= n_) throw_error(VBH_ERR_ARG, "vbh_assemble: eq index out of range");
 
-  DBuf<double> d_xyz, d_E, d_nu;
+  DBuf<double> d_xyz, d_E, d_nu, d_t;
   DBuf<int> d_eq, d_et, d_ep, d_conn, d_gd, d_err;
   d_xyz.upload(xyz_h, static_cast<size_t>(nnodes) * 3, s);
   d_eq.upload(eq_h, static_cast<size_t>(nnodes) * 6, s);
@@ -885,13 +868,14 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   d_conn.upload(conn_h, static_cast<size_t>(nconn), s);
   d_E.upload(E_h, static_cast<size_t>(nelem), s);
   d_nu.upload(nu_h, static_cast<size_t>(nelem), s);
+  if (!groups[kQuad4].empty()) d_t.upload(thick_h, static_cast<size_t>(nelem), s);
   d_gd.resize(static_cast<size_t>(nelem) * kGdStride);
   d_err.resize(4);
   hipLaunchKernelGGL(k_init_err, dim3(1), dim3(1), 0, s, d_err.p);
   hipLaunchKernelGGL(k_gdofs, dim3((nelem + 255) / 256), dim3(256), 0, s, nelem, d_et.p, d_ep.p,
                      d_conn.p, d_eq.p, d_gd.p);
-  DBuf<int> d_sel[4];
-  for (int k = 0; k < 4; ++k)
+  DBuf<int> d_sel[5];
+  for (int k = 0; k < 5; ++k)
     if (!groups[k].empty()) d_sel[k].upload(groups[k].data(), groups[k].size(), s);
 
   hipEvent_t e0, e1, e2;
@@ -903,8 +887,8 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   // 2. topological pattern
   const int bits = bits_for(n_);
   size_t m = 0;
-  const int ndk[4] = {24, 24, 18, 12};
-  for (int k = 0; k < 4; ++k) m += groups[k].size() * static_cast<size_t>(ndk[k] * (ndk[k] + 1) / 2);
+  const int ndk[5] = {24, 24, 18, 12, 24};
+  for (int k = 0; k < 5; ++k) m += groups[k].size() * static_cast<size_t>(ndk[k] * (ndk[k] + 1) / 2);
   DBuf<int> t_row, t_col;
   int nnz_topo = 0;
   {
@@ -912,7 +896,7 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
     keys.resize(m);
     sorted.resize(m);
     size_t off = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 5; ++k) {
       const int ns = static_cast<int>(groups[k].size());
       if (!ns) continue;
       const int NL = ndk[k] * (ndk[k] + 1) / 2;
@@ -943,9 +927,14 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   touched.resize(static_cast<size_t>(nnz_topo));
   dzero(t_val, s);
   dzero(touched, s);
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     const int ns = static_cast<int>(groups[k].size());
     if (!ns) continue;
+    if (k == kQuad4) {
+      launch_quad4_stiffness(ns, d_sel[k].p, d_ep.p, d_conn.p, d_xyz.p, d_E.p, d_nu.p, d_t.p,
+                             shell_form, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p, s);
+      continue;
+    }
     const dim3 g(static_cast<unsigned>(std::min((ns + kElemWaves - 1) / kElemWaves, 65535)));
     const dim3 b(64 * kElemWaves);
     if (k == kHexEas)
@@ -972,10 +961,18 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(e2);
-  if (herr[0] & 1)
-    throw_error(VBH_ERR_SOLVER, "element " + std::to_string(herr[1]) + ": non-positive Jacobian");
-  if (herr[0] & 2) throw_error(VBH_ERR_SOLVER, "EAS element: Kaa factorization failed");
-  if (herr[0] & 4) throw_error(VBH_ERR_HIP, "assembly: CSR slot not found (internal error)");
+  const std::string bad_e = "element " + std::to_string(herr[1]);
+  if (herr[0] & kErrJacobian) throw_error(VBH_ERR_SOLVER, bad_e + ": non-positive Jacobian");
+  if (herr[0] & kErrKaa) throw_error(VBH_ERR_SOLVER, "EAS element: Kaa factorization failed");
+  if (herr[0] & kErrMitcGeometry)
+    throw_error(VBH_ERR_SOLVER, "CQUAD4 " + bad_e + ": MITC4+ characteristic geometry is singular");
+  if (herr[0] & kErrMitcDistort)
+    throw_error(VBH_ERR_SOLVER, "CQUAD4 " + bad_e + ": MITC4+ distortion denominator is near zero");
+  if (herr[0] & kErrShellDiag)
+    throw_error(VBH_ERR_SOLVER, "CQUAD4 " + bad_e +
+                                    ": stiffness matrix diagonal is near-zero; check material, "
+                                    "thickness, and geometry");
+  if (herr[0] & kErrSlot) throw_error(VBH_ERR_HIP, "assembly: CSR slot not found (internal error)");
 
   // 4. compaction of slots without a nonzero contribution
   DBuf<int> pos;
