@@ -134,6 +134,23 @@ int vbh_solve_block(vbh_ctx* ctx, int k, const double* F, double* U, int* iters,
                     double* rel_res);
 int vbh_solve_block_device(vbh_ctx* ctx, int k, const double* dF, double* dU, int* iters,
                            double* rel_res);
+/* SpMM configuration the block solver's autotuner chose for the last block
+ * solve (no reference counterpart): column tile (load cases per pass over K),
+ * load cases per lane, lanes per block-row group (64 / lanes_per_group block
+ * rows per wave), grid rows, whether block rows are split into one
+ * contiguous range per XCD, whether block values are staged through LDS,
+ * padded column count and the measured mean SpMM launch in ms. */
+typedef struct {
+  int col_tile;
+  int cols_per_lane;
+  int lanes_per_group;
+  int row_blocks;
+  int xcd_rows;
+  int lds_staged;
+  int ldk;
+  double spmm_ms;
+} vbh_block_config;
+int vbh_last_block_config(const vbh_ctx* ctx, vbh_block_config* out);
 int vbh_last_stats(const vbh_ctx* ctx, vbh_solve_stats* out);
 
 /* ---- operator application ------------------------------------------------

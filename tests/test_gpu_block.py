@@ -1,0 +1,164 @@
+"""Block PCG over k load cases through the C ABI: every column follows the
+single-load-case recurrence, so each column is checked against the CPU oracle
+solve of the same right-hand side (FP64: same iteration count within 2 and
+solution within the PCG tolerance; mixed: FP64 true residual below 1e-8)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from vibestran_amd import CsrData, SolverError, _lib as L, decks
+from vibestran_amd import model as M
+from vibestran_amd.assembly import assemble_model, dof_map_device
+
+pytestmark = pytest.mark.gpu
+
+
+def csr_of(oc):
+    rp, ci, v = oc.arrays()
+    return CsrData.from_arrays(rp, ci, v, lower_only=oc.stores_lower_triangle_only())
+
+
+@pytest.fixture(scope="module")
+def c1():
+    m = decks.config1_hex_cantilever()
+    eq, n = M.analysis_dof_map(m)
+    oc = O.assemble_model(n, m, eq)
+    rng = np.random.default_rng(21)
+    F = rng.standard_normal((n, 6))
+    F[:, 0] = M.assemble_rhs(m, eq, n, m.loads[0])
+    return m, eq, n, oc, F
+
+
+@pytest.mark.parametrize("fmt", [-1, 0, 3])
+def test_block_fp64_matches_single_solves(gpu_backend_factory, c1, fmt):
+    m, eq, n, oc, F = c1
+    be = gpu_backend_factory(tolerance=1e-10, max_iters=20000)
+    be.set_format(fmt)
+    K = csr_of(oc)
+    U = be.solve_block(K, F)
+    its = be.last_block_iterations
+    for j in range(F.shape[1]):
+        uo, it_o, _ = O.pcg_jacobi(oc, F[:, j], tol=1e-10, max_iters=20000)
+        assert abs(int(its[j]) - it_o) <= 2, (j, its[j], it_o)
+        assert np.abs(U[:, j] - uo).max() <= 1e-8 * np.abs(uo).max()
+        assert O.true_relative_residual(oc, U[:, j], F[:, j]) < 1e-9
+    cfg = be.block_config()
+    assert cfg.ldk == 8 and cfg.col_tile in (4, 8) and cfg.spmm_ms > 0
+
+
+def test_block_mixed_reaches_1e8(gpu_backend_factory, c1):
+    m, eq, n, oc, F = c1
+    be = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=20000)
+    U = be.solve_block(csr_of(oc), F)
+    for j in range(F.shape[1]):
+        rel = O.true_relative_residual(oc, U[:, j], F[:, j])
+        assert rel < 1e-8 and abs(rel - be.last_block_residuals[j]) <= 1e-3 * rel + 1e-15
+    assert be.stats().refinements >= F.shape[1]
+
+
+def test_block_zero_and_repeated_columns(gpu_backend_factory, c1):
+    m, eq, n, oc, F = c1
+    be = gpu_backend_factory(tolerance=1e-10, max_iters=20000)
+    G = np.stack([F[:, 1], np.zeros(n), F[:, 1], 2.0 * F[:, 1]], axis=1)
+    U = be.solve_block(csr_of(oc), G)
+    its = be.last_block_iterations
+    assert its[1] == 0 and np.all(U[:, 1] == 0.0) and be.last_block_residuals[1] == 0.0
+    assert np.array_equal(U[:, 0], U[:, 2]) and its[0] == its[2]
+    assert np.abs(U[:, 3] - 2.0 * U[:, 0]).max() <= 1e-8 * np.abs(U[:, 3]).max()
+
+
+@pytest.mark.parametrize("k", [1, 3, 32, 64])
+def test_block_widths(gpu_backend_factory, k):
+    oc = O.SparseMatrixBuilder(300)
+    for i in range(300):
+        oc.add(i, i, 4.0 + (i % 7))
+        if i:
+            oc.add(i, i - 1, -1.0)
+        if i > 5:
+            oc.add(i, i - 6, -0.5)
+    A = oc.build_csr()
+    be = gpu_backend_factory(tolerance=1e-12, max_iters=5000)
+    F = np.random.default_rng(k).standard_normal((300, k))
+    U = be.solve_block(csr_of(A), F)
+    Ad = A.to_scipy().toarray()
+    Ad = np.tril(Ad) + np.tril(Ad, -1).T
+    X = np.linalg.solve(Ad, F)
+    assert np.abs(U - X).max() <= 1e-9 * np.abs(X).max()
+
+
+def test_block_errors(gpu_backend_factory, c1):
+    m, eq, n, oc, F = c1
+    be = gpu_backend_factory(max_iters=5)
+    with pytest.raises(SolverError, match="load case 0: did not converge after 5 iterations"):
+        be.solve_block(csr_of(oc), F[:, :2])
+    assert np.all(be.last_block_iterations == 5)
+    be2 = gpu_backend_factory()
+    with pytest.raises(ValueError, match="k must be in 1..64"):
+        be2.solve_block(csr_of(oc), np.zeros((n, 65)))
+
+
+def test_block_device_pointers_tube(gpu_backend_factory):
+    """CQUAD4 tube (BSR6), device-assembled operator, device-resident F/U."""
+    be = gpu_backend_factory(tolerance=1e-9, max_iters=50000)
+    m = decks.cquad4_tube(n_circ=24, n_axial=12, n_cases=8)
+    eq, n = dof_map_device(be, m)
+    K = assemble_model(be, m, eq, n)
+    F = np.asfortranarray(np.stack([M.assemble_rhs(m, eq, n, lc) for lc in m.loads], axis=1))
+    lib = L.load()
+    nbytes = F.nbytes
+    dF, dU = C.c_void_p(), C.c_void_p()
+    assert lib.vbh_device_alloc(be._h, nbytes, C.byref(dF)) == 0
+    assert lib.vbh_device_alloc(be._h, nbytes, C.byref(dU)) == 0
+    try:
+        assert lib.vbh_memcpy_h2d(be._h, dF, F.ctypes.data_as(C.c_void_p), nbytes) == 0
+        its, rels = be.solve_block_device(dF.value, dU.value, F.shape[1])
+        U = np.empty_like(F)
+        assert lib.vbh_memcpy_d2h(be._h, U.ctypes.data_as(C.c_void_p), dU, nbytes) == 0
+    finally:
+        lib.vbh_device_free(be._h, dF)
+        lib.vbh_device_free(be._h, dU)
+    oc = O.assemble_model(n, m, eq)
+    for j in range(F.shape[1]):
+        uo, it_o, _ = O.pcg_jacobi(oc, F[:, j], tol=1e-9, max_iters=50000)
+        # shells are ill-conditioned: the SpMM summation order moves the
+        # count by a few iterations in a thousand
+        assert abs(int(its[j]) - it_o) <= max(2, it_o // 50), (j, its[j], it_o)
+        assert np.abs(U[:, j] - uo).max() <= 1e-6 * np.abs(uo).max()
+        assert O.true_relative_residual(oc, U[:, j], F[:, j]) < 2e-9
+
+
+def _variants():
+    out = []
+    for cw, lg in [(2, 16), (4, 8), (8, 4), (1, 32)]:
+        for xcd in (0, 1):
+            for stage in (0, 1):
+                out.append(f"{cw},{lg},1024,{xcd},{stage}")
+    return out
+
+
+@pytest.mark.parametrize("cfg", _variants())
+def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
+    """Each SpMM shape the autotuner can pick, forced, on a BSR6 shell
+    operator (FP64) and a BSR3 solid operator (mixed), k = 32."""
+    monkeypatch.setenv("VBH_BLOCK_TUNE", cfg)
+    m = decks.cquad4_tube(n_circ=16, n_axial=24, n_cases=32)
+    be = gpu_backend_factory(tolerance=1e-10, max_iters=50000)
+    eq, n = dof_map_device(be, m)
+    K = assemble_model(be, m, eq, n)
+    F = np.asfortranarray(np.stack([M.assemble_rhs(m, eq, n, lc) for lc in m.loads], axis=1))
+    U = be.solve_block(K, F)
+    cfg_used = be.block_config()
+    assert (cfg_used.cols_per_lane, cfg_used.lanes_per_group) == tuple(map(int, cfg.split(",")[:2]))
+    oc = O.assemble_model(n, m, eq)
+    for j in (0, 13, 31):
+        assert O.true_relative_residual(oc, U[:, j], F[:, j]) < 2e-10
+    hm = decks.config2_hex_jitter(nx=10, ny=4, nz=4, seed=7)
+    eq2, n2 = M.analysis_dof_map(hm)
+    oc2 = O.assemble_model(n2, hm, eq2)
+    G = np.random.default_rng(3).standard_normal((n2, 32))
+    be2 = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=50000)
+    U2 = be2.solve_block(csr_of(oc2), G)
+    for j in (0, 7, 31):
+        assert O.true_relative_residual(oc2, U2[:, j], G[:, j]) < 1e-8

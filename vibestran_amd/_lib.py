@@ -35,6 +35,19 @@ class vbh_assembly_stats(C.Structure):
     ]
 
 
+class vbh_block_config(C.Structure):
+    _fields_ = [
+        ("col_tile", C.c_int),
+        ("cols_per_lane", C.c_int),
+        ("lanes_per_group", C.c_int),
+        ("row_blocks", C.c_int),
+        ("xcd_rows", C.c_int),
+        ("lds_staged", C.c_int),
+        ("ldk", C.c_int),
+        ("spmm_ms", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -62,6 +75,7 @@ SIGNATURES = {
     "vbh_solve_device": (_I, [_P, _P, _P, _IP, _DP]),
     "vbh_solve_block": (_I, [_P, _I, _DP, _DP, _IP, _DP]),
     "vbh_solve_block_device": (_I, [_P, _I, _P, _P, _IP, _DP]),
+    "vbh_last_block_config": (_I, [_P, C.POINTER(vbh_block_config)]),
     "vbh_last_stats": (_I, [_P, C.POINTER(vbh_solve_stats)]),
     "vbh_spmv": (_I, [_P, _DP, _DP]),
     "vbh_bench_spmv": (_I, [_P, _I, _I, _DP]),

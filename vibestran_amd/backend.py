@@ -170,6 +170,23 @@ class HipPcgSolverBackend:
         _check(rc)
         return U
 
+    def solve_block_device(self, dF: int, dU: int, k: int):
+        """Block solve on device-resident column-major n x k arrays (HBM
+        addresses); returns (iterations[k], rel_res[k])."""
+        its = np.zeros(k, dtype=np.int32)
+        rels = np.zeros(k, dtype=np.float64)
+        rc = self._lib.vbh_solve_block_device(self._h, k, C.c_void_p(dF), C.c_void_p(dU),
+                                              L.ip(its), L.dp(rels))
+        self.last_block_iterations = its
+        self.last_block_residuals = rels
+        _check(rc)
+        return its, rels
+
+    def block_config(self) -> L.vbh_block_config:
+        cfg = L.vbh_block_config()
+        _check(self._lib.vbh_last_block_config(self._h, C.byref(cfg)))
+        return cfg
+
     def spmv(self, x) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)

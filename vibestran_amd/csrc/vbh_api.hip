@@ -185,15 +185,25 @@ void Ctx::upload_operator(int n_, int nnz, const int* rp, const int* ci, const d
   HostBsr bsr;
   if (fmt < 0) {
     fmt = 0;
-    for (int B : {6, 3}) {
-      if (n_ < 4 * B) continue;
-      HostBsr cand = to_bsr(full, B);
-      const double fill = static_cast<double>(cand.v.size()) / static_cast<double>(full.v.size());
-      if (fill <= (B == 6 ? 1.12 : 1.25)) {
-        bsr = std::move(cand);
-        fmt = B;
-        break;
-      }
+    // 6x6 node blocks when they carry no more padding than 3x3 blocks would
+    // (shell meshes: structural zeros make both about 1.2), else 3x3.
+    const double nv = static_cast<double>(full.v.size());
+    HostBsr b6, b3;
+    double f6 = 1e300, f3 = 1e300;
+    if (n_ >= 24) {
+      b6 = to_bsr(full, 6);
+      f6 = static_cast<double>(b6.v.size()) / nv;
+    }
+    if (f6 > 1.12 && n_ >= 12) {
+      b3 = to_bsr(full, 3);
+      f3 = static_cast<double>(b3.v.size()) / nv;
+    }
+    if (f6 <= 1.12 || (f6 <= 1.3 && f6 <= 1.05 * f3)) {
+      bsr = std::move(b6);
+      fmt = 6;
+    } else if (f3 <= 1.25) {
+      bsr = std::move(b3);
+      fmt = 3;
     }
   } else if (fmt == 3 || fmt == 6) {
     bsr = to_bsr(full, fmt);

@@ -1,14 +1,1263 @@
-// vibestran_amd/csrc/vbh_block.hip -- block PCG over k load cases (placeholder
-// until the fused block kernels land: reports VBH_ERR_STATE).
+// vibestran_amd/csrc/vbh_block.hip -- block Jacobi-PCG over k load cases that
+// share one stiffness operator (gfx950).
+//
+// The reference solves every SUBCASE separately (linear_static.cpp:202-320,
+// one SolverBackend::solve per load vector). Here the k recurrences advance
+// together so the operator is streamed from HBM once per iteration for all of
+// them:
+//   * vectors are row-major n_pad x ldk (ldk = k rounded up to a power of two,
+//     padding columns stay zero and inactive), so one row of all load cases is
+//     one contiguous run and lanes map to load cases;
+//   * the SpMM reads each stored B x B block once per column tile: a group of
+//     KT lanes (one per load case) owns one block row, gathers the B rows of
+//     the block column for its KT cases and accumulates B outputs per lane.
+//     With one group per wave the block row index is wave-uniform, so block
+//     values and indices are fetched by scalar loads; with 64/KT groups per
+//     wave every lane is busy but values go through vector loads. The choice,
+//     the column tile KT and the grid are picked per (operator, k, precision)
+//     by timing the candidates (autotune below);
+//   * every recurrence keeps the reference's scalar semantics
+//     (cuda_pcg_backend_impl.hpp:1795-1999: u0 = 0, r = F, z = r/diag, zero
+//     load -> 0 iterations, r.z <= 0 -> error, p.Ap <= 0 -> breakdown,
+//     converged when ||r||/||F|| < tol) independently per load case: a column
+//     that converges or breaks down is frozen while the others continue;
+//   * global sums are per-block partials reduced by a one-block-per-column
+//     scalar kernel (fixed order, deterministic), which also owns that
+//     column's state, so no kernel reads what another block of it writes;
+//   * iterations are replayed from a captured hipGraph; the host polls the
+//     column states between replays.
+// Mixed precision (FP32 r, p, Ap, Jacobi and SpMM, FP64 solution update)
+// pauses a column when its recursive residual fell by refresh_delta since its
+// last refresh (or below tol); the host then recomputes r64 = F - K u in FP64
+// for all columns and reloads the paused ones (reliable-update continuation),
+// deciding convergence on the FP64 true residual.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
 #include "vbh_ctx.hpp"
 
+namespace vbh {
+
+// Per-load-case solver state, owned by block `column` of the scalar kernels.
+struct BlkCol {
+  double rz_old, rz, rr, rel, norm_b, alpha, beta, pAp, rr_refresh, rel_true, hist_rel;
+  int iters, flags, state, refreshes, stalls, resumed;
+};
+enum : int { kColActive = 0, kColPaused = 1, kColDone = 2 };
+
+// Block operator in array-of-blocks layout, blocks column-major: entry (r, c)
+// of block q at v[q * B * B + c * B + r] (scalar CSR: B = 1, the CSR values).
+struct BlkOp {
+  int B = 0, nbr = 0;
+  const int* brp = nullptr;
+  const int* bci = nullptr;
+  const double* v64 = nullptr;
+  const float* v32 = nullptr;
+};
+
+struct BlkTune {
+  int cw = 0, lg = 0, nrb = 0;  // columns per lane, lanes per group, grid rows
+  int xcd = 0;                  // XCD-contiguous row ranges
+  int stage = 0;                // 1: block values staged through LDS
+  double ms = 0.0;
+};
+
+struct BlockState {
+  // operator copy in array-of-blocks layout
+  long long op_version = -1;
+  int B = 0;
+  DBuf<double> aos64;
+  DBuf<float> aos32;
+  BlkOp op;
+  // work space for one (ldk, n_pad)
+  int ldk = 0, lg2 = 0, npad = 0, k = 0;
+  long long work_version = -1;
+  DBuf<double> F, u, r64, rd, pd, Apd;
+  DBuf<float> rf, pf, Apf;
+  DBuf<double> partA, partB, partR;
+  DBuf<double> coefA, coefB;
+  DBuf<int> actD, actU, reload, ones;
+  DBuf<BlkCol> cols;
+  BlkCol* cols_host = nullptr;
+  int nvec = 1;  // grid of the vector kernels (partials per column)
+  // autotune cache for the current operator: (ldk, precision) -> config
+  std::map<std::pair<int, int>, BlkTune> tuned;
+  BlkTune cur;
+  // captured iteration graph
+  hipGraphExec_t graph = nullptr;
+  int g_prec = -1, g_ipg = 0, g_max_iters = 0, g_cw = 0, g_lg = 0, g_nrb = 0, g_xcd = 0, g_stage = 0;
+  double g_tol = 0.0, g_delta = 0.0;
+  ~BlockState() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (cols_host) (void)hipHostFree(cols_host);
+  }
+  void drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
+};
+
+void destroy_block_state(BlockState* b) { delete b; }
+
+namespace {
+
+constexpr int kBT = 256;  // threads per block of every block-PCG kernel
+
+// ---- block reductions -----------------------------------------------------------------
+__device__ __forceinline__ double block_total(double v, double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBT / 64; ++w) s += lds[w];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ bool any_active(const int* __restrict__ act, int ldk) {
+  int a = 0;
+  for (int j = 0; j < ldk; ++j) a |= act[j];
+  return a != 0;
+}
+
+// ---- wide loads ---------------------------------------------------------------------------
+template <typename T, int N>
+struct alignas(sizeof(T) * N) Vec {
+  T v[N];
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void vload(const T* __restrict__ p, T (&x)[N]) {
+  const Vec<T, N> t = *reinterpret_cast<const Vec<T, N>*>(p);
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = t.v[i];
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void vstore(T* __restrict__ p, const T (&x)[N]) {
+  Vec<T, N> t;
+#pragma unroll
+  for (int i = 0; i < N; ++i) t.v[i] = x[i];
+  *reinterpret_cast<Vec<T, N>*>(p) = t;
+}
+
+// ---- vector kernels -------------------------------------------------------------------------
+// Thread t handles V = 16 / sizeof(T) consecutive elements of one row, columns
+// c0 = (t V) mod ldk .. c0 + V - 1; the grid stride is a multiple of ldk, so
+// each thread keeps its columns. Element e = row * ldk + column, row = e >> lg2.
+template <typename T>
+constexpr int vwidth() {
+  return 16 / static_cast<int>(sizeof(T));
+}
+
+// Per-column block sums of NV values: lds holds kBT * V doubles; thread t < ldk
+// writes column t's sums to part[(i * ldk + t) * kMaxGrid + blockIdx.x].
+template <int V, int NV>
+__device__ __forceinline__ void column_partials(const double (&v)[NV][V], int ldk, double* lds,
+                                                double* __restrict__ part) {
+  const int per = ldk / V;  // threads sharing one set of columns repeat every `per`
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+#pragma unroll
+    for (int w = 0; w < V; ++w) lds[threadIdx.x * V + w] = v[i][w];
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < ldk) {
+      const int t = threadIdx.x, w = t % V;
+      double s = 0.0;
+      for (int th = t / V; th < kBT; th += per) s += lds[th * V + w];
+      part[(static_cast<size_t>(i) * ldk + t) * kMaxGrid + blockIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+#define VBH_BLK_LOOP(e)                                                                      \
+  for (long long e = (blockIdx.x * static_cast<long long>(kBT) + threadIdx.x) * V; e < m; \
+       e += static_cast<long long>(gridDim.x) * kBT * V)
+
+template <typename T>
+__global__ __launch_bounds__(kBT) void k_blk_init(long long m, int ldk, int lg2,
+                                                  const double* __restrict__ F,
+                                                  const T* __restrict__ diag,
+                                                  double* __restrict__ u, T* __restrict__ r,
+                                                  T* __restrict__ p, double* __restrict__ part) {
+  constexpr int V = vwidth<T>();
+  __shared__ double lds[kBT * V];
+  double acc[3][V] = {};
+  VBH_BLK_LOOP(e) {
+    const T d = diag[e >> lg2];
+    double f[V], uz[V];
+    T ri[V], zero[V];
+    vload<double, V>(F + e, f);
+#pragma unroll
+    for (int w = 0; w < V; ++w) {
+      ri[w] = static_cast<T>(f[w]);
+      const T zi = ri[w] / d;
+      acc[0][w] += static_cast<double>(ri[w]) * static_cast<double>(zi);
+      acc[1][w] += static_cast<double>(ri[w]) * static_cast<double>(ri[w]);
+      acc[2][w] += f[w] * f[w];
+      uz[w] = 0.0;
+      zero[w] = T(0);
+    }
+    vstore<double, V>(u + e, uz);
+    vstore<T, V>(r + e, ri);
+    vstore<T, V>(p + e, zero);
+  }
+  column_partials<V, 3>(acc, ldk, lds, part);
+}
+
+// p = z + beta p with z = r / diag, for active columns.
+template <typename T>
+__global__ __launch_bounds__(kBT) void k_blk_direction(long long m, int ldk, int lg2,
+                                                       const T* __restrict__ diag,
+                                                       const T* __restrict__ r, T* __restrict__ p,
+                                                       const double* __restrict__ coefB,
+                                                       const int* __restrict__ actD) {
+  constexpr int V = vwidth<T>();
+  if (!any_active(actD, ldk)) return;
+  const int c0 = (threadIdx.x * V) & (ldk - 1);
+  bool on[V];
+  T bt[V];
+  int any = 0;
+#pragma unroll
+  for (int w = 0; w < V; ++w) {
+    on[w] = actD[c0 + w] != 0;
+    any |= on[w];
+    bt[w] = static_cast<T>(coefB[c0 + w]);
+  }
+  if (!any) return;
+  VBH_BLK_LOOP(e) {
+    const T d = diag[e >> lg2];
+    T rv[V], pv[V];
+    vload<T, V>(r + e, rv);
+    vload<T, V>(p + e, pv);
+#pragma unroll
+    for (int w = 0; w < V; ++w) pv[w] = on[w] ? rv[w] / d + bt[w] * pv[w] : pv[w];
+    vstore<T, V>(p + e, pv);
+  }
+}
+
+// u += alpha p (FP64); r -= alpha Ap; partials r.z, r.r per column.
+template <typename T>
+__global__ __launch_bounds__(kBT) void k_blk_update(long long m, int ldk, int lg2,
+                                                    const T* __restrict__ diag,
+                                                    double* __restrict__ u, T* __restrict__ r,
+                                                    const T* __restrict__ p,
+                                                    const T* __restrict__ Ap,
+                                                    const double* __restrict__ coefA,
+                                                    const int* __restrict__ actU,
+                                                    double* __restrict__ part) {
+  constexpr int V = vwidth<T>();
+  __shared__ double lds[kBT * V];
+  if (!any_active(actU, ldk)) return;
+  const int c0 = (threadIdx.x * V) & (ldk - 1);
+  bool on[V];
+  double al[V];
+  T a[V];
+#pragma unroll
+  for (int w = 0; w < V; ++w) {
+    on[w] = actU[c0 + w] != 0;
+    al[w] = coefA[c0 + w];
+    a[w] = static_cast<T>(al[w]);
+  }
+  double acc[2][V] = {};
+  VBH_BLK_LOOP(e) {
+    const T d = diag[e >> lg2];
+    double uv[V];
+    T rv[V], pv[V], qv[V];
+    vload<double, V>(u + e, uv);
+    vload<T, V>(r + e, rv);
+    vload<T, V>(p + e, pv);
+    vload<T, V>(Ap + e, qv);
+#pragma unroll
+    for (int w = 0; w < V; ++w) {
+      if (on[w]) {
+        uv[w] += al[w] * static_cast<double>(pv[w]);
+        rv[w] = rv[w] - a[w] * qv[w];
+      }
+      const T zi = rv[w] / d;
+      acc[0][w] += static_cast<double>(rv[w]) * static_cast<double>(zi);
+      acc[1][w] += static_cast<double>(rv[w]) * static_cast<double>(rv[w]);
+    }
+    vstore<double, V>(u + e, uv);
+    vstore<T, V>(r + e, rv);
+  }
+  column_partials<V, 2>(acc, ldk, lds, part);
+}
+
+// Mixed refresh: reload r = (float) r64 in the columns the scalar kernel
+// released; partials r.z, r.r per column.
+__global__ __launch_bounds__(kBT) void k_blk_reload(long long m, int ldk, int lg2,
+                                                    const float* __restrict__ diag,
+                                                    const double* __restrict__ r64,
+                                                    float* __restrict__ r,
+                                                    const int* __restrict__ reload,
+                                                    double* __restrict__ part) {
+  constexpr int V = vwidth<float>();
+  __shared__ double lds[kBT * V];
+  const int c0 = (threadIdx.x * V) & (ldk - 1);
+  bool on[V];
+#pragma unroll
+  for (int w = 0; w < V; ++w) on[w] = reload[c0 + w] != 0;
+  double acc[2][V] = {};
+  VBH_BLK_LOOP(e) {
+    const float d = diag[e >> lg2];
+    double wv[V];
+    float rv[V];
+    vload<double, V>(r64 + e, wv);
+    vload<float, V>(r + e, rv);
+#pragma unroll
+    for (int w = 0; w < V; ++w) {
+      if (on[w]) rv[w] = static_cast<float>(wv[w]);
+      const float zi = rv[w] / d;
+      acc[0][w] += static_cast<double>(rv[w]) * static_cast<double>(zi);
+      acc[1][w] += static_cast<double>(rv[w]) * static_cast<double>(rv[w]);
+    }
+    vstore<float, V>(r + e, rv);
+  }
+  column_partials<V, 2>(acc, ldk, lds, part);
+}
+#undef VBH_BLK_LOOP
+
+// ---- SpMM ---------------------------------------------------------------------------------
+// MODE 0 (PCG):      Y = K X,      partial X.Y per column.
+// MODE 1 (refresh):  Y = F - K X,  partial Y.Y per column (FP64).
+// Lanes form groups of LG; a group owns one block row at a time and each of
+// its lanes CW consecutive columns (one wide load per block row of X), so a
+// wave covers 64 / LG block rows and a column tile of KT = LG * CW load cases.
+// Grid: (nrb, ldk / KT), 4 waves per block.
+template <typename T>
+__device__ __forceinline__ const T* op_vals(const BlkOp& A);
+template <>
+__device__ __forceinline__ const double* op_vals<double>(const BlkOp& A) { return A.v64; }
+template <>
+__device__ __forceinline__ const float* op_vals<float>(const BlkOp& A) { return A.v32; }
+
+// B consecutive values of a stored block (one block column; 16-byte loads
+// where the layout allows).
+template <typename T, int B>
+__device__ __forceinline__ void block_row(const T* __restrict__ v, T (&a)[B]) {
+  if constexpr (B == 6) {
+    T t[2];
+#pragma unroll
+    for (int h = 0; h < 3; ++h) {
+      vload<T, 2>(v + 2 * h, t);
+      a[2 * h] = t[0];
+      a[2 * h + 1] = t[1];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < B; ++s) a[s] = v[s];
+  }
+}
+
+// xcd != 0: workgroup w runs on XCD w mod 8 (round-robin dispatch), so the
+// block rows are split into 8 contiguous ranges, one per XCD, and each XCD's
+// workgroups sweep their range together: rows that share block columns (the
+// neighbours of a node) are then read through the same L2.
+template <typename T, int B, int CW, int LG, int MODE>
+__global__ __launch_bounds__(kBT) void k_blk_spmm(BlkOp A, int ldk, int xcd,
+                                                  const T* __restrict__ X, T* __restrict__ Y,
+                                                  const double* __restrict__ F,
+                                                  const int* __restrict__ act,
+                                                  double* __restrict__ part) {
+  constexpr int KT = CW * LG, GR = 64 / LG;
+  __shared__ double lds[kBT * CW];
+  const int lane = threadIdx.x & 63;
+  const int lg = lane % LG, g = lane / LG;
+  const int c0 = blockIdx.y * KT + lg * CW;
+  if (MODE == 0) {
+    int on = 0;
+#pragma unroll
+    for (int w = 0; w < CW; ++w) on |= act[c0 + w];
+    if (!__syncthreads_or(on)) return;
+  }
+  const T* __restrict__ val = op_vals<T>(A);
+  const int wave = threadIdx.x >> 6;
+  double dot[CW];
+#pragma unroll
+  for (int w = 0; w < CW; ++w) dot[w] = 0.0;
+  int rb = blockIdx.x, nrbx = gridDim.x, row_lo = 0, row_hi = A.nbr;
+  if (xcd) {
+    const int x = blockIdx.x & 7, per = (A.nbr + 7) >> 3;
+    rb = blockIdx.x >> 3;
+    nrbx = gridDim.x >> 3;
+    row_lo = min(A.nbr, x * per);
+    row_hi = min(A.nbr, row_lo + per);
+  }
+  for (int I0 = row_lo + (rb * (kBT / 64) + wave) * GR; I0 < row_hi;
+       I0 += nrbx * (kBT / 64) * GR) {
+    const int I = I0 + g;
+    if (I >= row_hi) continue;
+    T acc[B][CW];
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int w = 0; w < CW; ++w) acc[r][w] = T(0);
+    const int o = A.brp[I], e = A.brp[I + 1];
+    for (int q = o; q < e; ++q) {
+      const int J = A.bci[q];
+      const T* __restrict__ v = val + static_cast<size_t>(q) * (B * B);
+      const T* __restrict__ xj = X + static_cast<size_t>(B * J) * ldk + c0;
+      // column s of the block (stored contiguously) times row s of X_J
+#pragma unroll
+      for (int s = 0; s < B; ++s) {
+        T x[CW], a[B];
+        vload<T, CW>(xj + static_cast<size_t>(s) * ldk, x);
+        block_row<T, B>(v + s * B, a);
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+          for (int w = 0; w < CW; ++w) acc[r][w] += a[r] * x[w];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
+      if constexpr (MODE == 0) {
+        T xo[CW];
+        vload<T, CW>(X + idx, xo);
+        vstore<T, CW>(Y + idx, acc[r]);
+#pragma unroll
+        for (int w = 0; w < CW; ++w)
+          dot[w] += static_cast<double>(xo[w]) * static_cast<double>(acc[r][w]);
+      } else {
+        double f[CW];
+        T y[CW];
+        vload<double, CW>(F + idx, f);
+#pragma unroll
+        for (int w = 0; w < CW; ++w) {
+          const double yy = f[w] - static_cast<double>(acc[r][w]);
+          y[w] = static_cast<T>(yy);
+          dot[w] += yy * yy;
+        }
+        vstore<T, CW>(Y + idx, y);
+      }
+    }
+  }
+  // per-column block sum: threads with equal lane % LG share columns
+#pragma unroll
+  for (int w = 0; w < CW; ++w) lds[threadIdx.x * CW + w] = dot[w];
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < KT) {
+    const int t = threadIdx.x, w = t % CW;
+    double s = 0.0;
+    for (int th = t / CW; th < kBT; th += LG) s += lds[th * CW + w];
+    part[static_cast<size_t>(blockIdx.y * KT + t) * kMaxGrid + blockIdx.x] = s;
+  }
+}
+
+// ---- scalar kernels: block j owns load case j -------------------------------------------
+// mode 0: after init; 1: after an update; 2: after a refresh reload (only the
+// reloaded columns). Reduces r.z, r.r (and F.F in mode 0) and decides.
+__global__ __launch_bounds__(kBT) void k_blk_scal_beta(int mode, int ldk, int k, int mixed,
+                                                       const double* __restrict__ part, int nvec,
+                                                       BlkCol* __restrict__ cols,
+                                                       double* __restrict__ coefB,
+                                                       int* __restrict__ actD,
+                                                       const int* __restrict__ reload,
+                                                       PcgParams prm) {
+  __shared__ double lds[kBT];
+  const int j = blockIdx.x;
+  BlkCol& cj = cols[j];
+  if (mode == 2 && !reload[j]) return;  // untouched columns keep their coefficients
+  if (mode == 1 && cj.state != kColActive) {
+    if (threadIdx.x == 0) actD[j] = 0;
+    return;
+  }
+  double v[3] = {0.0, 0.0, 0.0};
+  const int nv = (mode == 0) ? 3 : 2;
+  for (int i = 0; i < nv; ++i) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nvec; b += kBT) t += part[(i * ldk + j) * kMaxGrid + b];
+    v[i] = block_total(t, lds);
+  }
+  if (threadIdx.x != 0) return;
+  const double rz = v[0], rr = v[1];
+  int stop = 0;
+  double beta = 0.0, rel;
+  if (mode == 0) {
+    cj.iters = 0;
+    cj.flags = 0;
+    cj.refreshes = 0;
+    cj.stalls = 0;
+    cj.resumed = 0;
+    cj.hist_rel = 1e300;
+    cj.alpha = cj.pAp = 0.0;
+    cj.rel_true = 1.0;
+    cj.rz_old = 0.0;
+    cj.norm_b = sqrt(fabs(v[2]));
+    rel = 1.0;
+    if (j >= k) {  // padding column
+      stop = kFlagZeroRhs;
+      rel = 0.0;
+    } else if (cj.norm_b < 1e-300) {  // zero load: u = 0 after 0 iterations
+      stop = kFlagZeroRhs;
+      rel = 0.0;
+    } else if (rz <= 0.0) {
+      stop = kFlagRz0;
+    } else if (prm.max_iters <= 0) {
+      stop = kFlagMaxIter;
+    }
+    if (!stop) cj.rr_refresh = rr;
+  } else {
+    rel = sqrt(fabs(rr)) / cj.norm_b;
+    if (mode == 1) {
+      if (!mixed) {
+        if (rel < prm.tol) stop = kFlagConverged;
+      } else if (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * cj.rr_refresh) {
+        stop = kFlagRefresh;
+      }
+    }
+    if (!stop && cj.iters >= prm.max_iters) stop = kFlagMaxIter;
+    if (!stop) beta = rz / cj.rz_old;
+    if (mode == 2) cj.rr_refresh = rr;
+  }
+  cj.rz = rz;
+  cj.rr = rr;
+  cj.rel = rel;
+  if (stop == kFlagRefresh) {
+    cj.state = kColPaused;
+    actD[j] = 0;
+    return;  // rz_old stays the predecessor's: beta after the reload uses it
+  }
+  if (stop) {
+    cj.flags |= stop;
+    cj.state = kColDone;
+    actD[j] = 0;
+    return;
+  }
+  cj.state = kColActive;
+  cj.resumed = 0;
+  cj.rz_old = rz;
+  cj.beta = beta;
+  coefB[j] = beta;
+  actD[j] = 1;
+}
+
+// alpha = r.z / p.Ap for columns whose direction was formed this iteration.
+__global__ __launch_bounds__(kBT) void k_blk_scal_alpha(int ldk, const double* __restrict__ part,
+                                                        int nrb, BlkCol* __restrict__ cols,
+                                                        double* __restrict__ coefA,
+                                                        int* __restrict__ actU,
+                                                        const int* __restrict__ actD) {
+  __shared__ double lds[kBT];
+  const int j = blockIdx.x;
+  if (!actD[j]) {
+    if (threadIdx.x == 0) {
+      actU[j] = 0;
+      coefA[j] = 0.0;
+    }
+    return;
+  }
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nrb; b += kBT) t += part[static_cast<size_t>(j) * kMaxGrid + b];
+  const double pAp = block_total(t, lds);
+  if (threadIdx.x != 0) return;
+  BlkCol& cj = cols[j];
+  cj.pAp = pAp;
+  if (!(pAp > 0.0)) {  // breakdown (cuda_pcg_backend_impl.hpp:691-696)
+    cj.flags |= kFlagBreakdown;
+    cj.state = kColDone;
+    cj.rel = 1e300;
+    cj.alpha = 0.0;
+    coefA[j] = 0.0;
+    actU[j] = 0;
+    return;
+  }
+  const double alpha = cj.rz_old / pAp;
+  cj.alpha = alpha;
+  cj.iters += 1;
+  coefA[j] = alpha;
+  actU[j] = 1;
+}
+
+// Mixed refresh decision for the paused columns from ||F - K u||.
+__global__ __launch_bounds__(kBT) void k_blk_scal_refresh(int ldk, const double* __restrict__ part,
+                                                          int nrb, BlkCol* __restrict__ cols,
+                                                          int* __restrict__ reload, PcgParams prm) {
+  __shared__ double lds[kBT];
+  const int j = blockIdx.x;
+  BlkCol& cj = cols[j];
+  if (cj.state != kColPaused) {
+    if (threadIdx.x == 0) reload[j] = 0;
+    return;
+  }
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nrb; b += kBT) t += part[static_cast<size_t>(j) * kMaxGrid + b];
+  const double rr = block_total(t, lds);
+  if (threadIdx.x != 0) return;
+  const double rel = sqrt(fabs(rr)) / cj.norm_b;
+  cj.refreshes += 1;
+  cj.rel_true = rel;
+  int stop = 0;
+  if (rel < prm.tol) {
+    stop = kFlagConverged;
+  } else {
+    cj.stalls = (rel > 0.9 * cj.hist_rel) ? cj.stalls + 1 : 0;
+    if (cj.stalls >= 4) stop = kFlagStall;
+  }
+  cj.hist_rel = rel;
+  if (stop) {
+    cj.rel = rel;
+    cj.flags |= stop;
+    cj.state = kColDone;
+    reload[j] = 0;
+    return;
+  }
+  cj.state = kColActive;
+  cj.resumed = 1;
+  reload[j] = 1;
+}
+
+// ---- layout helpers -------------------------------------------------------------------------
+// Column-major n x k (leading dimension n) -> row-major npad x ldk, zero padded.
+__global__ void k_to_rows(int n, int k, int npad, int ldk, const double* __restrict__ a,
+                          double* __restrict__ b) {
+  const long long m = static_cast<long long>(npad) * ldk;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int row = static_cast<int>(e / ldk), c = static_cast<int>(e % ldk);
+    b[e] = (row < n && c < k) ? a[static_cast<size_t>(c) * n + row] : 0.0;
+  }
+}
+
+__global__ void k_to_cols(int n, int k, int ldk, const double* __restrict__ b,
+                          double* __restrict__ a) {
+  const long long m = static_cast<long long>(n) * k;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int c = static_cast<int>(e / n), row = static_cast<int>(e % n);
+    a[e] = b[static_cast<size_t>(row) * ldk + c];
+  }
+}
+
+// Entry-major BSR (vbh BsrDev) -> array-of-blocks, each block column-major
+// (entry (r, c) at c * B + r), so the SpMM reads one block column per row of X.
+template <int B>
+__global__ void k_bsr_to_aos(int nbr, const int* __restrict__ brp, const double* __restrict__ em,
+                             double* __restrict__ a64, float* __restrict__ a32) {
+  for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nbr; I += gridDim.x * blockDim.x) {
+    const int o = brp[I], nb = brp[I + 1] - o;
+    for (int q = 0; q < nb; ++q)
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          const double x = em[static_cast<size_t>(B * B) * o + static_cast<size_t>(r * B + c) * nb + q];
+          a64[static_cast<size_t>(o + q) * (B * B) + c * B + r] = x;
+          a32[static_cast<size_t>(o + q) * (B * B) + c * B + r] = static_cast<float>(x);
+        }
+  }
+}
+
+__global__ void k_fill_i32(int n, int* a, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
+// LDS-staged variant: at step t every group of the wave needs the t-th block
+// of its row. The wave fetches those GR blocks with a few coalesced loads
+// (each lane a distinct value) into its LDS slice, and each lane then reads
+// its group's block column from LDS. A direct load of a block column would
+// be broadcast to the LG lanes of a group but still occupy the address unit
+// for all 64 lanes; through LDS the address unit sees B*B*GR values once.
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
+
+template <typename T, int B, int CW, int LG, int MODE>
+__global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
+                                                      const T* __restrict__ X, T* __restrict__ Y,
+                                                      const double* __restrict__ F,
+                                                      const int* __restrict__ act,
+                                                      double* __restrict__ part) {
+  constexpr int KT = CW * LG, GR = 64 / LG, NB2 = B * B, NV = GR * NB2;
+  constexpr int NJ = (NV + 63) / 64;  // staged values per lane and step
+  __shared__ T sA[kBT / 64][NV];
+  __shared__ double lds[kBT * CW];
+  const int lane = threadIdx.x & 63;
+  const int lg = lane % LG, g = lane / LG;
+  const int c0 = blockIdx.y * KT + lg * CW;
+  if (MODE == 0) {
+    int on = 0;
+#pragma unroll
+    for (int w = 0; w < CW; ++w) on |= act[c0 + w];
+    if (!__syncthreads_or(on)) return;
+  }
+  const T* __restrict__ val = op_vals<T>(A);
+  const int wave = threadIdx.x >> 6;
+  T* __restrict__ stage = sA[wave];
+  double dot[CW];
+#pragma unroll
+  for (int w = 0; w < CW; ++w) dot[w] = 0.0;
+  int rb = blockIdx.x, nrbx = gridDim.x, row_lo = 0, row_hi = A.nbr;
+  if (xcd) {
+    const int x = blockIdx.x & 7, per = (A.nbr + 7) >> 3;
+    rb = blockIdx.x >> 3;
+    nrbx = gridDim.x >> 3;
+    row_lo = min(A.nbr, x * per);
+    row_hi = min(A.nbr, row_lo + per);
+  }
+  for (int I0 = row_lo + (rb * (kBT / 64) + wave) * GR; I0 < row_hi;
+       I0 += nrbx * (kBT / 64) * GR) {
+    const int I = I0 + g;
+    const bool row_ok = I < row_hi;
+    int o = 0, nb = 0;
+    if (row_ok) {
+      o = A.brp[I];
+      nb = A.brp[I + 1] - o;
+    }
+    // owner group of each staged slot: its first block and block count
+    int so[NJ], sn[NJ], se[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int f = lane + 64 * j;
+      const int gj = (f < NV) ? f / NB2 : 0;
+      se[j] = f - gj * NB2;
+      // every lane takes part in the shuffles (a lane reading from an
+      // inactive lane would get an undefined value)
+      so[j] = __shfl(o, gj * LG, 64);
+      const int nbj = __shfl(nb, gj * LG, 64);
+      sn[j] = (f < NV) ? nbj : 0;
+    }
+    int nbmax = nb;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nbmax = max(nbmax, __shfl_xor(nbmax, off, 64));
+    T acc[B][CW];
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int w = 0; w < CW; ++w) acc[r][w] = T(0);
+    for (int t = 0; t < nbmax; ++t) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int f = lane + 64 * j;
+        if (f < NV)
+          stage[f] = (t < sn[j]) ? val[static_cast<size_t>(so[j] + t) * NB2 + se[j]] : T(0);
+      }
+      wave_lds_sync();
+      if (t < nb) {
+        const int J = A.bci[o + t];
+        const T* __restrict__ xj = X + static_cast<size_t>(B * J) * ldk + c0;
+        const T* ag = stage + g * NB2;
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+          T x[CW], a[B];
+          vload<T, CW>(xj + static_cast<size_t>(s) * ldk, x);
+#pragma unroll
+          for (int r = 0; r < B; ++r) a[r] = ag[s * B + r];
+#pragma unroll
+          for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int w = 0; w < CW; ++w) acc[r][w] += a[r] * x[w];
+        }
+      }
+      wave_lds_sync();
+    }
+    if (row_ok) {
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
+        if constexpr (MODE == 0) {
+          T xo[CW];
+          vload<T, CW>(X + idx, xo);
+          vstore<T, CW>(Y + idx, acc[r]);
+#pragma unroll
+          for (int w = 0; w < CW; ++w)
+            dot[w] += static_cast<double>(xo[w]) * static_cast<double>(acc[r][w]);
+        } else {
+          double f[CW];
+          T y[CW];
+          vload<double, CW>(F + idx, f);
+#pragma unroll
+          for (int w = 0; w < CW; ++w) {
+            const double yy = f[w] - static_cast<double>(acc[r][w]);
+            y[w] = static_cast<T>(yy);
+            dot[w] += yy * yy;
+          }
+          vstore<T, CW>(Y + idx, y);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < CW; ++w) lds[threadIdx.x * CW + w] = dot[w];
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < KT) {
+    const int t = threadIdx.x, w = t % CW;
+    double sum = 0.0;
+    for (int th = t / CW; th < kBT; th += LG) sum += lds[th * CW + w];
+    part[static_cast<size_t>(blockIdx.y * KT + t) * kMaxGrid + blockIdx.x] = sum;
+  }
+}
+
+// ---- SpMM dispatch ----------------------------------------------------------------------------
+template <typename T, int B, int MODE>
+void spmm_b(const BlkTune& t, const BlkOp& A, int ldk, const T* X, T* Y, const double* F,
+            const int* act, double* part, hipStream_t s) {
+  const dim3 grid(t.nrb, ldk / (t.cw * t.lg));
+#define VBH_SPMM_CASE(CW_, LG_)                                                                 \
+  if (t.cw == CW_ && t.lg == LG_) {                                                             \
+    if (t.stage)                                                                                \
+      hipLaunchKernelGGL((k_blk_spmm_lds<T, B, CW_, LG_, MODE>), grid, dim3(kBT), 0, s, A, ldk, \
+                         t.xcd, X, Y, F, act, part);                                            \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_blk_spmm<T, B, CW_, LG_, MODE>), grid, dim3(kBT), 0, s, A, ldk,    \
+                         t.xcd, X, Y, F, act, part);                                            \
+    return;                                                                                     \
+  }
+  VBH_SPMM_CASE(1, 8)
+  VBH_SPMM_CASE(1, 16)
+  VBH_SPMM_CASE(1, 32)
+  VBH_SPMM_CASE(1, 64)
+  if constexpr (B > 1) {
+    VBH_SPMM_CASE(2, 4)
+    VBH_SPMM_CASE(2, 8)
+    VBH_SPMM_CASE(2, 16)
+    VBH_SPMM_CASE(2, 32)
+    VBH_SPMM_CASE(4, 2)
+    VBH_SPMM_CASE(4, 4)
+    VBH_SPMM_CASE(4, 8)
+    VBH_SPMM_CASE(4, 16)
+    VBH_SPMM_CASE(8, 1)
+    VBH_SPMM_CASE(8, 2)
+    VBH_SPMM_CASE(8, 4)
+    VBH_SPMM_CASE(8, 8)
+  }
+#undef VBH_SPMM_CASE
+  throw_error(VBH_ERR_HIP, "block SpMM: no kernel for the tuned configuration");
+}
+
+template <typename T, int MODE>
+void spmm(const BlkTune& t, const BlkOp& A, int ldk, const T* X, T* Y, const double* F,
+          const int* act, double* part, hipStream_t s) {
+  if (A.B == 6) spmm_b<T, 6, MODE>(t, A, ldk, X, Y, F, act, part, s);
+  else if (A.B == 3) spmm_b<T, 3, MODE>(t, A, ldk, X, Y, F, act, part, s);
+  else spmm_b<T, 1, MODE>(t, A, ldk, X, Y, F, act, part, s);
+}
+
+int pow2_at_least(int k) {
+  int p = 8;  // narrowest column tile with a kernel
+  while (p < k) p <<= 1;
+  return p;
+}
+
+int log2_of(int p) {
+  int l = 0;
+  while ((1 << l) < p) ++l;
+  return l;
+}
+
+// SpMM candidates for ldk columns and block size B: every (CW, LG) with a
+// kernel whose tile is the full width (or half of it with CW = 4), two grids.
+std::vector<BlkTune> candidates(int ldk, int B) {
+  const int kt = std::min(ldk, 64);
+  std::vector<BlkTune> c;
+  auto add = [&](int cw, int lg) {
+    for (int stage : {1, 0})
+      for (int xcd : {1, 0})
+        for (int nrb : {2048, 1024}) {
+          BlkTune t;
+          t.cw = cw;
+          t.lg = lg;
+          t.nrb = nrb;
+          t.xcd = xcd;
+          t.stage = stage;
+          c.push_back(t);
+        }
+  };
+  for (int cw : {2, 4, 8, 1}) {
+    if (B == 1 && cw > 1) continue;
+    const int lg = kt / cw;
+    if (lg >= 1 && lg <= 64 && (cw == 1 ? lg >= 8 : true)) add(cw, lg);
+  }
+  return c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+static BlockState& block_state(Ctx& c) {
+  if (!c.blk) c.blk.reset(new BlockState());
+  return *c.blk;
+}
+
+// Array-of-blocks copy of the operator (once per operator version).
+static void ensure_block_operator(Ctx& c, BlockState& b) {
+  if (b.op_version == c.operator_version) return;
+  hipStream_t s = c.stream;
+  b.drop_graph();
+  b.tuned.clear();
+  if (c.format == 3 || c.format == 6) {
+    const int B = c.format;
+    const size_t nv = static_cast<size_t>(c.bsrd.nnzb) * B * B;
+    b.aos64.resize(nv);
+    b.aos32.resize(nv);
+    const int grid = grid_for(c.bsrd.nbr, kBT);
+    if (B == 3)
+      hipLaunchKernelGGL(k_bsr_to_aos<3>, dim3(grid), dim3(kBT), 0, s, c.bsrd.nbr, c.bsrd.brp,
+                         c.bsrd.v64, b.aos64.p, b.aos32.p);
+    else
+      hipLaunchKernelGGL(k_bsr_to_aos<6>, dim3(grid), dim3(kBT), 0, s, c.bsrd.nbr, c.bsrd.brp,
+                         c.bsrd.v64, b.aos64.p, b.aos32.p);
+    b.op.B = B;
+    b.op.nbr = c.bsrd.nbr;
+    b.op.brp = c.bsrd.brp;
+    b.op.bci = c.bsrd.bci;
+    b.op.v64 = b.aos64.p;
+    b.op.v32 = b.aos32.p;
+  } else {
+    // scalar CSR: blocks of 1 x 1, values already in array-of-blocks order
+    b.aos64.release();
+    b.aos32.release();
+    b.op.B = 1;
+    b.op.nbr = c.csr.n;
+    b.op.brp = c.csr.rp;
+    b.op.bci = c.csr.ci;
+    b.op.v64 = c.csr.v64;
+    b.op.v32 = c.csr.v32;
+  }
+  VBH_HIP(hipStreamSynchronize(s));
+  b.op_version = c.operator_version;
+}
+
+static void ensure_block_work(Ctx& c, BlockState& b, int k, int precision) {
+  const int ldk = pow2_at_least(k);
+  const int npad = c.n_pad;
+  if (!(b.ldk == ldk && b.npad == npad && b.work_version == c.operator_version)) {
+    const size_t m = static_cast<size_t>(npad) * ldk;
+    b.F.resize(m);
+    b.u.resize(m);
+    b.r64.resize(m);
+    b.rd.release();
+    b.pd.release();
+    b.Apd.release();
+    b.rf.release();
+    b.pf.release();
+    b.Apf.release();
+    b.partA.resize(static_cast<size_t>(ldk) * kMaxGrid);
+    b.partB.resize(static_cast<size_t>(3) * ldk * kMaxGrid);
+    b.partR.resize(static_cast<size_t>(ldk) * kMaxGrid);
+    b.coefA.resize(static_cast<size_t>(ldk));
+    b.coefB.resize(static_cast<size_t>(ldk));
+    b.actD.resize(static_cast<size_t>(ldk));
+    b.actU.resize(static_cast<size_t>(ldk));
+    b.reload.resize(static_cast<size_t>(ldk));
+    b.ones.resize(static_cast<size_t>(ldk));
+    b.cols.resize(static_cast<size_t>(ldk));
+    if (b.cols_host) (void)hipHostFree(b.cols_host);
+    b.cols_host = nullptr;
+    VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.cols_host), sizeof(BlkCol) * ldk, 0));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c.stream, ldk, b.ones.p, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c.stream, ldk, b.reload.p, 0);
+    b.ldk = ldk;
+    b.lg2 = log2_of(ldk);
+    b.npad = npad;
+    b.work_version = c.operator_version;
+    b.nvec = grid_for(static_cast<long long>(m), kBT * 8);
+    b.drop_graph();
+  }
+  const size_t m = static_cast<size_t>(npad) * ldk;
+  if (precision == VBH_PREC_FP64 && !b.rd.p) {
+    b.rd.resize(m);
+    b.pd.resize(m);
+    b.Apd.resize(m);
+  }
+  if (precision == VBH_PREC_MIXED && !b.rf.p) {
+    b.rf.resize(m);
+    b.pf.resize(m);
+    b.Apf.resize(m);
+  }
+  b.k = k;
+}
+
+// Autotune: time every candidate SpMM configuration for this operator, ldk
+// and precision on the current work vectors; keep the fastest.
+static BlkTune tune(Ctx& c, BlockState& b, int precision) {
+  const auto key = std::make_pair(b.ldk, precision);
+  auto it = b.tuned.find(key);
+  if (it != b.tuned.end()) return it->second;
+  std::vector<BlkTune> cand = candidates(b.ldk, b.op.B);
+  // VBH_BLOCK_TUNE=0 takes the first candidate; "cw,lg,nrb[,xcd[,stage]]" forces one.
+  const char* env = std::getenv("VBH_BLOCK_TUNE");
+  if (env && env[0] == '0') cand.resize(1);
+  if (env && std::strchr(env, ',')) {
+    BlkTune f;
+    if (std::sscanf(env, "%d,%d,%d,%d,%d", &f.cw, &f.lg, &f.nrb, &f.xcd, &f.stage) >= 3) {
+      bool known = false;
+      for (const auto& t : cand) known |= (t.cw == f.cw && t.lg == f.lg);
+      if (!known || f.nrb < 8 || f.nrb > kMaxGrid || f.nrb % 8)
+        throw_error(VBH_ERR_ARG, "VBH_BLOCK_TUNE: no such SpMM configuration for this width");
+      cand.assign(1, f);
+    }
+  }
+  hipStream_t s = c.stream;
+  // inputs: X = row-major F (finite values), output into the Ap buffer
+  auto run = [&](const BlkTune& t) {
+    if (precision == VBH_PREC_FP64)
+      spmm<double, 0>(t, b.op, b.ldk, b.F.p, b.Apd.p, nullptr, b.ones.p, b.partA.p, s);
+    else
+      spmm<float, 0>(t, b.op, b.ldk, b.pf.p, b.Apf.p, nullptr, b.ones.p, b.partA.p, s);
+  };
+  if (precision == VBH_PREC_MIXED) {
+    const long long m = static_cast<long long>(b.npad) * b.ldk;
+    hipLaunchKernelGGL(k_blk_init<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
+                       c.diag32.p, b.u.p, b.pf.p, b.Apf.p, b.partB.p);
+  }
+  const char* log_env = std::getenv("VBH_BLOCK_TUNE_LOG");
+  const bool log = log_env && log_env[0] == '1';
+  BlkTune best;
+  best.ms = 1e300;
+  for (BlkTune t : cand) {
+    run(t);  // warm-up (instruction cache, L2)
+    VBH_HIP(hipEventRecord(c.ev_start, s));
+    const int reps = 3;
+    for (int i = 0; i < reps; ++i) run(t);
+    VBH_HIP(hipEventRecord(c.ev_stop, s));
+    VBH_HIP(hipEventSynchronize(c.ev_stop));
+    float ms = 0.0f;
+    VBH_HIP(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
+    t.ms = static_cast<double>(ms) / reps;
+    if (log)
+      std::fprintf(stderr,
+                   "vbh block tune: B=%d ldk=%d prec=%d cw=%d lg=%d nrb=%d xcd=%d stage=%d  %.4f ms\n",
+                   b.op.B, b.ldk, precision, t.cw, t.lg, t.nrb, t.xcd, t.stage, t.ms);
+    if (t.ms < best.ms) best = t;
+  }
+  b.tuned[key] = best;
+  return best;
+}
+
+static void enqueue_block_iterations(Ctx& c, BlockState& b, int precision, int count) {
+  hipStream_t s = c.stream;
+  const long long m = static_cast<long long>(b.npad) * b.ldk;
+  const PcgParams prm = c.params();
+  const int mixed = precision == VBH_PREC_MIXED;
+  for (int i = 0; i < count; ++i) {
+    if (!mixed) {
+      hipLaunchKernelGGL(k_blk_direction<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
+                         c.diag64.p, b.rd.p, b.pd.p, b.coefB.p, b.actD.p);
+      spmm<double, 0>(b.cur, b.op, b.ldk, b.pd.p, b.Apd.p, nullptr, b.actD.p, b.partA.p, s);
+    } else {
+      hipLaunchKernelGGL(k_blk_direction<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
+                         c.diag32.p, b.rf.p, b.pf.p, b.coefB.p, b.actD.p);
+      spmm<float, 0>(b.cur, b.op, b.ldk, b.pf.p, b.Apf.p, nullptr, b.actD.p, b.partA.p, s);
+    }
+    hipLaunchKernelGGL(k_blk_scal_alpha, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partA.p,
+                       b.cur.nrb, b.cols.p, b.coefA.p, b.actU.p, b.actD.p);
+    if (!mixed)
+      hipLaunchKernelGGL(k_blk_update<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
+                         c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.Apd.p, b.coefA.p, b.actU.p,
+                         b.partB.p);
+    else
+      hipLaunchKernelGGL(k_blk_update<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
+                         c.diag32.p, b.u.p, b.rf.p, b.pf.p, b.Apf.p, b.coefA.p, b.actU.p,
+                         b.partB.p);
+    hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 1, b.ldk, b.k, mixed,
+                       b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
+  }
+}
+
+static hipGraphExec_t block_graph(Ctx& c, BlockState& b, int precision) {
+  if (b.graph && b.g_prec == precision && b.g_ipg == c.iters_per_graph && b.g_tol == c.tol &&
+      b.g_max_iters == c.max_iters && b.g_delta == c.refresh_delta && b.g_cw == b.cur.cw &&
+      b.g_lg == b.cur.lg && b.g_nrb == b.cur.nrb && b.g_xcd == b.cur.xcd &&
+      b.g_stage == b.cur.stage)
+    return b.graph;
+  b.drop_graph();
+  hipGraph_t graph = nullptr;
+  VBH_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  enqueue_block_iterations(c, b, precision, c.iters_per_graph);
+  VBH_HIP(hipStreamEndCapture(c.stream, &graph));
+  VBH_HIP(hipGraphInstantiate(&b.graph, graph, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(graph);
+  b.g_prec = precision;
+  b.g_ipg = c.iters_per_graph;
+  b.g_tol = c.tol;
+  b.g_max_iters = c.max_iters;
+  b.g_delta = c.refresh_delta;
+  b.g_cw = b.cur.cw;
+  b.g_lg = b.cur.lg;
+  b.g_nrb = b.cur.nrb;
+  b.g_xcd = b.cur.xcd;
+  b.g_stage = b.cur.stage;
+  return b.graph;
+}
+
+static void read_cols(Ctx& c, BlockState& b) {
+  VBH_HIP(hipMemcpyAsync(b.cols_host, b.cols.p, sizeof(BlkCol) * b.ldk, hipMemcpyDeviceToHost,
+                         c.stream));
+  VBH_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Runs the block solve on b.F (row-major) into b.u.
+static void run_block_solve(Ctx& c, BlockState& b, int precision) {
+  hipStream_t s = c.stream;
+  const long long m = static_cast<long long>(b.npad) * b.ldk;
+  const PcgParams prm = c.params();
+  const int mixed = precision == VBH_PREC_MIXED;
+  b.cur = tune(c, b, precision);
+  VBH_HIP(hipEventRecord(c.ev_start, s));
+  if (!mixed)
+    hipLaunchKernelGGL(k_blk_init<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
+                       c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.partB.p);
+  else
+    hipLaunchKernelGGL(k_blk_init<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
+                       c.diag32.p, b.u.p, b.rf.p, b.pf.p, b.partB.p);
+  hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 0, b.ldk, b.k, mixed,
+                     b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
+  const char* ge_env = std::getenv("VBH_USE_GRAPH");
+  const bool use_graph = !(ge_env && ge_env[0] == '0');
+  hipGraphExec_t ge = use_graph ? block_graph(c, b, precision) : nullptr;
+  // every pass either advances an active column or refreshes a paused one
+  const long long cap = 4ll * (static_cast<long long>(c.max_iters) + 2) + 64;
+  long long passes = 0;
+  for (;;) {
+    read_cols(c, b);
+    int active = 0, paused = 0;
+    for (int j = 0; j < b.ldk; ++j) {
+      active += b.cols_host[j].state == kColActive;
+      paused += b.cols_host[j].state == kColPaused;
+    }
+    if (!active && !paused) break;
+    if (++passes > cap) throw_error(VBH_ERR_HIP, "HIP block PCG: device state stopped advancing");
+    if (paused) {
+      spmm<double, 1>(b.cur, b.op, b.ldk, b.u.p, b.r64.p, b.F.p, b.ones.p, b.partR.p, s);
+      hipLaunchKernelGGL(k_blk_scal_refresh, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partR.p,
+                         b.cur.nrb, b.cols.p, b.reload.p, prm);
+      hipLaunchKernelGGL(k_blk_reload, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, c.diag32.p,
+                         b.r64.p, b.rf.p, b.reload.p, b.partB.p);
+      hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 2, b.ldk, b.k, mixed,
+                         b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
+      continue;
+    }
+    if (use_graph) VBH_HIP(hipGraphLaunch(ge, s));
+    else enqueue_block_iterations(c, b, precision, c.iters_per_graph);
+  }
+  VBH_HIP(hipEventRecord(c.ev_stop, s));
+  VBH_HIP(hipEventSynchronize(c.ev_stop));
+  float ms = 0.0f;
+  VBH_HIP(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
+  c.stats.solve_ms = ms;
+}
+
+static double column_rel(const BlkCol& cj, int precision) {
+  if (cj.flags & kFlagZeroRhs) return 0.0;
+  if (precision == VBH_PREC_MIXED && (cj.flags & (kFlagConverged | kFlagStall))) return cj.rel_true;
+  return cj.rel;
+}
+
+static void block_solve(Ctx& c, int k, const double* F, double* U, bool device_ptrs, int* iters,
+                        double* rel_res) {
+  if (!c.have_matrix) throw_error(VBH_ERR_STATE, "vbh_solve_block: no matrix set");
+  if (k <= 0 || k > 64) throw_error(VBH_ERR_ARG, "vbh_solve_block: k must be in 1..64");
+  if (!F || !U) throw_error(VBH_ERR_ARG, "vbh_solve_block: null array");
+  c.select_stream_device();
+  BlockState& b = block_state(c);
+  const int precision = c.precision;
+  ensure_block_operator(c, b);
+  ensure_block_work(c, b, k, precision);
+  hipStream_t s = c.stream;
+  const size_t bytes = static_cast<size_t>(c.n) * k * sizeof(double);
+  DBuf<double> stage;
+  const double* dF = F;
+  if (!device_ptrs) {
+    stage.resize(static_cast<size_t>(c.n) * k);
+    VBH_HIP(hipMemcpyAsync(stage.p, F, bytes, hipMemcpyHostToDevice, s));
+    dF = stage.p;
+  }
+  const long long m = static_cast<long long>(b.npad) * b.ldk;
+  hipLaunchKernelGGL(k_to_rows, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, s, c.n, k, b.npad, b.ldk,
+                     dF, b.F.p);
+  run_block_solve(c, b, precision);
+  double* dU = device_ptrs ? U : stage.p;
+  hipLaunchKernelGGL(k_to_cols, dim3(grid_for(static_cast<long long>(c.n) * k, kBT * 4)), dim3(kBT),
+                     0, s, c.n, k, b.ldk, b.u.p, dU);
+  if (!device_ptrs) VBH_HIP(hipMemcpyAsync(U, stage.p, bytes, hipMemcpyDeviceToHost, s));
+  read_cols(c, b);
+  int max_it = 0, refr = 0;
+  double max_rel = 0.0;
+  std::string failure;
+  for (int j = 0; j < k; ++j) {
+    const BlkCol& cj = b.cols_host[j];
+    const double rel = column_rel(cj, precision);
+    if (iters) iters[j] = cj.iters;
+    if (rel_res) rel_res[j] = rel;
+    max_it = std::max(max_it, cj.iters);
+    refr += cj.refreshes;
+    max_rel = std::max(max_rel, rel);
+    if (failure.empty()) {
+      if (cj.flags & kFlagRz0)
+        failure = "HIP PCG: load case " + std::to_string(j) + ": non-positive initial r*z=" +
+                  fmt_g(cj.rz) + " -- preconditioner is not SPD or matrix is singular";
+      else if (cj.flags & (kFlagBreakdown | kFlagMaxIter))
+        failure = "HIP PCG: load case " + std::to_string(j) + ": did not converge after " +
+                  std::to_string(c.max_iters) + " iterations (relative residual " +
+                  fmt_g((cj.flags & kFlagBreakdown) ? 1e300 : cj.rel) + " > tolerance " +
+                  fmt_g(c.tol) +
+                  "). Consider increasing max iterations or using a direct solver.";
+    }
+  }
+  c.stats.iterations = max_it;
+  c.stats.refinements = refr;
+  c.stats.rel_res = max_rel;
+  c.stats.format = c.format;
+  c.last_iters = max_it;
+  c.last_rel = max_rel;
+  if (!failure.empty()) throw_error(VBH_ERR_SOLVER, failure);
+}
+
+}  // namespace vbh
+
 extern "C" {
-int vbh_solve_block(vbh_ctx*, int, const double*, double*, int*, double*) {
-  vbh::set_error("vbh_solve_block: block PCG not available in this build");
-  return VBH_ERR_STATE;
+
+int vbh_solve_block(vbh_ctx* h, int k, const double* F, double* U, int* iters, double* rel_res) {
+  if (!h) {
+    vbh::set_error("vbh_solve_block: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { vbh::block_solve(*c, k, F, U, false, iters, rel_res); });
 }
-int vbh_solve_block_device(vbh_ctx*, int, const double*, double*, int*, double*) {
-  vbh::set_error("vbh_solve_block_device: block PCG not available in this build");
-  return VBH_ERR_STATE;
+
+int vbh_solve_block_device(vbh_ctx* h, int k, const double* dF, double* dU, int* iters,
+                           double* rel_res) {
+  if (!h) {
+    vbh::set_error("vbh_solve_block_device: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { vbh::block_solve(*c, k, dF, dU, true, iters, rel_res); });
 }
+
+int vbh_last_block_config(const vbh_ctx* h, vbh_block_config* out) {
+  if (!h || !out) {
+    vbh::set_error("vbh_last_block_config: null argument");
+    return VBH_ERR_ARG;
+  }
+  const auto* c = reinterpret_cast<const vbh::Ctx*>(h);
+  if (!c->blk) {
+    vbh::set_error("vbh_last_block_config: no block solve yet");
+    return VBH_ERR_STATE;
+  }
+  out->col_tile = c->blk->cur.cw * c->blk->cur.lg;
+  out->cols_per_lane = c->blk->cur.cw;
+  out->lanes_per_group = c->blk->cur.lg;
+  out->row_blocks = c->blk->cur.nrb;
+  out->xcd_rows = c->blk->cur.xcd;
+  out->lds_staged = c->blk->cur.stage;
+  out->ldk = c->blk->ldk;
+  out->spmm_ms = c->blk->cur.ms;
+  return VBH_OK;
 }
+
+}  // extern "C"

@@ -1,11 +1,15 @@
 // vibestran_amd/csrc/vbh_ctx.hpp -- the context behind vbh_ctx*.
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "vbh_internal.hpp"
 
 namespace vbh {
+
+struct BlockState;  // block-PCG work space (vbh_block.hip)
+void destroy_block_state(BlockState* b);
 
 struct GraphSlot {
   hipGraphExec_t exec = nullptr;
@@ -67,6 +71,9 @@ struct Ctx {
   DBuf<PcgScalars> st_dev;
   PcgScalars* st_host = nullptr;
   GraphSlot graphs[2];
+
+  // block PCG over several load cases
+  std::unique_ptr<BlockState, void (*)(BlockState*)> blk{nullptr, destroy_block_state};
 
   // diagnostics (SolverBackend::last_iteration_count / last_estimated_error)
   int last_iters = 0;

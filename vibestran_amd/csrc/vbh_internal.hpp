@@ -19,6 +19,7 @@ struct Error {
 };
 void set_error(const std::string& msg);
 [[noreturn]] void throw_error(int code, const std::string& msg);
+std::string fmt_g(double v);  // printf %g
 
 template <typename F>
 int guarded(F&& f) {

@@ -36,12 +36,6 @@ template <>
 __device__ __forceinline__ const double* vals<double>(const CsrDev& A) { return A.v64; }
 template <>
 __device__ __forceinline__ const float* vals<float>(const CsrDev& A) { return A.v32; }
-template <typename T>
-__device__ __forceinline__ const T* bvals(const BsrDev& A);
-template <>
-__device__ __forceinline__ const double* bvals<double>(const BsrDev& A) { return A.v64; }
-template <>
-__device__ __forceinline__ const float* bvals<float>(const BsrDev& A) { return A.v32; }
 
 // ---- deterministic reductions ---------------------------------------------------
 template <int NV>
