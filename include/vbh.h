@@ -122,6 +122,23 @@ int vbh_assemble(vbh_ctx* ctx, int n, int nnodes, const double* xyz, const int* 
                  const double* nu, const double* thickness, int solid_form, int shell_form);
 int vbh_last_assembly_stats(const vbh_ctx* ctx, vbh_assembly_stats* out);
 
+/* ---- adjoint sensitivities (no reference counterpart) ----------------------
+ * For every element e of the mesh given to the last vbh_assemble:
+ *   out[e] = -u_e^T (dK_e / dp_e) v_e,  p = E_e (param 0) or the CQUAD4
+ *   thickness t_e (param 1; solid elements give 0),
+ * with u_e, v_e the element's entries of the length-n vectors u, v
+ * (constrained DOFs contribute 0). K_e is linear in E; for CQUAD4 membrane and
+ * transverse shear scale with t, bending with t^3 and the drilling penalty
+ * with the rotational diagonal entry it is taken from. Compliance C = F^T u
+ * (K u = F, F independent of p): dC/dp_e = out with v = u. A functional
+ * L^T u: solve K lambda = L, then d(L^T u)/dp_e = out with v = lambda.
+ * out has nelem entries in element order. */
+int vbh_element_sensitivity(vbh_ctx* ctx, int param, const double* u, const double* v,
+                            double* out);
+/* Same with device-resident u, v and out. */
+int vbh_element_sensitivity_device(vbh_ctx* ctx, int param, const double* du, const double* dv,
+                                   double* dout);
+
 /* ---- solve: SolverBackend::solve(K, F) -> u (solver_backend.hpp:26-29) ----
  * F and u are host arrays of length n. iters / rel_res may be NULL. */
 int vbh_solve(vbh_ctx* ctx, const double* F, double* u, int* iters, double* rel_res);

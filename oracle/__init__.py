@@ -79,6 +79,8 @@ def _load():
         "orc_quad4_ke": (I, [DP, D, D, D, D, D, D, I, DP]),
         "orc_quad4_me": (I, [DP, D, D, DP]),
         "orc_assemble_model": (P, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I]),
+        "orc_quad4_dke_dt": (I, [DP, D, D, D, D, D, D, I, DP]),
+        "orc_element_sensitivity": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I, I, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -202,6 +204,11 @@ def penta6_me(xyz, rho):
 
 def quad4_ke(xyz, E, nu, t, formulation="MITC4", G=0.0, twelveI_t3=1.0, tst=0.833333):
     return _elem(lib().orc_quad4_ke, xyz, E, nu, G, t, twelveI_t3, tst,
+                 1 if formulation == "MINDLIN" else 0, nd=24)
+
+
+def quad4_dke_dt(xyz, E, nu, t, formulation="MITC4", G=0.0, twelveI_t3=1.0, tst=0.833333):
+    return _elem(lib().orc_quad4_dke_dt, xyz, E, nu, G, t, twelveI_t3, tst,
                  1 if formulation == "MINDLIN" else 0, nd=24)
 
 
@@ -331,6 +338,23 @@ def assemble_model(n, model, eq):
     if not h:
         raise OracleError(_err())
     return Csr(h)
+
+
+def element_sensitivity(n, model, eq, param, u, v):
+    """out[e] = -u_e^T (dK_e/dp_e) v_e with p = E (param 0) or CQUAD4
+    thickness (param 1); the FP64 CPU adjoint contraction."""
+    et, ep, cn = _i32(model.elem_type), _i32(model.elem_ptr), _i32(model.conn)
+    x, q = _f64(model.xyz).reshape(-1), _i32(eq).reshape(-1)
+    e, nv = _f64(model.E), _f64(model.nu)
+    th = _f64(model.thickness if model.thickness is not None else np.zeros(len(et)))
+    out = np.empty(len(et), dtype=np.float64)
+    rc = lib().orc_element_sensitivity(n, len(et), _ip(et), _ip(ep), _ip(cn), _dp(x), _ip(q),
+                                       _dp(e), _dp(nv), _dp(th), int(model.solid_form),
+                                       int(model.shell_form), int(param), _dp(_f64(u)),
+                                       _dp(_f64(v)), _dp(out))
+    if rc != 0:
+        raise OracleError(_err())
+    return out
 
 
 # ---------------------------------------------------------------- PCG

@@ -802,6 +802,59 @@ OrcCsr* orc_assemble_model(int n, int nelem, const int* et, const int* ep, const
   return orc_builder_build_csr(&b);
 }
 
+int orc_element_sensitivity(int n, int nelem, const int* et, const int* ep, const int* conn,
+                            const double* xyz, const int* eq, const double* E, const double* nu,
+                            const double* thick, int solid_form, int shell_form, int param,
+                            const double* u, const double* v, double* out) {
+  (void)n;
+  std::vector<double> ke(576), xe(24);
+  std::vector<int> gd(24);
+  for (int e = 0; e < nelem; ++e) {
+    const int nn = ep[e + 1] - ep[e];
+    const int dpn = (et[e] == ORC_CQUAD4) ? 6 : 3;
+    for (int a = 0; a < nn; ++a) {
+      const int node = conn[ep[e] + a];
+      for (int d = 0; d < 3; ++d) xe[static_cast<size_t>(3 * a + d)] = xyz[3 * node + d];
+      for (int d = 0; d < dpn; ++d) gd[static_cast<size_t>(dpn * a + d)] = eq[6 * node + d];
+    }
+    int rc = ORC_OK;
+    double scale = 1.0;
+    if (param == 1 && et[e] != ORC_CQUAD4) {
+      out[e] = 0.0;
+      continue;
+    }
+    if (et[e] == ORC_CHEXA8)
+      rc = solid_form == 1 ? orc_hexa8_ke_sri(xe.data(), E[e], nu[e], ke.data())
+                           : orc_hexa8_ke_eas(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CTETRA4)
+      rc = orc_tetra4_ke(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CPENTA6)
+      rc = orc_penta6_ke_eas(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CQUAD4 && param == 0)
+      rc = orc_quad4_ke(xe.data(), E[e], nu[e], 0.0, thick[e], 1.0, 0.833333, shell_form, ke.data());
+    else if (et[e] == ORC_CQUAD4)
+      rc = orc_quad4_dke_dt(xe.data(), E[e], nu[e], 0.0, thick[e], 1.0, 0.833333, shell_form,
+                            ke.data());
+    else
+      rc = fail(ORC_ERR_ARG, "orc_element_sensitivity: unsupported element type");
+    if (rc != ORC_OK) return rc;
+    if (param == 0) scale = 1.0 / E[e];  // K_e is linear in E
+    const int nd = dpn * nn;
+    double sum = 0.0;
+    for (int i = 0; i < nd; ++i) {
+      const int gi = gd[static_cast<size_t>(i)];
+      if (gi < 0) continue;
+      for (int j = 0; j < nd; ++j) {
+        const int gj = gd[static_cast<size_t>(j)];
+        if (gj < 0) continue;
+        sum += u[gi] * ke[static_cast<size_t>(i * nd + j)] * v[gj];
+      }
+    }
+    out[e] = -sum * scale;
+  }
+  return ORC_OK;
+}
+
 OrcCsr* orc_assemble_solids(int n, int nelem, const int* et, const int* ep, const int* conn,
                             const double* xyz, const int* eq, const double* E, const double* nu,
                             int formulation) {

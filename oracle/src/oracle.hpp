@@ -121,6 +121,11 @@ extern "C" {
 // (default), 1 Mindlin. G <= 0 selects E/(2(1+nu)). Output row-major 24x24.
 int orc_quad4_ke(const double* xyz12, double E, double nu, double G, double t,
                  double twelveI_t3, double tst, int formulation, double* ke576);
+// Analytic dK_e/dt of orc_quad4_ke (adjoint sensitivities; no reference
+// counterpart): membrane and transverse shear are linear in t, bending cubic,
+// the drilling penalty follows the rotational diagonal entry it is taken from.
+int orc_quad4_dke_dt(const double* xyz12, double E, double nu, double G, double t,
+                     double twelveI_t3, double tst, int formulation, double* dke576);
 // compute_shell_mass (cquad4.cpp:672-708): rho*t translational, rho*t^3/12
 // rotary, tiny drilling inertia.
 int orc_quad4_me(const double* xyz12, double rho, double t, double* me576);
@@ -130,4 +135,12 @@ OrcCsr* orc_assemble_model(int n, int nelem, const int* elem_type, const int* el
                            const int* conn, const double* xyz, const int* eq, const double* E,
                            const double* nu, const double* thickness, int solid_form,
                            int shell_form);
+// FP64 CPU adjoint contraction (no reference counterpart): out[e] =
+// -u_e^T (dK_e/dp_e) v_e, p = E (param 0) or CQUAD4 thickness (param 1;
+// solids 0). Compliance: v = u; a functional L^T u: v = lambda, K lambda = L.
+int orc_element_sensitivity(int n, int nelem, const int* elem_type, const int* elem_ptr,
+                            const int* conn, const double* xyz, const int* eq, const double* E,
+                            const double* nu, const double* thickness, int solid_form,
+                            int shell_form, int param, const double* u, const double* v,
+                            double* out);
 }

@@ -261,10 +261,30 @@ void btdb(const double (&B)[M][NC], const double (&D)[M][M], double w, double (&
 
 extern "C" void orc_internal_set_error(const char* msg);
 
+namespace {
+// deriv 0: K_e; deriv 1: dK_e/dt (membrane and shear scale with t, bending
+// with t^3, the drilling penalty with the rotational diagonal it is taken from).
+int quad4_ke_impl(const double* X, double E, double nu, double G, double t, double twelveI_t3,
+                  double tst, int form, int deriv, double* Ke_out);
+}  // namespace
+
 extern "C" {
 
 int orc_quad4_ke(const double* X, double E, double nu, double G, double t, double twelveI_t3,
                  double tst, int form, double* Ke_out) {
+  return quad4_ke_impl(X, E, nu, G, t, twelveI_t3, tst, form, 0, Ke_out);
+}
+
+int orc_quad4_dke_dt(const double* X, double E, double nu, double G, double t, double twelveI_t3,
+                     double tst, int form, double* dKe_out) {
+  return quad4_ke_impl(X, E, nu, G, t, twelveI_t3, tst, form, 1, dKe_out);
+}
+
+}  // extern "C"
+
+namespace {
+int quad4_ke_impl(const double* X, double E, double nu, double G, double t, double twelveI_t3,
+                  double tst, int form, int deriv, double* Ke_out) {
   // frame
   const V3 g[4] = {{X[0], X[1], X[2]}, {X[3], X[4], X[5]}, {X[6], X[7], X[8]}, {X[9], X[10], X[11]}};
   const V3 v12 = sub(g[1], g[0]), v14 = sub(g[3], g[0]);
@@ -304,6 +324,7 @@ int orc_quad4_ke(const double* X, double E, double nu, double G, double t, doubl
     mitc_shear_data(xl, yl, ms);
   }
   double Kl[24][24] = {};
+  double dKl[24][24] = {};  // d/dt of the membrane, bending and shear parts
   static const int MEM[2] = {0, 1}, PLT[3] = {3, 4, 2};
   const double gp = 1.0 / std::sqrt(3.0), gg[2] = {-gp, gp};
   for (int gi = 0; gi < 2; ++gi)
@@ -330,6 +351,13 @@ int orc_quad4_ke(const double* X, double E, double nu, double G, double t, doubl
             for (int b = 0; b < 3; ++b) Kl[6 * i + PLT[a]][6 * j + PLT[b]] += Kbb[3 * i + a][3 * j + b];
           for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) Kl[6 * i + PLT[a]][6 * j + PLT[b]] += Kss[3 * i + a][3 * j + b];
+          for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+              dKl[6 * i + MEM[a]][6 * j + MEM[b]] += Kmm[2 * i + a][2 * j + b] / t;
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              dKl[6 * i + PLT[a]][6 * j + PLT[b]] +=
+                  (3.0 * Kbb[3 * i + a][3 * j + b] + Kss[3 * i + a][3 * j + b]) / t;
         }
     }
   if (form == 1 && std::fabs(A[2][2]) > 0.0) {
@@ -341,7 +369,10 @@ int orc_quad4_ke(const double* X, double E, double nu, double G, double t, doubl
     for (int i = 0; i < 4; ++i)
       for (int j = 0; j < 4; ++j)
         for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b) Kl[6 * i + MEM[a]][6 * j + MEM[b]] += Kxy[2 * i + a][2 * j + b];
+          for (int b = 0; b < 2; ++b) {
+            Kl[6 * i + MEM[a]][6 * j + MEM[b]] += Kxy[2 * i + a][2 * j + b];
+            dKl[6 * i + MEM[a]][6 * j + MEM[b]] += Kxy[2 * i + a][2 * j + b] / t;
+          }
   }
   double maxd = 0.0;
   for (int i = 0; i < 24; ++i) maxd = std::max(maxd, std::fabs(Kl[i][i]));
@@ -350,12 +381,24 @@ int orc_quad4_ke(const double* X, double E, double nu, double G, double t, doubl
     return ORC_ERR_SOLVER;
   }
   double maxrot = 0.0;
-  for (int i = 0; i < 4; ++i) {
-    maxrot = std::max(maxrot, std::fabs(Kl[6 * i + 3][6 * i + 3]));
-    maxrot = std::max(maxrot, std::fabs(Kl[6 * i + 4][6 * i + 4]));
-  }
+  int krot = -1, kmax = 0;
+  for (int i = 0; i < 24; ++i)
+    if (std::fabs(Kl[i][i]) > std::fabs(Kl[kmax][kmax])) kmax = i;
+  for (int i = 0; i < 4; ++i)
+    for (int c = 3; c <= 4; ++c)
+      if (std::fabs(Kl[6 * i + c][6 * i + c]) > maxrot) {
+        maxrot = std::fabs(Kl[6 * i + c][6 * i + c]);
+        krot = 6 * i + c;
+      }
   const double drill = (maxrot > 0.0) ? 1e-6 * maxrot : 1e-12 * maxd;
   for (int i = 0; i < 4; ++i) Kl[6 * i + 5][6 * i + 5] += drill;
+  if (deriv) {
+    const int k = (maxrot > 0.0) ? krot : kmax;
+    const double sg = (Kl[k][k] < 0.0) ? -1.0 : 1.0;
+    const double ddrill = ((maxrot > 0.0) ? 1e-6 : 1e-12) * sg * dKl[k][k];
+    for (int i = 0; i < 4; ++i) dKl[6 * i + 5][6 * i + 5] += ddrill;
+    std::memcpy(Kl, dKl, sizeof Kl);
+  }
   // T^T Kl T with T = blockdiag(R, Rr) per node
   double T[24][24] = {};
   for (int n = 0; n < 4; ++n)
@@ -379,6 +422,9 @@ int orc_quad4_ke(const double* X, double E, double nu, double G, double t, doubl
     }
   return ORC_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int orc_quad4_me(const double* X, double rho, double t, double* Me_out) {
   const V3 g[4] = {{X[0], X[1], X[2]}, {X[3], X[4], X[5]}, {X[6], X[7], X[8]}, {X[9], X[10], X[11]}};

@@ -71,6 +71,8 @@ SIGNATURES = {
     "vbh_dof_map": (_I, [_P, _I, _IP, _I, _IP, _IP, _IP, _I, _IP, _IP, _IP, _IP]),
     "vbh_assemble": (_I, [_P, _I, _I, _DP, _IP, _I, _IP, _IP, _IP, _DP, _DP, _DP, _I, _I]),
     "vbh_last_assembly_stats": (_I, [_P, C.POINTER(vbh_assembly_stats)]),
+    "vbh_element_sensitivity": (_I, [_P, _I, _DP, _DP, _DP]),
+    "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),
     "vbh_solve": (_I, [_P, _DP, _DP, _IP, _DP]),
     "vbh_solve_device": (_I, [_P, _P, _P, _IP, _DP]),
     "vbh_solve_block": (_I, [_P, _I, _DP, _DP, _IP, _DP]),

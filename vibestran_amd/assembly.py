@@ -81,3 +81,32 @@ def assembly_stats(backend) -> L.vbh_assembly_stats:
     s = L.vbh_assembly_stats()
     _check(L.load().vbh_last_assembly_stats(backend._h, C.byref(s)))
     return s
+
+
+def element_sensitivity(backend, model: Model, param: int, u, v) -> np.ndarray:
+    """out[e] = -u_e^T (dK_e/dp_e) v_e over the mesh of the last
+    ``assemble_model`` on ``backend``: p = E (param 0) or CQUAD4 thickness
+    (param 1). Runs the GPU adjoint kernel (no reference counterpart)."""
+    lib = L.load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty(model.ne, dtype=np.float64)
+    _check(lib.vbh_element_sensitivity(backend._h, int(param), L.dp(u), L.dp(v), L.dp(out)))
+    return out
+
+
+def compliance_gradients(backend, model: Model, eq, n, F, tip_dof=None):
+    """Compliance C = F^T u and dC/dE_e; with ``tip_dof`` also u[tip_dof] and
+    d u[tip_dof] / dE_e from the adjoint solve K lambda = e_tip. Both load
+    vectors go through one block solve. Returns (C, dC, u_tip, du_tip)."""
+    K = assemble_model(backend, model, eq, n)
+    if tip_dof is None:
+        u = backend.solve(K, F)
+        return float(F @ u), element_sensitivity(backend, model, 0, u, u), None, None
+    R = np.zeros((n, 2))
+    R[:, 0] = F
+    R[tip_dof, 1] = 1.0
+    U = backend.solve_block(K, R)
+    u, lam = U[:, 0], U[:, 1]
+    return (float(F @ u), element_sensitivity(backend, model, 0, u, u), float(u[tip_dof]),
+            element_sensitivity(backend, model, 0, u, lam))

@@ -309,13 +309,16 @@ struct ElemLds {
   static constexpr int WORDS = Kind<K>::EAS ? (ND * 9 + 81 + 9 * ND) : 1;
 };
 
-template <int K>
+// MODE 0 scatters K_e into the lower CSR; MODE 1 writes
+// out[e] = -u_e^T (dK_e/dE_e) v_e = -u_e^T K_e v_e / E_e (K_e is linear in E).
+template <int K, int MODE = 0>
 __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
     int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
     const int* __restrict__ conn, const double* __restrict__ xyz, const double* __restrict__ Ev,
     const double* __restrict__ nuv, const int* __restrict__ gd, const int* __restrict__ rp,
     const int* __restrict__ ci, double* __restrict__ val, int* __restrict__ touched,
-    int* __restrict__ err) {
+    int* __restrict__ err, const double* __restrict__ uvec = nullptr,
+    const double* __restrict__ vvec = nullptr, double* __restrict__ out = nullptr) {
   using KD = Kind<K>;
   constexpr int NN = KD::NN, ND = KD::ND, NL = KD::NL;
   __shared__ double lds_all[kElemWaves][ElemLds<K>::WORDS];
@@ -523,7 +526,33 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
       __syncthreads();
     }
 
-    if (active) {
+    if constexpr (MODE == 1) {
+      const int* g = gd + static_cast<size_t>(e) * kGdStride;
+      double part = 0.0;
+#pragma unroll
+      for (int q = 0; q < KD::SU; ++q) {
+        const int t = lane + 64 * q;
+        if (t < NL) {
+          const unsigned short pr = pair_at<ND>(t);
+          const int i = pr >> 8, j = pr & 0xff;
+          const int gi = g[i], gj = g[j];
+          if (gi >= 0 && gj >= 0) {
+            double w = uvec[gi] * vvec[gj];
+            if (i != j) w += uvec[gj] * vvec[gi];
+            part += kuu[q] * w;
+          }
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+      if (active && lane == 0) {
+        out[e] = -part / E;
+        if (bad) {
+          atomicOr(&err[0], kErrJacobian);
+          atomicMin(&err[1], e);
+        }
+      }
+    }
+    if (MODE == 0 && active) {
       if (bad && lane == 0) {
         atomicOr(&err[0], kErrJacobian);
         atomicMin(&err[1], e);
@@ -1002,7 +1031,86 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
     for (int e = 0; e < nelem; ++e) (etype_h[e] == VBH_CQUAD4 ? shells : solids) = true;
     block_hint = (shells && !solids) ? 6 : (solids && !shells ? 3 : 0);
   }
+  // keep the mesh on the device for the sensitivity kernels
+  mesh.valid = false;
+  mesh.xyz.swap(d_xyz);
+  mesh.E.swap(d_E);
+  mesh.nu.swap(d_nu);
+  mesh.t.swap(d_t);
+  mesh.eq.swap(d_eq);
+  mesh.et.swap(d_et);
+  mesh.ep.swap(d_ep);
+  mesh.conn.swap(d_conn);
+  mesh.gd.swap(d_gd);
+  for (int k = 0; k < 5; ++k) {
+    mesh.sel[k].swap(d_sel[k]);
+    mesh.nsel[k] = static_cast<int>(groups[k].size());
+  }
+  mesh.n = n_;
+  mesh.nnodes = nnodes;
+  mesh.nelem = nelem;
+  mesh.solid_form = solid_form;
+  mesh.shell_form = shell_form;
+  mesh.valid = true;
   finish_device_operator();
+}
+
+// out[e] = -u_e^T (dK_e/dp_e) v_e over the retained mesh (param 0: E, 1: CQUAD4
+// thickness). Host or device vectors.
+void Ctx::element_sensitivity(int param, const double* u, const double* v, double* out, bool dev) {
+  if (!mesh.valid)
+    throw_error(VBH_ERR_STATE, "vbh_element_sensitivity: no assembled mesh in this context");
+  if (param != 0 && param != 1)
+    throw_error(VBH_ERR_ARG, "vbh_element_sensitivity: param must be 0 (E) or 1 (thickness)");
+  if (!u || !v || !out) throw_error(VBH_ERR_ARG, "vbh_element_sensitivity: null vector");
+  select_stream_device();
+  hipStream_t s = stream;
+  DBuf<double> du, dv, dout;
+  const double* pu = u;
+  const double* pv = v;
+  double* po = out;
+  if (!dev) {
+    du.upload(u, static_cast<size_t>(mesh.n), s);
+    dv.upload(v, static_cast<size_t>(mesh.n), s);
+    dout.resize(static_cast<size_t>(mesh.nelem));
+    pu = du.p;
+    pv = dv.p;
+    po = dout.p;
+  }
+  launch_fill_f64(mesh.nelem, po, 0.0, s);  // solids stay 0 for thickness
+  DBuf<int> d_err;
+  d_err.resize(4);
+  hipLaunchKernelGGL(k_init_err, dim3(1), dim3(1), 0, s, d_err.p);
+  for (int k = 0; k < 5; ++k) {
+    const int ns = mesh.nsel[k];
+    if (!ns) continue;
+    if (k == 4) {
+      launch_quad4_sensitivity(ns, mesh.sel[k].p, mesh.ep.p, mesh.conn.p, mesh.xyz.p, mesh.E.p,
+                               mesh.nu.p, mesh.t.p, mesh.shell_form, mesh.gd.p, param, pu, pv, po,
+                               d_err.p, s);
+      continue;
+    }
+    if (param != 0) continue;
+    const dim3 g(static_cast<unsigned>(std::min((ns + kElemWaves - 1) / kElemWaves, 65535)));
+    const dim3 b(64 * kElemWaves);
+#define VBH_SENS(KIND)                                                                          \
+  hipLaunchKernelGGL((k_element_stiffness<KIND, 1>), g, b, 0, s, ns, mesh.sel[k].p, mesh.ep.p,  \
+                     mesh.conn.p, mesh.xyz.p, mesh.E.p, mesh.nu.p, mesh.gd.p, nullptr, nullptr, \
+                     nullptr, nullptr, d_err.p, pu, pv, po)
+    if (k == kHexEas) VBH_SENS(kHexEas);
+    else if (k == kHexSri) VBH_SENS(kHexSri);
+    else if (k == kPentaEas) VBH_SENS(kPentaEas);
+    else VBH_SENS(kTet4);
+#undef VBH_SENS
+  }
+  int herr[4];
+  VBH_HIP(hipMemcpyAsync(herr, d_err.p, sizeof herr, hipMemcpyDeviceToHost, s));
+  if (!dev)
+    VBH_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * mesh.nelem, hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  if (herr[0]) throw_error(VBH_ERR_SOLVER, "vbh_element_sensitivity: element " +
+                                               std::to_string(herr[1]) + " failed (code " +
+                                               std::to_string(herr[0]) + ")");
 }
 
 // Device operator from the device lower CSR: diagonal checks, BSR build (or a
@@ -1156,6 +1264,26 @@ int vbh_assemble(vbh_ctx* h, int n, int nnodes, const double* xyz, const int* eq
     c->assemble(n, nnodes, xyz, eq, nelem, elem_type, elem_ptr, conn, E, nu, thickness, solid_form,
                 shell_form);
   });
+}
+
+int vbh_element_sensitivity(vbh_ctx* h, int param, const double* u, const double* v,
+                            double* out) {
+  if (!h) {
+    vbh::set_error("vbh_element_sensitivity: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { c->element_sensitivity(param, u, v, out, false); });
+}
+
+int vbh_element_sensitivity_device(vbh_ctx* h, int param, const double* du, const double* dv,
+                                   double* dout) {
+  if (!h) {
+    vbh::set_error("vbh_element_sensitivity_device: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { c->element_sensitivity(param, du, dv, dout, true); });
 }
 
 int vbh_last_assembly_stats(const vbh_ctx* h, vbh_assembly_stats* out) {

@@ -18,6 +18,17 @@ struct GraphSlot {
   int max_iters = 0;
 };
 
+// The mesh of the last vbh_assemble, kept on the device for the element
+// sensitivity kernels (and anything else that revisits the elements).
+struct MeshDev {
+  bool valid = false;
+  int n = 0, nnodes = 0, nelem = 0, solid_form = 0, shell_form = 0;
+  DBuf<double> xyz, E, nu, t;
+  DBuf<int> eq, et, ep, conn, gd;
+  DBuf<int> sel[5];
+  int nsel[5] = {0, 0, 0, 0, 0};
+};
+
 struct Ctx {
   // device / stream
   int device = 0;
@@ -72,6 +83,9 @@ struct Ctx {
   PcgScalars* st_host = nullptr;
   GraphSlot graphs[2];
 
+  // mesh of the last device assembly
+  MeshDev mesh;
+
   // block PCG over several load cases
   std::unique_ptr<BlockState, void (*)(BlockState*)> blk{nullptr, destroy_block_state};
 
@@ -96,6 +110,7 @@ struct Ctx {
                 const int* eptr, const int* conn, const double* E, const double* nu,
                 const double* thick, int solid_form, int shell_form);
   void finish_device_operator();
+  void element_sensitivity(int param, const double* u, const double* v, double* out, bool dev);
   void fetch_lower_to_host();
   int dof_map(int nnodes, const int* ids, int nelem, const int* etype, const int* eptr,
               const int* conn, int nspc, const int* spc_nodes, const int* spc_masks, int* eq_out);

@@ -52,4 +52,12 @@ void launch_quad4_stiffness(int nsel, const int* sel, const int* eptr, const int
                             const double* thick, int form, const int* gd, const int* rp,
                             const int* ci, double* val, int* touched, int* err, hipStream_t s);
 
+// out[e] = -u_e^T (dK_e/dp_e) v_e for the selected CQUAD4 elements; param 0:
+// p = E, 1: p = thickness.
+void launch_quad4_sensitivity(int nsel, const int* sel, const int* eptr, const int* conn,
+                              const double* xyz, const double* E, const double* nu,
+                              const double* thick, int form, const int* gd, int param,
+                              const double* u, const double* v, double* out, int* err,
+                              hipStream_t s);
+
 }  // namespace vbh

@@ -67,6 +67,14 @@ struct DBuf {
     VBH_HIP(hipMalloc(&p, count * sizeof(T)));
     n = count;
   }
+  void swap(DBuf& o) noexcept {
+    T* tp = p;
+    p = o.p;
+    o.p = tp;
+    const size_t tn = n;
+    n = o.n;
+    o.n = tn;
+  }
   void upload(const T* h, size_t count, hipStream_t s) {
     resize(count);
     if (count) VBH_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));

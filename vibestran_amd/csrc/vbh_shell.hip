@@ -246,6 +246,7 @@ struct ShellLds {
   double Km[8][8];
   double Kp[12][12];
   double drill;
+  double dg[20];  // Kl diagonal before differentiation (thickness mode)
 };
 
 // Bending curvature column of plate column k: (kx, ky, kxy).
@@ -293,12 +294,19 @@ __device__ __forceinline__ void tcol(const Frame& f, int a, double t[3], int l[3
   }
 }
 
-__global__ __launch_bounds__(64 * kElemWaves) void k_quad4_stiffness(
+// MODE 0: assemble (scatter into the lower CSR).
+// MODE 1: out[e] = -u_e^T (dK_e/dE_e) v_e = -u_e^T K_e v_e / E_e (K_e is linear in E).
+// MODE 2: out[e] = -u_e^T (dK_e/dt_e) v_e: membrane and transverse shear scale
+//         with t, bending with t^3, the drilling penalty with the rotational
+//         diagonal entry it was taken from.
+template <int MODE>
+__global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
     int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
     const int* __restrict__ conn, const double* __restrict__ xyz, const double* __restrict__ Ev,
     const double* __restrict__ nuv, const double* __restrict__ thick, int form,
     const int* __restrict__ gd, const int* __restrict__ rp, const int* __restrict__ ci,
-    double* __restrict__ val, int* __restrict__ touched, int* __restrict__ err) {
+    double* __restrict__ val, int* __restrict__ touched, int* __restrict__ err,
+    const double* __restrict__ uvec, const double* __restrict__ vvec, double* __restrict__ out) {
   __shared__ ShellLds lds_all[kElemWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   ShellLds& L = lds_all[wave];
@@ -369,12 +377,14 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4_stiffness(
         for (int g = 0; g < 4; ++g)
           sum += L.w[g] * dform(A00, A01, form == 0 ? A22 : 0.0, L.Bm[g][i], L.Bm[g][j]);
         if (form == 1) sum += L.w0 * A22 * L.c2[i] * L.c2[j];
+        if (MODE == 2 && i == j) L.dg[i] = sum;
+        if (MODE == 2) sum /= t;  // membrane stiffness is proportional to t
         L.Km[i][j] = sum;
         L.Km[j][i] = sum;
       } else {
         const unsigned short pr = c_spairs.v[it - 36];
         const int i = pr >> 8, j = pr & 0xff;
-        double sum = 0.0;
+        double sum = 0.0, dsum = 0.0;
         for (int g = 0; g < 4; ++g) {
           double bi[3], bj[3];
           bend_col(L, g, i, bi);
@@ -382,6 +392,11 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4_stiffness(
           const double kb = dform(D00, D01, D22, bi, bj);
           const double ks = ts * (L.Bs[g][i][0] * L.Bs[g][j][0] + L.Bs[g][i][1] * L.Bs[g][j][1]);
           sum += L.w[g] * (kb + ks);
+          dsum += L.w[g] * (3.0 * kb + ks);  // t^3 bending, t shear
+        }
+        if (MODE == 2) {
+          if (i == j) L.dg[8 + i] = sum;
+          sum = dsum / t;
         }
         L.Kp[i][j] = sum;
         L.Kp[j][i] = sum;
@@ -391,17 +406,71 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4_stiffness(
 
     // d. drilling penalty (cquad4.cpp:655-667 with 109-129)
     if (lane == 0) {
+      // diagonal of Kl before any derivative: membrane 0..7, plate 8..19
+      double dg[20];
+      for (int i = 0; i < 8; ++i) dg[i] = (MODE == 2) ? L.dg[i] : L.Km[i][i];
+      for (int i = 0; i < 12; ++i) dg[8 + i] = (MODE == 2) ? L.dg[8 + i] : L.Kp[i][i];
       double maxd = 0.0, maxrot = 0.0;
-      for (int i = 0; i < 8; ++i) maxd = fmax(maxd, fabs(L.Km[i][i]));
-      for (int i = 0; i < 12; ++i) maxd = fmax(maxd, fabs(L.Kp[i][i]));
+      int imax = 0, irot = -1;
+      for (int i = 0; i < 20; ++i)
+        if (fabs(dg[i]) > maxd) {
+          maxd = fabs(dg[i]);
+          imax = i;
+        }
       for (int n = 0; n < 4; ++n)
-        maxrot = fmax(maxrot, fmax(fabs(L.Kp[3 * n][3 * n]), fabs(L.Kp[3 * n + 1][3 * n + 1])));
+        for (int c = 0; c < 2; ++c)
+          if (fabs(dg[8 + 3 * n + c]) > maxrot) {
+            maxrot = fabs(dg[8 + 3 * n + c]);
+            irot = 8 + 3 * n + c;
+          }
       if (maxd < 1e-10) bad |= kErrShellDiag;
-      L.drill = (maxrot > 0.0) ? 1e-6 * maxrot : 1e-12 * maxd;
+      if (MODE != 2) {
+        L.drill = (maxrot > 0.0) ? 1e-6 * maxrot : 1e-12 * maxd;
+      } else {
+        // derivative of the penalty along the entry that defines it
+        const int k = (maxrot > 0.0) ? irot : imax;
+        const double dk = (k < 8) ? L.Km[k][k] : L.Kp[k - 8][k - 8];
+        const double sg = (L.dg[k] < 0.0) ? -1.0 : 1.0;  // d|x| = sign(x) dx
+        L.drill = ((maxrot > 0.0) ? 1e-6 : 1e-12) * sg * dk;
+      }
     }
     __syncthreads();
 
-    if (active) {
+    if (MODE != 0) {
+      // contraction with the element's parts of u and v
+      const int* g = gd + static_cast<size_t>(e) * kGdStride;
+      double part = 0.0;
+      for (int it = lane; it < 300; it += 64) {
+        const unsigned short pr = c_spairs.v[it];
+        const int I = pr >> 8, J = pr & 0xff;
+        const int gi = g[I], gj = g[J];
+        if (gi < 0 || gj < 0) continue;
+        const int ni = I / 6, a = I % 6, nj = J / 6, b = J % 6;
+        double ta[3], tb[3];
+        int la[3], lb[3];
+        tcol(f, a, ta, la);
+        tcol(f, b, tb, lb);
+        double kv = 0.0;
+        for (int p = 0; p < 3; ++p) {
+          double inner = 0.0;
+          for (int q = 0; q < 3; ++q) inner += klocal(L, ni, la[p], nj, lb[q]) * tb[q];
+          kv += ta[p] * inner;
+        }
+        double w = uvec[gi] * vvec[gj];
+        if (I != J) w += uvec[gj] * vvec[gi];
+        part += kv * w;
+      }
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+      if (active && lane == 0) {
+        out[e] = (MODE == 1) ? -part / E : -part;
+        if (bad) {
+          atomicOr(&err[0], bad);
+          atomicMin(&err[1], e);
+        }
+      }
+    }
+
+    if (MODE == 0 && active) {
       if (bad) {
         atomicOr(&err[0], bad);
         atomicMin(&err[1], e);
@@ -452,8 +521,26 @@ void launch_quad4_stiffness(int nsel, const int* sel, const int* eptr, const int
                             const int* ci, double* val, int* touched, int* err, hipStream_t s) {
   if (nsel <= 0) return;
   const int blocks = std::min((nsel + kElemWaves - 1) / kElemWaves, 65535);
-  hipLaunchKernelGGL(k_quad4_stiffness, dim3(blocks), dim3(64 * kElemWaves), 0, s, nsel, sel, eptr,
-                     conn, xyz, E, nu, thick, form, gd, rp, ci, val, touched, err);
+  hipLaunchKernelGGL(k_quad4<0>, dim3(blocks), dim3(64 * kElemWaves), 0, s, nsel, sel, eptr, conn,
+                     xyz, E, nu, thick, form, gd, rp, ci, val, touched, err, nullptr, nullptr,
+                     nullptr);
+}
+
+void launch_quad4_sensitivity(int nsel, const int* sel, const int* eptr, const int* conn,
+                              const double* xyz, const double* E, const double* nu,
+                              const double* thick, int form, const int* gd, int param,
+                              const double* u, const double* v, double* out, int* err,
+                              hipStream_t s) {
+  if (nsel <= 0) return;
+  const int blocks = std::min((nsel + kElemWaves - 1) / kElemWaves, 65535);
+  if (param == 0)
+    hipLaunchKernelGGL(k_quad4<1>, dim3(blocks), dim3(64 * kElemWaves), 0, s, nsel, sel, eptr,
+                       conn, xyz, E, nu, thick, form, gd, nullptr, nullptr, nullptr, nullptr, err,
+                       u, v, out);
+  else
+    hipLaunchKernelGGL(k_quad4<2>, dim3(blocks), dim3(64 * kElemWaves), 0, s, nsel, sel, eptr,
+                       conn, xyz, E, nu, thick, form, gd, nullptr, nullptr, nullptr, nullptr, err,
+                       u, v, out);
 }
 
 }  // namespace vbh
