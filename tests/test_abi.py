@@ -50,3 +50,15 @@ def test_try_create_without_device_returns_none():
         pytest.skip("a GPU is visible")
     from vibestran_amd import HipPcgSolverBackend
     assert HipPcgSolverBackend.try_create() is None
+
+
+CPP_TEST = os.path.join(os.path.dirname(__file__), "cpp", "backend_test.bin")
+
+
+def test_cpp_host_mirror_builds_and_runs():
+    """include/vbh_backend.hpp compiles into a program linked against libvbh
+    (built by the library Makefile); without a GPU it reports so (77)."""
+    import subprocess
+    assert os.path.exists(CPP_TEST), "run __graft_entry__.build() (or make) first"
+    r = subprocess.run([CPP_TEST], capture_output=True, text=True, timeout=120)
+    assert r.returncode in (0, 77), r.stdout + r.stderr
