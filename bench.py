@@ -9,7 +9,9 @@ u=0 to a true FP64 relative residual ||F-Ku||/||F|| < 1e-8, in FP32 SpMV /
 Jacobi with FP64 refreshes and FP64 solution update. The plain FP64 GPU PCG
 is timed on the same device matrix for comparison.
 
-Prints one JSON line (see the README of the harness contract).
+Other workloads (--workload tube | simp) print their own JSON line: the
+32-load-case block PCG on the CQUAD4 tube, and the adjoint gradients of the
+SIMP block. Prints one JSON line per run.
 """
 from __future__ import annotations
 
@@ -34,6 +36,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="hex", choices=["hex", "tube", "simp"],
+                    help="hex: C1/C2 single-RHS mixed vs FP64 (the headline line); "
+                         "tube: 32-load-case block PCG vs sequential solves; "
+                         "simp: adjoint compliance / tip-displacement gradients")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
@@ -73,13 +79,31 @@ def cpu_baseline(model, eq, n, F, seconds):
                       f"(oracle restatement, OpenMP); oracle assembly took {t_asm:.1f} s"}
 
 
+def profiled_traffic(profile, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprof summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration)."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        for line in open(path):
+            if line.startswith("| `" + kernel + "` |"):
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                return int((float(cells[3]) + float(cells[4])) * 1e6), f"profiles/{profile}"
+    except OSError:
+        pass
+    return None, None
+
+
 def main():
     args = parse()
+    if args.workload == "tube":
+        return bench_tube(args)
+    if args.workload == "simp":
+        return bench_simp(args)
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.spmv_reps = 1, 0, True, 5
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
-              "sharding is described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharded solve is a design, not part of this build", file=sys.stderr)
         sys.exit(2)
     from vibestran_amd import HipMixedPcgSolverBackend, _lib
     from vibestran_amd import model as M
@@ -163,9 +187,12 @@ def main():
     # column per nonzero, row pointers) plus the kernel's vector streams:
     #   cg_spmv  reads r, diag, p_old and writes p_new, Ap   (5 streams)
     #   cg_update reads p, Ap, r, diag, u(f64), writes u(f64), r
-    spmv32_bytes = nnz_full * (4 + 4) + 4 * (n + 1) + 5 * 4 * n
-    spmv64_bytes = nnz_full * (8 + 4) + 4 * (n + 1) + 5 * 8 * n
-    upd32_bytes = (4 * 4 + 8 + 8 + 4) * n
+    #   cg_spmv   reads p (gathered once) and writes Ap         (2 streams)
+    #   cg_update reads u(f64), p, r, Ap, diag, writes u(f64), r (z is stored
+    #             too, but the algorithmic count leaves it out)
+    spmv32_bytes = nnz_full * (4 + 4) + 4 * (n + 1) + 2 * 4 * n
+    spmv64_bytes = nnz_full * (8 + 4) + 4 * (n + 1) + 2 * 8 * n
+    upd32_bytes = (8 + 4 * 4 + 8 + 4) * n
     ach32 = spmv32_bytes / (ms32 * 1e-3) / 1e9
     ach64 = spmv64_bytes / (ms64 * 1e-3) / 1e9
     # whole-iteration algorithmic bytes (13 vector streams, FP32 inner)
@@ -183,6 +210,9 @@ def main():
         except Exception as e:  # the oracle is optional on the box
             log(f"[bench] cpu baseline skipped: {e}")
 
+    traffic, traffic_src = profiled_traffic("pcg_c2_hex605k.md", f"k_cg_spmv_bsr<float, {fmt}")
+    if args.config != "c2":
+        traffic, traffic_src = None, None
     value = iters_mixed / t_mixed
     out = {
         "metric": "time-to-1e-8 residual + PCG iters/s, 620k-DOF hex; % HBM roofline vs FP64",
@@ -207,8 +237,11 @@ def main():
         "speedup_vs_fp64": round(fp64["time_s"] / t_mixed, 3) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_cg_spmv_bsr<float,3> (fused FP32 PCG SpMV)",
-                     "bytes_per_launch": int(spmv32_bytes), "us_per_launch": round(ms32 * 1e3, 2)},
+                     "traffic": traffic,
+                     "kernel": f"k_cg_spmv_bsr<float,{fmt}> (FP32 PCG SpMV + p.Ap)",
+                     "bytes_per_launch": int(spmv32_bytes), "us_per_launch": round(ms32 * 1e3, 2),
+                     "measured_GBs": round(traffic / (ms32 * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_source": traffic_src},
         "kernels_us": {"cg_spmv_f32": round(ms32 * 1e3, 2), "cg_spmv_f64": round(ms64 * 1e3, 2),
                        "cg_update_f32": round(ms_upd32 * 1e3, 2),
                        "cg_update_f64": round(ms_upd64 * 1e3, 2),
@@ -223,6 +256,133 @@ def main():
         "assembly_ms": {"pattern": ast.pattern_ms, "elements": ast.element_ms},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def _device_columns(lib, h, A):
+    """Column-major host array -> device buffer (caller frees)."""
+    ptr = C.c_void_p()
+    A = np.asfortranarray(A, dtype=np.float64)
+    assert lib.vbh_device_alloc(h, A.nbytes, C.byref(ptr)) == 0
+    assert lib.vbh_memcpy_h2d(h, ptr, A.ctypes.data_as(C.c_void_p), A.nbytes) == 0
+    return ptr
+
+
+def bench_tube(args):
+    """C3: CQUAD4 tube R=50, L=400, 400 x 800 elements (1.92M DOF), 32 load
+    cases; one block-PCG launch vs single-load-case solves of the same
+    operator (a sample of the sequential solves, scaled to 32)."""
+    from vibestran_amd import HipPcgSolverBackend, _lib, decks
+    from vibestran_amd import model as M
+    from vibestran_amd.assembly import assemble_model, dof_map_device
+
+    lib = _lib.load()
+    m = decks.cquad4_tube(n_cases=32)
+    k = len(m.loads)
+    res = {}
+    for prec, label in ((_lib.VBH_PREC_FP64, "fp64"), (_lib.VBH_PREC_MIXED, "mixed")):
+        be = HipPcgSolverBackend.try_create(tolerance=args.tol, max_iters=200000)
+        lib.vbh_set_precision(be._h, prec)
+        eq, n = dof_map_device(be, m)
+        K = assemble_model(be, m, eq, n)
+        F = np.stack([M.assemble_rhs(m, eq, n, lc) for lc in m.loads], axis=1)
+        h = be._h
+        dF, dU = _device_columns(lib, h, F), _device_columns(lib, h, np.zeros_like(F))
+        be.solve_block_device(dF.value, dU.value, k)  # tunes the SpMM, builds the graph
+        t = time.perf_counter()
+        its, rels = be.solve_block_device(dF.value, dU.value, k)
+        t_block = time.perf_counter() - t
+        cfg = be.block_config()
+        st = be.stats()
+        U = np.empty((n, k), order="F")
+        lib.vbh_memcpy_d2h(h, U.ctypes.data_as(C.c_void_p), dU, U.nbytes)
+        true = [float(np.linalg.norm(F[:, j] - be.spmv(U[:, j])) / np.linalg.norm(F[:, j]))
+                for j in (0, k - 1)]
+        sample = list(range(0, k, 8))
+        t_seq, it_seq = 0.0, []
+        for j in sample:
+            t = time.perf_counter()
+            be.solve(K, F[:, j])
+            t_seq += time.perf_counter() - t
+            it_seq.append(be.last_iteration_count())
+        t_seq32 = t_seq * k / len(sample)
+        nnz_full = 2 * K.nnz - n
+        sv = 8 if prec == _lib.VBH_PREC_FP64 else 4
+        spmm_bytes = nnz_full * (sv + 4) + 4 * (n + 1) + 2 * cfg.ldk * n * sv
+        res[label] = {"block_s": round(t_block, 3), "sequential_32_s": round(t_seq32, 3),
+                      "speedup": round(t_seq32 / t_block, 3),
+                      "block_iters_min_max": [int(its.min()), int(its.max())],
+                      "sequential_sample_iters": it_seq,
+                      "block_iters_of_sample": [int(its[j]) for j in sample],
+                      "refreshes": st.refinements, "true_rel_res_first_last": true,
+                      "spmm": {"cols_per_lane": cfg.cols_per_lane,
+                               "lanes_per_group": cfg.lanes_per_group,
+                               "row_blocks": cfg.row_blocks, "xcd_rows": cfg.xcd_rows,
+                               "lds_staged": cfg.lds_staged, "ms": round(cfg.spmm_ms, 4),
+                               "algorithmic_GBs": round(spmm_bytes / (cfg.spmm_ms * 1e-3) / 1e9, 1)},
+                      "ms_per_iteration": round(t_block / int(its.max()) * 1e3, 4)}
+        log(f"[bench] tube {label}: block {t_block:.2f} s, 32 sequential ~{t_seq32:.2f} s "
+            f"({t_seq32 / t_block:.2f}x), iters {its.min()}-{its.max()}")
+        lib.vbh_device_free(h, dF)
+        lib.vbh_device_free(h, dU)
+        be.close()
+    best = max(res.values(), key=lambda r: r["speedup"])
+    out = {"metric": "32-RHS block-PCG throughput vs 32 sequential solves, CQUAD4 tube 1.92M DOF",
+           "value": best["speedup"], "unit": "x (sequential time / block time)", "n_gpus": 1,
+           "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (fp64 mode); f32 SpMM/Jacobi + f64 refresh (mixed mode)",
+           "data": "synthetic (seeded tube, 32 random end force/moment cases)",
+           "config": {"workload": m.meta["name"], "n_dof": int(n), "load_cases": k, "tol": args.tol},
+           "fp64": res["fp64"], "mixed": res["mixed"]}
+    print(json.dumps(out), flush=True)
+
+
+def bench_simp(args):
+    """C4: CHEXA8 80x40x20 SIMP block; compliance and tip-displacement
+    gradients w.r.t. every element density from one 2-column block solve
+    ([F, e_tip]) and two adjoint contraction launches; the GPU gradients are
+    compared with the oracle's FP64 CPU adjoint on all elements."""
+    import oracle as O
+    from vibestran_amd import HipPcgSolverBackend, _lib, decks
+    from vibestran_amd import model as M
+    from vibestran_amd.assembly import assemble_model, dof_map_device, element_sensitivity
+
+    m = decks.config4_simp()
+    be = HipPcgSolverBackend.try_create(tolerance=1e-10, max_iters=400000)
+    eq, n = dof_map_device(be, m)
+    tip = int(eq[m.meta["tip_node"], 2])
+    F = M.assemble_rhs(m, eq, n, m.loads[0])
+    R = np.zeros((n, 2))
+    R[:, 0] = F
+    R[tip, 1] = 1.0
+    dE = 3.0 * m.meta["density"] ** 2 * (m.meta["E0"] - m.meta["Emin"])
+    times = []
+    for step in range(2):
+        t0 = time.perf_counter()
+        K = assemble_model(be, m, eq, n)
+        t1 = time.perf_counter()
+        U = be.solve_block(K, R)
+        t2 = time.perf_counter()
+        gC = element_sensitivity(be, m, 0, U[:, 0], U[:, 0]) * dE
+        gT = element_sensitivity(be, m, 0, U[:, 0], U[:, 1]) * dE
+        t3 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1, t3 - t2))
+    its = be.last_block_iterations
+    mE = M.Model(**{**m.__dict__})
+    oC = O.element_sensitivity(n, mE, eq, 0, U[:, 0], U[:, 0]) * dE
+    oT = O.element_sensitivity(n, mE, eq, 0, U[:, 0], U[:, 1]) * dE
+    errC = float(np.abs(gC - oC).max() / np.abs(oC).max())
+    errT = float(np.abs(gT - oT).max() / np.abs(oT).max())
+    ta, ts, tg = times[-1]
+    out = {"metric": "adjoint dC/drho and du_tip/drho for all elements, CHEXA8 80x40x20 SIMP",
+           "value": round(ta + ts + tg, 4), "unit": "s (assembly + 2-column solve + 2 contractions)",
+           "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SIMP densities)",
+           "config": {"workload": m.meta["name"], "n_dof": int(n), "elements": int(m.ne), "tol": 1e-10},
+           "assembly_s": round(ta, 4), "solve_s": round(ts, 4), "contractions_s": round(tg, 4),
+           "iterations": [int(x) for x in its],
+           "max_rel_diff_vs_cpu_adjoint": {"compliance": errC, "tip": errT},
+           "compliance": float(F @ U[:, 0]), "u_tip": float(U[tip, 0])}
     print(json.dumps(out), flush=True)
 
 

@@ -650,7 +650,7 @@ int vbh_spmv(vbh_ctx* h, const double* x, double* y) {
 }
 
 int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
-  if (!h || reps <= 0 || !ms_per_launch) {
+  if (!h || reps <= 0 || reps > 1000 || !ms_per_launch) {
     vbh::set_error("vbh_bench_spmv: bad argument");
     return VBH_ERR_ARG;
   }
@@ -705,13 +705,24 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
       }
     };
     one();
-    VBH_HIP(hipEventRecord(c->ev_start, c->stream));
-    for (int i = 0; i < reps; ++i) one();
-    VBH_HIP(hipEventRecord(c->ev_stop, c->stream));
-    VBH_HIP(hipEventSynchronize(c->ev_stop));
-    float ms = 0.0f;
-    VBH_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
-    *ms_per_launch = static_cast<double>(ms) / reps;
+    // one event pair around every launch: the mean is the kernel's own
+    // duration, without the gaps between back-to-back launches
+    std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps), nullptr);
+    for (auto& e : ev) VBH_HIP(hipEventCreate(&e));
+    for (int i = 0; i < reps; ++i) {
+      VBH_HIP(hipEventRecord(ev[2 * static_cast<size_t>(i)], c->stream));
+      one();
+      VBH_HIP(hipEventRecord(ev[2 * static_cast<size_t>(i) + 1], c->stream));
+    }
+    VBH_HIP(hipEventSynchronize(ev.back()));
+    double total = 0.0;
+    for (int i = 0; i < reps; ++i) {
+      float ms = 0.0f;
+      VBH_HIP(hipEventElapsedTime(&ms, ev[2 * static_cast<size_t>(i)], ev[2 * static_cast<size_t>(i) + 1]));
+      total += ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms_per_launch = total / reps;
   });
 }
 
