@@ -122,6 +122,23 @@ int vbh_assemble(vbh_ctx* ctx, int n, int nnodes, const double* xyz, const int* 
                  const double* nu, const double* thickness, int solid_form, int shell_form);
 int vbh_last_assembly_stats(const vbh_ctx* ctx, vbh_assembly_stats* out);
 
+/* ---- load vector of one load case on the device ----------------------------
+ * Over the mesh of the last vbh_assemble (reference LinearStaticSolver load
+ * assembly): FORCE/MOMENT (linear_static.cpp:344-398): nodal[6*nnodes]
+ * components per node in the basic frame, or NULL; PLOAD4 on solid faces
+ * (linear_static_pressure.cpp:154-196): nface faces given as 4 node indices in
+ * outward corner order (the face choice of 420-517 is the caller's) and 4
+ * corner pressures, or nface = 0; GRAV (linear_static.cpp:400-503): grav[3]
+ * acceleration in the basic frame applied to every node with the consistent
+ * mass, rho[nelem] densities already scaled by WTMASS, or grav = NULL.
+ * F (length n) is overwritten. */
+int vbh_assemble_rhs(vbh_ctx* ctx, const double* nodal, int nface, const int* face_nodes,
+                     const double* face_p, const double* grav, const double* rho, double* F);
+/* Same with F a device pointer (nodal, faces, grav and rho stay host arrays). */
+int vbh_assemble_rhs_device(vbh_ctx* ctx, const double* nodal, int nface, const int* face_nodes,
+                            const double* face_p, const double* grav, const double* rho,
+                            double* dF);
+
 /* ---- adjoint sensitivities (no reference counterpart) ----------------------
  * For every element e of the mesh given to the last vbh_assemble:
  *   out[e] = -u_e^T (dK_e / dp_e) v_e,  p = E_e (param 0) or the CQUAD4

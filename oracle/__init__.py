@@ -81,6 +81,7 @@ def _load():
         "orc_assemble_model": (P, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I]),
         "orc_quad4_dke_dt": (I, [DP, D, D, D, D, D, D, I, DP]),
         "orc_element_sensitivity": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I, I, DP, DP, DP]),
+        "orc_grav_rhs": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -338,6 +339,19 @@ def assemble_model(n, model, eq):
     if not h:
         raise OracleError(_err())
     return Csr(h)
+
+
+def grav_rhs(n, model, eq, g, rho=None):
+    """F = sum_e M_e a (consistent masses), a = g at every node."""
+    et, ep, cn = _i32(model.elem_type), _i32(model.elem_ptr), _i32(model.conn)
+    x, q = _f64(model.xyz).reshape(-1), _i32(eq).reshape(-1)
+    r = _f64(model.rho if rho is None else rho)
+    th = _f64(model.thickness if model.thickness is not None else np.zeros(len(et)))
+    F = np.empty(n, dtype=np.float64)
+    if lib().orc_grav_rhs(n, len(et), _ip(et), _ip(ep), _ip(cn), _dp(x), _ip(q), _dp(r), _dp(th),
+                          _dp(_f64(g)), _dp(F)) != 0:
+        raise OracleError(_err())
+    return F
 
 
 def element_sensitivity(n, model, eq, param, u, v):

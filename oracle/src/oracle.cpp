@@ -802,6 +802,48 @@ OrcCsr* orc_assemble_model(int n, int nelem, const int* et, const int* ep, const
   return orc_builder_build_csr(&b);
 }
 
+int orc_grav_rhs(int n, int nelem, const int* et, const int* ep, const int* conn,
+                 const double* xyz, const int* eq, const double* rho, const double* thick,
+                 const double* g, double* F) {
+  std::fill(F, F + n, 0.0);
+  std::vector<double> me(576), xe(24), ae(24), fe(24);
+  for (int e = 0; e < nelem; ++e) {
+    const int nn = ep[e + 1] - ep[e];
+    const int dpn = (et[e] == ORC_CQUAD4) ? 6 : 3;
+    const int nd = dpn * nn;
+    for (int a = 0; a < nn; ++a) {
+      const int node = conn[ep[e] + a];
+      for (int d = 0; d < 3; ++d) xe[static_cast<size_t>(3 * a + d)] = xyz[3 * node + d];
+      for (int d = 0; d < dpn; ++d) ae[static_cast<size_t>(dpn * a + d)] = (d < 3) ? g[d] : 0.0;
+    }
+    int rc;
+    if (et[e] == ORC_CHEXA8) rc = orc_hexa8_me(xe.data(), rho[e], me.data());
+    else if (et[e] == ORC_CTETRA4) rc = orc_tetra4_me(xe.data(), rho[e], me.data());
+    else if (et[e] == ORC_CPENTA6) rc = orc_penta6_me(xe.data(), rho[e], me.data());
+    else if (et[e] == ORC_CQUAD4) rc = orc_quad4_me(xe.data(), rho[e], thick[e], me.data());
+    else rc = fail(ORC_ERR_ARG, "orc_grav_rhs: unsupported element type");
+    if (rc != ORC_OK) return rc;
+    double mmax = 0.0, fmax = 0.0;
+    for (int i = 0; i < nd * nd; ++i) mmax = std::max(mmax, std::fabs(me[static_cast<size_t>(i)]));
+    if (mmax < 1e-30) continue;  // linear_static.cpp:452-453
+    for (int i = 0; i < nd; ++i) {
+      double sv = 0.0;
+      for (int j = 0; j < nd; ++j) sv += me[static_cast<size_t>(i * nd + j)] * ae[static_cast<size_t>(j)];
+      fe[static_cast<size_t>(i)] = sv;
+      fmax = std::max(fmax, std::fabs(sv));
+    }
+    if (fmax < 1e-30) continue;  // 495-497
+    for (int a = 0; a < nn; ++a) {
+      const int node = conn[ep[e] + a];
+      for (int d = 0; d < dpn; ++d) {
+        const int q = eq[6 * node + d];
+        if (q >= 0) F[q] += fe[static_cast<size_t>(dpn * a + d)];
+      }
+    }
+  }
+  return ORC_OK;
+}
+
 int orc_element_sensitivity(int n, int nelem, const int* et, const int* ep, const int* conn,
                             const double* xyz, const int* eq, const double* E, const double* nu,
                             const double* thick, int solid_form, int shell_form, int param,

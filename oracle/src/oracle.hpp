@@ -135,6 +135,12 @@ OrcCsr* orc_assemble_model(int n, int nelem, const int* elem_type, const int* el
                            const int* conn, const double* xyz, const int* eq, const double* E,
                            const double* nu, const double* thickness, int solid_form,
                            int shell_form);
+// GRAV load vector (LinearStaticSolver::apply_inertial_loads,
+// linear_static.cpp:400-503): F = sum_e M_e a with the consistent element
+// masses, acceleration g[3] at every node (basic frame, WTMASS folded into rho).
+int orc_grav_rhs(int n, int nelem, const int* elem_type, const int* elem_ptr, const int* conn,
+                 const double* xyz, const int* eq, const double* rho, const double* thickness,
+                 const double* g, double* F);
 // FP64 CPU adjoint contraction (no reference counterpart): out[e] =
 // -u_e^T (dK_e/dp_e) v_e, p = E (param 0) or CQUAD4 thickness (param 1;
 // solids 0). Compliance: v = u; a functional L^T u: v = lambda, K lambda = L.

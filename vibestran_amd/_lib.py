@@ -71,6 +71,8 @@ SIGNATURES = {
     "vbh_dof_map": (_I, [_P, _I, _IP, _I, _IP, _IP, _IP, _I, _IP, _IP, _IP, _IP]),
     "vbh_assemble": (_I, [_P, _I, _I, _DP, _IP, _I, _IP, _IP, _IP, _DP, _DP, _DP, _I, _I]),
     "vbh_last_assembly_stats": (_I, [_P, C.POINTER(vbh_assembly_stats)]),
+    "vbh_assemble_rhs": (_I, [_P, _DP, _I, _IP, _DP, _DP, _DP, _DP]),
+    "vbh_assemble_rhs_device": (_I, [_P, _DP, _I, _IP, _DP, _DP, _DP, _P]),
     "vbh_element_sensitivity": (_I, [_P, _I, _DP, _DP, _DP]),
     "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),
     "vbh_solve": (_I, [_P, _DP, _DP, _IP, _DP]),

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .model import CHEXA8, CQUAD4, LoadCase, Model, Pload4
+from .model import CHEXA8, CPENTA6, CQUAD4, CTETRA4, LoadCase, Model, Pload4
 
 
 def _hex_grid(nx, ny, nz, h=1.0):
@@ -144,4 +144,61 @@ def cquad4_tube(n_circ=400, n_axial=800, radius=50.0, length=400.0, t=2.0, E=70e
         nodal[tip, :] = fm / len(tip)
         m.loads.append(LoadCase(nodal=nodal))
     m.meta.update(dict(name=f"cquad4-tube-{n_circ}x{n_axial}", n_circ=n_circ, n_axial=n_axial))
+    return m
+
+
+_TET_A = np.array([[0, 1, 3, 4], [1, 2, 3, 6], [1, 4, 5, 6], [3, 4, 6, 7], [1, 3, 4, 6]])
+_TET_B = np.array([[0, 1, 2, 5], [0, 2, 3, 7], [0, 5, 7, 4], [2, 5, 6, 7], [0, 2, 5, 7]])
+_WEDGES = np.array([[0, 1, 2, 4, 5, 6], [0, 2, 3, 4, 6, 7]])
+
+
+def mixed_block(nx=120, ny=60, nz=54, jitter=0.1, seed=5, n_cases=8, hex_frac=0.4,
+                wedge_frac=0.1, n_force_nodes=16, force_scale=1e3):
+    """Mixed-mesh workload: a jittered nx x ny x nz cell block (h = 1) whose
+    cells are CHEXA8 for i < hex_frac*nx, two CPENTA6 each for the next
+    wedge_frac*nx layers, and five CTETRA4 each beyond (alternating split for
+    conformity). Every element takes one of two materials by a seeded draw
+    (E 200e3 / nu 0.3 / rho 7.85e-9, or 70e3 / 0.33 / 2.7e-9). The base z = 0
+    is clamped (123). Load case j: GRAV (0, 0, -9810) plus FORCE components
+    U[-1,1]*force_scale on n_force_nodes seeded top-surface nodes."""
+    ijk, xyz, ids = _hex_grid(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    interior = ((ijk[:, 0] > 0) & (ijk[:, 0] < nx) & (ijk[:, 1] > 0) & (ijk[:, 1] < ny)
+                & (ijk[:, 2] > 0) & (ijk[:, 2] < nz))
+    xyz = xyz + rng.uniform(-jitter, jitter, size=xyz.shape) * interior[:, None]
+    cells = _hex_conn(nx, ny, nz)                        # [nc,8], x-major
+    ci = np.arange(len(cells)) // (ny * nz)             # cell i index
+    cjk = np.arange(len(cells)) % (ny * nz)
+    parity = (ci + cjk // nz + cjk % nz) % 2
+    i_hex, i_wedge = int(round(hex_frac * nx)), int(round((hex_frac + wedge_frac) * nx))
+    hexes = cells[ci < i_hex]
+    wsel = cells[(ci >= i_hex) & (ci < i_wedge)]
+    wedges = wsel[:, _WEDGES].reshape(-1, 6)
+    tsel = (ci >= i_wedge)
+    tets = np.concatenate([cells[tsel & (parity == 0)][:, _TET_A].reshape(-1, 4),
+                           cells[tsel & (parity == 1)][:, _TET_B].reshape(-1, 4)])
+    X = xyz[tets]
+    neg = np.einsum("ij,ij->i", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), X[:, 3] - X[:, 0]) < 0
+    tets[neg] = tets[neg][:, [0, 2, 1, 3]]
+    et = np.concatenate([np.full(len(hexes), CHEXA8), np.full(len(wedges), CPENTA6),
+                         np.full(len(tets), CTETRA4)]).astype(np.int32)
+    conn = np.concatenate([hexes.reshape(-1), wedges.reshape(-1), tets.reshape(-1)]).astype(np.int32)
+    npe = np.where(et == CHEXA8, 8, np.where(et == CPENTA6, 6, 4))
+    ep = np.concatenate([[0], np.cumsum(npe)]).astype(np.int32)
+    ne = len(et)
+    mat = rng.random(ne) < 0.5
+    spc_nodes = np.nonzero(ijk[:, 2] == 0)[0].astype(np.int32)
+    m = Model(node_ids=ids, xyz=xyz, elem_type=et, elem_ptr=ep, conn=conn,
+              E=np.where(mat, 200e3, 70e3), nu=np.where(mat, 0.3, 0.33),
+              rho=np.where(mat, 7.85e-9, 2.7e-9), spc_nodes=spc_nodes,
+              spc_masks=np.full(len(spc_nodes), 0b000111, np.int32))
+    top = np.nonzero(ijk[:, 2] == nz)[0]
+    for j in range(n_cases):
+        r = np.random.default_rng(seed * 1000 + j)
+        nodal = np.zeros((m.nn, 6))
+        pick = r.choice(top, n_force_nodes, replace=False)
+        nodal[pick, 0:3] = r.uniform(-1.0, 1.0, (n_force_nodes, 3)) * force_scale
+        m.loads.append(LoadCase(nodal=nodal, grav=np.array([0.0, 0.0, -9810.0])))
+    m.meta.update(dict(name=f"mixed{nx}x{ny}x{nz}-hex-penta-tet", nx=nx, ny=ny, nz=nz, ijk=ijk,
+                       counts=dict(CHEXA8=len(hexes), CPENTA6=len(wedges), CTETRA4=len(tets))))
     return m
