@@ -491,6 +491,7 @@ int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
     c->max_iters = max_iters > 0 ? max_iters : 10000;
     if (const char* e = std::getenv("VBH_REFRESH_DELTA")) c->refresh_delta = std::atof(e);
     if (const char* e = std::getenv("VBH_SPMV_ACC64")) c->spmv_acc64 = std::atoi(e);
+    if (const char* e = std::getenv("VBH_ROWSUM")) c->rowsum_fp32 = std::atoi(e);
     VBH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     VBH_HIP(hipEventCreate(&c->ev_start));
     VBH_HIP(hipEventCreate(&c->ev_stop));

@@ -66,6 +66,9 @@ struct Ctx {
   DBuf<int> sell_koff, sell_idx;
   DBuf<double> sell_v64;
   DBuf<float> sell_v32;
+  DBuf<double> sell_dblk;   // FP64 corrected diagonal node blocks (row-sum preserving FP32)
+  DBuf<int> sell_kdiag;     // slot of the diagonal block in each block row
+  int rowsum_fp32 = 1;      // split the FP32 stream as above (VBH_ROWSUM=0 disables)
   long long sell_padded_blocks = 0;
   DBuf<double> diag64;
   DBuf<float> diag32;

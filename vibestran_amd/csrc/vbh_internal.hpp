@@ -167,6 +167,11 @@ struct BsrDev {
   const int* sidx = nullptr;
   const double* s64 = nullptr;
   const float* s32 = nullptr;
+  // Row-sum preserving FP32 stream: when dblk is set, s32 holds zeros in
+  // every diagonal node block and dblk (FP64, B*B per block row, row-major)
+  // holds that block corrected so each scalar row's per-component sums equal
+  // the FP64 operator's; FP32 SpMVs then accumulate in FP64.
+  const double* dblk = nullptr;
 };
 
 constexpr int kSliceRows = 64;

@@ -133,6 +133,19 @@ __device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ v
   return acc;
 }
 
+// FP64 diagonal node block of row g (row-sum preserving FP32 stream, see
+// BsrDev::dblk) times x.
+template <typename T, int B>
+__device__ __forceinline__ double diag_block_row(const double* __restrict__ dblk,
+                                                 const T* __restrict__ x, int g) {
+  const int node = g / B, r = g - node * B;
+  const double* d = dblk + (static_cast<size_t>(node) * B + r) * B;
+  double acc = 0.0;
+#pragma unroll
+  for (int c = 0; c < B; ++c) acc += d[c] * static_cast<double>(x[B * node + c]);
+  return acc;
+}
+
 // ---- plain SpMV -------------------------------------------------------------------
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_csr(CsrDev A, const T* __restrict__ x,
@@ -146,17 +159,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv_csr(CsrDev A, const T* __restri
   }
 }
 
-template <typename T, int B>
+template <typename T, int B, typename Acc = T>
 __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restrict__ x,
                                                      T* __restrict__ y) {
   const T* val = svals<T>(A);
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
+  const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
   for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
        s += gridDim.x * (blockDim.x / kSliceRows)) {
-    const T v = sell_row<T, B>(A, val, x, s, l);
+    Acc acc = sell_row<T, B, Acc>(A, val, x, s, l);
     const int g = kSliceRows * s + l;
-    if (g < nrows) y[g] = v;
+    if (g < nrows) {
+      if (split) acc += static_cast<Acc>(diag_block_row<T, B>(A.dblk, x, g));
+      y[g] = static_cast<T>(acc);
+    }
   }
 }
 
@@ -359,12 +376,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   const T* val = svals<T>(A);
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
+  const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
   double d[1] = {0.0};
   for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
        s += gridDim.x * (blockDim.x / kSliceRows)) {
-    const T v = static_cast<T>(sell_row<T, B, Acc>(A, val, b.p, s, l));
+    Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
     const int g = kSliceRows * s + l;
     if (g < nrows) {
+      if (split) acc += static_cast<Acc>(diag_block_row<T, B>(A.dblk, b.p, g));
+      const T v = static_cast<T>(acc);
       b.Ap[g] = v;
       d[0] += static_cast<double>(b.p[g]) * static_cast<double>(v);
     }
@@ -544,12 +564,17 @@ void launch_spmv_csr(const CsrDev& A, const T* x, T* y, hipStream_t s) {
 
 template <typename T>
 void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s) {
-  if (A.B == 3)
-    hipLaunchKernelGGL((k_spmv_bsr<T, 3>), dim3(grid_for(A.nslices, kBlock / kSliceRows)),
-                       dim3(kBlock), 0, s, A, x, y);
+  const dim3 g(grid_for(A.nslices, kBlock / kSliceRows));
+  // the split FP32 stream accumulates in FP64 (the diagonal block is FP64)
+  const bool acc64 = sizeof(T) == 4 && A.dblk != nullptr;
+  if (A.B == 3 && acc64)
+    hipLaunchKernelGGL((k_spmv_bsr<T, 3, double>), g, dim3(kBlock), 0, s, A, x, y);
+  else if (A.B == 3)
+    hipLaunchKernelGGL((k_spmv_bsr<T, 3>), g, dim3(kBlock), 0, s, A, x, y);
+  else if (acc64)
+    hipLaunchKernelGGL((k_spmv_bsr<T, 6, double>), g, dim3(kBlock), 0, s, A, x, y);
   else
-    hipLaunchKernelGGL((k_spmv_bsr<T, 6>), dim3(grid_for(A.nslices, kBlock / kSliceRows)),
-                       dim3(kBlock), 0, s, A, x, y);
+    hipLaunchKernelGGL((k_spmv_bsr<T, 6>), g, dim3(kBlock), 0, s, A, x, y);
 }
 
 template <typename T>
@@ -560,6 +585,7 @@ void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
                     PcgScalars* st, const CgRed& red, hipStream_t s, int acc64) {
+  if (sizeof(T) == 4 && Ab.dblk != nullptr) acc64 = 1;  // split FP32 stream: FP64 accumulation
   if (format == 0)
     hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red);
   else if (format == 3 && acc64)

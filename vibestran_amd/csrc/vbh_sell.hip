@@ -60,6 +60,53 @@ __global__ void k_sell_fill(int nslices, int nbr, const int* __restrict__ brp,
   }
 }
 
+// Diagonal node block of block row I in FP64 plus the symmetrised correction
+// (e + e^T)/2, e[r][c] = sum over the off-diagonal blocks of the row of
+// (v - (double)(float)v) at entry (r, c): with FP32 off-diagonal blocks and
+// this block, the per-component sums of every scalar row equal the FP64
+// operator's (up to the symmetrisation), so fields that are locally constant
+// per component -- the smooth vectors late PCG iterations consist of -- are
+// multiplied without the FP32 rounding of the blocks around them.
+template <int B>
+__global__ void k_diag_rowsum(int nbr, const int* __restrict__ brp, const int* __restrict__ bci,
+                              const double* __restrict__ bv, double* __restrict__ dblk,
+                              int* __restrict__ kdiag) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const int o = brp[I], nb = brp[I + 1] - o;
+  double e[B][B], a[B][B];
+  for (int r = 0; r < B; ++r)
+    for (int c = 0; c < B; ++c) e[r][c] = a[r][c] = 0.0;
+  int kd = -1;
+  for (int q = 0; q < nb; ++q) {
+    const bool diag = bci[o + q] == I;
+    if (diag) kd = q;
+    for (int r = 0; r < B; ++r)
+      for (int c = 0; c < B; ++c) {
+        const double x = bv[static_cast<size_t>(B * B) * o + static_cast<size_t>(r * B + c) * nb + q];
+        if (diag) a[r][c] = x;
+        else e[r][c] += x - static_cast<double>(static_cast<float>(x));
+      }
+  }
+  for (int r = 0; r < B; ++r)
+    for (int c = 0; c < B; ++c)
+      dblk[(static_cast<size_t>(I) * B + r) * B + c] = a[r][c] + 0.5 * (e[r][c] + e[c][r]);
+  kdiag[I] = kd;
+}
+
+// Zero the diagonal node block of every row in the FP32 sliced-ELL values.
+template <int B>
+__global__ void k_sell_zero_diag(int nslices, int nbr, const int* __restrict__ koff,
+                                 const int* __restrict__ kdiag, float* __restrict__ s32) {
+  const int s = blockIdx.x, l = threadIdx.x;
+  if (s >= nslices) return;
+  const int g = kSliceRows * s + l;
+  if (g >= nbr * B) return;
+  const int kd = kdiag[g / B];
+  if (kd < 0) return;
+  for (int c = 0; c < B; ++c) s32[(static_cast<size_t>(koff[s] + kd) * kSliceRows + l) * B + c] = 0.0f;
+}
+
 __global__ void k_to_f32(long long m, const double* __restrict__ a, float* __restrict__ b) {
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -105,6 +152,26 @@ void build_sell(Ctx& c) {
                        A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
   hipLaunchKernelGGL(k_to_f32, dim3(grid_for(static_cast<long long>(nval), 1024)), dim3(256), 0, s,
                      static_cast<long long>(nval), c.sell_v64.p, c.sell_v32.p);
+  A.dblk = nullptr;
+  if (c.rowsum_fp32) {
+    c.sell_dblk.resize(static_cast<size_t>(A.nbr) * B * B);
+    c.sell_kdiag.resize(static_cast<size_t>(A.nbr));
+    if (B == 3) {
+      hipLaunchKernelGGL(k_diag_rowsum<3>, dim3((A.nbr + 255) / 256), dim3(256), 0, s, A.nbr, A.brp,
+                         A.bci, A.v64, c.sell_dblk.p, c.sell_kdiag.p);
+      hipLaunchKernelGGL(k_sell_zero_diag<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_kdiag.p, c.sell_v32.p);
+    } else {
+      hipLaunchKernelGGL(k_diag_rowsum<6>, dim3((A.nbr + 255) / 256), dim3(256), 0, s, A.nbr, A.brp,
+                         A.bci, A.v64, c.sell_dblk.p, c.sell_kdiag.p);
+      hipLaunchKernelGGL(k_sell_zero_diag<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_kdiag.p, c.sell_v32.p);
+    }
+    A.dblk = c.sell_dblk.p;
+  } else {
+    c.sell_dblk.release();
+    c.sell_kdiag.release();
+  }
   VBH_HIP(hipStreamSynchronize(s));
   A.nslices = nslices;
   A.koff = c.sell_koff.p;
