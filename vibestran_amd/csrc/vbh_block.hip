@@ -38,6 +38,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -60,6 +61,10 @@ struct BlkOp {
   const int* bci = nullptr;
   const double* v64 = nullptr;
   const float* v32 = nullptr;
+  // FP64 diagonal node blocks corrected for row-sum preserving FP32 rounding
+  // (row-major B*B per block row); the FP32 SpMM skips the FP32 diagonal
+  // block and adds this block's product in FP64 after the row.
+  const double* dblk = nullptr;
 };
 
 struct BlkTune {
@@ -381,7 +386,12 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm(BlkOp A, int ldk, int xcd,
     for (int w = 0; w < CW; ++w) on |= act[c0 + w];
     if (!__syncthreads_or(on)) return;
   }
+  // FP32 products of 6x6 (shell) blocks are summed in FP64: their rows mix
+  // membrane and bending terms of very different size, and FP32 sums cost
+  // the row-sum preserving operator a large share of its iteration savings
+  using Acc = typename std::conditional<(MODE == 0 && sizeof(T) == 4 && B == 6), double, T>::type;
   const T* __restrict__ val = op_vals<T>(A);
+  const double* __restrict__ dblk = (MODE == 0 && sizeof(T) == 4) ? A.dblk : nullptr;
   const int wave = threadIdx.x >> 6;
   double dot[CW];
 #pragma unroll
@@ -398,16 +408,17 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm(BlkOp A, int ldk, int xcd,
        I0 += nrbx * (kBT / 64) * GR) {
     const int I = I0 + g;
     if (I >= row_hi) continue;
-    T acc[B][CW];
+    Acc acc[B][CW];
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
-      for (int w = 0; w < CW; ++w) acc[r][w] = T(0);
+      for (int w = 0; w < CW; ++w) acc[r][w] = Acc(0);
     const int o = A.brp[I], e = A.brp[I + 1];
     for (int q = o; q < e; ++q) {
       const int J = A.bci[q];
       const T* __restrict__ v = val + static_cast<size_t>(q) * (B * B);
       const T* __restrict__ xj = X + static_cast<size_t>(B * J) * ldk + c0;
+      if (dblk && J == I) continue;  // FP64 epilogue below
       // column s of the block (stored contiguously) times row s of X_J
 #pragma unroll
       for (int s = 0; s < B; ++s) {
@@ -417,19 +428,35 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm(BlkOp A, int ldk, int xcd,
 #pragma unroll
         for (int r = 0; r < B; ++r)
 #pragma unroll
-          for (int w = 0; w < CW; ++w) acc[r][w] += a[r] * x[w];
+          for (int w = 0; w < CW; ++w) acc[r][w] += static_cast<Acc>(a[r]) * static_cast<Acc>(x[w]);
       }
     }
 #pragma unroll
     for (int r = 0; r < B; ++r) {
       const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
       if constexpr (MODE == 0) {
-        T xo[CW];
+        T xo[CW], yo[CW];
         vload<T, CW>(X + idx, xo);
-        vstore<T, CW>(Y + idx, acc[r]);
+        for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(acc[r][w]);
+        if (dblk) {  // FP64 corrected diagonal block: y = acc + D_II x_I in FP64
+          double dsum[CW];
+#pragma unroll
+          for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
+#pragma unroll
+          for (int s = 0; s < B; ++s) {
+            T xi[CW];
+            vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xi);
+            const double dv = dblk[(static_cast<size_t>(I) * B + r) * B + s];
+#pragma unroll
+            for (int w = 0; w < CW; ++w) dsum[w] += dv * static_cast<double>(xi[w]);
+          }
+#pragma unroll
+          for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(dsum[w]);
+        }
+        vstore<T, CW>(Y + idx, yo);
 #pragma unroll
         for (int w = 0; w < CW; ++w)
-          dot[w] += static_cast<double>(xo[w]) * static_cast<double>(acc[r][w]);
+          dot[w] += static_cast<double>(xo[w]) * static_cast<double>(yo[w]);
       } else {
         double f[CW];
         T y[CW];
@@ -691,7 +718,12 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
     for (int w = 0; w < CW; ++w) on |= act[c0 + w];
     if (!__syncthreads_or(on)) return;
   }
+  // FP32 products of 6x6 (shell) blocks are summed in FP64: their rows mix
+  // membrane and bending terms of very different size, and FP32 sums cost
+  // the row-sum preserving operator a large share of its iteration savings
+  using Acc = typename std::conditional<(MODE == 0 && sizeof(T) == 4 && B == 6), double, T>::type;
   const T* __restrict__ val = op_vals<T>(A);
+  const double* __restrict__ dblk = (MODE == 0 && sizeof(T) == 4) ? A.dblk : nullptr;
   const int wave = threadIdx.x >> 6;
   T* __restrict__ stage = sA[wave];
   double dot[CW];
@@ -730,11 +762,11 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
     int nbmax = nb;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nbmax = max(nbmax, __shfl_xor(nbmax, off, 64));
-    T acc[B][CW];
+    Acc acc[B][CW];
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
-      for (int w = 0; w < CW; ++w) acc[r][w] = T(0);
+      for (int w = 0; w < CW; ++w) acc[r][w] = Acc(0);
     for (int t = 0; t < nbmax; ++t) {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
@@ -747,16 +779,17 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
         const int J = A.bci[o + t];
         const T* __restrict__ xj = X + static_cast<size_t>(B * J) * ldk + c0;
         const T* ag = stage + g * NB2;
+        if (!(dblk && J == I)) {  // the diagonal block goes through the FP64 epilogue
 #pragma unroll
-        for (int s = 0; s < B; ++s) {
-          T x[CW], a[B];
-          vload<T, CW>(xj + static_cast<size_t>(s) * ldk, x);
+          for (int s = 0; s < B; ++s) {
+            T x[CW];
+            vload<T, CW>(xj + static_cast<size_t>(s) * ldk, x);
 #pragma unroll
-          for (int r = 0; r < B; ++r) a[r] = ag[s * B + r];
+            for (int r = 0; r < B; ++r)
 #pragma unroll
-          for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int w = 0; w < CW; ++w) acc[r][w] += a[r] * x[w];
+              for (int w = 0; w < CW; ++w)
+                acc[r][w] += static_cast<Acc>(ag[s * B + r]) * static_cast<Acc>(x[w]);
+          }
         }
       }
       wave_lds_sync();
@@ -766,12 +799,28 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
       for (int r = 0; r < B; ++r) {
         const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
         if constexpr (MODE == 0) {
-          T xo[CW];
+          T xo[CW], yo[CW];
           vload<T, CW>(X + idx, xo);
-          vstore<T, CW>(Y + idx, acc[r]);
+          for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(acc[r][w]);
+        if (dblk) {  // FP64 corrected diagonal block: y = acc + D_II x_I in FP64
+          double dsum[CW];
+#pragma unroll
+          for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
+#pragma unroll
+          for (int s = 0; s < B; ++s) {
+            T xi[CW];
+            vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xi);
+            const double dv = dblk[(static_cast<size_t>(I) * B + r) * B + s];
+#pragma unroll
+            for (int w = 0; w < CW; ++w) dsum[w] += dv * static_cast<double>(xi[w]);
+          }
+#pragma unroll
+          for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(dsum[w]);
+        }
+          vstore<T, CW>(Y + idx, yo);
 #pragma unroll
           for (int w = 0; w < CW; ++w)
-            dot[w] += static_cast<double>(xo[w]) * static_cast<double>(acc[r][w]);
+            dot[w] += static_cast<double>(xo[w]) * static_cast<double>(yo[w]);
         } else {
           double f[CW];
           T y[CW];
@@ -915,6 +964,7 @@ static void ensure_block_operator(Ctx& c, BlockState& b) {
     b.op.bci = c.bsrd.bci;
     b.op.v64 = b.aos64.p;
     b.op.v32 = b.aos32.p;
+    b.op.dblk = c.bsrd.dblk;
   } else {
     // scalar CSR: blocks of 1 x 1, values already in array-of-blocks order
     b.aos64.release();
@@ -925,6 +975,7 @@ static void ensure_block_operator(Ctx& c, BlockState& b) {
     b.op.bci = c.csr.ci;
     b.op.v64 = c.csr.v64;
     b.op.v32 = c.csr.v32;
+    b.op.dblk = nullptr;
   }
   VBH_HIP(hipStreamSynchronize(s));
   b.op_version = c.operator_version;

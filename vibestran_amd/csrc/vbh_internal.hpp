@@ -170,7 +170,8 @@ struct BsrDev {
   // Row-sum preserving FP32 stream: when dblk is set, s32 holds zeros in
   // every diagonal node block and dblk (FP64, B*B per block row, row-major)
   // holds that block corrected so each scalar row's per-component sums equal
-  // the FP64 operator's; FP32 SpMVs then accumulate in FP64.
+  // the FP64 operator's; FP32 SpMVs sum the off-diagonal blocks in FP32 and
+  // add the diagonal block's product in FP64.
   const double* dblk = nullptr;
 };
 

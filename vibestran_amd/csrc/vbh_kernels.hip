@@ -168,12 +168,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
   const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
   for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
        s += gridDim.x * (blockDim.x / kSliceRows)) {
-    Acc acc = sell_row<T, B, Acc>(A, val, x, s, l);
+    const Acc acc = sell_row<T, B, Acc>(A, val, x, s, l);
     const int g = kSliceRows * s + l;
-    if (g < nrows) {
-      if (split) acc += static_cast<Acc>(diag_block_row<T, B>(A.dblk, x, g));
-      y[g] = static_cast<T>(acc);
-    }
+    if (g < nrows)
+      y[g] = split ? static_cast<T>(static_cast<double>(acc) + diag_block_row<T, B>(A.dblk, x, g))
+                   : static_cast<T>(acc);
   }
 }
 
@@ -380,11 +379,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   double d[1] = {0.0};
   for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
        s += gridDim.x * (blockDim.x / kSliceRows)) {
-    Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
+    const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
     const int g = kSliceRows * s + l;
     if (g < nrows) {
-      if (split) acc += static_cast<Acc>(diag_block_row<T, B>(A.dblk, b.p, g));
-      const T v = static_cast<T>(acc);
+      // split FP32 stream: off-diagonal blocks summed in FP32, the corrected
+      // FP64 diagonal block added in FP64
+      const T v = split ? static_cast<T>(static_cast<double>(acc) +
+                                         diag_block_row<T, B>(A.dblk, b.p, g))
+                        : static_cast<T>(acc);
       b.Ap[g] = v;
       d[0] += static_cast<double>(b.p[g]) * static_cast<double>(v);
     }
@@ -565,7 +567,8 @@ void launch_spmv_csr(const CsrDev& A, const T* x, T* y, hipStream_t s) {
 template <typename T>
 void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s) {
   const dim3 g(grid_for(A.nslices, kBlock / kSliceRows));
-  // the split FP32 stream accumulates in FP64 (the diagonal block is FP64)
+  // the split FP32 stream sums in FP64 (cheap next to the matrix stream; an
+  // FP32 sum gives back part of the iterations the split saves)
   const bool acc64 = sizeof(T) == 4 && A.dblk != nullptr;
   if (A.B == 3 && acc64)
     hipLaunchKernelGGL((k_spmv_bsr<T, 3, double>), g, dim3(kBlock), 0, s, A, x, y);
@@ -585,7 +588,7 @@ void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
                     PcgScalars* st, const CgRed& red, hipStream_t s, int acc64) {
-  if (sizeof(T) == 4 && Ab.dblk != nullptr) acc64 = 1;  // split FP32 stream: FP64 accumulation
+  if (sizeof(T) == 4 && Ab.dblk != nullptr) acc64 = 1;  // split FP32 stream: FP64 sums
   if (format == 0)
     hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red);
   else if (format == 3 && acc64)
