@@ -214,7 +214,7 @@ def main():
         except Exception as e:  # the oracle is optional on the box
             log(f"[bench] cpu baseline skipped: {e}")
 
-    traffic, traffic_src = profiled_traffic("pcg_c2_hex605k.md", f"k_cg_spmv_bsr<float, {fmt}")
+    traffic, traffic_src = profiled_traffic("pcg_c2_hex605k.md", f"k_cg_spmv_bsr<float, {fmt}, double>")
     if args.config != "c2":
         traffic, traffic_src = None, None
     value = iters_mixed / t_mixed
@@ -284,6 +284,8 @@ def bench_tube(args):
     m = decks.cquad4_tube(n_cases=32)
     k = len(m.loads)
     res = {}
+    if args.profile:
+        return profile_tube(args, m)
     for prec, label in ((_lib.VBH_PREC_FP64, "fp64"), (_lib.VBH_PREC_MIXED, "mixed")):
         be = HipPcgSolverBackend.try_create(tolerance=args.tol, max_iters=200000)
         lib.vbh_set_precision(be._h, prec)
@@ -339,6 +341,36 @@ def bench_tube(args):
            "config": {"workload": m.meta["name"], "n_dof": int(n), "load_cases": k, "tol": args.tol},
            "fp64": res["fp64"], "mixed": res["mixed"]}
     print(json.dumps(out), flush=True)
+
+
+def profile_tube(args, m):
+    """Short tube run for profilers: 200 block iterations per precision (the
+    solve stops at the iteration limit), then a 256 MiB copy that calibrates
+    the HBM byte counters."""
+    from vibestran_amd import HipPcgSolverBackend, SolverError, _lib
+    from vibestran_amd import model as M
+    from vibestran_amd.assembly import assemble_model, dof_map_device
+
+    lib = _lib.load()
+    k = len(m.loads)
+    for prec in (_lib.VBH_PREC_FP64, _lib.VBH_PREC_MIXED):
+        be = HipPcgSolverBackend.try_create(tolerance=args.tol, max_iters=200)
+        lib.vbh_set_precision(be._h, prec)
+        eq, n = dof_map_device(be, m)
+        K = assemble_model(be, m, eq, n)
+        F = np.stack([M.assemble_rhs(m, eq, n, lc) for lc in m.loads], axis=1)
+        try:
+            be.solve_block(K, F)
+        except SolverError:
+            pass  # stopped at the 200-iteration limit, as intended
+        cfg = be.block_config()
+        log(f"[bench] tube profile prec={prec}: {be.last_block_iterations.max()} iterations, "
+            f"SpMM cw={cfg.cols_per_lane} lg={cfg.lanes_per_group} stage={cfg.lds_staged} "
+            f"{cfg.spmm_ms*1e3:.1f} us")
+        ms = C.c_double(0.0)
+        lib.vbh_bench_spmv(be._h, 6, 5, C.byref(ms))
+        be.close()
+    print(json.dumps({"profile": "tube", "load_cases": k}), flush=True)
 
 
 def bench_simp(args):
