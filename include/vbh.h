@@ -156,6 +156,34 @@ int vbh_element_sensitivity(vbh_ctx* ctx, int param, const double* u, const doub
 int vbh_element_sensitivity_device(vbh_ctx* ctx, int param, const double* du, const double* dv,
                                    double* dout);
 
+/* ---- result recovery: LinearStaticSolver::recover_results
+ * (linear_static.cpp:566-1201) over the mesh of the last vbh_assemble.
+ * u is the solution (n free DOFs). Element rows hold the element CSV columns
+ * [sx, sy, sxy, sz, syz, szx, mx, my, mxy, von_mises] (csv_writer.cpp:66-85);
+ * components that do not apply to the element are 0. centroid is nelem x 10
+ * (solids: centroid stress, 1047-1197; CQUAD4: centroid membrane stress and
+ * moments, 712-736). corner (nelem x 8 x 10, may be NULL) holds the per-node
+ * points in element node order (CHEXA8 8, CPENTA6 6, CTETRA4 4 copies of the
+ * constant stress, CQUAD4 4; unused rows 0). Solids use the displacement B at
+ * the point, EAS formulations included, as the reference does. */
+int vbh_recover_stress(vbh_ctx* ctx, const double* u, double* centroid, double* corner);
+int vbh_recover_stress_device(vbh_ctx* ctx, const double* du, double* dcentroid,
+                              double* dcorner);
+/* Node displacements T1..R3 (recover_results 568-612, basic frame): disp is
+ * nnodes x 6 in the order of the assembled mesh's nodes; constrained and
+ * inactive DOFs are 0. */
+int vbh_recover_displacements(vbh_ctx* ctx, const double* u, double* disp);
+int vbh_recover_displacements_device(vbh_ctx* ctx, const double* du, double* ddisp);
+/* CSV result files of CsvWriter::write (csv_writer.cpp:21-93), host only (no
+ * device needed). append = 0 truncates the file and writes the header line;
+ * append != 0 adds the rows of another subcase. Node rows go out in ascending
+ * node ID order; element rows CQUAD4 first, then solids, each in element
+ * order; values as "%.8e". Errors: VBH_ERR_SOLVER "Cannot write CSV: <path>". */
+int vbh_write_node_csv(const char* path, int append, int subcase_id, int nnodes,
+                       const int* node_ids, const double* disp);
+int vbh_write_elem_csv(const char* path, int append, int subcase_id, int nelem,
+                       const int* elem_ids, const int* etype, const double* rows);
+
 /* ---- solve: SolverBackend::solve(K, F) -> u (solver_backend.hpp:26-29) ----
  * F and u are host arrays of length n. iters / rel_res may be NULL. */
 int vbh_solve(vbh_ctx* ctx, const double* F, double* u, int* iters, double* rel_res);

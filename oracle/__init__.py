@@ -82,6 +82,8 @@ def _load():
         "orc_quad4_dke_dt": (I, [DP, D, D, D, D, D, D, I, DP]),
         "orc_element_sensitivity": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I, I, DP, DP, DP]),
         "orc_grav_rhs": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, DP]),
+        "orc_quad4_recover": (I, [DP, D, D, D, D, I, DP, D, D, DP]),
+        "orc_recover_stress": (I, [I, IP, IP, IP, DP, IP, DP, DP, DP, I, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -369,6 +371,38 @@ def element_sensitivity(n, model, eq, param, u, v):
     if rc != 0:
         raise OracleError(_err())
     return out
+
+
+def quad4_recover(xyz, E, nu, t, ue, xi, eta, formulation="MITC4", twelveI_t3=1.0):
+    """recover_shell_response at (xi, eta): (sx, sy, sxy, mx, my, mxy)."""
+    x = _f64(xyz).reshape(-1)
+    u = _f64(ue).reshape(-1)
+    out = np.empty(6, dtype=np.float64)
+    rc = lib().orc_quad4_recover(_dp(x), E, nu, t, twelveI_t3,
+                                 0 if formulation == "MITC4" else 1, _dp(u), xi, eta, _dp(out))
+    if rc != 0:
+        raise OracleError(_err())
+    return out
+
+
+STRESS_COLUMNS = ("sx", "sy", "sxy", "sz", "syz", "szx", "mx", "my", "mxy", "von_mises")
+
+
+def recover_stress(model, eq, u, corners=True):
+    """Element stresses of recover_results: (centroid [nelem, 10], corner
+    [nelem, 8, 10] or None), columns STRESS_COLUMNS."""
+    et, ep, cn = _i32(model.elem_type), _i32(model.elem_ptr), _i32(model.conn)
+    x, q = _f64(model.xyz).reshape(-1), _i32(eq).reshape(-1)
+    e, nv = _f64(model.E), _f64(model.nu)
+    th = _f64(model.thickness if model.thickness is not None else np.zeros(len(et)))
+    cen = np.empty((len(et), 10), dtype=np.float64)
+    cor = np.empty((len(et), 8, 10), dtype=np.float64) if corners else None
+    rc = lib().orc_recover_stress(len(et), _ip(et), _ip(ep), _ip(cn), _dp(x), _ip(q), _dp(e),
+                                  _dp(nv), _dp(th), int(model.shell_form), _dp(_f64(u)),
+                                  _dp(cen), _dp(cor) if corners else None)
+    if rc != 0:
+        raise OracleError(_err())
+    return cen, cor
 
 
 # ---------------------------------------------------------------- PCG

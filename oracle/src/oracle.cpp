@@ -844,6 +844,176 @@ int orc_grav_rhs(int n, int nelem, const int* et, const int* ep, const int* conn
   return ORC_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Element stress recovery: LinearStaticSolver::recover_results, element part
+// (linear_static.cpp:676-1197). Rows hold the CSV columns
+// [sx, sy, sxy, sz, syz, szx, mx, my, mxy, von_mises] (csv_writer.cpp:66-85).
+// ---------------------------------------------------------------------------
+namespace {
+
+// solid_von_mises: linear_static.cpp:140-151
+double solid_vm(const double* s) {
+  return std::sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) +
+                          (s[2] - s[0]) * (s[2] - s[0]) +
+                          6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+}
+
+// sigma = D (B ue) (solid_sigma_from_B 186-196, no thermal strain) as a row.
+void solid_row(int nn, const double* dnx, const double* dny, const double* dnz, double E,
+               double nu, const double* ue, double* row) {
+  std::vector<double> B(6 * 3 * static_cast<size_t>(nn));
+  fill_B(nn, dnx, dny, dnz, B.data());
+  double D[6][6];
+  isotropic_D(E, nu, D);
+  double eps[6], sig[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < 3 * nn; ++j) s += B[static_cast<size_t>(i * 3 * nn + j)] * ue[j];
+    eps[i] = s;
+  }
+  for (int i = 0; i < 6; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < 6; ++j) s += D[i][j] * eps[j];
+    sig[i] = s;
+  }
+  const double r[10] = {sig[0], sig[1], sig[3], sig[2], sig[4], sig[5], 0.0, 0.0, 0.0,
+                        solid_vm(sig)};
+  std::memcpy(row, r, sizeof r);
+}
+
+// eval_chexa8 / eval_cpenta6 (1058-1117, 996-1056): displacement B at a point.
+void iso_point_row(int et, const double* xe, double E, double nu, const double* ue, double r0,
+                   double r1, double r2, double* row) {
+  double d0[8], d1[8], d2[8];
+  int nn;
+  if (et == ORC_CHEXA8) {
+    const HexShape s = hex_shape(r0, r1, r2);
+    nn = 8;
+    std::memcpy(d0, s.dxi, sizeof s.dxi);
+    std::memcpy(d1, s.deta, sizeof s.deta);
+    std::memcpy(d2, s.dzeta, sizeof s.dzeta);
+  } else {
+    const PentaShape s = penta_shape(r0, r1, r2);
+    nn = 6;
+    std::memcpy(d0, s.dL1, sizeof s.dL1);
+    std::memcpy(d1, s.dL2, sizeof s.dL2);
+    std::memcpy(d2, s.dz, sizeof s.dz);
+  }
+  const M3 J = jacobian(nn, d0, d1, d2, xe);
+  const M3 Ji = inv3(J);
+  double dnx[8], dny[8], dnz[8];
+  phys_grad(nn, Ji, d0, d1, d2, dnx, dny, dnz);
+  solid_row(nn, dnx, dny, dnz, E, nu, ue, row);
+}
+
+// eval_ctetra4 (830-882): gradients of the CST shape functions from the
+// cofactors of [1 x y z] divided by its determinant.
+void tet_row(const double* xe, double E, double nu, const double* ue, double* row) {
+  double A[4][4];
+  for (int r = 0; r < 4; ++r) {
+    A[r][0] = 1.0;
+    for (int d = 0; d < 3; ++d) A[r][1 + d] = xe[3 * r + d];
+  }
+  auto minor_det = [&](int skip_r, int skip_c) {
+    M3 m;
+    int ri = 0;
+    for (int r = 0; r < 4; ++r) {
+      if (r == skip_r) continue;
+      int ci = 0;
+      for (int c = 0; c < 4; ++c) {
+        if (c == skip_c) continue;
+        m.a[ri][ci++] = A[r][c];
+      }
+      ++ri;
+    }
+    return det3(m);
+  };
+  double V6 = 0.0;
+  for (int c = 0; c < 4; ++c) V6 += ((c & 1) ? -1.0 : 1.0) * A[0][c] * minor_det(0, c);
+  double dnx[4], dny[4], dnz[4];
+  for (int n = 0; n < 4; ++n) {
+    // cofA(i, n) = (-1)^(i+n) det(A without row n and column i)
+    const double sg = (n & 1) ? -1.0 : 1.0;
+    dnx[n] = -sg * minor_det(n, 1) / V6;
+    dny[n] = sg * minor_det(n, 2) / V6;
+    dnz[n] = -sg * minor_det(n, 3) / V6;
+  }
+  solid_row(4, dnx, dny, dnz, E, nu, ue, row);
+}
+
+// make_plate_stress_point (168-184) from orc_quad4_recover's six values.
+void plate_row(const double* v6, double* row) {
+  const double sx = v6[0], sy = v6[1], sxy = v6[2];
+  const double r[10] = {sx, sy, sxy, 0.0, 0.0, 0.0, v6[3], v6[4], v6[5],
+                        std::sqrt(sx * sx - sx * sy + sy * sy + 3.0 * sxy * sxy)};
+  std::memcpy(row, r, sizeof r);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_recover_stress(int nelem, const int* et, const int* ep, const int* conn,
+                       const double* xyz, const int* eq, const double* E, const double* nu,
+                       const double* thick, int shell_form, const double* u, double* centroid,
+                       double* corner) {
+  static const double kHex[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  static const double kPenta[6][3] = {{1, 0, -1}, {0, 1, -1}, {0, 0, -1},
+                                      {1, 0, 1},  {0, 1, 1},  {0, 0, 1}};
+  static const double kQuad[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  double xe[24], ue[24];
+  for (int e = 0; e < nelem; ++e) {
+    const int nn = ep[e + 1] - ep[e];
+    const int dpn = (et[e] == ORC_CQUAD4) ? 6 : 3;
+    for (int a = 0; a < nn; ++a) {
+      const int node = conn[ep[e] + a];
+      for (int d = 0; d < 3; ++d) xe[3 * a + d] = xyz[3 * node + d];
+      for (int d = 0; d < dpn; ++d) {
+        const int q = eq[6 * node + d];
+        ue[dpn * a + d] = q >= 0 ? u[q] : 0.0;
+      }
+    }
+    double* cen = centroid + 10 * static_cast<size_t>(e);
+    double* cor = corner ? corner + 80 * static_cast<size_t>(e) : nullptr;
+    if (cor) std::fill(cor, cor + 80, 0.0);
+    if (et[e] == ORC_CHEXA8) {
+      iso_point_row(et[e], xe, E[e], nu[e], ue, 0.0, 0.0, 0.0, cen);
+      if (cor)
+        for (int c = 0; c < 8; ++c)
+          iso_point_row(et[e], xe, E[e], nu[e], ue, kHex[c][0], kHex[c][1], kHex[c][2],
+                        cor + 10 * c);
+    } else if (et[e] == ORC_CPENTA6) {
+      iso_point_row(et[e], xe, E[e], nu[e], ue, 1.0 / 3.0, 1.0 / 3.0, 0.0, cen);
+      if (cor)
+        for (int c = 0; c < 6; ++c)
+          iso_point_row(et[e], xe, E[e], nu[e], ue, kPenta[c][0], kPenta[c][1], kPenta[c][2],
+                        cor + 10 * c);
+    } else if (et[e] == ORC_CTETRA4) {
+      tet_row(xe, E[e], nu[e], ue, cen);
+      if (cor)
+        for (int c = 0; c < 4; ++c) std::memcpy(cor + 10 * c, cen, 10 * sizeof(double));
+    } else if (et[e] == ORC_CQUAD4) {
+      double v6[6];
+      int rc = orc_quad4_recover(xe, E[e], nu[e], thick[e], 1.0, shell_form, ue, 0.0, 0.0, v6);
+      if (rc != ORC_OK) return rc;
+      plate_row(v6, cen);
+      if (cor)
+        for (int c = 0; c < 4; ++c) {
+          rc = orc_quad4_recover(xe, E[e], nu[e], thick[e], 1.0, shell_form, ue, kQuad[c][0],
+                                 kQuad[c][1], v6);
+          if (rc != ORC_OK) return rc;
+          plate_row(v6, cor + 10 * c);
+        }
+    } else {
+      return fail(ORC_ERR_ARG, "orc_recover_stress: unsupported element type");
+    }
+  }
+  return ORC_OK;
+}
+
 int orc_element_sensitivity(int n, int nelem, const int* et, const int* ep, const int* conn,
                             const double* xyz, const int* eq, const double* E, const double* nu,
                             const double* thick, int solid_form, int shell_form, int param,

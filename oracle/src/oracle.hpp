@@ -141,6 +141,21 @@ OrcCsr* orc_assemble_model(int n, int nelem, const int* elem_type, const int* el
 int orc_grav_rhs(int n, int nelem, const int* elem_type, const int* elem_ptr, const int* conn,
                  const double* xyz, const int* eq, const double* rho, const double* thickness,
                  const double* g, double* F);
+// recover_shell_response (cquad4.cpp:758-801) at (xi, eta) without thermal
+// strain: out6 = membrane stress (sx, sy, sxy) and moments (mx, my, mxy) from
+// the 24 global element displacements ue.
+int orc_quad4_recover(const double* xyz12, double E, double nu, double t, double twelveI_t3,
+                      int formulation, const double* ue24, double xi, double eta, double* out6);
+// Element stress recovery (LinearStaticSolver::recover_results element part,
+// linear_static.cpp:676-1197): centroid row per element and up to 8 corner
+// rows (element node order; unused rows 0), each
+// [sx, sy, sxy, sz, syz, szx, mx, my, mxy, von_mises] as in the element CSV
+// (csv_writer.cpp:66-85). Solids use the displacement B at the point, also for
+// EAS elements. centroid is nelem*10, corner nelem*80 or null.
+int orc_recover_stress(int nelem, const int* elem_type, const int* elem_ptr, const int* conn,
+                       const double* xyz, const int* eq, const double* E, const double* nu,
+                       const double* thickness, int shell_form, const double* u,
+                       double* centroid, double* corner);
 // FP64 CPU adjoint contraction (no reference counterpart): out[e] =
 // -u_e^T (dK_e/dp_e) v_e, p = E (param 0) or CQUAD4 thickness (param 1;
 // solids 0). Compliance: v = u; a functional L^T u: v = lambda, K lambda = L.

@@ -482,5 +482,85 @@ int orc_quad4_me(const double* X, double rho, double t, double* Me_out) {
   return ORC_OK;
 }
 
+// recover_shell_response (cquad4.cpp:758-801) at (xi, eta), no thermal strain:
+// u_local = T u (to_local_displacements 491-502), membrane DOFs (u, v) and
+// plate DOFs (rx, ry, w) of each node (gather_* 504-522), membrane strain
+// Bm u_m (MITC4+ tied B, or the standard B for Mindlin), curvature Bb u_p.
+// out6 = membrane stress Dp eps (sx, sy, sxy) then moments D kappa (mx, my,
+// mxy), D = (12I/t^3) t^3/12 Dp (build_shell_section 189-218; no MID4
+// coupling, so the section's B term is zero).
+int orc_quad4_recover(const double* X, double E, double nu, double t, double twelveI_t3,
+                      int form, const double* ue, double xi, double eta, double* out6) {
+  const V3 g[4] = {{X[0], X[1], X[2]}, {X[3], X[4], X[5]}, {X[6], X[7], X[8]}, {X[9], X[10], X[11]}};
+  const V3 v12 = sub(g[1], g[0]), v14 = sub(g[3], g[0]);
+  const V3 e3 = normalized(cross(v12, v14));
+  const V3 e1 = normalized(v12);
+  const V3 e2 = cross(e3, e1);
+  double xl[4], yl[4];
+  for (int n = 0; n < 4; ++n) {
+    xl[n] = dot(g[n], e1);
+    yl[n] = dot(g[n], e2);
+  }
+  const double R[3][3] = {{e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, {e3.x, e3.y, e3.z}};
+  const double Rr[3][3] = {{-R[1][0], -R[1][1], -R[1][2]}, {R[0][0], R[0][1], R[0][2]},
+                           {R[2][0], R[2][1], R[2][2]}};
+  double ul[24];
+  for (int n = 0; n < 4; ++n)
+    for (int a = 0; a < 3; ++a) {
+      double st = 0.0, sr = 0.0;
+      for (int b = 0; b < 3; ++b) {
+        st += R[a][b] * ue[6 * n + b];
+        sr += Rr[a][b] * ue[6 * n + 3 + b];
+      }
+      ul[6 * n + a] = st;
+      ul[6 * n + 3 + a] = sr;
+    }
+  double um[8], up[12];
+  for (int n = 0; n < 4; ++n) {
+    um[2 * n] = ul[6 * n];
+    um[2 * n + 1] = ul[6 * n + 1];
+    up[3 * n] = ul[6 * n + 3];
+    up[3 * n + 1] = ul[6 * n + 4];
+    up[3 * n + 2] = ul[6 * n + 2];
+  }
+  QPoint q;
+  if (!eval_point(xl, yl, xi, eta, q)) {
+    orc_internal_set_error("CQUAD4: negative Jacobian");
+    return ORC_ERR_SOLVER;
+  }
+  double Bm[3][8], Bb[3][12];
+  if (form == 0) {
+    MitcM mm{};
+    const int rc = mitc_membrane_data(xl, yl, mm);
+    if (rc == 1) { orc_internal_set_error("CQUAD4: negative Jacobian"); return ORC_ERR_SOLVER; }
+    if (rc == 2) { orc_internal_set_error("CQUAD4: MITC4+ characteristic geometry is singular"); return ORC_ERR_SOLVER; }
+    if (rc == 3) { orc_internal_set_error("CQUAD4: MITC4+ distortion denominator is near zero"); return ORC_ERR_SOLVER; }
+    mitc_membrane_B(mm, q, Bm);
+  } else {
+    std_membrane_B(q, Bm);
+  }
+  bending_B(q, Bb);
+  double eps[3], kap[3];
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0, k = 0.0;
+    for (int j = 0; j < 8; ++j) s += Bm[i][j] * um[j];
+    for (int j = 0; j < 12; ++j) k += Bb[i][j] * up[j];
+    eps[i] = s;
+    kap[i] = k;
+  }
+  const double c = E / (1.0 - nu * nu);
+  const double Dp[3][3] = {{c, c * nu, 0.0}, {c * nu, c, 0.0}, {0.0, 0.0, c * (1.0 - nu) / 2.0}};
+  const double ib = twelveI_t3 * t * t * t / 12.0;
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0, m = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      s += Dp[i][j] * eps[j];
+      m += ib * Dp[i][j] * kap[j];
+    }
+    out6[i] = s;
+    out6[3 + i] = m;
+  }
+  return ORC_OK;
+}
 
 }  // extern "C"

@@ -73,6 +73,12 @@ SIGNATURES = {
     "vbh_last_assembly_stats": (_I, [_P, C.POINTER(vbh_assembly_stats)]),
     "vbh_assemble_rhs": (_I, [_P, _DP, _I, _IP, _DP, _DP, _DP, _DP]),
     "vbh_assemble_rhs_device": (_I, [_P, _DP, _I, _IP, _DP, _DP, _DP, _P]),
+    "vbh_recover_stress": (_I, [_P, _DP, _DP, _DP]),
+    "vbh_recover_stress_device": (_I, [_P, _P, _P, _P]),
+    "vbh_recover_displacements": (_I, [_P, _DP, _DP]),
+    "vbh_recover_displacements_device": (_I, [_P, _P, _P]),
+    "vbh_write_node_csv": (_I, [C.c_char_p, _I, _I, _I, _IP, _DP]),
+    "vbh_write_elem_csv": (_I, [C.c_char_p, _I, _I, _I, _IP, _IP, _DP]),
     "vbh_element_sensitivity": (_I, [_P, _I, _DP, _DP, _DP]),
     "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),
     "vbh_solve": (_I, [_P, _DP, _DP, _IP, _DP]),

@@ -202,4 +202,6 @@ def read_bdf(path_or_text: str, is_text=False) -> Model:
                         p=np.array([r[3] for r in rows]))
         m.loads.append(LoadCase(nodal=nodal, pload4=pl))
     m.meta["params"] = params
+    m.elem_ids = np.array([eid for eid, *_ in elems], dtype=np.int32)
+    m.subcase_ids = [s["id"] for s in subcases] if subcases else [1]
     return m

@@ -116,6 +116,8 @@ struct Ctx {
   void element_sensitivity(int param, const double* u, const double* v, double* out, bool dev);
   void assemble_rhs(const double* nodal, int nface, const int* face_nodes, const double* face_p,
                     const double* grav, const double* rho, double* F, bool dev);
+  void recover_stress(const double* u, double* centroid, double* corner, bool dev);
+  void recover_displacements(const double* u, double* disp, bool dev);
   void fetch_lower_to_host();
   int dof_map(int nnodes, const int* ids, int nelem, const int* etype, const int* eptr,
               const int* conn, int nspc, const int* spc_nodes, const int* spc_masks, int* eq_out);

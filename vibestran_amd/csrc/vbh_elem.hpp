@@ -60,4 +60,12 @@ void launch_quad4_sensitivity(int nsel, const int* sel, const int* eptr, const i
                               const double* u, const double* v, double* out, int* err,
                               hipStream_t s);
 
+// Stress recovery of the selected CQUAD4 elements: centroid rows cen[e*10..]
+// and, when cor is not null, corner rows cor[e*80 + 10*c..] (c < 4; rows 4..7
+// are zeroed). Columns [sx, sy, sxy, sz, syz, szx, mx, my, mxy, von_mises].
+void launch_quad4_recover(int nsel, const int* sel, const int* eptr, const int* conn,
+                          const double* xyz, const double* E, const double* nu,
+                          const double* thick, int form, const int* gd, const double* u,
+                          double* cen, double* cor, int* err, hipStream_t s);
+
 }  // namespace vbh

@@ -513,7 +513,150 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
   }
 }
 
+// Covariant membrane strains (e_rr, e_ss, e_rs) of the membrane DOFs um at a
+// tying point: the contraction of cov_membrane's columns with um.
+__device__ void cov_membrane_strain(const Frame& f, double xi, double eta, const double* um,
+                                    double& rr, double& ss, double& rs) {
+  Point q;
+  point(f, xi, eta, q);
+  rr = ss = rs = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    const int n = k >> 1, c = k & 1;
+    const double gr = q.J[0][c], gs = q.J[1][c];
+    rr += gr * q.s.dxi[n] * um[k];
+    ss += gs * q.s.deta[n] * um[k];
+    rs += 0.5 * (gr * q.s.deta[n] + gs * q.s.dxi[n]) * um[k];
+  }
+}
+
+// Stress recovery of recover_shell_response (cquad4.cpp:758-801), one thread
+// per element: the displacements go to the local frame (T u,
+// to_local_displacements 491-502), the membrane strain is the MITC4+ tied
+// strain (the tying-point strains contracted once, then interpolated as in
+// mitc4plus_membrane_B) or the standard one (Mindlin), the curvature the
+// bending B of the plate DOFs (rx, ry, w). Rows are
+// [sx, sy, sxy, 0, 0, 0, mx, my, mxy, von_mises] at the centroid and at the
+// four corners (recover_results, linear_static.cpp:712-753); no thermal strain.
+__global__ __launch_bounds__(256) void k_quad4_recover(
+    int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
+    const int* __restrict__ conn, const double* __restrict__ xyz, const double* __restrict__ E,
+    const double* __restrict__ nu, const double* __restrict__ thick, int form,
+    const int* __restrict__ gd, const double* __restrict__ u, double* __restrict__ cen,
+    double* __restrict__ cor, int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsel) return;
+  const int e = sel[i];
+  const int o = eptr[e];
+  double X[4][3];
+  for (int a = 0; a < 4; ++a) {
+    const int node = conn[o + a];
+    for (int d = 0; d < 3; ++d) X[a][d] = xyz[3 * node + d];
+  }
+  Frame f;
+  make_frame(X, f);
+  const int* g = gd + static_cast<size_t>(e) * kGdStride;
+  double um[8], up[12];
+  for (int n = 0; n < 4; ++n) {
+    double ut[3], th[3];
+    for (int d = 0; d < 3; ++d) {
+      const int qt = g[6 * n + d], qr = g[6 * n + 3 + d];
+      ut[d] = qt >= 0 ? u[qt] : 0.0;
+      th[d] = qr >= 0 ? u[qr] : 0.0;
+    }
+    double loc[3], rot[3];
+    for (int a = 0; a < 3; ++a) {
+      loc[a] = f.R[a][0] * ut[0] + f.R[a][1] * ut[1] + f.R[a][2] * ut[2];
+      rot[a] = f.R[a][0] * th[0] + f.R[a][1] * th[1] + f.R[a][2] * th[2];
+    }
+    um[2 * n] = loc[0];
+    um[2 * n + 1] = loc[1];
+    up[3 * n] = -rot[1];      // local rx = -(e2 . theta)
+    up[3 * n + 1] = rot[0];   // local ry = e1 . theta
+    up[3 * n + 2] = loc[2];   // w
+  }
+  int bad = 0;
+  MitcCoef a{};
+  double T_rr = 0, B_rr = 0, R_ss = 0, L_ss = 0, C_rs = 0;
+  if (form == 0) {
+    bad |= mitc_coefficients(f, a);
+    double x1, x2;
+    cov_membrane_strain(f, 0.0, 1.0, um, T_rr, x1, x2);
+    cov_membrane_strain(f, 0.0, -1.0, um, B_rr, x1, x2);
+    cov_membrane_strain(f, 1.0, 0.0, um, x1, R_ss, x2);
+    cov_membrane_strain(f, -1.0, 0.0, um, x1, L_ss, x2);
+    cov_membrane_strain(f, 0.0, 0.0, um, x1, x2, C_rs);
+  }
+  const double c = E[e] / (1.0 - nu[e] * nu[e]);
+  const double d01 = c * nu[e], d22 = c * (1.0 - nu[e]) / 2.0;
+  const double t = thick[e], ib = t * t * t / 12.0;  // 12I/t^3 = 1
+  const double pts[5][2] = {{0.0, 0.0}, {-1.0, -1.0}, {1.0, -1.0}, {1.0, 1.0}, {-1.0, 1.0}};
+  const int npt = cor ? 5 : 1;
+  for (int p = 0; p < npt; ++p) {
+    const double r = pts[p][0], s = pts[p][1];
+    Point q;
+    point(f, r, s, q);
+    if (!(q.det > 0.0)) bad |= kErrJacobian;
+    double eps[3];
+    if (form == 0) {
+      const double err_ = 0.5 * (1.0 - 2.0 * a.top + s + 2.0 * a.top * s * s) * T_rr +
+                          0.5 * (1.0 - 2.0 * a.bottom - s + 2.0 * a.bottom * s * s) * B_rr +
+                          a.right * (-1.0 + s * s) * R_ss + a.left * (-1.0 + s * s) * L_ss +
+                          a.center * (-1.0 + s * s) * C_rs;
+      const double ess_ = a.top * (-1.0 + r * r) * T_rr + a.bottom * (-1.0 + r * r) * B_rr +
+                          0.5 * (1.0 - 2.0 * a.right + r + 2.0 * a.right * r * r) * R_ss +
+                          0.5 * (1.0 - 2.0 * a.left - r + 2.0 * a.left * r * r) * L_ss +
+                          a.center * (-1.0 + r * r) * C_rs;
+      const double ers_ = 0.25 * (r + 4.0 * a.top * r * s) * T_rr +
+                          0.25 * (-r + 4.0 * a.bottom * r * s) * B_rr +
+                          0.25 * (s + 4.0 * a.right * r * s) * R_ss +
+                          0.25 * (-s + 4.0 * a.left * r * s) * L_ss +
+                          (1.0 + a.center * r * s) * C_rs;
+      const double A = q.Ji[0][0], B = q.Ji[1][0], C = q.Ji[0][1], D = q.Ji[1][1];
+      eps[0] = A * A * err_ + C * C * ess_ + 2.0 * A * C * ers_;
+      eps[1] = B * B * err_ + D * D * ess_ + 2.0 * B * D * ers_;
+      eps[2] = 2.0 * A * B * err_ + 2.0 * C * D * ess_ + 2.0 * (A * D + B * C) * ers_;
+    } else {
+      eps[0] = eps[1] = eps[2] = 0.0;
+      for (int n = 0; n < 4; ++n) {
+        eps[0] += q.dN[0][n] * um[2 * n];
+        eps[1] += q.dN[1][n] * um[2 * n + 1];
+        eps[2] += q.dN[1][n] * um[2 * n] + q.dN[0][n] * um[2 * n + 1];
+      }
+    }
+    double kap[3] = {0.0, 0.0, 0.0};
+    for (int n = 0; n < 4; ++n) {
+      kap[0] += q.dN[0][n] * up[3 * n];
+      kap[1] += q.dN[1][n] * up[3 * n + 1];
+      kap[2] += q.dN[1][n] * up[3 * n] + q.dN[0][n] * up[3 * n + 1];
+    }
+    const double sx = c * eps[0] + d01 * eps[1], sy = d01 * eps[0] + c * eps[1],
+                 sxy = d22 * eps[2];
+    const double row[10] = {sx, sy, sxy, 0.0, 0.0, 0.0,
+                            ib * (c * kap[0] + d01 * kap[1]), ib * (d01 * kap[0] + c * kap[1]),
+                            ib * d22 * kap[2],
+                            sqrt(sx * sx - sx * sy + sy * sy + 3.0 * sxy * sxy)};
+    double* out = p == 0 ? cen + 10 * static_cast<size_t>(e)
+                         : cor + 80 * static_cast<size_t>(e) + 10 * (p - 1);
+    for (int k = 0; k < 10; ++k) out[k] = row[k];
+  }
+  if (cor)
+    for (int k = 40; k < 80; ++k) cor[80 * static_cast<size_t>(e) + k] = 0.0;
+  if (bad) {
+    atomicOr(&err[0], bad);
+    atomicMin(&err[1], e);
+  }
+}
+
 }  // namespace
+
+void launch_quad4_recover(int nsel, const int* sel, const int* eptr, const int* conn,
+                          const double* xyz, const double* E, const double* nu,
+                          const double* thick, int form, const int* gd, const double* u,
+                          double* cen, double* cor, int* err, hipStream_t s) {
+  if (nsel <= 0) return;
+  hipLaunchKernelGGL(k_quad4_recover, dim3((nsel + 255) / 256), dim3(256), 0, s, nsel, sel, eptr,
+                     conn, xyz, E, nu, thick, form, gd, u, cen, cor, err);
+}
 
 void launch_quad4_stiffness(int nsel, const int* sel, const int* eptr, const int* conn,
                             const double* xyz, const double* E, const double* nu,

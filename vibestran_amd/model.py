@@ -8,6 +8,7 @@ Data layout (host side, handed to libvbh as flat arrays):
   conn      int32[...]      0-based node indices, reference node order
   E, nu, rho float64[ne]    per-element MAT1 data
   thickness float64[ne]     PSHELL T (shells only)
+  elem_ids  int32[ne]       external element IDs (optional)
 """
 from __future__ import annotations
 
@@ -59,6 +60,16 @@ class Model:
     spc_masks: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     loads: List[LoadCase] = field(default_factory=list)
     meta: dict = field(default_factory=dict)
+    elem_ids: Optional[np.ndarray] = None  # external element IDs (default 1..ne)
+    subcase_ids: Optional[List[int]] = None  # SUBCASE IDs of `loads` (default 1..)
+
+    def element_ids(self) -> np.ndarray:
+        if self.elem_ids is not None:
+            return np.asarray(self.elem_ids, dtype=np.int32)
+        return np.arange(1, self.ne + 1, dtype=np.int32)
+
+    def subcase_id(self, k: int) -> int:
+        return int(self.subcase_ids[k]) if self.subcase_ids is not None else k + 1
 
     @property
     def nn(self) -> int:
