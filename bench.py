@@ -82,17 +82,27 @@ def cpu_baseline(model, eq, n, F, seconds):
 
 
 def profiled_traffic(profile, kernel):
-    """HBM bytes per launch of `kernel` from a committed rocprof summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration)."""
+    """From a committed rocprof summary: HBM bytes per launch of `kernel`
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) and its average
+    duration in us (kernel-trace statistics)."""
     path = os.path.join(ROOT, "profiles", profile)
+    traffic = avg_us = None
     try:
+        table = None
         for line in open(path):
+            if line.startswith("| kernel | calls | avg us"):
+                table = "dur"
+            elif line.startswith("| kernel | FETCH_SIZE"):
+                table = "traffic"
             if line.startswith("| `" + kernel + "` |"):
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                return int((float(cells[3]) + float(cells[4])) * 1e6), f"profiles/{profile}"
+                if table == "traffic":
+                    traffic = int((float(cells[3]) + float(cells[4])) * 1e6)
+                elif table == "dur" and avg_us is None:
+                    avg_us = float(cells[2])
     except OSError:
-        pass
-    return None, None
+        return None, None, None
+    return traffic, avg_us, f"profiles/{profile}"
 
 
 def main():
@@ -173,24 +183,27 @@ def main():
         fp64 = {"time_s": t64, "iters": it64, "iters_per_s": it64 / t64, "rel_res": rel64}
         log(f"[bench] fp64 solve: {it64} iters, rel {rel64:.3e}, {t64*1e3:.1f} ms")
 
-    # dominant kernel: the fused FP32 PCG SpMV (search direction + K p + p.Ap),
-    # timed with hipEvents on the context stream; plus the update kernel
+    # dominant kernel: the fused FP32 PCG SpMV (K p + p.Ap partials), timed with
+    # hipEvents on the context stream: `reps` back-to-back launches between one
+    # event pair (the per-launch event cost, ~5 us here, would otherwise be
+    # counted). The per-launch figures inside directly launched PCG iterations
+    # (direction, SpMV, update; dispatch start/stop events) are reported beside.
     def kt(which, reps=args.spmv_reps):
         ms = C.c_double(0.0)
         if lib.vbh_bench_spmv(h, which, reps, C.byref(ms)) != 0:
             raise RuntimeError(_lib.last_error())
         return ms.value
     lib.vbh_memcpy_h2d(h, dF, F.ctypes.data_as(C.c_void_p), nbytes)
-    ms32, ms64 = kt(3), kt(2)
-    ms_upd32, ms_upd64 = kt(5), kt(4)
+    ms32, ms64 = kt(14), kt(13)  # back to back, one event pair around all launches
+    ms_ctx32, ms_ctx64 = kt(8), kt(7)  # inside iterations, per-launch dispatch events
+    ms_upd32, ms_upd64 = kt(10), kt(9)
+    ms_dir32, ms_dir64 = kt(12), kt(11)
     ms_plain32 = kt(1)
     ms_copy = kt(6, 5)
     copy_gbs = 2 * (256 << 20) / (ms_copy * 1e-3) / 1e9
     nnz_full = 2 * K.nnz - n
-    # algorithmic bytes per launch, full-CSR-equivalent matrix (value + int32
-    # column per nonzero, row pointers) plus the kernel's vector streams:
-    #   cg_spmv  reads r, diag, p_old and writes p_new, Ap   (5 streams)
-    #   cg_update reads p, Ap, r, diag, u(f64), writes u(f64), r
+    # algorithmic bytes per launch: full-CSR-equivalent matrix (value + int32
+    # column per nonzero, row pointers) plus the kernel's vector streams
     #   cg_spmv   reads p (gathered once) and writes Ap         (2 streams)
     #   cg_update reads u(f64), p, r, Ap, diag, writes u(f64), r (z is stored
     #             too, but the algorithmic count leaves it out)
@@ -202,9 +215,41 @@ def main():
     # whole-iteration algorithmic bytes (13 vector streams, FP32 inner)
     b_iter32 = nnz_full * (4 + 4) + 4 * (n + 1) + 13 * n * 4
     iter_gbs = b_iter32 * iters_mixed / (devs[-1]) / 1e9
-    log(f"[bench] cg_spmv fp32 {ms32*1e3:.1f} us, fp64 {ms64*1e3:.1f} us; cg_update fp32 "
-        f"{ms_upd32*1e3:.1f} us, fp64 {ms_upd64*1e3:.1f} us; plain spmv fp32 {ms_plain32*1e3:.1f} us; "
-        f"iteration {devs[-1]/iters_mixed*1e6:.1f} us")
+    log(f"[bench] cg_spmv back to back (us): fp32 {ms32*1e3:.1f} fp64 {ms64*1e3:.1f}; in-iteration "
+        f"events: cg_spmv fp32 {ms_ctx32*1e3:.1f} fp64 {ms_ctx64*1e3:.1f}, cg_update fp32 "
+        f"{ms_upd32*1e3:.1f} fp64 {ms_upd64*1e3:.1f}, cg_direction fp32 {ms_dir32*1e3:.1f} fp64 "
+        f"{ms_dir64*1e3:.1f}; iteration {devs[-1]/iters_mixed*1e6:.1f} us")
+
+    # result recovery of the converged u: centroid + corner stresses on the
+    # device (device-resident u and outputs), node displacements, CSV files
+    post = None
+    if not args.profile:
+        from vibestran_amd import results as R
+        dcen, dcor, ddisp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        assert lib.vbh_device_alloc(h, m.ne * 80, C.byref(dcen)) == 0
+        assert lib.vbh_device_alloc(h, m.ne * 640, C.byref(dcor)) == 0
+        assert lib.vbh_device_alloc(h, m.nn * 48, C.byref(ddisp)) == 0
+        lib.vbh_recover_stress_device(h, du, dcen, dcor)
+        t = time.perf_counter()
+        rc = lib.vbh_recover_stress_device(h, du, dcen, dcor)
+        t_rec = time.perf_counter() - t
+        rc |= lib.vbh_recover_displacements_device(h, du, ddisp)
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        cen = np.empty((m.ne, 10))
+        disp = np.empty((m.nn, 6))
+        lib.vbh_memcpy_d2h(h, cen.ctypes.data_as(C.c_void_p), dcen, cen.nbytes)
+        lib.vbh_memcpy_d2h(h, disp.ctypes.data_as(C.c_void_p), ddisp, disp.nbytes)
+        for d in (dcen, dcor, ddisp):
+            lib.vbh_device_free(h, d)
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            t = time.perf_counter()
+            R.write_csv(os.path.join(td, "c2"), m, [(1, disp, cen)])
+            t_csv = time.perf_counter() - t
+        post = {"recover_stress_ms": round(t_rec * 1e3, 3), "write_csv_ms": round(t_csv * 1e3, 1),
+                "elements": m.ne, "nodes": m.nn}
+        log(f"[bench] recovery (centroid + corners) {t_rec*1e3:.2f} ms, CSV {t_csv*1e3:.0f} ms")
 
     cpu = None
     if not args.no_cpu:
@@ -214,9 +259,10 @@ def main():
         except Exception as e:  # the oracle is optional on the box
             log(f"[bench] cpu baseline skipped: {e}")
 
-    traffic, traffic_src = profiled_traffic("pcg_c2_hex605k.md", f"k_cg_spmv_bsr<float, {fmt}, double>")
+    traffic, prof_us, traffic_src = profiled_traffic("pcg_c2_hex605k.md",
+                                                     f"k_cg_spmv_bsr<float, {fmt}, double>")
     if args.config != "c2":
-        traffic, traffic_src = None, None
+        traffic, prof_us, traffic_src = None, None, None
     value = iters_mixed / t_mixed
     out = {
         "metric": "time-to-1e-8 residual + PCG iters/s, 620k-DOF hex; % HBM roofline vs FP64",
@@ -242,13 +288,18 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"k_cg_spmv_bsr<float,{fmt}> (FP32 PCG SpMV + p.Ap)",
+                     "kernel": f"k_cg_spmv_bsr<float, {fmt}, double> (FP32 PCG SpMV + p.Ap)",
                      "bytes_per_launch": int(spmv32_bytes), "us_per_launch": round(ms32 * 1e3, 2),
                      "measured_GBs": round(traffic / (ms32 * 1e-3) / 1e9, 1) if traffic else None,
+                     "rocprof_us_per_launch": prof_us,
                      "traffic_source": traffic_src},
         "kernels_us": {"cg_spmv_f32": round(ms32 * 1e3, 2), "cg_spmv_f64": round(ms64 * 1e3, 2),
                        "cg_update_f32": round(ms_upd32 * 1e3, 2),
                        "cg_update_f64": round(ms_upd64 * 1e3, 2),
+                       "cg_direction_f32": round(ms_dir32 * 1e3, 2),
+                       "cg_direction_f64": round(ms_dir64 * 1e3, 2),
+                       "cg_spmv_f32_in_iteration": round(ms_ctx32 * 1e3, 2),
+                       "cg_spmv_f64_in_iteration": round(ms_ctx64 * 1e3, 2),
                        "spmv_plain_f32": round(ms_plain32 * 1e3, 2),
                        "cg_update_f32_GBs": round(upd32_bytes / (ms_upd32 * 1e-3) / 1e9, 1),
                        "cg_spmv_f64_GBs": round(ach64, 1),
@@ -258,6 +309,7 @@ def main():
                                "frac": round(iter_gbs / HBM_PEAK_GBS, 4),
                                "us_per_iter": round(devs[-1] / iters_mixed * 1e6, 2)},
         "assembly_ms": {"pattern": ast.pattern_ms, "elements": ast.element_ms},
+        "results": post,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

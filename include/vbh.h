@@ -224,7 +224,13 @@ int vbh_spmv(vbh_ctx* ctx, const double* x, double* y);
  * the context stream. which: 0/1 plain SpMV FP64/FP32, 2/3 the fused PCG
  * SpMV kernel (search direction + K p + p.Ap) FP64/FP32, 4/5 the fused PCG
  * update kernel FP64/FP32, 6 a 256 MiB device-to-device streaming copy
- * (calibration of byte counters). 2..5 use the F of the last solve. */
+ * (calibration of byte counters); 7..12 time one kernel inside `reps`
+ * directly launched PCG iterations (direction, SpMV, update) from the start
+ * and stop timestamps of that kernel's own dispatch (hipExtLaunchKernelGGL
+ * events): 7/8 the fused SpMV FP64/FP32, 9/10 the update, 11/12 the
+ * direction; 13/14 the fused SpMV FP64/FP32 launched `reps` times back to
+ * back between one event pair (kernel plus dispatch gap, no per-launch event
+ * cost). 2..5 and 7..14 use the F of the last solve. */
 int vbh_bench_spmv(vbh_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* ---- device memory helpers for device-resident workflows ---------------- */

@@ -191,14 +191,18 @@ int rows_per_block(int format);
 template <typename T>
 void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
 // acc64: accumulate the SpMV rows in FP64 (products of the stored values).
+// t0/t1 (optional): events that receive the kernel's own start and end times.
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64);
+                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64,
+                    hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 template <typename T>
 void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
-                         const PcgParams& prm, int mixed, hipStream_t s);
+                         const PcgParams& prm, int mixed, hipStream_t s, hipEvent_t t0 = nullptr,
+                         hipEvent_t t1 = nullptr);
 template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s,
+                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // Mixed refresh: r64 = F - K64 u64 (FP64), then decide and reload r32.
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
                              const double* u64, double* r64, PcgScalars* st, const CgRed& red,

@@ -15,6 +15,8 @@
 //    only occasionally.
 // Reference semantics (not code) followed: Jacobi-PCG of
 // cuda_pcg_backend_impl.hpp:1795-1999, kernels 629-751.
+#include <hip/hip_ext.h>
+
 #include "vbh_internal.hpp"
 
 #include <cmath>
@@ -585,33 +587,47 @@ void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
   hipLaunchKernelGGL((k_cg_init<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red);
 }
 
+// Launch with optional kernel timestamps: hipExtLaunchKernelGGL's start/stop
+// events take the dispatch's own begin and end times (what a profiler reports),
+// without the command-processor gap an event record adds around a launch.
+#define VBH_LAUNCH(K, G, S, T0, T1, ...)                                        \
+  do {                                                                          \
+    if (T0) hipExtLaunchKernelGGL(K, G, dim3(kBlock), 0, S, T0, T1, 0u, __VA_ARGS__); \
+    else hipLaunchKernelGGL(K, G, dim3(kBlock), 0, S, __VA_ARGS__);             \
+  } while (0)
+
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64) {
+                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64, hipEvent_t t0,
+                    hipEvent_t t1) {
   if (sizeof(T) == 4 && Ab.dblk != nullptr) acc64 = 1;  // split FP32 stream: FP64 sums
+  const dim3 g(red.nA);
   if (format == 0)
-    hipLaunchKernelGGL((k_cg_spmv_csr<T, kCsrG>), dim3(red.nA), dim3(kBlock), 0, s, A, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_csr<T, kCsrG>), g, s, t0, t1, A, b, st, red);
   else if (format == 3 && acc64)
-    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3, double>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 3, double>), g, s, t0, t1, Ab, b, st, red);
   else if (format == 3)
-    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 3>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 3>), g, s, t0, t1, Ab, b, st, red);
   else if (acc64)
-    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6, double>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red);
   else
-    hipLaunchKernelGGL((k_cg_spmv_bsr<T, 6>), dim3(red.nA), dim3(kBlock), 0, s, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red);
 }
 
 template <typename T>
 void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
-                         const PcgParams& prm, int mixed, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_direction<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red, prm,
-                     mixed);
+                         const PcgParams& prm, int mixed, hipStream_t s, hipEvent_t t0,
+                         hipEvent_t t1) {
+  VBH_LAUNCH((k_cg_direction<T>), dim3(red.nB), s, t0, t1, n, b, st, red, prm, mixed);
 }
 
 template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_update<T>), dim3(red.nB), dim3(kBlock), 0, s, n, b, st, red);
+void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s,
+                      hipEvent_t t0, hipEvent_t t1) {
+  VBH_LAUNCH((k_cg_update<T>), dim3(red.nB), s, t0, t1, n, b, st, red);
 }
+
+#undef VBH_LAUNCH
 
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
                              const double* u64, double* r64, PcgScalars* st, const CgRed& red,
@@ -658,14 +674,20 @@ template void launch_spmv_bsr<float>(const BsrDev&, const float*, float*, hipStr
 template void launch_cg_init<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_init<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const CgBufs<double>&,
-                                     PcgScalars*, const CgRed&, hipStream_t, int);
+                                     PcgScalars*, const CgRed&, hipStream_t, int, hipEvent_t,
+                                     hipEvent_t);
 template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
-                                    PcgScalars*, const CgRed&, hipStream_t, int);
+                                    PcgScalars*, const CgRed&, hipStream_t, int, hipEvent_t,
+                                    hipEvent_t);
 template void launch_cg_direction<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
-                                          const PcgParams&, int, hipStream_t);
+                                          const PcgParams&, int, hipStream_t, hipEvent_t,
+                                          hipEvent_t);
 template void launch_cg_direction<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
-                                         const PcgParams&, int, hipStream_t);
-template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
-template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
+                                         const PcgParams&, int, hipStream_t, hipEvent_t,
+                                         hipEvent_t);
+template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
+                                       hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
+                                      hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace vbh
