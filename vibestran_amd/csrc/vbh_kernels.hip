@@ -39,6 +39,32 @@ __device__ __forceinline__ const double* vals<double>(const CsrDev& A) { return 
 template <>
 __device__ __forceinline__ const float* vals<float>(const CsrDev& A) { return A.v32; }
 
+// ---- four consecutive rows per thread ----------------------------------------------
+// The PCG vector kernels stream 3-8 vectors; giving each thread 4 consecutive
+// rows with one 16/32-byte access per vector puts every load of the thread in
+// flight at once (one memory round trip instead of one per grid-stride step).
+// Work vectors come from hipMalloc (256-byte aligned), so row 4q is aligned.
+template <typename T>
+struct Quad {
+  T v[4];
+};
+__device__ __forceinline__ Quad<float> ld4(const float* p) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  return {{a.x, a.y, a.z, a.w}};
+}
+__device__ __forceinline__ Quad<double> ld4(const double* p) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  return {{a.x, a.y, b.x, b.y}};
+}
+__device__ __forceinline__ void st4(float* p, const Quad<float>& q) {
+  *reinterpret_cast<float4*>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+}
+__device__ __forceinline__ void st4(double* p, const Quad<double>& q) {
+  *reinterpret_cast<double2*>(p) = make_double2(q.v[0], q.v[1]);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(q.v[2], q.v[3]);
+}
+
 // ---- deterministic reductions ---------------------------------------------------
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
@@ -246,6 +272,75 @@ __device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
   __syncthreads();
 }
 
+// The head of a vector kernel, split in two so the caller can put its own
+// stream loads between them:
+//   head_issue   every load the head needs, all in flight at once: this
+//                thread's kMaxGrid/kBlock partial sums (unrolled, so one
+//                memory round trip instead of one per grid-stride step) and
+//                the state words (thread 0);
+//   head_finish  the block sum of the partials and the broadcast of the words.
+// Partials are issued before any stream load, so they return first (vector
+// loads complete in order) and are read while they are still in L2.
+// Only the four state words go through the head: the double scalars
+// (norm_b, rr_refresh, rz) are read after the decision that needs them --
+// loading them up front with the words made both vector kernels ~12 us slower
+// per launch on MI355X.
+struct Head {
+  int done, iter_done, iter_cur, resume_iter;
+};
+
+constexpr int kPartPerThread = kMaxGrid / kBlock;
+
+template <int NV>
+struct HeadLoads {
+  double part[NV][kPartPerThread];
+  Head st;
+};
+
+template <int NV>
+__device__ __forceinline__ void head_issue(const PcgScalars* st, const double* __restrict__ part,
+                                           int nprod, HeadLoads<NV>& hl) {
+  // every slot of the kMaxGrid-wide partial arrays is allocated, so the loads
+  // are unconditional (no branch between them) and the unused ones masked
+#pragma unroll
+  for (int j = 0; j < kPartPerThread; ++j)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) hl.part[i][j] = part[i * kMaxGrid + threadIdx.x + j * kBlock];
+#pragma unroll
+  for (int j = 0; j < kPartPerThread; ++j)
+    if (threadIdx.x + j * kBlock >= nprod)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) hl.part[i][j] = 0.0;
+  if (threadIdx.x == 0) {
+    hl.st.done = st->done;
+    hl.st.iter_done = st->iter_done;
+    hl.st.iter_cur = st->iter_cur;
+    hl.st.resume_iter = st->resume_iter;
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ void head_finish(HeadLoads<NV>& hl, double (&out)[NV], Head& h,
+                                            double* lds, Head* sh) {
+  double t[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    t[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPartPerThread; ++j) t[i] += hl.part[i][j];
+  }
+  if (threadIdx.x == 0) *sh = hl.st;
+  block_sum<NV>(t, lds);  // its barriers also publish *sh
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lds[4 * kMaxRed + i] = t[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) out[i] = lds[4 * kMaxRed + i];
+  h = *sh;
+  __syncthreads();
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScalars* st,
                                                     CgRed red) {
@@ -284,14 +379,24 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, PcgScalars* st,
                                                          CgRed red, PcgParams prm, int mixed) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[4];
-  read_words(st, s_i);
-  if (s_i[0]) return;
-  const int i = s_i[1];
-  const bool resumed = (s_i[3] == i);
-  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  __shared__ Head sh;
+  const int n4 = n >> 2;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  HeadLoads<3> hl;
+  head_issue<3>(st, red.partB, red.nB, hl);
+  // this thread's first rows go out behind the head loads
+  Quad<T> z0{}, p0{};
+  if (t0 < n4) {
+    z0 = ld4(b.z + 4 * t0);
+    p0 = ld4(b.p + 4 * t0);
+  }
   double v[3];
-  reduce_partials<3>(red.partB, red.nB, v, lds);
+  Head h;
+  head_finish<3>(hl, v, h, lds, &sh);
+  if (h.done) return;
+  const int i = h.iter_done;
+  const bool resumed = (h.resume_iter == i);
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
   const double rz = v[0], rr = v[1];
   double norm_b, beta = 0.0, rel;
   int stop = 0;
@@ -330,8 +435,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, Pcg
     return;
   }
   const T bt = static_cast<T>(beta);
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    b.p[k] = b.z[k] + bt * b.p[k];
+  if (t0 < n4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p0.v[j] = z0.v[j] + bt * p0.v[j];
+    st4(b.p + 4 * t0, p0);
+  }
+  for (int q = t0 + nt; q < n4; q += nt) {
+    const Quad<T> z = ld4(b.z + 4 * q);
+    Quad<T> p = ld4(b.p + 4 * q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p.v[j] = z.v[j] + bt * p.v[j];
+    st4(b.p + 4 * q, p);
+  }
+  for (int k = 4 * n4 + t0; k < n; k += nt) b.p[k] = b.z[k] + bt * b.p[k];
   if (lead) {
     if (i == 0 && !resumed) {
       st->norm_b = norm_b;
@@ -400,13 +516,28 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgScalars* st,
                                                       CgRed red) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[4];
-  read_words(st, s_i);
-  if (s_i[0]) return;
-  const int i = s_i[2];
-  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  __shared__ Head sh;
+  const int n4 = n >> 2;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  HeadLoads<1> hl;
+  head_issue<1>(st, red.partA, red.nA, hl);
+  // this thread's first rows go out behind the head loads
+  Quad<double> u0{};
+  Quad<T> p0{}, ap0{}, d0{}, r0{};
+  if (t0 < n4) {
+    const int k = 4 * t0;
+    u0 = ld4(b.u64 + k);
+    p0 = ld4(b.p + k);
+    ap0 = ld4(b.Ap + k);
+    d0 = ld4(b.diag + k);
+    r0 = ld4(b.r + k);
+  }
   double v1[1];
-  reduce_partials<1>(red.partA, red.nA, v1, lds);
+  Head h;
+  head_finish<1>(hl, v1, h, lds, &sh);
+  if (h.done) return;
+  const int i = h.iter_cur;
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
   const double pAp = v1[0];
   if (!(pAp > 0.0)) {  // breakdown (impl:691-696)
     if (lead) {
@@ -421,7 +552,30 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgSca
   const double alpha = st->rz[i & 1] / pAp;
   const T a = static_cast<T>(alpha);
   double v[3] = {0.0, 0.0, 0.0};
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+  auto rows4 = [&](int k, Quad<double>& u, const Quad<T>& p, const Quad<T>& ap,
+                   const Quad<T>& d, Quad<T>& r) {
+    Quad<T> z;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      u.v[j] += alpha * static_cast<double>(p.v[j]);
+      r.v[j] = r.v[j] - a * ap.v[j];
+      z.v[j] = r.v[j] / d.v[j];
+      v[0] += static_cast<double>(r.v[j]) * static_cast<double>(z.v[j]);
+      v[1] += static_cast<double>(r.v[j]) * static_cast<double>(r.v[j]);
+    }
+    st4(b.u64 + k, u);
+    st4(b.r + k, r);
+    st4(b.z + k, z);
+  };
+  if (t0 < n4) rows4(4 * t0, u0, p0, ap0, d0, r0);
+  for (int q = t0 + nt; q < n4; q += nt) {
+    const int k = 4 * q;
+    Quad<double> u = ld4(b.u64 + k);
+    const Quad<T> p = ld4(b.p + k), ap = ld4(b.Ap + k), d = ld4(b.diag + k);
+    Quad<T> r = ld4(b.r + k);
+    rows4(k, u, p, ap, d, r);
+  }
+  for (int k = 4 * n4 + t0; k < n; k += nt) {
     b.u64[k] += alpha * static_cast<double>(b.p[k]);
     const T r = b.r[k] - a * b.Ap[k];
     const T z = r / b.diag[k];
