@@ -289,25 +289,25 @@ struct Head {
   int done, iter_done, iter_cur, resume_iter;
 };
 
-constexpr int kPartPerThread = kMaxGrid / kBlock;
-
-template <int NV>
+// NJ partial sums per thread: the launchers pick the smallest NJ with
+// NJ * kBlock >= the producer's grid (1, 2, 4 or kMaxGrid / kBlock).
+template <int NV, int NJ>
 struct HeadLoads {
-  double part[NV][kPartPerThread];
+  double part[NV][NJ];
   Head st;
 };
 
-template <int NV>
+template <int NV, int NJ>
 __device__ __forceinline__ void head_issue(const PcgScalars* st, const double* __restrict__ part,
-                                           int nprod, HeadLoads<NV>& hl) {
+                                           int nprod, HeadLoads<NV, NJ>& hl) {
   // every slot of the kMaxGrid-wide partial arrays is allocated, so the loads
   // are unconditional (no branch between them) and the unused ones masked
 #pragma unroll
-  for (int j = 0; j < kPartPerThread; ++j)
+  for (int j = 0; j < NJ; ++j)
 #pragma unroll
     for (int i = 0; i < NV; ++i) hl.part[i][j] = part[i * kMaxGrid + threadIdx.x + j * kBlock];
 #pragma unroll
-  for (int j = 0; j < kPartPerThread; ++j)
+  for (int j = 0; j < NJ; ++j)
     if (threadIdx.x + j * kBlock >= nprod)
 #pragma unroll
       for (int i = 0; i < NV; ++i) hl.part[i][j] = 0.0;
@@ -319,15 +319,15 @@ __device__ __forceinline__ void head_issue(const PcgScalars* st, const double* _
   }
 }
 
-template <int NV>
-__device__ __forceinline__ void head_finish(HeadLoads<NV>& hl, double (&out)[NV], Head& h,
+template <int NV, int NJ>
+__device__ __forceinline__ void head_finish(HeadLoads<NV, NJ>& hl, double (&out)[NV], Head& h,
                                             double* lds, Head* sh) {
   double t[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     t[i] = 0.0;
 #pragma unroll
-    for (int j = 0; j < kPartPerThread; ++j) t[i] += hl.part[i][j];
+    for (int j = 0; j < NJ; ++j) t[i] += hl.part[i][j];
   }
   if (threadIdx.x == 0) *sh = hl.st;
   block_sum<NV>(t, lds);  // its barriers also publish *sh
@@ -375,15 +375,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScala
   }
 }
 
-template <typename T>
+template <typename T, int NJ>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, PcgScalars* st,
                                                          CgRed red, PcgParams prm, int mixed) {
   __shared__ double lds[kLds];
   __shared__ Head sh;
   const int n4 = n >> 2;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  HeadLoads<3> hl;
-  head_issue<3>(st, red.partB, red.nB, hl);
+  HeadLoads<3, NJ> hl;
+  head_issue<3, NJ>(st, red.partB, red.nB, hl);
   // this thread's first rows go out behind the head loads
   Quad<T> z0{}, p0{};
   if (t0 < n4) {
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, Pcg
   }
   double v[3];
   Head h;
-  head_finish<3>(hl, v, h, lds, &sh);
+  head_finish<3, NJ>(hl, v, h, lds, &sh);
   if (h.done) return;
   const int i = h.iter_done;
   const bool resumed = (h.resume_iter == i);
@@ -512,15 +512,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   write_partials<1>(d, red.partA, lds);
 }
 
-template <typename T>
+template <typename T, int NJ>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgScalars* st,
                                                       CgRed red) {
   __shared__ double lds[kLds];
   __shared__ Head sh;
   const int n4 = n >> 2;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  HeadLoads<1> hl;
-  head_issue<1>(st, red.partA, red.nA, hl);
+  HeadLoads<1, NJ> hl;
+  head_issue<1, NJ>(st, red.partA, red.nA, hl);
   // this thread's first rows go out behind the head loads
   Quad<double> u0{};
   Quad<T> p0{}, ap0{}, d0{}, r0{};
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgSca
   }
   double v1[1];
   Head h;
-  head_finish<1>(hl, v1, h, lds, &sh);
+  head_finish<1, NJ>(hl, v1, h, lds, &sh);
   if (h.done) return;
   const int i = h.iter_cur;
   const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
@@ -768,17 +768,35 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red);
 }
 
+// partial sums per thread for a producer grid of nprod blocks
+static int head_nj(int nprod) {
+  return nprod <= kBlock ? 1 : nprod <= 2 * kBlock ? 2 : nprod <= 4 * kBlock ? 4 : kMaxGrid / kBlock;
+}
+static_assert(kMaxGrid / kBlock == 8, "head_nj covers up to 8 partials per thread");
+
 template <typename T>
 void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
                          const PcgParams& prm, int mixed, hipStream_t s, hipEvent_t t0,
                          hipEvent_t t1) {
-  VBH_LAUNCH((k_cg_direction<T>), dim3(red.nB), s, t0, t1, n, b, st, red, prm, mixed);
+  const dim3 g(red.nB);
+  switch (head_nj(red.nB)) {
+    case 1: VBH_LAUNCH((k_cg_direction<T, 1>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
+    case 2: VBH_LAUNCH((k_cg_direction<T, 2>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
+    case 4: VBH_LAUNCH((k_cg_direction<T, 4>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
+    default: VBH_LAUNCH((k_cg_direction<T, 8>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
+  }
 }
 
 template <typename T>
 void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s,
                       hipEvent_t t0, hipEvent_t t1) {
-  VBH_LAUNCH((k_cg_update<T>), dim3(red.nB), s, t0, t1, n, b, st, red);
+  const dim3 g(red.nB);
+  switch (head_nj(red.nA)) {
+    case 1: VBH_LAUNCH((k_cg_update<T, 1>), g, s, t0, t1, n, b, st, red); break;
+    case 2: VBH_LAUNCH((k_cg_update<T, 2>), g, s, t0, t1, n, b, st, red); break;
+    case 4: VBH_LAUNCH((k_cg_update<T, 4>), g, s, t0, t1, n, b, st, red); break;
+    default: VBH_LAUNCH((k_cg_update<T, 8>), g, s, t0, t1, n, b, st, red); break;
+  }
 }
 
 #undef VBH_LAUNCH
