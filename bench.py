@@ -36,12 +36,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--workload", default="hex", choices=["hex", "tube", "simp", "mixed"],
+    ap.add_argument("--workload", default="hex", choices=["hex", "tube", "simp", "mixed", "deck"],
                     help="hex: C1/C2 single-RHS mixed vs FP64 (the headline line); "
                          "tube: 32-load-case block PCG vs sequential solves; "
                          "simp: adjoint compliance / tip-displacement gradients; "
                          "mixed: 1.2M-DOF hex/penta/tet mesh, 8 GRAV+FORCE cases, block "
-                         "PCG and compliance adjoint")
+                         "PCG and compliance adjoint; "
+                         "deck: BDF input throughput of the 605k-DOF hex deck (native "
+                         "reader vs the Python reader), host only")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
@@ -113,6 +115,8 @@ def main():
         return bench_simp(args)
     if args.workload == "mixed":
         return bench_mixed(args)
+    if args.workload == "deck":
+        return bench_deck(args)
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.spmv_reps = 1, 0, True, 5
     if args.gpus != 1:
@@ -471,6 +475,55 @@ def bench_simp(args):
            "iterations": [int(x) for x in its],
            "max_rel_diff_vs_cpu_adjoint": {"compliance": errC, "tip": errT},
            "compliance": float(F @ U[:, 0]), "u_tip": float(U[tip, 0])}
+    print(json.dumps(out), flush=True)
+
+
+def bench_deck(args):
+    """Input throughput: the 605k-DOF jittered hex workload written as a BDF
+    deck (one MAT1 + PSOLID per element, SPC1 THRU, 4,800 PLOAD4; free
+    field, and a mixed large/small/free field copy), read by the native
+    reader (vbh_deck_read) and, for the free-field copy, by the pure-Python
+    reader; both must give the same model. Host only."""
+    import tempfile
+    from vibestran_amd import bdf, deck, decks
+
+    m = decks.config2_hex_jitter()
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        for fmt in ("free", "mixed"):
+            path = os.path.join(td, f"c2_{fmt}.bdf")
+            text = decks.bdf_text(m, fmt)
+            with open(path, "w") as fh:
+                fh.write(text)
+            size = os.path.getsize(path)
+            nlines = text.count("\n")
+            del text
+            deck.read_deck(path)  # warm the page cache
+            t = time.perf_counter()
+            a = deck.read_deck(path)
+            t_native = time.perf_counter() - t
+            r = {"bytes": size, "lines": nlines, "native_s": round(t_native, 4),
+                 "native_MBs": round(size / t_native / 1e6, 1)}
+            if fmt == "free":
+                t = time.perf_counter()
+                b = bdf.read_bdf(path)
+                t_py = time.perf_counter() - t
+                same = (np.array_equal(a.xyz, b.xyz) and np.array_equal(a.conn, b.conn)
+                        and np.array_equal(a.E, b.E) and np.array_equal(a.loads[0].pload4.p,
+                                                                        b.loads[0].pload4.p))
+                r.update({"python_s": round(t_py, 3), "speedup": round(t_py / t_native, 1),
+                          "same_model": bool(same)})
+                base = r
+            res[fmt] = r
+            log(f"[bench] deck {fmt}: {size/1e6:.1f} MB, {nlines} lines, native {t_native:.3f} s"
+                + (f", python {r['python_s']:.2f} s" if "python_s" in r else ""))
+    out = {"metric": "BDF input throughput, 605k-DOF hex deck (192k CHEXA + 192k MAT1/PSOLID)",
+           "value": base["native_MBs"], "unit": "MB/s (native reader, free field)", "n_gpus": 0,
+           "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 text parsing",
+           "data": "synthetic (the hex120x40x40 jittered workload written as BDF text)",
+           "config": {"workload": m.meta["name"] + "-bdf", "nodes": m.nn, "elements": m.ne},
+           "free": res["free"], "mixed": res["mixed"]}
     print(json.dumps(out), flush=True)
 
 

@@ -184,6 +184,47 @@ int vbh_write_node_csv(const char* path, int append, int subcase_id, int nnodes,
 int vbh_write_elem_csv(const char* path, int append, int subcase_id, int nelem,
                        const int* elem_ids, const int* etype, const double* rows);
 
+/* ---- BDF input: BdfParser::parse_file (src/io/bdf_parser.cpp:78-633) for
+ * the cards of this path (GRID, MAT1, PSOLID, PSHELL, CHEXA, CPENTA, CTETRA,
+ * CQUAD4, FORCE, MOMENT, PLOAD4, GRAV, SPC, SPC1, PARAM SHELLFORM; case
+ * control SUBCASE, LOAD, SPC, DISPLACEMENT, STRESS) in small, large and free
+ * field with continuations. Host only, no device needed; parsed in parallel
+ * chunks. Nodes come out in ascending ID order (node indices below are
+ * 0-based into that order), elements in file order. Coordinate systems,
+ * higher-order solids and PLOAD4 direction vectors are rejected with
+ * VBH_ERR_SOLVER "Line N: ..."; other cards are skipped and listed by
+ * vbh_deck_unsupported. */
+typedef struct vbh_deck vbh_deck;
+typedef struct {
+  int nnodes, nelem, nconn, nsubcase;
+  int solid_form;  /* VBH_SOLID_EAS / VBH_SOLID_SRI (PSOLID ISOP) */
+  int shell_form;  /* VBH_SHELL_MITC4 / VBH_SHELL_MINDLIN (PARAM SHELLFORM) */
+  int nunsupported;
+} vbh_deck_sizes;
+typedef struct {
+  int id, load_set, spc_set;
+  int nspc;          /* constrained nodes (masks OR-ed per node) */
+  int npload4;       /* PLOAD4 faces of the load set */
+  int disp_print, stress_print;  /* DISPLACEMENT / STRESS requests */
+  double grav[3];    /* GRAV of the load set: scale x direction, basic frame */
+} vbh_deck_subcase_info;
+int vbh_deck_read(const char* path, vbh_deck** out);
+int vbh_deck_parse(const char* text, long long len, vbh_deck** out);
+void vbh_deck_free(vbh_deck* deck);
+int vbh_deck_info(const vbh_deck* deck, vbh_deck_sizes* out);
+/* Any output pointer may be NULL. xyz nnodes x 3; eptr nelem + 1; E, nu, rho
+ * from the element's MAT1 (NU blank = 0), thickness = PSHELL T (solids 0). */
+int vbh_deck_mesh(const vbh_deck* deck, int* node_ids, double* xyz, int* elem_ids, int* etype,
+                  int* eptr, int* conn, double* E, double* nu, double* rho, double* thickness);
+int vbh_deck_subcase(const vbh_deck* deck, int k, vbh_deck_subcase_info* out);
+/* Subcase k: SPC nodes and DOF masks (bit d = DOF d+1), nodal loads nnodes x 6
+ * (FORCE into T1-T3, MOMENT into R1-R3), PLOAD4 element indices, face grids
+ * G1 / G3-G4 as node indices (-1 = none) and 4 corner pressures each. */
+int vbh_deck_subcase_data(const vbh_deck* deck, int k, int* spc_nodes, int* spc_masks,
+                          double* nodal, int* pl_elem, int* pl_g1, int* pl_g34, double* pl_p);
+/* Keyword of the i-th unsupported bulk card, NULL past the end. */
+const char* vbh_deck_unsupported(const vbh_deck* deck, int i);
+
 /* ---- solve: SolverBackend::solve(K, F) -> u (solver_backend.hpp:26-29) ----
  * F and u are host arrays of length n. iters / rel_res may be NULL. */
 int vbh_solve(vbh_ctx* ctx, const double* F, double* u, int* iters, double* rel_res);

@@ -48,6 +48,31 @@ class vbh_block_config(C.Structure):
     ]
 
 
+class vbh_deck_sizes(C.Structure):
+    _fields_ = [
+        ("nnodes", C.c_int),
+        ("nelem", C.c_int),
+        ("nconn", C.c_int),
+        ("nsubcase", C.c_int),
+        ("solid_form", C.c_int),
+        ("shell_form", C.c_int),
+        ("nunsupported", C.c_int),
+    ]
+
+
+class vbh_deck_subcase_info(C.Structure):
+    _fields_ = [
+        ("id", C.c_int),
+        ("load_set", C.c_int),
+        ("spc_set", C.c_int),
+        ("nspc", C.c_int),
+        ("npload4", C.c_int),
+        ("disp_print", C.c_int),
+        ("stress_print", C.c_int),
+        ("grav", C.c_double * 3),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -79,6 +104,14 @@ SIGNATURES = {
     "vbh_recover_displacements_device": (_I, [_P, _P, _P]),
     "vbh_write_node_csv": (_I, [C.c_char_p, _I, _I, _I, _IP, _DP]),
     "vbh_write_elem_csv": (_I, [C.c_char_p, _I, _I, _I, _IP, _IP, _DP]),
+    "vbh_deck_read": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "vbh_deck_parse": (_I, [C.c_char_p, _LL, C.POINTER(_P)]),
+    "vbh_deck_free": (None, [_P]),
+    "vbh_deck_info": (_I, [_P, C.POINTER(vbh_deck_sizes)]),
+    "vbh_deck_mesh": (_I, [_P, _IP, _DP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP]),
+    "vbh_deck_subcase": (_I, [_P, _I, C.POINTER(vbh_deck_subcase_info)]),
+    "vbh_deck_subcase_data": (_I, [_P, _I, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
+    "vbh_deck_unsupported": (C.c_char_p, [_P, _I]),
     "vbh_element_sensitivity": (_I, [_P, _I, _DP, _DP, _DP]),
     "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),
     "vbh_solve": (_I, [_P, _DP, _DP, _IP, _DP]),

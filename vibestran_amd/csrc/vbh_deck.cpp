@@ -1,0 +1,895 @@
+// vibestran_amd/csrc/vbh_deck.cpp -- native Nastran BDF reader for the cards
+// the linear-static path uses (host C++, no device code).
+//
+// Restates the reference's BdfParser (src/io/bdf_parser.cpp) for:
+//   sections         BEGIN BULK / ENDDATA, case control SUBCASE, LOAD =, SPC =,
+//                    DISPLACEMENT / STRESS requests (93-395)
+//   logical cards    comment and continuation rules (425-533): '+' lines, bare
+//                    '*' lines (large field), lines whose first 8 columns are
+//                    blank; free field when the line has a comma, large field
+//                    when it starts with '*' or its keyword field ends in '*';
+//                    continuation fields 1..8 appended, free-field trailing
+//                    '+...' markers dropped (709-778)
+//   numbers          parse_double: D -> E, an implicit exponent sign gets an
+//                    'E' (780-812); parse_int as std::stoi (814-821)
+//   cards            GRID (1091-1102), MAT1 (1104-1120), PSHELL (1265-1287),
+//                    PSOLID (1289-1306), CQUAD4 (1433-1451), CHEXA / CTETRA /
+//                    CPENTA (1473-1532), FORCE / MOMENT (1663-1689), GRAV
+//                    (1717-1730), PLOAD4 (1835-1935), SPC / SPC1 (1937-1995),
+//                    PARAM SHELLFORM (574-586, 1997-)
+// and builds the flat arrays the device path takes: nodes in ascending ID
+// order (the DofMap order), elements in file order with per-element E, nu,
+// rho and PSHELL T, and per subcase its SPC masks, nodal FORCE/MOMENT
+// components, PLOAD4 faces and GRAV vector.
+//
+// Large decks (192k MAT1 + PSOLID pairs at 605k DOF) are the reason this is
+// native: the text is cut into chunks at card boundaries and the chunks are
+// tokenised and converted in parallel, then merged in file order.
+//
+// Not covered (error, not silently wrong): coordinate systems (GRID CP/CD,
+// FORCE/MOMENT/GRAV CID), CHEXA20 / CTETRA10 / CPENTA15, PLOAD4 direction
+// vectors, mixed PSOLID ISOP within one deck, SPC enforced values. Other
+// cards are counted as unsupported and skipped, as the reference logs them.
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <string_view>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include <memory>
+
+#include "vbh_error.hpp"
+
+namespace vbh {
+namespace {
+
+struct DeckError {
+  std::string msg;
+};
+[[noreturn]] void deck_fail(int line, const std::string& m) {
+  throw DeckError{"Line " + std::to_string(line) + ": " + m};
+}
+
+std::string_view trim(std::string_view s) {
+  size_t a = 0, b = s.size();
+  while (a < b && (s[a] == ' ' || s[a] == '\t')) ++a;
+  while (b > a && (s[b - 1] == ' ' || s[b - 1] == '\t' || s[b - 1] == '\r')) --b;
+  return s.substr(a, b - a);
+}
+
+bool is_blank(std::string_view s) {
+  for (char c : s)
+    if (c != ' ' && c != '\t' && c != '\r') return false;
+  return true;
+}
+
+std::string upper(std::string_view s) {
+  std::string u(s);
+  for (auto& c : u) c = static_cast<char>(std::toupper(static_cast<unsigned char>(c)));
+  return u;
+}
+
+// parse_double (bdf_parser.cpp:780-812): blank -> 0; D/d -> E; the first
+// '+'/'-' after position 0 not preceded by E/e gets an 'E'; then the longest
+// numeric prefix (std::stod semantics).
+double parse_double(std::string_view s, int line) {
+  if (s.empty()) return 0.0;
+  char buf[64];
+  if (s.size() > 60) deck_fail(line, "cannot parse double: '" + std::string(s) + "'");
+  size_t n = 0;
+  for (char c : s) buf[n++] = (c == 'D' || c == 'd') ? 'E' : c;
+  buf[n] = 0;
+  for (size_t i = 1; i < n; ++i) {
+    if ((buf[i] == '+' || buf[i] == '-') &&
+        std::toupper(static_cast<unsigned char>(buf[i - 1])) != 'E') {
+      std::memmove(buf + i + 1, buf + i, n - i + 1);
+      buf[i] = 'E';
+      ++n;
+      break;
+    }
+  }
+  char* end = nullptr;
+  errno = 0;
+  const double v = std::strtod(buf, &end);
+  if (end == buf || errno == ERANGE)
+    deck_fail(line, "cannot parse double: '" + std::string(s) + "'");
+  return v;
+}
+
+// parse_int (bdf_parser.cpp:814-821): blank -> 0, else std::stoi semantics
+// (optional sign, digits, the rest ignored; no digits or overflow is an error).
+int parse_int(std::string_view s, int line) {
+  if (s.empty()) return 0;
+  char buf[64];
+  if (s.size() > 60) deck_fail(line, "cannot parse int: '" + std::string(s) + "'");
+  std::memcpy(buf, s.data(), s.size());
+  buf[s.size()] = 0;
+  char* end = nullptr;
+  errno = 0;
+  const long v = std::strtol(buf, &end, 10);
+  if (end == buf || errno == ERANGE || v < INT32_MIN || v > INT32_MAX)
+    deck_fail(line, "cannot parse int: '" + std::string(s) + "'");
+  return static_cast<int>(v);
+}
+
+bool integer_like(std::string_view s) {  // field_is_integer_like (844-856)
+  if (s.empty()) return false;
+  size_t i = 0;
+  if (s[0] == '+' || s[0] == '-') ++i;
+  if (i >= s.size()) return false;
+  for (; i < s.size(); ++i)
+    if (!std::isdigit(static_cast<unsigned char>(s[i]))) return false;
+  return true;
+}
+
+int dof_mask(int c) {  // DofSet::from_int (types.hpp:114-124)
+  int m = 0;
+  for (int t = c; t > 0; t /= 10) {
+    const int d = t % 10;
+    if (d >= 1 && d <= 6) m |= 1 << (d - 1);
+  }
+  return m;
+}
+
+// ---- records produced by the chunk workers -----------------------------------
+struct GridRec { int id; double x, y, z; };
+struct MatRec { int id; double E, nu, rho; };
+struct PropRec { int id, kind, mid; double t; int isop; };  // kind 0 PSOLID, 1 PSHELL
+struct ElemRec { int id, type, pid, nn; int nodes[8]; int line; };
+struct NodalRec { int sid, node, off; double v[3]; };  // off 0 FORCE, 3 MOMENT
+struct GravRec { int sid; double a[3]; };
+struct PloadRec { int sid, eid, g1, g34; double p[4]; int line; };
+struct SpcRec { int sid, node, mask; };
+
+struct Chunk {
+  std::vector<GridRec> grids;
+  std::vector<MatRec> mats;
+  std::vector<PropRec> props;
+  std::vector<ElemRec> elems;
+  std::vector<NodalRec> nodal;
+  std::vector<GravRec> grav;
+  std::vector<PloadRec> pload;
+  std::vector<SpcRec> spc;
+  std::vector<std::pair<std::string, std::string>> params;
+  std::set<std::string> unsupported;
+  std::string error;
+};
+
+// ---- line classification (bdf_parser.cpp:425-460) ----------------------------
+bool line_is_comment(std::string_view l) {
+  size_t f = 0;
+  while (f < l.size() && (l[f] == ' ' || l[f] == '\t')) ++f;
+  return f == l.size() || l[f] == '$' || is_blank(l);
+}
+
+bool line_is_continuation(std::string_view l) {
+  if (l.empty()) return false;
+  const char c0 = l[0];
+  if (c0 == '+') return true;
+  if (c0 == '*') {
+    for (size_t i = 1; i < std::min<size_t>(l.size(), 8); ++i)
+      if (!std::isspace(static_cast<unsigned char>(l[i]))) return false;
+    return true;
+  }
+  if (std::isspace(static_cast<unsigned char>(c0))) {
+    const size_t kend = std::min<size_t>(l.size(), 8);
+    for (size_t i = 0; i < kend; ++i)
+      if (!std::isspace(static_cast<unsigned char>(l[i]))) return false;
+    for (size_t i = kend; i < l.size(); ++i)
+      if (!std::isspace(static_cast<unsigned char>(l[i]))) return true;
+  }
+  return false;
+}
+
+// ---- field splitting (709-778) -------------------------------------------------
+void split_small(std::string_view l, std::vector<std::string_view>& f) {
+  auto get = [&](size_t start, size_t width) -> std::string_view {
+    if (start >= l.size()) return {};
+    return trim(l.substr(start, std::min(width, l.size() - start)));
+  };
+  f.push_back(get(0, 8));
+  for (int i = 0; i < 8; ++i) f.push_back(get(8 + 8 * i, 8));
+}
+
+void split_large(std::string_view l, std::vector<std::string_view>& f) {
+  auto get = [&](size_t start, size_t width) -> std::string_view {
+    if (start >= l.size()) return {};
+    return trim(l.substr(start, std::min(width, l.size() - start)));
+  };
+  f.push_back(get(0, 8));
+  for (int i = 0; i < 4; ++i) f.push_back(get(8 + 16 * i, 16));
+}
+
+void split_free(std::string_view l, std::vector<std::string_view>& f) {
+  const size_t d = l.find('$');
+  if (d != std::string_view::npos) l = l.substr(0, d);
+  const size_t first = f.size();
+  // tokens as std::getline(ss, tok, ',') yields them: no token after a final ','
+  size_t a = 0;
+  while (a < l.size()) {
+    const size_t c = l.find(',', a);
+    if (c == std::string_view::npos) {
+      f.push_back(trim(l.substr(a)));
+      break;
+    }
+    f.push_back(trim(l.substr(a, c - a)));
+    a = c + 1;
+  }
+  // a 10th field starting with '+' or '*' is a continuation label
+  if (f.size() - first >= 10) {
+    const std::string_view f9 = f[first + 9];
+    if (!f9.empty() && (f9[0] == '+' || f9[0] == '*')) f.erase(f.begin() + first + 9);
+  }
+}
+
+void split_any(std::string_view l, std::vector<std::string_view>& f) {
+  if (l.find(',') != std::string_view::npos) {
+    split_free(l, f);
+    return;
+  }
+  bool large = !l.empty() && l[0] == '*';
+  if (!large) {
+    const size_t kend = std::min<size_t>(l.size(), 8);
+    size_t last = std::string_view::npos;
+    for (size_t i = 0; i < kend; ++i)
+      if (l[i] != ' ' && l[i] != '\t') last = i;
+    if (last != std::string_view::npos && l[last] == '*') large = true;
+  }
+  if (large) split_large(l, f);
+  else split_small(l, f);
+}
+
+std::string_view fld(const std::vector<std::string_view>& f, size_t i) {
+  return i < f.size() ? f[i] : std::string_view{};
+}
+
+// ---- card processors -------------------------------------------------------------
+void process(const std::vector<std::string_view>& f, int line, Chunk& ck) {
+  std::string kw = upper(f[0]);
+  if (!kw.empty() && kw[0] == '*') kw.erase(0, 1);
+  while (!kw.empty() && (kw.back() == '*' || std::isspace(static_cast<unsigned char>(kw.back()))))
+    kw.pop_back();
+  if (kw == "GRID") {
+    const int cp = parse_int(fld(f, 2), line), cd = parse_int(fld(f, 6), line);
+    if (cp != 0 || cd != 0) deck_fail(line, "GRID coordinate systems (CP/CD) are not supported");
+    ck.grids.push_back({parse_int(fld(f, 1), line), parse_double(fld(f, 3), line),
+                        parse_double(fld(f, 4), line), parse_double(fld(f, 5), line)});
+  } else if (kw == "MAT1") {
+    ck.mats.push_back({parse_int(fld(f, 1), line), parse_double(fld(f, 2), line),
+                       fld(f, 4).empty() ? 0.0 : parse_double(fld(f, 4), line),
+                       fld(f, 5).empty() ? 0.0 : parse_double(fld(f, 5), line)});
+  } else if (kw == "PSOLID") {
+    int isop = -1;
+    if (!fld(f, 6).empty()) {
+      const std::string u = upper(fld(f, 6));
+      isop = (u == "SRI") ? 1 : (u == "EAS" ? 0 : -1);
+    }
+    ck.props.push_back({parse_int(fld(f, 1), line), 0, parse_int(fld(f, 2), line), 0.0, isop});
+  } else if (kw == "PSHELL") {
+    ck.props.push_back({parse_int(fld(f, 1), line), 1, parse_int(fld(f, 2), line),
+                        parse_double(fld(f, 3), line), -1});
+  } else if (kw == "CQUAD4" || kw == "CHEXA" || kw == "CTETRA" || kw == "CPENTA") {
+    ElemRec e{};
+    e.id = parse_int(fld(f, 1), line);
+    e.pid = parse_int(fld(f, 2), line);
+    e.line = line;
+    if (kw == "CQUAD4") {
+      e.type = VBH_CQUAD4;
+      e.nn = 4;
+      for (int i = 0; i < 4; ++i) e.nodes[i] = parse_int(fld(f, 3 + i), line);
+    } else {
+      int nn = 0;
+      for (size_t i = 3; i < f.size(); ++i) {
+        if (f[i].empty()) continue;
+        if (nn == 8) {
+          nn = 9;
+          break;
+        }
+        e.nodes[nn++] = parse_int(f[i], line);
+      }
+      const int want = kw == "CHEXA" ? 8 : kw == "CPENTA" ? 6 : 4;
+      if (nn != want)
+        deck_fail(line, kw + " has " + (nn > 8 ? std::string("more than 8") : std::to_string(nn)) +
+                            " nodes; expected " + std::to_string(want));
+      e.nn = nn;
+      e.type = kw == "CHEXA" ? VBH_CHEXA8 : kw == "CPENTA" ? VBH_CPENTA6 : VBH_CTETRA4;
+    }
+    ck.elems.push_back(e);
+  } else if (kw == "FORCE" || kw == "MOMENT") {
+    if (parse_int(fld(f, 3), line) != 0)
+      deck_fail(line, kw + " in a local coordinate system (CID) is not supported");
+    const double s = parse_double(fld(f, 4), line);
+    ck.nodal.push_back({parse_int(fld(f, 1), line), parse_int(fld(f, 2), line),
+                        kw == "FORCE" ? 0 : 3,
+                        {s * parse_double(fld(f, 5), line), s * parse_double(fld(f, 6), line),
+                         s * parse_double(fld(f, 7), line)}});
+  } else if (kw == "GRAV") {
+    if (parse_int(fld(f, 2), line) != 0)
+      deck_fail(line, "GRAV in a local coordinate system (CID) is not supported");
+    for (size_t i = 7; i < f.size(); ++i)
+      if (!f[i].empty())
+        deck_fail(line, "GRAV rotational acceleration components are not yet supported");
+    const double s = fld(f, 3).empty() ? 0.0 : parse_double(fld(f, 3), line);
+    ck.grav.push_back({parse_int(fld(f, 1), line),
+                       {s * parse_double(fld(f, 4), line), s * parse_double(fld(f, 5), line),
+                        s * parse_double(fld(f, 6), line)}});
+  } else if (kw == "PLOAD4") {
+    const int sid = parse_int(fld(f, 1), line);
+    const bool alt = f.size() > 3 && !fld(f, 2).empty() && !integer_like(fld(f, 2)) &&
+                     integer_like(fld(f, 3));
+    for (size_t i = 9; i < std::min<size_t>(f.size(), 13); ++i)
+      if (!f[i].empty()) deck_fail(line, "PLOAD4 direction vectors are not supported");
+    double p[4];
+    std::vector<int> eids;
+    int g1 = -1, g34 = -1;
+    const bool thru = upper(fld(f, 7)) == "THRU";
+    if (alt) {
+      p[0] = p[1] = p[2] = p[3] = parse_double(fld(f, 2), line);
+      if (thru) {
+        // expand_id_list from field 3: ids with THRU ranges
+        int prev = 0;
+        bool in_thru = false;
+        for (size_t i = 3; i < f.size(); ++i) {
+          if (f[i].empty()) continue;
+          if (upper(f[i]) == "THRU") {
+            in_thru = true;
+            continue;
+          }
+          const int v = parse_int(f[i], line);
+          if (in_thru) {
+            for (int e = prev + 1; e <= v; ++e) eids.push_back(e);
+            in_thru = false;
+          } else {
+            eids.push_back(v);
+          }
+          prev = v;
+        }
+      } else {
+        eids.push_back(parse_int(fld(f, 3), line));
+      }
+    } else {
+      if (!integer_like(fld(f, 2)))
+        deck_fail(line, "PLOAD4 element field must be an integer element ID");
+      p[0] = parse_double(fld(f, 3), line);
+      for (int k = 1; k < 4; ++k)
+        p[k] = fld(f, 3 + k).empty() ? p[0] : parse_double(fld(f, 3 + k), line);
+      if (thru) {
+        const int e1 = parse_int(fld(f, 2), line), e2 = parse_int(fld(f, 8), line);
+        if (e2 < e1)
+          deck_fail(line, "PLOAD4 THRU range end " + std::to_string(e2) +
+                              " is less than start " + std::to_string(e1));
+        for (int e = e1; e <= e2; ++e) eids.push_back(e);
+      } else {
+        eids.push_back(parse_int(fld(f, 2), line));
+        const bool h7 = !fld(f, 7).empty(), h8 = !fld(f, 8).empty();
+        if (h7 && h8) {
+          g1 = parse_int(fld(f, 7), line);
+          g34 = parse_int(fld(f, 8), line);
+        } else if (h7 || h8) {
+          deck_fail(line, "PLOAD4 solid face selection requires both face grid fields");
+        }
+      }
+    }
+    if (eids.empty()) deck_fail(line, "PLOAD4 requires at least one element ID");
+    for (int e : eids) ck.pload.push_back({sid, e, g1, g34, {p[0], p[1], p[2], p[3]}, line});
+  } else if (kw == "SPC") {
+    const int sid = parse_int(fld(f, 1), line);
+    for (int i = 0; i < 2; ++i) {
+      const size_t ni = 2 + 3 * i;
+      if (ni >= f.size() || f[ni].empty()) break;
+      if (!fld(f, ni + 2).empty() && parse_double(fld(f, ni + 2), line) != 0.0)
+        deck_fail(line, "SPC enforced displacements are not supported");
+      ck.spc.push_back({sid, parse_int(f[ni], line), dof_mask(parse_int(fld(f, ni + 1), line))});
+    }
+  } else if (kw == "SPC1") {
+    const int sid = parse_int(fld(f, 1), line);
+    const int mask = dof_mask(parse_int(fld(f, 2), line));
+    std::vector<int> ids;
+    bool thru = false;
+    int start = 0;
+    for (size_t i = 3; i < f.size(); ++i) {
+      if (f[i].empty()) continue;
+      if (upper(f[i]) == "THRU") {
+        thru = true;
+        if (!ids.empty()) ids.pop_back();
+      } else if (thru) {
+        const int end = parse_int(f[i], line);
+        for (int g = start; g <= end; ++g) ids.push_back(g);
+        thru = false;
+      } else {
+        start = parse_int(f[i], line);
+        ids.push_back(start);
+      }
+    }
+    for (int g : ids) ck.spc.push_back({sid, g, mask});
+  } else if (kw == "PARAM") {
+    if (f.size() >= 3 && !f[1].empty())
+      ck.params.emplace_back(upper(f[1]), upper(f[2]));
+  } else if (!kw.empty()) {
+    ck.unsupported.insert(kw);
+  }
+}
+
+// Cards of lines [a, b) (a starts a card or a comment).
+void parse_chunk(const std::vector<std::string_view>& lines, size_t a, size_t b, size_t line0,
+                 Chunk& ck) {
+  std::vector<std::string_view> f;
+  f.reserve(64);
+  try {
+    for (size_t i = a; i < b; ++i) {
+      const std::string_view l = lines[i];
+      if (line_is_comment(l) || line_is_continuation(l)) continue;
+      const size_t first = i;
+      f.clear();
+      const bool free = l.find(',') != std::string_view::npos;
+      split_any(l, f);
+      if (free && f.size() > 1 && !f.back().empty() && f.back()[0] == '+') f.pop_back();
+      size_t j = i + 1;
+      std::vector<std::string_view> cf;
+      while (j < b) {
+        const std::string_view nx = lines[j];
+        if (line_is_comment(nx)) {
+          ++j;
+          continue;
+        }
+        if (!line_is_continuation(nx)) break;
+        cf.clear();
+        split_any(nx, cf);
+        const bool nfree = nx.find(',') != std::string_view::npos;
+        if (nfree && !cf.empty() && !cf.back().empty() && cf.back()[0] == '+') cf.pop_back();
+        for (size_t k = 1; k < cf.size() && k <= 8; ++k) f.push_back(cf[k]);
+        ++j;
+      }
+      i = j - 1;
+      while (f.size() < 10) f.push_back({});
+      process(f, static_cast<int>(line0 + first + 1), ck);
+    }
+  } catch (const DeckError& e) {
+    ck.error = e.msg;
+  }
+}
+
+}  // namespace
+
+// ---- the parsed deck -------------------------------------------------------------
+struct Deck {
+  std::vector<int> node_ids;
+  std::vector<double> xyz;
+  std::vector<int> elem_ids, etype, eptr, conn;
+  std::vector<double> E, nu, rho, thick;
+  int solid_form = 0, shell_form = 0;
+  struct Sub {
+    int id = 1, load = 0, spc = 0, disp = 0, stress = 0;
+    std::vector<int> spc_nodes, spc_masks;
+    std::vector<double> nodal;  // nnodes x 6
+    std::vector<int> pl_elem, pl_g1, pl_g34;
+    std::vector<double> pl_p;
+    double grav[3] = {0, 0, 0};
+  };
+  std::vector<Sub> subs;
+  std::vector<std::string> unsupported;
+};
+
+namespace {
+
+// ID -> index lookup: a dense table when the IDs are compact (the usual
+// numbering), a hash map otherwise. A later definition of an ID replaces the
+// earlier one, as the reference's map assignments do.
+class IdIndex {
+ public:
+  template <class GetId>
+  void build(size_t n, GetId id_of) {
+    int lo = 0, hi = -1;
+    for (size_t i = 0; i < n; ++i) {
+      const int id = id_of(i);
+      if (i == 0 || id < lo) lo = id;
+      if (i == 0 || id > hi) hi = id;
+    }
+    dense_ = n > 0 && lo >= 0 && static_cast<long long>(hi) < 4ll * static_cast<long long>(n) + 1024;
+    if (dense_) {
+      table_.assign(static_cast<size_t>(hi) + 1, -1);
+      for (size_t i = 0; i < n; ++i) table_[static_cast<size_t>(id_of(i))] = static_cast<int>(i);
+    } else {
+      map_.reserve(2 * n);
+      for (size_t i = 0; i < n; ++i) map_[id_of(i)] = static_cast<int>(i);
+    }
+  }
+  int find(int id) const {  // -1 when absent
+    if (dense_) return (id >= 0 && static_cast<size_t>(id) < table_.size()) ? table_[static_cast<size_t>(id)] : -1;
+    const auto it = map_.find(id);
+    return it == map_.end() ? -1 : it->second;
+  }
+
+ private:
+  bool dense_ = false;
+  std::vector<int> table_;
+  std::unordered_map<int, int> map_;
+};
+
+void parse_text(std::string_view text, Deck& d) {
+  // lines
+  std::vector<std::string_view> lines;
+  lines.reserve(text.size() / 40 + 16);
+  size_t a = 0;
+  while (a < text.size()) {
+    size_t e = text.find('\n', a);
+    if (e == std::string_view::npos) e = text.size();
+    std::string_view l = text.substr(a, e - a);
+    if (!l.empty() && l.back() == '\r') l.remove_suffix(1);
+    lines.push_back(l);
+    a = e + 1;
+  }
+  // sections
+  size_t begin_bulk = static_cast<size_t>(-1), enddata = lines.size();
+  for (size_t i = 0; i < lines.size(); ++i) {
+    std::string_view l = lines[i];
+    size_t f = 0;
+    while (f < l.size() && (l[f] == ' ' || l[f] == '\t')) ++f;
+    if (f == l.size()) continue;
+    const char c0 = static_cast<char>(std::toupper(static_cast<unsigned char>(l[f])));
+    if (c0 != 'B' && c0 != 'E') continue;  // only BEGIN BULK / ENDDATA matter here
+    const std::string kwl = upper(l.substr(f, std::min<size_t>(l.size() - f, 12)));
+    if (kwl.rfind("BEGIN BULK", 0) == 0 || kwl.rfind("BEGIN  BULK", 0) == 0) {
+      begin_bulk = i + 1;
+    } else if (kwl.rfind("ENDDATA", 0) == 0) {
+      enddata = i;
+      break;
+    }
+  }
+  if (begin_bulk == static_cast<size_t>(-1)) begin_bulk = 0;
+  // case control (93-395, the requests the linear-static path reads)
+  {
+    Deck::Sub global, cur;
+    bool in_sc = false;
+    for (size_t i = 0; i < begin_bulk; ++i) {
+      std::string u = upper(lines[i]);
+      const size_t dl = u.find('$');
+      if (dl != std::string::npos) u.resize(dl);
+      const std::string_view kw = trim(u);
+      if (kw.empty()) continue;
+      Deck::Sub& sc = in_sc ? cur : global;
+      auto value_after_eq = [&](int line) -> int {
+        const size_t eq = kw.find('=');
+        if (eq == std::string_view::npos) return -1;
+        std::string_view v = trim(kw.substr(eq + 1));
+        if (v.empty() || !(std::isdigit(static_cast<unsigned char>(v[0])) || v[0] == '+' || v[0] == '-'))
+          return -1;
+        return parse_int(v, line);
+      };
+      // DISPLACEMENT / STRESS [(modifiers)] = ALL | NONE | set: NONE clears the
+      // print request; with a modifier list, PRINT asks for it; with neither
+      // PRINT nor PLOT listed, STRESS still prints (parse_print_plot_modifiers
+      // default) and DISPLACEMENT does not; without a list both print.
+      auto request = [&](bool& flag_print, bool default_print) {
+        const size_t eq = kw.find('=');
+        if (eq == std::string_view::npos) return;
+        const std::string_view v = trim(kw.substr(eq + 1));
+        if (v.rfind("NONE", 0) == 0) {
+          flag_print = false;
+          return;
+        }
+        const size_t lp = kw.find('('), rp = kw.find(')');
+        if (lp != std::string_view::npos && rp != std::string_view::npos && rp > lp) {
+          const std::string_view mods = kw.substr(lp + 1, rp - lp - 1);
+          const bool pr = mods.find("PRINT") != std::string_view::npos;
+          const bool pl = mods.find("PLOT") != std::string_view::npos;
+          if (pr || (default_print && !pl)) flag_print = true;
+        } else {
+          flag_print = true;
+        }
+      };
+      const int line = static_cast<int>(i + 1);
+      try {
+        if (kw.rfind("SUBCASE", 0) == 0) {
+          if (in_sc) d.subs.push_back(cur);
+          cur = global;
+          cur.id = 1;
+          in_sc = true;
+          const size_t eq = kw.find('=');
+          std::string_view v = eq != std::string_view::npos ? kw.substr(eq + 1) : kw.substr(7);
+          v = trim(v);
+          if (!v.empty()) cur.id = parse_int(v, line);
+        } else if (kw.rfind("LOAD", 0) == 0) {
+          const int v = value_after_eq(line);
+          if (v >= 0) sc.load = v;
+        } else if (kw.rfind("SPC", 0) == 0 && kw.rfind("SPCADD", 0) != 0) {
+          const int v = value_after_eq(line);
+          if (v >= 0) sc.spc = v;
+        } else if (kw.rfind("DISPLACEMENT", 0) == 0) {
+          bool f = sc.disp != 0;
+          request(f, false);
+          sc.disp = f;
+        } else if (kw.rfind("STRESS", 0) == 0 || kw.rfind("STRAIN", 0) == 0) {
+          bool f = sc.stress != 0;
+          request(f, true);
+          sc.stress = f;
+        }
+      } catch (const DeckError&) {
+        // malformed numbers in case control are ignored, as the reference's
+        // try { std::stoi } catch (...) {} does
+      }
+    }
+    if (in_sc) d.subs.push_back(cur);
+    if (d.subs.empty()) {
+      const bool any = global.load || global.spc || global.disp || global.stress;
+      if (!any) {
+        global.load = 1;
+        global.spc = 1;
+      }
+      d.subs.push_back(global);
+    }
+  }
+  // bulk: chunks cut at card starts
+  const size_t nb = enddata > begin_bulk ? enddata - begin_bulk : 0;
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t nchunk = std::max<size_t>(1, std::min<size_t>(hw, nb / 20000 + 1));
+  std::vector<size_t> cut(nchunk + 1, enddata);
+  cut[0] = begin_bulk;
+  for (size_t c = 1; c < nchunk; ++c) {
+    size_t k = begin_bulk + nb * c / nchunk;
+    while (k < enddata && (line_is_comment(lines[k]) || line_is_continuation(lines[k]))) ++k;
+    cut[c] = std::max(k, cut[c - 1]);
+  }
+  std::vector<Chunk> chunks(nchunk);
+  {
+    std::vector<std::thread> pool;
+    for (size_t c = 1; c < nchunk; ++c)
+      pool.emplace_back(parse_chunk, std::cref(lines), cut[c], cut[c + 1], 0, std::ref(chunks[c]));
+    parse_chunk(lines, cut[0], cut[1], 0, chunks[0]);
+    for (auto& t : pool) t.join();
+  }
+  for (const auto& ck : chunks)
+    if (!ck.error.empty()) throw DeckError{ck.error};
+
+  // merge: nodes sorted by ID
+  std::vector<GridRec> grids;
+  std::vector<MatRec> mats;
+  std::vector<PropRec> props;
+  std::set<std::string> unsupported;
+  std::map<std::string, std::string> params;
+  size_t ne = 0;
+  for (auto& ck : chunks) {
+    grids.insert(grids.end(), ck.grids.begin(), ck.grids.end());
+    mats.insert(mats.end(), ck.mats.begin(), ck.mats.end());
+    props.insert(props.end(), ck.props.begin(), ck.props.end());
+    for (const auto& [k, v] : ck.params) params[k] = v;
+    unsupported.insert(ck.unsupported.begin(), ck.unsupported.end());
+    ne += ck.elems.size();
+  }
+  IdIndex mat_ix, prop_ix;
+  mat_ix.build(mats.size(), [&](size_t i) { return mats[i].id; });
+  prop_ix.build(props.size(), [&](size_t i) { return props[i].id; });
+  // a later GRID with the same ID replaces the earlier one (model.nodes[id] = g)
+  std::stable_sort(grids.begin(), grids.end(), [](const GridRec& x, const GridRec& y) { return x.id < y.id; });
+  std::vector<GridRec> uniq;
+  uniq.reserve(grids.size());
+  for (const auto& g : grids) {
+    if (!uniq.empty() && uniq.back().id == g.id) uniq.back() = g;
+    else uniq.push_back(g);
+  }
+  const int nn = static_cast<int>(uniq.size());
+  d.node_ids.resize(nn);
+  d.xyz.resize(3 * static_cast<size_t>(nn));
+  for (int i = 0; i < nn; ++i) {
+    d.node_ids[i] = uniq[i].id;
+    d.xyz[3 * i] = uniq[i].x;
+    d.xyz[3 * i + 1] = uniq[i].y;
+    d.xyz[3 * i + 2] = uniq[i].z;
+  }
+  IdIndex node_ix;
+  node_ix.build(d.node_ids.size(), [&](size_t i) { return d.node_ids[i]; });
+  auto node_index = [&](int id, int line) {
+    const int k = node_ix.find(id);
+    if (k < 0) deck_fail(line, "GRID " + std::to_string(id) + " is not defined");
+    return k;
+  };
+  auto it_shell = params.find("SHELLFORM");
+  d.shell_form = (it_shell != params.end() && it_shell->second == "MINDLIN") ? 1 : 0;
+  // elements in file order
+  d.elem_ids.reserve(ne);
+  d.etype.reserve(ne);
+  d.eptr.reserve(ne + 1);
+  d.eptr.push_back(0);
+  d.E.reserve(ne);
+  d.nu.reserve(ne);
+  d.rho.reserve(ne);
+  d.thick.reserve(ne);
+  int isop_seen = -2;
+  for (const auto& ck : chunks)
+    for (const auto& e : ck.elems) {
+      const int ip = prop_ix.find(e.pid);
+      if (ip < 0) deck_fail(e.line, "property " + std::to_string(e.pid) + " is not defined");
+      const PropRec& p = props[static_cast<size_t>(ip)];
+      if ((e.type == VBH_CQUAD4) != (p.kind == 1))
+        deck_fail(e.line, "element " + std::to_string(e.id) + " references a property of the wrong kind");
+      const int im = mat_ix.find(p.mid);
+      if (im < 0) deck_fail(e.line, "MAT1 " + std::to_string(p.mid) + " is not defined");
+      const MatRec& mt = mats[static_cast<size_t>(im)];
+      if (p.kind == 0) {
+        const int isop = p.isop == 1 ? 1 : 0;
+        if (isop_seen >= 0 && isop_seen != isop)
+          deck_fail(e.line, "mixed PSOLID ISOP (EAS and SRI) in one deck is not supported");
+        isop_seen = isop;
+      }
+      d.elem_ids.push_back(e.id);
+      d.etype.push_back(e.type);
+      for (int k = 0; k < e.nn; ++k) d.conn.push_back(node_index(e.nodes[k], e.line));
+      d.eptr.push_back(static_cast<int>(d.conn.size()));
+      d.E.push_back(mt.E);
+      d.nu.push_back(mt.nu);
+      d.rho.push_back(mt.rho);
+      d.thick.push_back(p.kind == 1 ? p.t : 0.0);
+    }
+  d.solid_form = isop_seen == 1 ? 1 : 0;
+  IdIndex elem_ix;  // element IDs are looked up by PLOAD4 only
+  bool any_pload = false;
+  for (const auto& ck : chunks) any_pload = any_pload || !ck.pload.empty();
+  if (any_pload) elem_ix.build(d.elem_ids.size(), [&](size_t i) { return d.elem_ids[i]; });
+  // subcases: SPC set, FORCE/MOMENT, PLOAD4, GRAV of the LOAD set
+  for (auto& s : d.subs) {
+    std::vector<int> mask(static_cast<size_t>(nn), 0);
+    s.nodal.assign(6 * static_cast<size_t>(nn), 0.0);
+    for (const auto& ck : chunks) {
+      for (const auto& r : ck.spc)
+        if (r.sid == s.spc) mask[static_cast<size_t>(node_index(r.node, 0))] |= r.mask;
+      for (const auto& r : ck.nodal)
+        if (r.sid == s.load) {
+          const size_t i = static_cast<size_t>(node_index(r.node, 0));
+          for (int k = 0; k < 3; ++k) s.nodal[6 * i + r.off + k] += r.v[k];
+        }
+      for (const auto& r : ck.grav)
+        if (r.sid == s.load)
+          for (int k = 0; k < 3; ++k) s.grav[k] += r.a[k];
+      for (const auto& r : ck.pload)
+        if (r.sid == s.load) {
+          const int ie = elem_ix.find(r.eid);
+          if (ie < 0)
+            deck_fail(r.line, "PLOAD4 element " + std::to_string(r.eid) + " is not defined");
+          s.pl_elem.push_back(ie);
+          s.pl_g1.push_back(r.g1 < 0 ? -1 : node_index(r.g1, r.line));
+          s.pl_g34.push_back(r.g34 < 0 ? -1 : node_index(r.g34, r.line));
+          for (int k = 0; k < 4; ++k) s.pl_p.push_back(r.p[k]);
+        }
+    }
+    for (int i = 0; i < nn; ++i)
+      if (mask[static_cast<size_t>(i)]) {
+        s.spc_nodes.push_back(i);
+        s.spc_masks.push_back(mask[static_cast<size_t>(i)]);
+      }
+  }
+  d.unsupported.assign(unsupported.begin(), unsupported.end());
+}
+
+}  // namespace
+}  // namespace vbh
+
+struct vbh_deck : vbh::Deck {};
+
+extern "C" {
+
+int vbh_deck_parse(const char* text, long long len, vbh_deck** out) {
+  return vbh::guarded([&] {
+    if (!text || len < 0 || !out) vbh::throw_error(VBH_ERR_ARG, "vbh_deck_parse: bad argument");
+    auto d = std::make_unique<vbh_deck>();
+    try {
+      vbh::parse_text(std::string_view(text, static_cast<size_t>(len)), *d);
+    } catch (const vbh::DeckError& e) {
+      vbh::throw_error(VBH_ERR_SOLVER, e.msg);
+    }
+    *out = d.release();
+  });
+}
+
+int vbh_deck_read(const char* path, vbh_deck** out) {
+  return vbh::guarded([&] {
+    if (!path || !out) vbh::throw_error(VBH_ERR_ARG, "vbh_deck_read: bad argument");
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) vbh::throw_error(VBH_ERR_SOLVER, std::string("Cannot open BDF file: ") + path);
+    const std::streamoff size = f.tellg();
+    std::string text(static_cast<size_t>(size > 0 ? size : 0), '\0');
+    f.seekg(0);
+    if (size > 0 && !f.read(text.data(), size))
+      vbh::throw_error(VBH_ERR_SOLVER, std::string("Cannot read BDF file: ") + path);
+    auto d = std::make_unique<vbh_deck>();
+    try {
+      vbh::parse_text(text, *d);
+    } catch (const vbh::DeckError& e) {
+      vbh::throw_error(VBH_ERR_SOLVER, e.msg);
+    }
+    *out = d.release();
+  });
+}
+
+void vbh_deck_free(vbh_deck* d) { delete d; }
+
+int vbh_deck_info(const vbh_deck* d, vbh_deck_sizes* s) {
+  if (!d || !s) {
+    vbh::set_error("vbh_deck_info: null argument");
+    return VBH_ERR_ARG;
+  }
+  s->nnodes = static_cast<int>(d->node_ids.size());
+  s->nelem = static_cast<int>(d->elem_ids.size());
+  s->nconn = static_cast<int>(d->conn.size());
+  s->nsubcase = static_cast<int>(d->subs.size());
+  s->solid_form = d->solid_form;
+  s->shell_form = d->shell_form;
+  s->nunsupported = static_cast<int>(d->unsupported.size());
+  return VBH_OK;
+}
+
+int vbh_deck_mesh(const vbh_deck* d, int* node_ids, double* xyz, int* elem_ids, int* etype,
+                  int* eptr, int* conn, double* E, double* nu, double* rho, double* thickness) {
+  if (!d) {
+    vbh::set_error("vbh_deck_mesh: null deck");
+    return VBH_ERR_ARG;
+  }
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+  };
+  cp(node_ids, d->node_ids);
+  cp(xyz, d->xyz);
+  cp(elem_ids, d->elem_ids);
+  cp(etype, d->etype);
+  cp(eptr, d->eptr);
+  cp(conn, d->conn);
+  cp(E, d->E);
+  cp(nu, d->nu);
+  cp(rho, d->rho);
+  cp(thickness, d->thick);
+  return VBH_OK;
+}
+
+int vbh_deck_subcase(const vbh_deck* d, int k, vbh_deck_subcase_info* s) {
+  if (!d || !s || k < 0 || k >= static_cast<int>(d->subs.size())) {
+    vbh::set_error("vbh_deck_subcase: bad argument");
+    return VBH_ERR_ARG;
+  }
+  const auto& u = d->subs[static_cast<size_t>(k)];
+  s->id = u.id;
+  s->load_set = u.load;
+  s->spc_set = u.spc;
+  s->nspc = static_cast<int>(u.spc_nodes.size());
+  s->npload4 = static_cast<int>(u.pl_elem.size());
+  s->disp_print = u.disp;
+  s->stress_print = u.stress;
+  for (int i = 0; i < 3; ++i) s->grav[i] = u.grav[i];
+  return VBH_OK;
+}
+
+int vbh_deck_subcase_data(const vbh_deck* d, int k, int* spc_nodes, int* spc_masks,
+                          double* nodal, int* pl_elem, int* pl_g1, int* pl_g34, double* pl_p) {
+  if (!d || k < 0 || k >= static_cast<int>(d->subs.size())) {
+    vbh::set_error("vbh_deck_subcase_data: bad argument");
+    return VBH_ERR_ARG;
+  }
+  const auto& u = d->subs[static_cast<size_t>(k)];
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+  };
+  cp(spc_nodes, u.spc_nodes);
+  cp(spc_masks, u.spc_masks);
+  cp(nodal, u.nodal);
+  cp(pl_elem, u.pl_elem);
+  cp(pl_g1, u.pl_g1);
+  cp(pl_g34, u.pl_g34);
+  cp(pl_p, u.pl_p);
+  return VBH_OK;
+}
+
+const char* vbh_deck_unsupported(const vbh_deck* d, int i) {
+  if (!d || i < 0 || i >= static_cast<int>(d->unsupported.size())) return nullptr;
+  return d->unsupported[static_cast<size_t>(i)].c_str();
+}
+
+}  // extern "C"

@@ -9,34 +9,9 @@
 #include <vector>
 
 #include "../../include/vbh.h"
+#include "vbh_error.hpp"
 
 namespace vbh {
-
-// ---- errors ----------------------------------------------------------------
-struct Error {
-  int code;
-  std::string msg;
-};
-void set_error(const std::string& msg);
-[[noreturn]] void throw_error(int code, const std::string& msg);
-std::string fmt_g(double v);  // printf %g
-
-template <typename F>
-int guarded(F&& f) {
-  try {
-    f();
-    return VBH_OK;
-  } catch (const Error& e) {
-    set_error(e.msg);
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    set_error("host out of memory");
-    return VBH_ERR_HIP;
-  } catch (const std::exception& e) {
-    set_error(e.what());
-    return VBH_ERR_ARG;
-  }
-}
 
 #define VBH_HIP(call)                                                              \
   do {                                                                             \

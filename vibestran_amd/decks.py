@@ -202,3 +202,95 @@ def mixed_block(nx=120, ny=60, nz=54, jitter=0.1, seed=5, n_cases=8, hex_frac=0.
     m.meta.update(dict(name=f"mixed{nx}x{ny}x{nz}-hex-penta-tet", nx=nx, ny=ny, nz=nz, ijk=ijk,
                        counts=dict(CHEXA8=len(hexes), CPENTA6=len(wedges), CTETRA4=len(tets))))
     return m
+
+
+def bdf_text(m: Model, fmt: str = "free") -> str:
+    """The model as BDF text: SOL 101 case control with one SUBCASE per load
+    case (LOAD = k+1, SPC = 1, DISPLACEMENT/STRESS = ALL), one MAT1 + PSOLID (or
+    PSHELL) per element so per-element E survives, SPC1 (THRU for runs of
+    consecutive IDs), FORCE/MOMENT, PLOAD4 with its face grids, GRAV.
+    fmt "free": every card comma-separated; "mixed": GRID in large field
+    (GRID* + '*' continuation), elements and PLOAD4 in small field (8-column,
+    blank-label continuations), the rest free field. Values are written with
+    repr (free field) or must fit the fixed fields exactly (the caller rounds
+    coordinates for "mixed")."""
+    ids = np.asarray(m.node_ids)
+    eids = m.element_ids()
+    out = ["SOL 101", "CEND"]
+    for k in range(max(1, len(m.loads))):
+        out += [f"SUBCASE {m.subcase_id(k)}", f"  LOAD = {k + 1}", "  SPC = 1",
+                "  DISPLACEMENT = ALL", "  STRESS = ALL"]
+    out.append("BEGIN BULK")
+    if m.shell_form == 1:
+        out.append("PARAM,SHELLFORM,MINDLIN")
+
+    def small(*fields):
+        return "".join(f"{str(x):<8s}" for x in fields).rstrip()
+
+    for i, g in enumerate(ids):
+        x, y, z = (repr(float(v)) for v in m.xyz[i])
+        if fmt == "mixed":
+            out.append(f"{'GRID*':<8s}{g:<16d}{'':16s}{float(m.xyz[i, 0]):<16.9E}"
+                       f"{float(m.xyz[i, 1]):<16.9E}".rstrip())
+            out.append(f"{'*':<8s}{float(m.xyz[i, 2]):<16.9E}".rstrip())
+        else:
+            out.append(f"GRID,{g},,{x},{y},{z}")
+    names = {CHEXA8: "CHEXA", CPENTA6: "CPENTA", CTETRA4: "CTETRA", CQUAD4: "CQUAD4"}
+    for e in range(m.ne):
+        t = int(m.elem_type[e])
+        nodes = [int(ids[c]) for c in m.conn[m.elem_ptr[e]:m.elem_ptr[e + 1]]]
+        eid = int(eids[e])
+        if fmt == "mixed":
+            head = [names[t], eid, eid] + nodes[:6]
+            out.append(small(*head))
+            if len(nodes) > 6:
+                out.append(small("", *nodes[6:]))
+        else:
+            line = f"{names[t]},{eid},{eid}," + ",".join(map(str, nodes[:6]))
+            if len(nodes) > 6:
+                out.append(line + ",+")
+                out.append("+," + ",".join(map(str, nodes[6:])))
+            else:
+                out.append(line)
+        rho = repr(float(m.rho[e])) if m.rho is not None else ""
+        out.append(f"MAT1,{eid},{repr(float(m.E[e]))},,{repr(float(m.nu[e]))},{rho}")
+        if t == CQUAD4:
+            out.append(f"PSHELL,{eid},{eid},{repr(float(m.thickness[e]))}")
+        else:
+            out.append(f"PSOLID,{eid},{eid}" + (",,,,SRI" if m.solid_form == 1 else ""))
+    # SPC1 per mask, THRU for runs of consecutive IDs
+    for mask in sorted(set(int(x) for x in m.spc_masks)):
+        comp = "".join(str(d + 1) for d in range(6) if mask >> d & 1)
+        g = np.sort(ids[m.spc_nodes[m.spc_masks == mask]])
+        runs = np.split(g, np.nonzero(np.diff(g) != 1)[0] + 1)
+        for r in runs:
+            if len(r) >= 3:
+                out.append(f"SPC1,1,{comp},{r[0]},THRU,{r[-1]}")
+            else:
+                for v in r:
+                    out.append(f"SPC1,1,{comp},{v}")
+    for k, lc in enumerate(m.loads):
+        sid = k + 1
+        if lc.nodal is not None:
+            for i in np.nonzero(np.any(lc.nodal != 0.0, axis=1))[0]:
+                for name, off in (("FORCE", 0), ("MOMENT", 3)):
+                    v = lc.nodal[i, off:off + 3]
+                    if np.any(v != 0.0):
+                        out.append(f"{name},{sid},{ids[i]},0,1.0," +
+                                   ",".join(repr(float(a)) for a in v))
+        if lc.pload4 is not None:
+            pl = lc.pload4
+            for j, e in enumerate(pl.elem):
+                c = m.conn[m.elem_ptr[e]:m.elem_ptr[e + 1]]
+                g1 = "" if pl.g1[j] < 0 else int(ids[c[pl.g1[j]]])
+                g34 = "" if pl.g34[j] < 0 else int(ids[c[pl.g34[j]]])
+                ps = [repr(float(p)) for p in pl.p[j]]
+                if fmt == "mixed":
+                    out.append(small("PLOAD4", sid, int(eids[e]), *[f"{float(p):.6g}" for p in pl.p[j]],
+                                     g1, g34))
+                else:
+                    out.append(f"PLOAD4,{sid},{int(eids[e])}," + ",".join(ps) + f",{g1},{g34}")
+        if lc.grav is not None:
+            out.append(f"GRAV,{sid},0,1.0," + ",".join(repr(float(a)) for a in lc.grav))
+    out.append("ENDDATA")
+    return "\n".join(out) + "\n"
