@@ -174,6 +174,18 @@ int vbh_recover_stress_device(vbh_ctx* ctx, const double* du, double* dcentroid,
  * inactive DOFs are 0. */
 int vbh_recover_displacements(vbh_ctx* ctx, const double* u, double* disp);
 int vbh_recover_displacements_device(vbh_ctx* ctx, const double* du, double* ddisp);
+/* Internal nodal forces f = sum_e K_e u_e over the mesh of the last
+ * vbh_assemble, at every node DOF (nnodes x 6, basic frame, constrained DOFs
+ * included; no reference counterpart). */
+int vbh_element_forces(vbh_ctx* ctx, const double* u, double* f);
+int vbh_element_forces_device(vbh_ctx* ctx, const double* du, double* df);
+/* R = f - F_applied at every node DOF (nnodes x 6) for one load case given as
+ * in vbh_assemble_rhs: at constrained DOFs the support reactions
+ * (K_full u - F, no reference counterpart), at free DOFs minus the
+ * equilibrium residual. Host arrays. */
+int vbh_reactions(vbh_ctx* ctx, const double* u, const double* nodal, int nface,
+                  const int* face_nodes, const double* face_p, const double* grav,
+                  const double* rho, double* R);
 /* CSV result files of CsvWriter::write (csv_writer.cpp:21-93), host only (no
  * device needed). append = 0 truncates the file and writes the header line;
  * append != 0 adds the rows of another subcase. Node rows go out in ascending

@@ -83,6 +83,7 @@ def _load():
         "orc_element_sensitivity": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, I, I, I, DP, DP, DP]),
         "orc_grav_rhs": (I, [I, I, IP, IP, IP, DP, IP, DP, DP, DP, DP]),
         "orc_quad4_recover": (I, [DP, D, D, D, D, I, DP, D, D, DP]),
+        "orc_element_forces": (I, [I, IP, IP, IP, DP, IP, DP, DP, DP, I, I, DP, I, DP]),
         "orc_recover_stress": (I, [I, IP, IP, IP, DP, IP, DP, DP, DP, I, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
@@ -383,6 +384,21 @@ def quad4_recover(xyz, E, nu, t, ue, xi, eta, formulation="MITC4", twelveI_t3=1.
     if rc != 0:
         raise OracleError(_err())
     return out
+
+
+def element_forces(model, eq, u):
+    """sum_e K_e u_e at every node DOF, [nn, 6]."""
+    et, ep, cn = _i32(model.elem_type), _i32(model.elem_ptr), _i32(model.conn)
+    x, q = _f64(model.xyz).reshape(-1), _i32(eq).reshape(-1)
+    e, nv = _f64(model.E), _f64(model.nu)
+    th = _f64(model.thickness if model.thickness is not None else np.zeros(len(et)))
+    f = np.empty((model.nn, 6), dtype=np.float64)
+    rc = lib().orc_element_forces(len(et), _ip(et), _ip(ep), _ip(cn), _dp(x), _ip(q), _dp(e),
+                                  _dp(nv), _dp(th), int(model.solid_form), int(model.shell_form),
+                                  _dp(_f64(u)), model.nn, _dp(f))
+    if rc != 0:
+        raise OracleError(_err())
+    return f
 
 
 STRESS_COLUMNS = ("sx", "sy", "sxy", "sz", "syz", "szx", "mx", "my", "mxy", "von_mises")

@@ -1067,6 +1067,46 @@ int orc_element_sensitivity(int n, int nelem, const int* et, const int* ep, cons
   return ORC_OK;
 }
 
+int orc_element_forces(int nelem, const int* et, const int* ep, const int* conn,
+                       const double* xyz, const int* eq, const double* E, const double* nu,
+                       const double* thick, int solid_form, int shell_form, const double* u,
+                       int nnodes, double* f) {
+  std::fill(f, f + 6 * static_cast<size_t>(nnodes), 0.0);
+  std::vector<double> ke(576), xe(24), ue(24);
+  for (int e = 0; e < nelem; ++e) {
+    const int nn = ep[e + 1] - ep[e];
+    const int dpn = (et[e] == ORC_CQUAD4) ? 6 : 3;
+    const int nd = dpn * nn;
+    for (int a = 0; a < nn; ++a) {
+      const int node = conn[ep[e] + a];
+      for (int d = 0; d < 3; ++d) xe[static_cast<size_t>(3 * a + d)] = xyz[3 * node + d];
+      for (int d = 0; d < dpn; ++d) {
+        const int q = eq[6 * node + d];
+        ue[static_cast<size_t>(dpn * a + d)] = q >= 0 ? u[q] : 0.0;
+      }
+    }
+    int rc;
+    if (et[e] == ORC_CHEXA8)
+      rc = solid_form == 1 ? orc_hexa8_ke_sri(xe.data(), E[e], nu[e], ke.data())
+                           : orc_hexa8_ke_eas(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CTETRA4)
+      rc = orc_tetra4_ke(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CPENTA6)
+      rc = orc_penta6_ke_eas(xe.data(), E[e], nu[e], ke.data());
+    else if (et[e] == ORC_CQUAD4)
+      rc = orc_quad4_ke(xe.data(), E[e], nu[e], 0.0, thick[e], 1.0, 0.833333, shell_form, ke.data());
+    else
+      rc = fail(ORC_ERR_ARG, "orc_element_forces: unsupported element type");
+    if (rc != ORC_OK) return rc;
+    for (int i = 0; i < nd; ++i) {
+      double sv = 0.0;
+      for (int j = 0; j < nd; ++j) sv += ke[static_cast<size_t>(i * nd + j)] * ue[static_cast<size_t>(j)];
+      f[6 * static_cast<size_t>(conn[ep[e] + i / dpn]) + i % dpn] += sv;
+    }
+  }
+  return ORC_OK;
+}
+
 OrcCsr* orc_assemble_solids(int n, int nelem, const int* et, const int* ep, const int* conn,
                             const double* xyz, const int* eq, const double* E, const double* nu,
                             int formulation) {

@@ -141,6 +141,12 @@ OrcCsr* orc_assemble_model(int n, int nelem, const int* elem_type, const int* el
 int orc_grav_rhs(int n, int nelem, const int* elem_type, const int* elem_ptr, const int* conn,
                  const double* xyz, const int* eq, const double* rho, const double* thickness,
                  const double* g, double* F);
+// Internal nodal forces f[6 node + dof] = sum_e K_e u_e (no reference
+// counterpart): every DOF of every node, constrained ones included.
+int orc_element_forces(int nelem, const int* elem_type, const int* elem_ptr, const int* conn,
+                       const double* xyz, const int* eq, const double* E, const double* nu,
+                       const double* thickness, int solid_form, int shell_form, const double* u,
+                       int nnodes, double* f);
 // recover_shell_response (cquad4.cpp:758-801) at (xi, eta) without thermal
 // strain: out6 = membrane stress (sx, sy, sxy) and moments (mx, my, mxy) from
 // the 24 global element displacements ue.

@@ -104,6 +104,9 @@ SIGNATURES = {
     "vbh_recover_displacements_device": (_I, [_P, _P, _P]),
     "vbh_write_node_csv": (_I, [C.c_char_p, _I, _I, _I, _IP, _DP]),
     "vbh_write_elem_csv": (_I, [C.c_char_p, _I, _I, _I, _IP, _IP, _DP]),
+    "vbh_element_forces": (_I, [_P, _DP, _DP]),
+    "vbh_element_forces_device": (_I, [_P, _P, _P]),
+    "vbh_reactions": (_I, [_P, _DP, _DP, _I, _IP, _DP, _DP, _DP, _DP]),
     "vbh_deck_read": (_I, [C.c_char_p, C.POINTER(_P)]),
     "vbh_deck_parse": (_I, [C.c_char_p, _LL, C.POINTER(_P)]),
     "vbh_deck_free": (None, [_P]),

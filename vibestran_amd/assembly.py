@@ -112,15 +112,12 @@ def compliance_gradients(backend, model: Model, eq, n, F, tip_dof=None):
             element_sensitivity(backend, model, 0, u, lam))
 
 
-def assemble_rhs_device(backend, model: Model, eq, n, case, F_device=None):
-    """Load vector of one load case on the GPU (FORCE/MOMENT, PLOAD4 on CHEXA8
-    faces, GRAV with consistent masses) over the mesh of the last
-    ``assemble_model``. Returns F on the host, or writes the device buffer
-    ``F_device`` (an HBM address) when given."""
+def load_arrays(model: Model, case):
+    """The C-ABI load description of one load case: nodal [nn, 6] (or None),
+    PLOAD4 faces (corner node indices in outward order, [nface, 4]) with
+    their corner pressures, GRAV vector (or None), element densities."""
     from .model import hexa_pload4_faces
-    lib = L.load()
     nodal = np.ascontiguousarray(case.nodal, dtype=np.float64) if case.nodal is not None else None
-    nface = 0
     fn = fp = None
     if case.pload4 is not None and len(case.pload4.elem):
         pl = case.pload4
@@ -128,12 +125,29 @@ def assemble_rhs_device(backend, model: Model, eq, n, case, F_device=None):
         fn = np.ascontiguousarray(np.take_along_axis(model.conn2d()[pl.elem], faces, axis=1),
                                   dtype=np.int32)
         fp = np.ascontiguousarray(pl.p, dtype=np.float64)
-        nface = len(fn)
     g = None if case.grav is None else np.ascontiguousarray(case.grav, dtype=np.float64)
     rho = np.ascontiguousarray(model.rho, dtype=np.float64)
-    args = (backend._h, L.dp(nodal) if nodal is not None else None, nface,
+    return nodal, fn, fp, g, rho
+
+
+def load_args(arrays):
+    """ctypes arguments (nodal, nface, face_nodes, face_p, grav, rho) of
+    ``load_arrays`` output; the arrays must outlive the call."""
+    nodal, fn, fp, g, rho = arrays
+    nface = 0 if fn is None else len(fn)
+    return (L.dp(nodal) if nodal is not None else None, nface,
             L.ip(fn) if nface else None, L.dp(fp) if nface else None,
             L.dp(g) if g is not None else None, L.dp(rho))
+
+
+def assemble_rhs_device(backend, model: Model, eq, n, case, F_device=None):
+    """Load vector of one load case on the GPU (FORCE/MOMENT, PLOAD4 on CHEXA8
+    faces, GRAV with consistent masses) over the mesh of the last
+    ``assemble_model``. Returns F on the host, or writes the device buffer
+    ``F_device`` (an HBM address) when given."""
+    lib = L.load()
+    arrays = load_arrays(model, case)
+    args = (backend._h, *load_args(arrays))
     if F_device is not None:
         _check(lib.vbh_assemble_rhs_device(*args, C.c_void_p(F_device)))
         return None

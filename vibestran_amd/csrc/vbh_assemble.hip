@@ -310,7 +310,10 @@ struct ElemLds {
 };
 
 // MODE 0 scatters K_e into the lower CSR; MODE 1 writes
-// out[e] = -u_e^T (dK_e/dE_e) v_e = -u_e^T K_e v_e / E_e (K_e is linear in E).
+// out[e] = -u_e^T (dK_e/dE_e) v_e = -u_e^T K_e v_e / E_e (K_e is linear in E);
+// MODE 2 adds the element's internal force K_e u_e to out[6 * node + dof]
+// (every DOF of the element's nodes, constrained ones included; u_e is 0 at
+// DOFs without an equation).
 template <int K, int MODE = 0>
 __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
     int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
@@ -322,8 +325,10 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
   using KD = Kind<K>;
   constexpr int NN = KD::NN, ND = KD::ND, NL = KD::NL;
   __shared__ double lds_all[kElemWaves][ElemLds<K>::WORDS];
+  __shared__ double facc_all[kElemWaves][MODE == 2 ? KD::ND : 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double* lds = lds_all[wave];
+  double* facc = facc_all[wave];
   for (int base = blockIdx.x * kElemWaves; base < nsel; base += gridDim.x * kElemWaves) {
     const int s = base + wave;
     const bool active = s < nsel;
@@ -551,6 +556,30 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
           atomicMin(&err[1], e);
         }
       }
+    }
+    if constexpr (MODE == 2) {
+      // f_e = K_e u_e from the lane-owned lower entries, summed in LDS
+      const int* g = gd + static_cast<size_t>(e) * kGdStride;
+      if (lane < ND) facc[lane] = 0.0;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < KD::SU; ++q) {
+        const int t = lane + 64 * q;
+        if (t < NL) {
+          const unsigned short pr = pair_at<ND>(t);
+          const int i = pr >> 8, j = pr & 0xff;
+          const double ui = g[i] >= 0 ? uvec[g[i]] : 0.0, uj = g[j] >= 0 ? uvec[g[j]] : 0.0;
+          atomicAdd(&facc[i], kuu[q] * uj);
+          if (i != j) atomicAdd(&facc[j], kuu[q] * ui);
+        }
+      }
+      __syncthreads();
+      if (active && lane < ND) atomicAdd(&out[6 * conn[o + lane / 3] + lane % 3], facc[lane]);
+      if (active && lane == 0 && bad) {
+        atomicOr(&err[0], kErrJacobian);
+        atomicMin(&err[1], e);
+      }
+      __syncthreads();
     }
     if (MODE == 0 && active) {
       if (bad && lane == 0) {
@@ -1113,6 +1142,57 @@ void Ctx::element_sensitivity(int param, const double* u, const double* v, doubl
                                                std::to_string(herr[0]) + ")");
 }
 
+// f[6 node + dof] = sum_e K_e u_e over the retained mesh (internal nodal
+// forces at every DOF, constrained ones included). Host or device vectors.
+void Ctx::element_forces(const double* u, double* fout, bool dev) {
+  if (!mesh.valid) throw_error(VBH_ERR_STATE, "vbh_element_forces: no assembled mesh in this context");
+  if (!u || !fout) throw_error(VBH_ERR_ARG, "vbh_element_forces: null vector");
+  select_stream_device();
+  hipStream_t s = stream;
+  const size_t m = 6 * static_cast<size_t>(mesh.nnodes);
+  DBuf<double> du, dout;
+  const double* pu = u;
+  double* po = fout;
+  if (!dev) {
+    du.upload(u, static_cast<size_t>(mesh.n), s);
+    dout.resize(m);
+    pu = du.p;
+    po = dout.p;
+  }
+  launch_fill_f64(static_cast<long long>(m), po, 0.0, s);
+  DBuf<int> d_err;
+  d_err.resize(4);
+  hipLaunchKernelGGL(k_init_err, dim3(1), dim3(1), 0, s, d_err.p);
+  for (int k = 0; k < 5; ++k) {
+    const int ns = mesh.nsel[k];
+    if (!ns) continue;
+    if (k == 4) {
+      launch_quad4_forces(ns, mesh.sel[k].p, mesh.ep.p, mesh.conn.p, mesh.xyz.p, mesh.E.p,
+                          mesh.nu.p, mesh.t.p, mesh.shell_form, mesh.gd.p, pu, po, d_err.p, s);
+      continue;
+    }
+    const dim3 g(static_cast<unsigned>(std::min((ns + kElemWaves - 1) / kElemWaves, 65535)));
+    const dim3 b(64 * kElemWaves);
+#define VBH_FORCES(KIND)                                                                        \
+  hipLaunchKernelGGL((k_element_stiffness<KIND, 2>), g, b, 0, s, ns, mesh.sel[k].p, mesh.ep.p,  \
+                     mesh.conn.p, mesh.xyz.p, mesh.E.p, mesh.nu.p, mesh.gd.p, nullptr, nullptr, \
+                     nullptr, nullptr, d_err.p, pu, nullptr, po)
+    if (k == kHexEas) VBH_FORCES(kHexEas);
+    else if (k == kHexSri) VBH_FORCES(kHexSri);
+    else if (k == kPentaEas) VBH_FORCES(kPentaEas);
+    else VBH_FORCES(kTet4);
+#undef VBH_FORCES
+  }
+  VBH_HIP(hipGetLastError());
+  int herr[4];
+  VBH_HIP(hipMemcpyAsync(herr, d_err.p, sizeof herr, hipMemcpyDeviceToHost, s));
+  if (!dev) VBH_HIP(hipMemcpyAsync(fout, dout.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  if (herr[0]) throw_error(VBH_ERR_SOLVER, "vbh_element_forces: element " +
+                                               std::to_string(herr[1]) + " failed (code " +
+                                               std::to_string(herr[0]) + ")");
+}
+
 // Device operator from the device lower CSR: diagonal checks, BSR build (or a
 // host round trip for matrices without block structure).
 void Ctx::finish_device_operator() {
@@ -1284,6 +1364,24 @@ int vbh_element_sensitivity_device(vbh_ctx* h, int param, const double* du, cons
   }
   auto* c = reinterpret_cast<vbh::Ctx*>(h);
   return vbh::guarded([&] { c->element_sensitivity(param, du, dv, dout, true); });
+}
+
+int vbh_element_forces(vbh_ctx* h, const double* u, double* f) {
+  if (!h) {
+    vbh::set_error("vbh_element_forces: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { c->element_forces(u, f, false); });
+}
+
+int vbh_element_forces_device(vbh_ctx* h, const double* du, double* df) {
+  if (!h) {
+    vbh::set_error("vbh_element_forces_device: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { c->element_forces(du, df, true); });
 }
 
 int vbh_last_assembly_stats(const vbh_ctx* h, vbh_assembly_stats* out) {

@@ -114,9 +114,14 @@ struct Ctx {
                 const double* thick, int solid_form, int shell_form);
   void finish_device_operator();
   void element_sensitivity(int param, const double* u, const double* v, double* out, bool dev);
+  // full: F in node-DOF layout (6 per node, every DOF) instead of equations
   void assemble_rhs(const double* nodal, int nface, const int* face_nodes, const double* face_p,
-                    const double* grav, const double* rho, double* F, bool dev);
+                    const double* grav, const double* rho, double* F, bool dev,
+                    bool full = false);
   void recover_stress(const double* u, double* centroid, double* corner, bool dev);
+  void element_forces(const double* u, double* fout, bool dev);
+  void reactions(const double* u, const double* nodal, int nface, const int* face_nodes,
+                 const double* face_p, const double* grav, const double* rho, double* R);
   void recover_displacements(const double* u, double* disp, bool dev);
   void fetch_lower_to_host();
   int dof_map(int nnodes, const int* ids, int nelem, const int* etype, const int* eptr,

@@ -60,6 +60,13 @@ void launch_quad4_sensitivity(int nsel, const int* sel, const int* eptr, const i
                               const double* u, const double* v, double* out, int* err,
                               hipStream_t s);
 
+// Internal forces K_e u_e of the selected CQUAD4 elements added to
+// fout[6 * node + dof] (basic frame).
+void launch_quad4_forces(int nsel, const int* sel, const int* eptr, const int* conn,
+                         const double* xyz, const double* E, const double* nu, const double* thick,
+                         int form, const int* gd, const double* u, double* fout, int* err,
+                         hipStream_t s);
+
 // Stress recovery of the selected CQUAD4 elements: centroid rows cen[e*10..]
 // and, when cor is not null, corner rows cor[e*80 + 10*c..] (c < 4; rows 4..7
 // are zeroed). Columns [sx, sy, sxy, sz, syz, szx, mx, my, mxy, von_mises].

@@ -21,6 +21,7 @@
 // reference's element order differs from ours only by rounding).
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "vbh_ctx.hpp"
 #include "vbh_elem.hpp"
@@ -238,10 +239,15 @@ __global__ void k_init_err4(int* err) {
   err[1] = 0x7fffffff;
 }
 
+__global__ void k_iota(int m, int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) a[i] = i;
+}
+
 }  // namespace
 
 void Ctx::assemble_rhs(const double* nodal, int nface, const int* face_nodes, const double* face_p,
-                       const double* grav, const double* rho, double* F, bool dev) {
+                       const double* grav, const double* rho, double* F, bool dev, bool full) {
   if (!mesh.valid) throw_error(VBH_ERR_STATE, "vbh_assemble_rhs: no assembled mesh in this context");
   if (!F) throw_error(VBH_ERR_ARG, "vbh_assemble_rhs: null F");
   if (nface < 0 || (nface > 0 && (!face_nodes || !face_p)))
@@ -253,46 +259,79 @@ void Ctx::assemble_rhs(const double* nodal, int nface, const int* face_nodes, co
   select_stream_device();
   hipStream_t s = stream;
   DBuf<double> dF, dnodal, dfp, drho;
-  DBuf<int> dfn, derr;
+  DBuf<int> dfn, derr, dfull;
+  // full layout: every node DOF is its own equation (index 6 node + dof)
+  const int nout = full ? 6 * mesh.nnodes : mesh.n;
+  const int* eq = mesh.eq.p;
+  if (full) {
+    dfull.resize(static_cast<size_t>(nout));
+    if (nout > 0) hipLaunchKernelGGL(k_iota, dim3((nout + 255) / 256), dim3(256), 0, s, nout, dfull.p);
+    eq = dfull.p;
+  }
   double* pF = F;
   if (!dev) {
-    dF.resize(static_cast<size_t>(mesh.n));
+    dF.resize(static_cast<size_t>(nout));
     pF = dF.p;
   }
-  launch_fill_f64(mesh.n, pF, 0.0, s);
+  launch_fill_f64(nout, pF, 0.0, s);
   derr.resize(2);
   hipLaunchKernelGGL(k_init_err4, dim3(1), dim3(1), 0, s, derr.p);
   if (nodal) {
     dnodal.upload(nodal, static_cast<size_t>(6) * mesh.nnodes, s);
     hipLaunchKernelGGL(k_rhs_nodal, dim3((6 * mesh.nnodes + 255) / 256), dim3(256), 0, s,
-                       mesh.nnodes, mesh.eq.p, dnodal.p, pF);
+                       mesh.nnodes, eq, dnodal.p, pF);
   }
   if (nface > 0) {
     dfn.upload(face_nodes, static_cast<size_t>(4) * nface, s);
     dfp.upload(face_p, static_cast<size_t>(4) * nface, s);
     hipLaunchKernelGGL(k_rhs_pload4, dim3((nface + 255) / 256), dim3(256), 0, s, nface, dfn.p,
-                       dfp.p, mesh.xyz.p, mesh.eq.p, pF);
+                       dfp.p, mesh.xyz.p, eq, pF);
   }
   if (grav && (grav[0] != 0.0 || grav[1] != 0.0 || grav[2] != 0.0)) {
     drho.upload(rho, static_cast<size_t>(mesh.nelem), s);
     hipLaunchKernelGGL(k_rhs_grav, dim3((mesh.nelem + 255) / 256), dim3(256), 0, s, mesh.nelem,
                        mesh.et.p, mesh.ep.p, mesh.conn.p, mesh.xyz.p, drho.p,
-                       mesh.t.n ? mesh.t.p : nullptr, mesh.eq.p, grav[0], grav[1], grav[2], pF,
+                       mesh.t.n ? mesh.t.p : nullptr, eq, grav[0], grav[1], grav[2], pF,
                        derr.p);
   }
   int herr[2];
   VBH_HIP(hipMemcpyAsync(herr, derr.p, sizeof herr, hipMemcpyDeviceToHost, s));
   if (!dev)
-    VBH_HIP(hipMemcpyAsync(F, dF.p, sizeof(double) * mesh.n, hipMemcpyDeviceToHost, s));
+    VBH_HIP(hipMemcpyAsync(F, dF.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   VBH_HIP(hipStreamSynchronize(s));
   if (herr[0])
     throw_error(VBH_ERR_SOLVER,
                 "element " + std::to_string(herr[1]) + ": non-positive Jacobian in mass_matrix");
 }
 
+// R[6 node + dof] = (sum_e K_e u_e) - F_applied over every node DOF: the
+// support reaction at constrained DOFs, minus the equilibrium residual at free
+// ones (no reference counterpart).
+void Ctx::reactions(const double* u, const double* nodal, int nface, const int* face_nodes,
+                    const double* face_p, const double* grav, const double* rho, double* R) {
+  if (!R) throw_error(VBH_ERR_ARG, "vbh_reactions: null R");
+  const size_t m = 6 * static_cast<size_t>(mesh.nnodes);
+  std::vector<double> fa(m);
+  element_forces(u, R, false);
+  assemble_rhs(nodal, nface, face_nodes, face_p, grav, rho, fa.data(), false, true);
+  for (size_t i = 0; i < m; ++i) R[i] -= fa[i];
+}
+
 }  // namespace vbh
 
 extern "C" {
+
+int vbh_reactions(vbh_ctx* h, const double* u, const double* nodal, int nface,
+                  const int* face_nodes, const double* face_p, const double* grav,
+                  const double* rho, double* R) {
+  if (!h) {
+    vbh::set_error("vbh_reactions: null ctx");
+    return VBH_ERR_ARG;
+  }
+  auto* c = reinterpret_cast<vbh::Ctx*>(h);
+  return vbh::guarded([&] { c->reactions(u, nodal, nface, face_nodes, face_p, grav, rho, R); });
+}
+
 
 int vbh_assemble_rhs(vbh_ctx* h, const double* nodal, int nface, const int* face_nodes,
                      const double* face_p, const double* grav, const double* rho, double* F) {

@@ -294,6 +294,7 @@ __device__ __forceinline__ void tcol(const Frame& f, int a, double t[3], int l[3
   }
 }
 
+// MODE 3: out[6 node + dof] += (K_e u_e) in the basic frame.
 // MODE 0: assemble (scatter into the lower CSR).
 // MODE 1: out[e] = -u_e^T (dK_e/dE_e) v_e = -u_e^T K_e v_e / E_e (K_e is linear in E).
 // MODE 2: out[e] = -u_e^T (dK_e/dt_e) v_e: membrane and transverse shear scale
@@ -308,8 +309,10 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
     double* __restrict__ val, int* __restrict__ touched, int* __restrict__ err,
     const double* __restrict__ uvec, const double* __restrict__ vvec, double* __restrict__ out) {
   __shared__ ShellLds lds_all[kElemWaves];
+  __shared__ double facc_all[kElemWaves][MODE == 3 ? 24 : 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   ShellLds& L = lds_all[wave];
+  double* facc = facc_all[wave];
   for (int base = blockIdx.x * kElemWaves; base < nsel; base += gridDim.x * kElemWaves) {
     const int s = base + wave;
     const bool active = s < nsel;
@@ -436,7 +439,39 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
     }
     __syncthreads();
 
-    if (MODE != 0) {
+    if (MODE == 3) {
+      // f_e = K_e u_e in the basic frame, summed in LDS, added to out[6 node + dof]
+      const int* g = gd + static_cast<size_t>(e) * kGdStride;
+      if (lane < 24) facc[lane] = 0.0;
+      __syncthreads();
+      for (int it = lane; it < 300; it += 64) {
+        const unsigned short pr = c_spairs.v[it];
+        const int I = pr >> 8, J = pr & 0xff;
+        const int ni = I / 6, a = I % 6, nj = J / 6, b = J % 6;
+        double ta[3], tb[3];
+        int la[3], lb[3];
+        tcol(f, a, ta, la);
+        tcol(f, b, tb, lb);
+        double kv = 0.0;
+        for (int p = 0; p < 3; ++p) {
+          double inner = 0.0;
+          for (int q = 0; q < 3; ++q) inner += klocal(L, ni, la[p], nj, lb[q]) * tb[q];
+          kv += ta[p] * inner;
+        }
+        const double uI = g[I] >= 0 ? uvec[g[I]] : 0.0, uJ = g[J] >= 0 ? uvec[g[J]] : 0.0;
+        atomicAdd(&facc[I], kv * uJ);
+        if (I != J) atomicAdd(&facc[J], kv * uI);
+      }
+      __syncthreads();
+      if (active && lane < 24) atomicAdd(&out[6 * conn[o + lane / 6] + lane % 6], facc[lane]);
+      if (active && lane == 0 && bad) {
+        atomicOr(&err[0], bad);
+        atomicMin(&err[1], e);
+      }
+      __syncthreads();
+    }
+
+    if (MODE == 1 || MODE == 2) {
       // contraction with the element's parts of u and v
       const int* g = gd + static_cast<size_t>(e) * kGdStride;
       double part = 0.0;
@@ -656,6 +691,17 @@ void launch_quad4_recover(int nsel, const int* sel, const int* eptr, const int* 
   if (nsel <= 0) return;
   hipLaunchKernelGGL(k_quad4_recover, dim3((nsel + 255) / 256), dim3(256), 0, s, nsel, sel, eptr,
                      conn, xyz, E, nu, thick, form, gd, u, cen, cor, err);
+}
+
+void launch_quad4_forces(int nsel, const int* sel, const int* eptr, const int* conn,
+                         const double* xyz, const double* E, const double* nu, const double* thick,
+                         int form, const int* gd, const double* u, double* fout, int* err,
+                         hipStream_t s) {
+  if (nsel <= 0) return;
+  const int blocks = std::min((nsel + kElemWaves - 1) / kElemWaves, 65535);
+  hipLaunchKernelGGL(k_quad4<3>, dim3(blocks), dim3(64 * kElemWaves), 0, s, nsel, sel, eptr, conn,
+                     xyz, E, nu, thick, form, gd, nullptr, nullptr, nullptr, nullptr, err, u,
+                     nullptr, fout);
 }
 
 void launch_quad4_stiffness(int nsel, const int* sel, const int* eptr, const int* conn,

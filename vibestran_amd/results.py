@@ -47,6 +47,29 @@ def recover_displacements(backend, model: Model, u) -> np.ndarray:
     return disp
 
 
+def element_forces(backend, model: Model, u) -> np.ndarray:
+    """Internal nodal forces sum_e K_e u_e, [nn, 6] in the basic frame, at every
+    node DOF (constrained ones included)."""
+    lib = L.load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.empty((model.nn, 6), dtype=np.float64)
+    _check(lib.vbh_element_forces(backend._h, L.dp(u), L.dp(f)))
+    return f
+
+
+def reactions(backend, model: Model, u, case) -> np.ndarray:
+    """R = sum_e K_e u_e - F_applied, [nn, 6]: support reactions at
+    constrained DOFs, minus the equilibrium residual at free DOFs (no
+    reference counterpart)."""
+    from .assembly import load_args, load_arrays
+    lib = L.load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    arrays = load_arrays(model, case)
+    R = np.empty((model.nn, 6), dtype=np.float64)
+    _check(lib.vbh_reactions(backend._h, L.dp(u), *load_args(arrays), L.dp(R)))
+    return R
+
+
 def csv_paths(stem: str):
     """<stem without extension>.node.csv / .elem.csv, as CsvWriter::write
     derives them (csv_writer.cpp:36-38, 60-62)."""
