@@ -70,6 +70,7 @@ typedef struct {
   long long nnz;              /* stored lower nonzeros after exact-zero drops */
   double pattern_ms;          /* device time: pair keys, radix sort, unique */
   double element_ms;          /* device time: element matrices + scatter */
+  double mpc_ms;              /* wall time of K_red = T^T K T (0 without MPCs) */
 } vbh_assembly_stats;
 
 /* ---- errors / devices ---------------------------------------------------- */
@@ -122,6 +123,38 @@ int vbh_assemble(vbh_ctx* ctx, int n, int nnodes, const double* xyz, const int* 
                  const double* nu, const double* thickness, int solid_form, int shell_form);
 int vbh_last_assembly_stats(const vbh_ctx* ctx, vbh_assembly_stats* out);
 
+/* ---- multi-point constraints: MpcHandler (reference src/core/mpc_handler.cpp)
+ * vbh_set_mpc: MpcHandler::build (mpc_handler.cpp:14-164) over the numbering
+ * eq (nnodes x 6, -1 = constrained, n_full equations, as given to
+ * vbh_assemble). Equation k has the terms mpc_ptr[k] .. mpc_ptr[k+1]-1 of
+ * term_node (0-based node index), term_dof (1..6) and term_coeff: the term of
+ * largest |coefficient| is dependent and eliminated; an equation whose
+ * dependent DOF is constrained is skipped (message on stderr); terms on
+ * constrained or dependent DOFs drop out; a dependency cycle fails with
+ * VBH_ERR_SOLVER. The elimination applies from the next vbh_assemble on:
+ * that call assembles K_full (apply_to_stiffness, 188-278) and keeps
+ * K_red = T^T K_full T as the operator (n_reduced equations);
+ * vbh_assemble_rhs then gives F_red = T^T F (apply_to_force, 280-310), and the
+ * recovery, force and sensitivity entry points take u in the reduced numbering
+ * and expand it by u_full = T u_red (recover_dependent_dofs, 312-330).
+ * nmpc = 0 clears the elimination. */
+typedef struct {
+  int n_full, n_reduced;
+  int n_eliminations;  /* equations eliminated */
+  int n_skipped;       /* equations skipped: dependent DOF constrained */
+  int t_nnz;           /* entries of T */
+} vbh_mpc_stats;
+int vbh_set_mpc(vbh_ctx* ctx, int nnodes, const int* eq, int n_full, int nmpc, const int* mpc_ptr,
+                const int* term_node, const int* term_dof, const double* term_coeff,
+                int* n_reduced);
+int vbh_mpc_info(const vbh_ctx* ctx, vbh_mpc_stats* out);
+/* T of the elimination set by vbh_set_mpc as CSR over the n_full rows:
+ * t_ptr n_full + 1, t_col / t_val t_nnz (any pointer may be NULL). */
+int vbh_mpc_transform(const vbh_ctx* ctx, int* t_ptr, int* t_col, double* t_val);
+/* u_full (n_full, the mesh numbering) = T u_red over the mesh of the last
+ * vbh_assemble (identity copy when that mesh has no elimination). */
+int vbh_mpc_expand(vbh_ctx* ctx, const double* u_red, double* u_full);
+
 /* ---- load vector of one load case on the device ----------------------------
  * Over the mesh of the last vbh_assemble (reference LinearStaticSolver load
  * assembly): FORCE/MOMENT (linear_static.cpp:344-398): nodal[6*nnodes]
@@ -131,7 +164,7 @@ int vbh_last_assembly_stats(const vbh_ctx* ctx, vbh_assembly_stats* out);
  * corner pressures, or nface = 0; GRAV (linear_static.cpp:400-503): grav[3]
  * acceleration in the basic frame applied to every node with the consistent
  * mass, rho[nelem] densities already scaled by WTMASS, or grav = NULL.
- * F (length n) is overwritten. */
+ * F (length n; n_reduced of vbh_set_mpc under an elimination) is overwritten. */
 int vbh_assemble_rhs(vbh_ctx* ctx, const double* nodal, int nface, const int* face_nodes,
                      const double* face_p, const double* grav, const double* rho, double* F);
 /* Same with F a device pointer (nodal, faces, grav and rho stay host arrays). */
@@ -198,14 +231,16 @@ int vbh_write_elem_csv(const char* path, int append, int subcase_id, int nelem,
 
 /* ---- BDF input: BdfParser::parse_file (src/io/bdf_parser.cpp:78-633) for
  * the cards of this path (GRID, MAT1, PSOLID, PSHELL, CHEXA, CPENTA, CTETRA,
- * CQUAD4, FORCE, MOMENT, PLOAD4, GRAV, SPC, SPC1, PARAM SHELLFORM; case
- * control SUBCASE, LOAD, SPC, DISPLACEMENT, STRESS) in small, large and free
- * field with continuations. Host only, no device needed; parsed in parallel
- * chunks. Nodes come out in ascending ID order (node indices below are
- * 0-based into that order), elements in file order. Coordinate systems,
- * higher-order solids and PLOAD4 direction vectors are rejected with
- * VBH_ERR_SOLVER "Line N: ..."; other cards are skipped and listed by
- * vbh_deck_unsupported. */
+ * CQUAD4, FORCE, MOMENT, PLOAD4, GRAV, SPC, SPC1, MPC, MPCADD, CORD1R/C,
+ * CORD2R/C/S, PARAM SHELLFORM; case control SUBCASE, LOAD, SPC, MPC,
+ * DISPLACEMENT, STRESS) in small, large and free field with continuations.
+ * Host only, no device needed; parsed in parallel chunks. Nodes come out in
+ * ascending ID order (node indices below are 0-based into that order) with
+ * positions in the basic frame (GRID CP resolved, Model::resolve_coordinates,
+ * src/core/model.cpp:175-250), elements in file order, FORCE / MOMENT in a CID
+ * frame rotated into basic. Higher-order solids, PLOAD4 direction vectors and
+ * GRAV in a CID frame are rejected with VBH_ERR_SOLVER "Line N: ..."; other
+ * cards are skipped and listed by vbh_deck_unsupported. */
 typedef struct vbh_deck vbh_deck;
 typedef struct {
   int nnodes, nelem, nconn, nsubcase;
@@ -219,6 +254,8 @@ typedef struct {
   int npload4;       /* PLOAD4 faces of the load set */
   int disp_print, stress_print;  /* DISPLACEMENT / STRESS requests */
   double grav[3];    /* GRAV of the load set: scale x direction, basic frame */
+  int mpc_set;       /* case control MPC = (0 = none) */
+  int nmpc, nmpc_terms;  /* MPC equations of that set and their terms */
 } vbh_deck_subcase_info;
 int vbh_deck_read(const char* path, vbh_deck** out);
 int vbh_deck_parse(const char* text, long long len, vbh_deck** out);
@@ -234,6 +271,15 @@ int vbh_deck_subcase(const vbh_deck* deck, int k, vbh_deck_subcase_info* out);
  * G1 / G3-G4 as node indices (-1 = none) and 4 corner pressures each. */
 int vbh_deck_subcase_data(const vbh_deck* deck, int k, int* spc_nodes, int* spc_masks,
                           double* nodal, int* pl_elem, int* pl_g1, int* pl_g34, double* pl_p);
+/* MPC equations of subcase k's MPC set in model order (MPC cards in file
+ * order, MPCADD copies appended where the MPCADD card stands): mpc_ptr
+ * nmpc + 1 offsets into term_node (node index), term_dof (1..6), term_coeff. */
+int vbh_deck_subcase_mpc(const vbh_deck* deck, int k, int* mpc_ptr, int* term_node, int* term_dof,
+                         double* term_coeff);
+/* Displacement frames: cd[nnodes] the GRID CD (0 = basic) and rot[nnodes x 9]
+ * the frame's rotation matrix at the node, row-major, columns the local unit
+ * vectors in basic (identity for CD = 0). Node positions are already basic. */
+int vbh_deck_frames(const vbh_deck* deck, int* cd, double* rot);
 /* Keyword of the i-th unsupported bulk card, NULL past the end. */
 const char* vbh_deck_unsupported(const vbh_deck* deck, int i);
 

@@ -30,6 +30,13 @@ def load_csv_results(node_csv, elem_csv):
     return node_data, elem_data
 
 
+def uses_frames_or_mpc(name):
+    """Whether a deck has coordinate systems or MPC cards (read only by the
+    native reader)."""
+    with open(os.path.join(GOLD, name + ".bdf")) as fh:
+        return any(ln.upper().startswith(("MPC", "CORD")) for ln in fh)
+
+
 def deck_checks(name):
     with open(os.path.join(GOLD, name + ".expected.json")) as fh:
         return json.load(fh)["checks"]

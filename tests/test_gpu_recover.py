@@ -11,10 +11,11 @@ import pytest
 
 import oracle as O
 from e2e_csv import GOLD, apply_checks, load_csv_results
-from vibestran_amd import bdf, decks
+from vibestran_amd import deck, decks
 from vibestran_amd import model as M
 from vibestran_amd import results as R
-from vibestran_amd.assembly import assemble_model, assemble_rhs_device, dof_map_device
+from vibestran_amd.assembly import (analysis_numbering, assemble_model, assemble_rhs_device,
+                                    dof_map_device)
 from tests.test_gpu_assembly import mixed_mesh
 from tests.test_gpu_shell import warped_mesh
 
@@ -61,13 +62,17 @@ DECKS = sorted(f[:-4] for f in os.listdir(GOLD) if f.endswith(".bdf"))
 
 @pytest.mark.parametrize("name", DECKS)
 def test_e2e_deck_on_gpu_through_csv(gpu_backend_factory, name, tmp_path):
-    m = bdf.read_bdf(os.path.join(GOLD, name + ".bdf"))
+    # the native reader and the analysis numbering with MPC elimination and
+    # CD frames (the cquad4_ring_* decks have CORD2C nodes and oblique MPCs)
+    m = deck.read_deck(os.path.join(GOLD, name + ".bdf"))
     be = gpu_backend_factory(tolerance=1e-12, max_iters=200000)
-    eq, n = dof_map_device(be, m)
+    eq, n, n_red = analysis_numbering(be, m)
+    assert (n_red < n) == (m.mpc is not None)
     K = assemble_model(be, m, eq, n)
-    F = assemble_rhs_device(be, m, eq, n, m.loads[0])
+    assert K.n == n_red
+    F = assemble_rhs_device(be, m, eq, n_red, m.loads[0])
     u = be.solve(K, F)
-    disp = R.recover_displacements(be, m, u)
+    disp = M.to_output_frames(m, R.recover_displacements(be, m, u))
     cen, _ = R.recover_stress(be, m, u, corners=False)
     nc, ec = R.write_csv(str(tmp_path / name), m, [(m.subcase_id(0), disp, cen)])
     node_data, elem_data = load_csv_results(nc, ec)

@@ -32,6 +32,7 @@ class vbh_assembly_stats(C.Structure):
         ("nnz", C.c_longlong),
         ("pattern_ms", C.c_double),
         ("element_ms", C.c_double),
+        ("mpc_ms", C.c_double),
     ]
 
 
@@ -70,6 +71,19 @@ class vbh_deck_subcase_info(C.Structure):
         ("disp_print", C.c_int),
         ("stress_print", C.c_int),
         ("grav", C.c_double * 3),
+        ("mpc_set", C.c_int),
+        ("nmpc", C.c_int),
+        ("nmpc_terms", C.c_int),
+    ]
+
+
+class vbh_mpc_stats(C.Structure):
+    _fields_ = [
+        ("n_full", C.c_int),
+        ("n_reduced", C.c_int),
+        ("n_eliminations", C.c_int),
+        ("n_skipped", C.c_int),
+        ("t_nnz", C.c_int),
     ]
 
 
@@ -96,6 +110,10 @@ SIGNATURES = {
     "vbh_dof_map": (_I, [_P, _I, _IP, _I, _IP, _IP, _IP, _I, _IP, _IP, _IP, _IP]),
     "vbh_assemble": (_I, [_P, _I, _I, _DP, _IP, _I, _IP, _IP, _IP, _DP, _DP, _DP, _I, _I]),
     "vbh_last_assembly_stats": (_I, [_P, C.POINTER(vbh_assembly_stats)]),
+    "vbh_set_mpc": (_I, [_P, _I, _IP, _I, _I, _IP, _IP, _IP, _DP, _IP]),
+    "vbh_mpc_info": (_I, [_P, C.POINTER(vbh_mpc_stats)]),
+    "vbh_mpc_transform": (_I, [_P, _IP, _IP, _DP]),
+    "vbh_mpc_expand": (_I, [_P, _DP, _DP]),
     "vbh_assemble_rhs": (_I, [_P, _DP, _I, _IP, _DP, _DP, _DP, _DP]),
     "vbh_assemble_rhs_device": (_I, [_P, _DP, _I, _IP, _DP, _DP, _DP, _P]),
     "vbh_recover_stress": (_I, [_P, _DP, _DP, _DP]),
@@ -114,6 +132,8 @@ SIGNATURES = {
     "vbh_deck_mesh": (_I, [_P, _IP, _DP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP]),
     "vbh_deck_subcase": (_I, [_P, _I, C.POINTER(vbh_deck_subcase_info)]),
     "vbh_deck_subcase_data": (_I, [_P, _I, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
+    "vbh_deck_subcase_mpc": (_I, [_P, _I, _IP, _IP, _IP, _DP]),
+    "vbh_deck_frames": (_I, [_P, _IP, _DP]),
     "vbh_deck_unsupported": (C.c_char_p, [_P, _I]),
     "vbh_element_sensitivity": (_I, [_P, _I, _DP, _DP, _DP]),
     "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),

@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _lib as L
 from .backend import CsrData, _check
-from .model import Model
+from . import model as M
+from .model import Model, Mpc
 
 
 class DeviceCsr(CsrData):
@@ -75,6 +76,65 @@ def assemble_model(backend, model: Model, eq: np.ndarray, n: int) -> DeviceCsr:
     K = DeviceCsr(backend, nn_.value, nnz.value)
     backend._uploaded = K
     return K
+
+
+def set_mpc(backend, model: Model, eq: np.ndarray, n: int, mpc) -> int:
+    """MpcHandler::build (reference src/core/mpc_handler.cpp:14-164) in the
+    backend context for the equations ``mpc`` (an ``Mpc`` or None to clear)
+    over the numbering ``eq`` of ``n`` equations. From the next
+    ``assemble_model`` on, the operator is K_red = T^T K T, load vectors are
+    T^T F and solutions are expanded by T for recovery. Returns n_reduced."""
+    lib = L.load()
+    eqc = np.ascontiguousarray(eq, dtype=np.int32)
+    nr = C.c_int(0)
+    if mpc is None or mpc.n == 0:
+        _check(lib.vbh_set_mpc(backend._h, model.nn, L.ip(eqc), int(n), 0, None, None, None, None,
+                               C.byref(nr)))
+        return nr.value
+    ptr = np.ascontiguousarray(mpc.ptr, dtype=np.int32)
+    node = np.ascontiguousarray(mpc.node, dtype=np.int32)
+    dof = np.ascontiguousarray(mpc.dof, dtype=np.int32)
+    coeff = np.ascontiguousarray(mpc.coeff, dtype=np.float64)
+    _check(lib.vbh_set_mpc(backend._h, model.nn, L.ip(eqc), int(n), mpc.n, L.ip(ptr), L.ip(node),
+                           L.ip(dof), L.dp(coeff), C.byref(nr)))
+    return nr.value
+
+
+def mpc_stats(backend) -> L.vbh_mpc_stats:
+    s = L.vbh_mpc_stats()
+    _check(L.load().vbh_mpc_info(backend._h, C.byref(s)))
+    return s
+
+
+def mpc_transform(backend):
+    """T of the elimination set by ``set_mpc`` as CSR arrays over the full
+    equations: (t_ptr, t_col, t_val)."""
+    st = mpc_stats(backend)
+    ptr = np.empty(st.n_full + 1, np.int32)
+    col = np.empty(max(st.t_nnz, 1), np.int32)
+    val = np.empty(max(st.t_nnz, 1))
+    _check(L.load().vbh_mpc_transform(backend._h, L.ip(ptr), L.ip(col), L.dp(val)))
+    return ptr, col[:st.t_nnz], val[:st.t_nnz]
+
+
+def mpc_expand(backend, u_red, n_full: int) -> np.ndarray:
+    """u_full = T u_red over the mesh of the last ``assemble_model``
+    (MpcHandler::recover_dependent_dofs, mpc_handler.cpp:312-330)."""
+    u = np.ascontiguousarray(u_red, dtype=np.float64)
+    out = np.empty(n_full)
+    _check(L.load().vbh_mpc_expand(backend._h, L.dp(u), L.dp(out)))
+    return out
+
+
+def analysis_numbering(backend, model: Model):
+    """Equation numbering and MPC elimination of the linear-static analysis
+    (build_analysis_dof_map + build_analysis_mpc_system + MpcHandler::build,
+    reference src/solver/analysis_support.cpp:176-282): returns (eq, n_full,
+    n_reduced) with the elimination installed for the next ``assemble_model``.
+    Without MPCs or CD frames n_reduced == n_full."""
+    eq, n = M.analysis_dof_map(model)
+    eq, n, mpc = M.analysis_mpc_system(model, eq, n)
+    return eq, n, set_mpc(backend, model, eq, n, mpc)
 
 
 def assembly_stats(backend) -> L.vbh_assembly_stats:
@@ -143,8 +203,9 @@ def load_args(arrays):
 def assemble_rhs_device(backend, model: Model, eq, n, case, F_device=None):
     """Load vector of one load case on the GPU (FORCE/MOMENT, PLOAD4 on CHEXA8
     faces, GRAV with consistent masses) over the mesh of the last
-    ``assemble_model``. Returns F on the host, or writes the device buffer
-    ``F_device`` (an HBM address) when given."""
+    ``assemble_model``. Returns F (length n: the operator's equations,
+    n_reduced under an MPC elimination) on the host, or writes the device
+    buffer ``F_device`` (an HBM address) when given."""
     lib = L.load()
     arrays = load_arrays(model, case)
     args = (backend._h, *load_args(arrays))

@@ -2,6 +2,8 @@
 
 Cards: GRID, CHEXA, CPENTA, CTETRA, CQUAD4, MAT1, PSOLID, PSHELL, SPC1, SPC,
 FORCE, MOMENT, PLOAD4, GRAV, PARAM (SHELLFORM); case control SUBCASE/LOAD/SPC.
+GRID CP/CD and MPC cards are rejected: the native reader (``deck.read_deck``)
+handles coordinate systems and MPC equations.
 Free field (commas) and small fixed field with '+'/'*'/blank continuations.
 Real parsing follows the reference's Nastran rules: implicit exponents
 ("1.-3"), D exponents, trailing dots (reference src/io/bdf_parser.cpp:780-830).
@@ -102,7 +104,12 @@ def read_bdf(path_or_text: str, is_text=False) -> Model:
     for name, f in _cards(lines[bulk_at:]):
         f = f + [""] * 16
         if name == "GRID":
+            if any(len(f) > k and f[k].strip() not in ("", "0") for k in (1, 5)):
+                raise ValueError(f"GRID {f[0]}: coordinate systems (CP/CD) need the native "
+                                 "reader (vibestran_amd.deck.read_deck)")
             grids[int(f[0])] = (nastran_real(f[2]), nastran_real(f[3]), nastran_real(f[4]))
+        elif name in ("MPC", "MPCADD"):
+            raise ValueError(f"{name} needs the native reader (vibestran_amd.deck.read_deck)")
         elif name in ("CHEXA", "CPENTA", "CTETRA", "CQUAD4"):
             nn = {"CHEXA": 8, "CPENTA": 6, "CTETRA": 4, "CQUAD4": 4}[name]
             nodes = [int(x) for x in f[2:2 + nn]]

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -891,6 +892,8 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   hipStream_t s = stream;
   if (n_ <= 0) throw_error(VBH_ERR_SOLVER, "HIP PCG: stiffness matrix is empty -- no free DOFs");
   if (nelem <= 0 || nnodes <= 0) throw_error(VBH_ERR_ARG, "vbh_assemble: empty mesh");
+  if (mpc.active && mpc.n_full != n_)
+    throw_error(VBH_ERR_ARG, "vbh_assemble: n differs from the n_full of vbh_set_mpc");
   // host-side grouping by element kind
   constexpr int kQuad4 = 4;  // group index of CQUAD4 shells
   std::vector<int> groups[5];
@@ -1080,6 +1083,19 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   mesh.nelem = nelem;
   mesh.solid_form = solid_form;
   mesh.shell_form = shell_form;
+  asm_stats.mpc_ms = 0.0;
+  if (mpc.active) {
+    // K_red = T^T K_full T: node blocks no longer line up with the equations
+    const auto t0 = std::chrono::steady_clock::now();
+    mesh.mpc.upload(mpc, n_, s);
+    apply_mpc_to_lower();
+    asm_stats.mpc_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    asm_stats.nnz = l_nnz;
+    block_hint = 0;
+  } else {
+    mesh.mpc.upload(mpc, n_, s);
+  }
   mesh.valid = true;
   finish_device_operator();
 }
@@ -1094,16 +1110,12 @@ void Ctx::element_sensitivity(int param, const double* u, const double* v, doubl
   if (!u || !v || !out) throw_error(VBH_ERR_ARG, "vbh_element_sensitivity: null vector");
   select_stream_device();
   hipStream_t s = stream;
-  DBuf<double> du, dv, dout;
-  const double* pu = u;
-  const double* pv = v;
+  DBuf<double> du, dv, dout, fu, fv;
+  const double* pu = mesh_solution(u, dev, du, fu);
+  const double* pv = mesh_solution(v, dev, dv, fv);
   double* po = out;
   if (!dev) {
-    du.upload(u, static_cast<size_t>(mesh.n), s);
-    dv.upload(v, static_cast<size_t>(mesh.n), s);
     dout.resize(static_cast<size_t>(mesh.nelem));
-    pu = du.p;
-    pv = dv.p;
     po = dout.p;
   }
   launch_fill_f64(mesh.nelem, po, 0.0, s);  // solids stay 0 for thickness
@@ -1150,13 +1162,11 @@ void Ctx::element_forces(const double* u, double* fout, bool dev) {
   select_stream_device();
   hipStream_t s = stream;
   const size_t m = 6 * static_cast<size_t>(mesh.nnodes);
-  DBuf<double> du, dout;
-  const double* pu = u;
+  DBuf<double> du, dout, fu;
+  const double* pu = mesh_solution(u, dev, du, fu);
   double* po = fout;
   if (!dev) {
-    du.upload(u, static_cast<size_t>(mesh.n), s);
     dout.resize(m);
-    pu = du.p;
     po = dout.p;
   }
   launch_fill_f64(static_cast<long long>(m), po, 0.0, s);

@@ -18,8 +18,33 @@ struct GraphSlot {
   int max_iters = 0;
 };
 
+// Multi-point constraint elimination on the host (vbh_mpc.hip): the reduced
+// numbering and T (n_full x n_red, CSR by full rows) with its transpose.
+struct MpcSystem {
+  bool active = false;
+  int n_full = 0, n_red = 0, n_elim = 0, n_skipped = 0;
+  std::vector<int> index_map;                // full -> reduced equation, -1 dependent
+  std::vector<int> t_ptr, t_col, tt_ptr, tt_row;
+  std::vector<double> t_val, tt_val;
+  void build(int nnodes, const int* eq, int n_full, int nmpc, const int* ptr, const int* node,
+             const int* dof, const double* coeff);
+};
+
+// T of the mesh's elimination on the device (inactive: identity, n_red = n_full).
+struct MpcDev {
+  bool active = false;
+  int n_full = 0, n_red = 0;
+  DBuf<int> t_ptr, t_col, tt_ptr, tt_row;
+  DBuf<double> t_val, tt_val;
+  void upload(const MpcSystem& h, int n_mesh, hipStream_t s);
+  void expand(const double* u_red, double* u_full, hipStream_t s) const;   // T u
+  void reduce(const double* f_full, double* f_red, hipStream_t s) const;   // T^T f
+};
+
 // The mesh of the last vbh_assemble, kept on the device for the element
-// sensitivity kernels (and anything else that revisits the elements).
+// sensitivity kernels (and anything else that revisits the elements). n is
+// the size of the mesh's equation numbering (eq); under MPCs the operator has
+// mpc.n_red equations.
 struct MeshDev {
   bool valid = false;
   int n = 0, nnodes = 0, nelem = 0, solid_form = 0, shell_form = 0;
@@ -27,6 +52,7 @@ struct MeshDev {
   DBuf<int> eq, et, ep, conn, gd;
   DBuf<int> sel[5];
   int nsel[5] = {0, 0, 0, 0, 0};
+  MpcDev mpc;
 };
 
 struct Ctx {
@@ -88,6 +114,8 @@ struct Ctx {
 
   // mesh of the last device assembly
   MeshDev mesh;
+  // MPC elimination applied by the next vbh_assemble (vbh_set_mpc)
+  MpcSystem mpc;
 
   // block PCG over several load cases
   std::unique_ptr<BlockState, void (*)(BlockState*)> blk{nullptr, destroy_block_state};
@@ -113,6 +141,8 @@ struct Ctx {
                 const int* eptr, const int* conn, const double* E, const double* nu,
                 const double* thick, int solid_form, int shell_form);
   void finish_device_operator();
+  void apply_mpc_to_lower();
+  const double* mesh_solution(const double* u, bool dev, DBuf<double>& up, DBuf<double>& full);
   void element_sensitivity(int param, const double* u, const double* v, double* out, bool dev);
   // full: F in node-DOF layout (6 per node, every DOF) instead of equations
   void assemble_rhs(const double* nodal, int nface, const int* face_nodes, const double* face_p,

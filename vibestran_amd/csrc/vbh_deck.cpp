@@ -16,20 +16,30 @@
 //                    PSOLID (1289-1306), CQUAD4 (1433-1451), CHEXA / CTETRA /
 //                    CPENTA (1473-1532), FORCE / MOMENT (1663-1689), GRAV
 //                    (1717-1730), PLOAD4 (1835-1935), SPC / SPC1 (1937-1995),
-//                    PARAM SHELLFORM (574-586, 1997-)
+//                    PARAM SHELLFORM (574-586, 1997-), CORD2R/C/S (2008-2027),
+//                    CORD1R/C (2029-2043), MPC (2047-2068), MPCADD (2070-2095);
+//                    case control MPC = (242-248)
+//   coordinates      Model::resolve_coordinates (src/core/model.cpp:175-250):
+//                    CORD1x from basic node positions, CORD2x through their
+//                    RID chains, GRID CP positions into the basic frame
+//                    (build_axes / to_basic, src/core/coord_sys.cpp:10-69); the
+//                    node's CD frame as its rotation matrix at the node
+//                    (rotation_matrix, coord_sys.cpp:71-146), FORCE / MOMENT
+//                    in a CID frame rotated into basic (linear_static.cpp:351-392)
 // and builds the flat arrays the device path takes: nodes in ascending ID
 // order (the DofMap order), elements in file order with per-element E, nu,
 // rho and PSHELL T, and per subcase its SPC masks, nodal FORCE/MOMENT
-// components, PLOAD4 faces and GRAV vector.
+// components, PLOAD4 faces, GRAV vector and MPC equations.
 //
 // Large decks (192k MAT1 + PSOLID pairs at 605k DOF) are the reason this is
 // native: the text is cut into chunks at card boundaries and the chunks are
 // tokenised and converted in parallel, then merged in file order.
 //
-// Not covered (error, not silently wrong): coordinate systems (GRID CP/CD,
-// FORCE/MOMENT/GRAV CID), CHEXA20 / CTETRA10 / CPENTA15, PLOAD4 direction
-// vectors, mixed PSOLID ISOP within one deck, SPC enforced values. Other
-// cards are counted as unsupported and skipped, as the reference logs them.
+// Not covered (error, not silently wrong): GRAV in a CID frame, CHEXA20 /
+// CTETRA10 / CPENTA15, PLOAD4 direction vectors, mixed PSOLID ISOP within one
+// deck, SPC enforced values, a GRID CD naming an undefined frame. Other cards
+// (RBE2 / RBE3 among them) are counted as unsupported and skipped, as the
+// reference logs them.
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
@@ -47,6 +57,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include <array>
 #include <memory>
 
 #include "vbh_error.hpp"
@@ -143,11 +154,24 @@ int dof_mask(int c) {  // DofSet::from_int (types.hpp:114-124)
 }
 
 // ---- records produced by the chunk workers -----------------------------------
-struct GridRec { int id; double x, y, z; };
+struct GridRec { int id, cp, cd, line; double x, y, z; };
+struct CoordRec {  // CORD2x: defining points A, B, C in the RID frame; CORD1x: node IDs
+  int id, rid, type, line;  // type 0 rectangular, 1 cylindrical, 2 spherical
+  bool cord1;
+  double a[3], b[3], c[3];
+  int na, nb, nc;
+};
+struct MpcTerm { int node, dof; double coeff; };
+struct MpcRec {  // MPC equation, or MPCADD (add = true: sid <- each of srcs)
+  bool add;
+  int sid, line;
+  std::vector<MpcTerm> terms;
+  std::vector<int> srcs;
+};
 struct MatRec { int id; double E, nu, rho; };
 struct PropRec { int id, kind, mid; double t; int isop; };  // kind 0 PSOLID, 1 PSHELL
 struct ElemRec { int id, type, pid, nn; int nodes[8]; int line; };
-struct NodalRec { int sid, node, off; double v[3]; };  // off 0 FORCE, 3 MOMENT
+struct NodalRec { int sid, node, off, cid; double v[3]; };  // off 0 FORCE, 3 MOMENT
 struct GravRec { int sid; double a[3]; };
 struct PloadRec { int sid, eid, g1, g34; double p[4]; int line; };
 struct SpcRec { int sid, node, mask; };
@@ -161,6 +185,8 @@ struct Chunk {
   std::vector<GravRec> grav;
   std::vector<PloadRec> pload;
   std::vector<SpcRec> spc;
+  std::vector<CoordRec> coords;
+  std::vector<MpcRec> mpcs;
   std::vector<std::pair<std::string, std::string>> params;
   std::set<std::string> unsupported;
   std::string error;
@@ -261,10 +287,12 @@ void process(const std::vector<std::string_view>& f, int line, Chunk& ck) {
   while (!kw.empty() && (kw.back() == '*' || std::isspace(static_cast<unsigned char>(kw.back()))))
     kw.pop_back();
   if (kw == "GRID") {
-    const int cp = parse_int(fld(f, 2), line), cd = parse_int(fld(f, 6), line);
-    if (cp != 0 || cd != 0) deck_fail(line, "GRID coordinate systems (CP/CD) are not supported");
-    ck.grids.push_back({parse_int(fld(f, 1), line), parse_double(fld(f, 3), line),
-                        parse_double(fld(f, 4), line), parse_double(fld(f, 5), line)});
+    const int id = parse_int(fld(f, 1), line);
+    const int cp = fld(f, 2).empty() ? 0 : parse_int(fld(f, 2), line);
+    const double x = parse_double(fld(f, 3), line), y = parse_double(fld(f, 4), line),
+                 z = parse_double(fld(f, 5), line);
+    const int cd = fld(f, 6).empty() ? 0 : parse_int(fld(f, 6), line);
+    ck.grids.push_back({id, cp, cd, line, x, y, z});
   } else if (kw == "MAT1") {
     ck.mats.push_back({parse_int(fld(f, 1), line), parse_double(fld(f, 2), line),
                        fld(f, 4).empty() ? 0.0 : parse_double(fld(f, 4), line),
@@ -307,11 +335,10 @@ void process(const std::vector<std::string_view>& f, int line, Chunk& ck) {
     }
     ck.elems.push_back(e);
   } else if (kw == "FORCE" || kw == "MOMENT") {
-    if (parse_int(fld(f, 3), line) != 0)
-      deck_fail(line, kw + " in a local coordinate system (CID) is not supported");
+    const int sid = parse_int(fld(f, 1), line), g = parse_int(fld(f, 2), line);
+    const int cid = fld(f, 3).empty() ? 0 : parse_int(fld(f, 3), line);
     const double s = parse_double(fld(f, 4), line);
-    ck.nodal.push_back({parse_int(fld(f, 1), line), parse_int(fld(f, 2), line),
-                        kw == "FORCE" ? 0 : 3,
+    ck.nodal.push_back({sid, g, kw == "FORCE" ? 0 : 3, cid,
                         {s * parse_double(fld(f, 5), line), s * parse_double(fld(f, 6), line),
                          s * parse_double(fld(f, 7), line)}});
   } else if (kw == "GRAV") {
@@ -413,6 +440,47 @@ void process(const std::vector<std::string_view>& f, int line, Chunk& ck) {
       }
     }
     for (int g : ids) ck.spc.push_back({sid, g, mask});
+  } else if (kw == "CORD2R" || kw == "CORD2C" || kw == "CORD2S") {
+    CoordRec c{};
+    c.id = parse_int(fld(f, 1), line);
+    c.rid = fld(f, 2).empty() ? 0 : parse_int(fld(f, 2), line);
+    c.type = kw[5] == 'R' ? 0 : (kw[5] == 'C' ? 1 : 2);
+    c.line = line;
+    for (int k = 0; k < 3; ++k) {
+      c.a[k] = parse_double(fld(f, 3 + k), line);
+      c.b[k] = parse_double(fld(f, 6 + k), line);
+      c.c[k] = parse_double(fld(f, 9 + k), line);
+    }
+    ck.coords.push_back(c);
+  } else if (kw == "CORD1R" || kw == "CORD1C") {
+    CoordRec c{};
+    c.id = parse_int(fld(f, 1), line);
+    c.type = kw[5] == 'R' ? 0 : 1;
+    c.cord1 = true;
+    c.line = line;
+    c.na = parse_int(fld(f, 2), line);
+    c.nb = parse_int(fld(f, 3), line);
+    c.nc = parse_int(fld(f, 4), line);
+    ck.coords.push_back(c);
+  } else if (kw == "MPC") {
+    MpcRec m{};
+    m.sid = parse_int(fld(f, 1), line);
+    m.line = line;
+    for (size_t i = 2; i + 2 < f.size(); i += 3) {
+      if (f[i].empty()) break;
+      const int g = parse_int(f[i], line), c = parse_int(f[i + 1], line);
+      if (c < 1 || c > 6) deck_fail(line, "MPC component " + std::to_string(c) + " is not 1..6");
+      m.terms.push_back({g, c, parse_double(f[i + 2], line)});
+    }
+    if (!m.terms.empty()) ck.mpcs.push_back(std::move(m));
+  } else if (kw == "MPCADD") {
+    MpcRec m{};
+    m.add = true;
+    m.sid = parse_int(fld(f, 1), line);
+    m.line = line;
+    for (size_t i = 2; i < f.size(); ++i)
+      if (!f[i].empty()) m.srcs.push_back(parse_int(f[i], line));
+    ck.mpcs.push_back(std::move(m));
   } else if (kw == "PARAM") {
     if (f.size() >= 3 && !f[1].empty())
       ck.params.emplace_back(upper(f[1]), upper(f[2]));
@@ -469,9 +537,13 @@ struct Deck {
   std::vector<int> elem_ids, etype, eptr, conn;
   std::vector<double> E, nu, rho, thick;
   int solid_form = 0, shell_form = 0;
+  std::vector<int> cd;       // CD frame ID per node (0 = basic)
+  std::vector<double> cd_T;  // per node: rotation of the CD frame at the node (3x3 row-major)
   struct Sub {
-    int id = 1, load = 0, spc = 0, disp = 0, stress = 0;
+    int id = 1, load = 0, spc = 0, disp = 0, stress = 0, mpc = 0;
     std::vector<int> spc_nodes, spc_masks;
+    std::vector<int> mpc_ptr{0}, mpc_node, mpc_dof;  // MPC equations of the MPC set
+    std::vector<double> mpc_coeff;
     std::vector<double> nodal;  // nnodes x 6
     std::vector<int> pl_elem, pl_g1, pl_g34;
     std::vector<double> pl_p;
@@ -516,6 +588,165 @@ class IdIndex {
   std::vector<int> table_;
   std::unordered_map<int, int> map_;
 };
+
+// ---- coordinate frames (src/core/coord_sys.cpp) -------------------------------------
+struct Frame {
+  int type = 0;           // 0 rectangular, 1 cylindrical, 2 spherical
+  double o[3] = {0, 0, 0};
+  double ax[3][3] = {};   // ax[k] = local axis k in basic
+};
+
+void unit(const double* v, double* out, int line) {  // Vec3::normalized (types.hpp:173-178)
+  const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (n < 1e-15) deck_fail(line, "Cannot normalize zero vector");
+  for (int k = 0; k < 3; ++k) out[k] = v[k] / n;
+}
+
+void cross(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// build_axes (coord_sys.cpp:10-30): z along B - A, y = z x (C - A), x = y x z.
+void build_axes(Frame& fr, const double* a, const double* b, const double* c, int line) {
+  double d[3], ez[3], ac[3], ey[3], ex[3];
+  for (int k = 0; k < 3; ++k) d[k] = b[k] - a[k];
+  unit(d, ez, line);
+  for (int k = 0; k < 3; ++k) d[k] = c[k] - a[k];
+  unit(d, ac, line);
+  cross(ez, ac, d);
+  const double ly = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (ly < 1e-15) deck_fail(line, "Coordinate system defining points are collinear");
+  for (int k = 0; k < 3; ++k) ey[k] = d[k] / ly;
+  cross(ey, ez, d);
+  const double lx = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (int k = 0; k < 3; ++k) ex[k] = d[k] / lx;
+  for (int k = 0; k < 3; ++k) {
+    fr.o[k] = a[k];
+    fr.ax[0][k] = ex[k];
+    fr.ax[1][k] = ey[k];
+    fr.ax[2][k] = ez[k];
+  }
+}
+
+// to_basic (coord_sys.cpp:32-69): (r, theta deg, z) or (rho, theta deg, phi deg).
+void to_basic(const Frame& fr, const double* p, double* out) {
+  double q[3] = {p[0], p[1], p[2]};
+  constexpr double kDeg = 3.14159265358979323846 / 180.0;
+  if (fr.type == 1) {
+    const double th = p[1] * kDeg;
+    q[0] = p[0] * std::cos(th);
+    q[1] = p[0] * std::sin(th);
+    q[2] = p[2];
+  } else if (fr.type == 2) {
+    const double th = p[1] * kDeg, ph = p[2] * kDeg;
+    q[0] = p[0] * std::sin(ph) * std::cos(th);
+    q[1] = p[0] * std::sin(ph) * std::sin(th);
+    q[2] = p[0] * std::cos(ph);
+  }
+  for (int i = 0; i < 3; ++i)
+    out[i] = fr.o[i] + fr.ax[0][i] * q[0] + fr.ax[1][i] * q[1] + fr.ax[2][i] * q[2];
+}
+
+// rotation_matrix (coord_sys.cpp:71-146): columns are the frame's local unit
+// vectors at basic position pos (for curvilinear frames e_r, e_theta, e_z or
+// e_rho, e_phi, e_theta).
+void rotation(const Frame& fr, const double* pos, double T[3][3]) {
+  double R[3][3];  // R(i, k) = ax[k][i]
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) R[i][k] = fr.ax[k][i];
+  if (fr.type == 0) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) T[i][j] = R[i][j];
+    return;
+  }
+  double dp[3], l[3];
+  for (int k = 0; k < 3; ++k) dp[k] = pos[k] - fr.o[k];
+  for (int k = 0; k < 3; ++k) l[k] = fr.ax[k][0] * dp[0] + fr.ax[k][1] * dp[1] + fr.ax[k][2] * dp[2];
+  const double th = std::atan2(l[1], l[0]);
+  const double ct = std::cos(th), st = std::sin(th);
+  if (fr.type == 1) {
+    for (int i = 0; i < 3; ++i) {
+      T[i][0] = R[i][0] * ct + R[i][1] * st;
+      T[i][1] = R[i][0] * (-st) + R[i][1] * ct;
+      T[i][2] = R[i][2];
+    }
+    return;
+  }
+  const double ph = std::atan2(std::sqrt(l[0] * l[0] + l[1] * l[1]), l[2]);
+  const double cp = std::cos(ph), sp = std::sin(ph);
+  for (int i = 0; i < 3; ++i) {
+    T[i][0] = R[i][0] * (sp * ct) + R[i][1] * (sp * st) + R[i][2] * cp;
+    T[i][1] = R[i][0] * (cp * ct) + R[i][1] * (cp * st) + R[i][2] * (-sp);
+    T[i][2] = R[i][0] * (-st) + R[i][1] * ct;
+  }
+}
+
+// Model::resolve_coordinates (model.cpp:175-250) over the merged records: the
+// frames by ID, and GRID CP positions moved into the basic frame in place.
+std::map<int, Frame> resolve_frames(const std::vector<CoordRec>& recs, std::vector<GridRec>& grids) {
+  std::map<int, const CoordRec*> last;  // a later definition replaces an earlier one
+  for (const auto& r : recs) last[r.id] = &r;
+  std::map<int, Frame> fr;
+  auto raw_node = [&](int id, const CoordRec& r, const char* role) {
+    const auto it = std::lower_bound(grids.begin(), grids.end(), id,
+                                     [](const GridRec& g, int v) { return g.id < v; });
+    if (it == grids.end() || it->id != id)
+      deck_fail(r.line, "CORD1x " + std::to_string(r.id) + ": " + role + " node " +
+                            std::to_string(id) + " not found");
+    return std::array<double, 3>{it->x, it->y, it->z};  // as given: basic
+  };
+  for (const auto& [id, r] : last) {
+    Frame f;
+    f.type = r->type;
+    if (r->cord1) {
+      const auto a = raw_node(r->na, *r, "A"), b = raw_node(r->nb, *r, "B"), c = raw_node(r->nc, *r, "C");
+      build_axes(f, a.data(), b.data(), c.data(), r->line);
+      fr[id] = f;
+    } else if (r->rid == 0) {
+      build_axes(f, r->a, r->b, r->c, r->line);
+      fr[id] = f;
+    }
+  }
+  for (bool progress = true; progress;) {
+    progress = false;
+    for (const auto& [id, r] : last) {
+      if (fr.count(id)) continue;
+      const auto p = fr.find(r->rid);
+      if (p == fr.end()) continue;
+      double a[3], b[3], c[3];
+      to_basic(p->second, r->a, a);
+      to_basic(p->second, r->b, b);
+      to_basic(p->second, r->c, c);
+      Frame f;
+      f.type = r->type;
+      build_axes(f, a, b, c, r->line);
+      fr[id] = f;
+      progress = true;
+    }
+  }
+  for (const auto& [id, r] : last)
+    if (!fr.count(id))
+      deck_fail(r->line, "Coordinate system " + std::to_string(id) +
+                             ": circular or unresolvable reference chain (RID=" +
+                             std::to_string(r->rid) + " is part of a cycle)");
+  for (auto& g : grids) {
+    if (g.cp == 0) continue;
+    const auto it = fr.find(g.cp);
+    if (it == fr.end())
+      deck_fail(g.line, "Node " + std::to_string(g.id) + ": CP=" + std::to_string(g.cp) +
+                            " references undefined coordinate system");
+    const double p[3] = {g.x, g.y, g.z};
+    double q[3];
+    to_basic(it->second, p, q);
+    g.x = q[0];
+    g.y = q[1];
+    g.z = q[2];
+    g.cp = 0;
+  }
+  return fr;
+}
 
 void parse_text(std::string_view text, Deck& d) {
   // lines
@@ -603,6 +834,10 @@ void parse_text(std::string_view text, Deck& d) {
         } else if (kw.rfind("LOAD", 0) == 0) {
           const int v = value_after_eq(line);
           if (v >= 0) sc.load = v;
+        } else if (kw.rfind("MPC", 0) == 0 && kw.find('=') != std::string_view::npos &&
+                   kw.rfind("MPCADD", 0) != 0) {
+          const int v = value_after_eq(line);
+          if (v >= 0) sc.mpc = v;
         } else if (kw.rfind("SPC", 0) == 0 && kw.rfind("SPCADD", 0) != 0) {
           const int v = value_after_eq(line);
           if (v >= 0) sc.spc = v;
@@ -678,7 +913,28 @@ void parse_text(std::string_view text, Deck& d) {
     if (!uniq.empty() && uniq.back().id == g.id) uniq.back() = g;
     else uniq.push_back(g);
   }
+  std::vector<CoordRec> coords;
+  for (const auto& ck : chunks) coords.insert(coords.end(), ck.coords.begin(), ck.coords.end());
+  const std::map<int, Frame> frames = resolve_frames(coords, uniq);
   const int nn = static_cast<int>(uniq.size());
+  d.cd.assign(static_cast<size_t>(nn), 0);
+  d.cd_T.assign(9 * static_cast<size_t>(nn), 0.0);
+  for (int i = 0; i < nn; ++i) {
+    double* T = &d.cd_T[9 * static_cast<size_t>(i)];
+    T[0] = T[4] = T[8] = 1.0;
+    const GridRec& g = uniq[static_cast<size_t>(i)];
+    if (g.cd == 0) continue;
+    const auto it = frames.find(g.cd);
+    if (it == frames.end())
+      deck_fail(g.line, "Node " + std::to_string(g.id) + ": CD=" + std::to_string(g.cd) +
+                            " references undefined coordinate system");
+    const double pos[3] = {g.x, g.y, g.z};
+    double R[3][3];
+    rotation(it->second, pos, R);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) T[3 * a + b] = R[a][b];
+    d.cd[static_cast<size_t>(i)] = g.cd;
+  }
   d.node_ids.resize(nn);
   d.xyz.resize(3 * static_cast<size_t>(nn));
   for (int i = 0; i < nn; ++i) {
@@ -736,8 +992,39 @@ void parse_text(std::string_view text, Deck& d) {
   bool any_pload = false;
   for (const auto& ck : chunks) any_pload = any_pload || !ck.pload.empty();
   if (any_pload) elem_ix.build(d.elem_ids.size(), [&](size_t i) { return d.elem_ids[i]; });
-  // subcases: SPC set, FORCE/MOMENT, PLOAD4, GRAV of the LOAD set
+  // model.mpcs: MPC cards in file order, each MPCADD appending relabelled
+  // copies of the equations listed so far (process_mpcadd, 2070-2095)
+  std::vector<const MpcRec*> mpcs;
+  std::vector<int> mpc_sid;
+  for (const auto& ck : chunks)
+    for (const auto& r : ck.mpcs) {
+      if (!r.add) {
+        mpcs.push_back(&r);
+        mpc_sid.push_back(r.sid);
+        continue;
+      }
+      for (int src : r.srcs) {
+        if (src == r.sid) continue;
+        const size_t n0 = mpcs.size();
+        for (size_t k = 0; k < n0; ++k)
+          if (mpc_sid[k] == src) {
+            mpcs.push_back(mpcs[k]);
+            mpc_sid.push_back(r.sid);
+          }
+      }
+    }
+  // subcases: SPC set, FORCE/MOMENT, PLOAD4, GRAV of the LOAD set, MPC set
   for (auto& s : d.subs) {
+    if (s.mpc != 0)
+      for (size_t k = 0; k < mpcs.size(); ++k) {
+        if (mpc_sid[k] != s.mpc) continue;
+        for (const auto& t : mpcs[k]->terms) {
+          s.mpc_node.push_back(node_index(t.node, mpcs[k]->line));
+          s.mpc_dof.push_back(t.dof);
+          s.mpc_coeff.push_back(t.coeff);
+        }
+        s.mpc_ptr.push_back(static_cast<int>(s.mpc_node.size()));
+      }
     std::vector<int> mask(static_cast<size_t>(nn), 0);
     s.nodal.assign(6 * static_cast<size_t>(nn), 0.0);
     for (const auto& ck : chunks) {
@@ -746,7 +1033,14 @@ void parse_text(std::string_view text, Deck& d) {
       for (const auto& r : ck.nodal)
         if (r.sid == s.load) {
           const size_t i = static_cast<size_t>(node_index(r.node, 0));
-          for (int k = 0; k < 3; ++k) s.nodal[6 * i + r.off + k] += r.v[k];
+          double v[3] = {r.v[0], r.v[1], r.v[2]};
+          const auto fit = r.cid != 0 ? frames.find(r.cid) : frames.end();
+          if (fit != frames.end()) {  // a CID without a frame leaves the vector as given
+            double R[3][3];
+            rotation(fit->second, &d.xyz[3 * i], R);
+            for (int a = 0; a < 3; ++a) v[a] = R[a][0] * r.v[0] + R[a][1] * r.v[1] + R[a][2] * r.v[2];
+          }
+          for (int k = 0; k < 3; ++k) s.nodal[6 * i + r.off + k] += v[k];
         }
       for (const auto& r : ck.grav)
         if (r.sid == s.load)
@@ -863,6 +1157,9 @@ int vbh_deck_subcase(const vbh_deck* d, int k, vbh_deck_subcase_info* s) {
   s->npload4 = static_cast<int>(u.pl_elem.size());
   s->disp_print = u.disp;
   s->stress_print = u.stress;
+  s->mpc_set = u.mpc;
+  s->nmpc = static_cast<int>(u.mpc_ptr.size()) - 1;
+  s->nmpc_terms = static_cast<int>(u.mpc_node.size());
   for (int i = 0; i < 3; ++i) s->grav[i] = u.grav[i];
   return VBH_OK;
 }
@@ -884,6 +1181,33 @@ int vbh_deck_subcase_data(const vbh_deck* d, int k, int* spc_nodes, int* spc_mas
   cp(pl_g1, u.pl_g1);
   cp(pl_g34, u.pl_g34);
   cp(pl_p, u.pl_p);
+  return VBH_OK;
+}
+
+int vbh_deck_subcase_mpc(const vbh_deck* d, int k, int* mpc_ptr, int* term_node, int* term_dof,
+                         double* term_coeff) {
+  if (!d || k < 0 || k >= static_cast<int>(d->subs.size())) {
+    vbh::set_error("vbh_deck_subcase_mpc: bad argument");
+    return VBH_ERR_ARG;
+  }
+  const auto& u = d->subs[static_cast<size_t>(k)];
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+  };
+  cp(mpc_ptr, u.mpc_ptr);
+  cp(term_node, u.mpc_node);
+  cp(term_dof, u.mpc_dof);
+  cp(term_coeff, u.mpc_coeff);
+  return VBH_OK;
+}
+
+int vbh_deck_frames(const vbh_deck* d, int* cd, double* rot) {
+  if (!d) {
+    vbh::set_error("vbh_deck_frames: null deck");
+    return VBH_ERR_ARG;
+  }
+  if (cd) std::copy(d->cd.begin(), d->cd.end(), cd);
+  if (rot) std::copy(d->cd_T.begin(), d->cd_T.end(), rot);
   return VBH_OK;
 }
 

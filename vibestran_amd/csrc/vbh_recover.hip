@@ -208,15 +208,13 @@ void Ctx::recover_stress(const double* u, double* centroid, double* corner, bool
   select_stream_device();
   hipStream_t s = stream;
   const size_t ne = static_cast<size_t>(mesh.nelem);
-  DBuf<double> du, dcen, dcor;
+  DBuf<double> du, fu, dcen, dcor;
   DBuf<int> derr;
-  const double* pu = u;
+  const double* pu = mesh_solution(u, dev, du, fu);
   double* pc = centroid;
   double* pk = corner;
   if (!dev) {
-    du.upload(u, static_cast<size_t>(mesh.n), s);
     dcen.resize(10 * ne);
-    pu = du.p;
     pc = dcen.p;
     if (corner) {
       dcor.resize(80 * ne);
@@ -269,13 +267,11 @@ void Ctx::recover_displacements(const double* u, double* disp, bool dev) {
   select_stream_device();
   hipStream_t s = stream;
   const int m = 6 * mesh.nnodes;
-  DBuf<double> du, dd;
-  const double* pu = u;
+  DBuf<double> du, fu, dd;
+  const double* pu = mesh_solution(u, dev, du, fu);
   double* pd = disp;
   if (!dev) {
-    du.upload(u, static_cast<size_t>(mesh.n), s);
     dd.resize(static_cast<size_t>(m));
-    pu = du.p;
     pd = dd.p;
   }
   if (m > 0)

@@ -258,7 +258,7 @@ void Ctx::assemble_rhs(const double* nodal, int nface, const int* face_nodes, co
       throw_error(VBH_ERR_ARG, "vbh_assemble_rhs: face node index out of range");
   select_stream_device();
   hipStream_t s = stream;
-  DBuf<double> dF, dnodal, dfp, drho;
+  DBuf<double> dF, dnodal, dfp, drho, dFfull;
   DBuf<int> dfn, derr, dfull;
   // full layout: every node DOF is its own equation (index 6 node + dof)
   const int nout = full ? 6 * mesh.nnodes : mesh.n;
@@ -268,10 +268,18 @@ void Ctx::assemble_rhs(const double* nodal, int nface, const int* face_nodes, co
     if (nout > 0) hipLaunchKernelGGL(k_iota, dim3((nout + 255) / 256), dim3(256), 0, s, nout, dfull.p);
     eq = dfull.p;
   }
+  // under MPCs the loads go to the mesh numbering first, then F_red = T^T F
+  const bool reduce = !full && mesh.mpc.active;
+  const int nres = reduce ? mesh.mpc.n_red : nout;
   double* pF = F;
   if (!dev) {
-    dF.resize(static_cast<size_t>(nout));
+    dF.resize(static_cast<size_t>(std::max(nres, 1)));
     pF = dF.p;
+  }
+  double* pOut = pF;
+  if (reduce) {
+    dFfull.resize(static_cast<size_t>(std::max(nout, 1)));
+    pF = dFfull.p;
   }
   launch_fill_f64(nout, pF, 0.0, s);
   derr.resize(2);
@@ -294,10 +302,11 @@ void Ctx::assemble_rhs(const double* nodal, int nface, const int* face_nodes, co
                        mesh.t.n ? mesh.t.p : nullptr, eq, grav[0], grav[1], grav[2], pF,
                        derr.p);
   }
+  if (reduce) mesh.mpc.reduce(pF, pOut, s);
   int herr[2];
   VBH_HIP(hipMemcpyAsync(herr, derr.p, sizeof herr, hipMemcpyDeviceToHost, s));
   if (!dev)
-    VBH_HIP(hipMemcpyAsync(F, dF.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+    VBH_HIP(hipMemcpyAsync(F, dF.p, sizeof(double) * nres, hipMemcpyDeviceToHost, s));
   VBH_HIP(hipStreamSynchronize(s));
   if (herr[0])
     throw_error(VBH_ERR_SOLVER,

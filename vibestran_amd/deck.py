@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib as L
 from .backend import _check
-from .model import LoadCase, Model, Pload4
+from .model import LoadCase, Model, Mpc, Pload4
 
 
 def _read(handle) -> Model:
@@ -47,8 +47,14 @@ def _read(handle) -> Model:
         pp = np.empty((si.npload4, 4))
         _check(lib.vbh_deck_subcase_data(handle, k, L.ip(sn), L.ip(sm), L.dp(nodal), L.ip(pe),
                                          L.ip(g1), L.ip(g34), L.dp(pp)))
-        if k == 0:  # the model's SPC set is the first subcase's (one K per deck)
+        if k == 0:  # the model's SPC and MPC sets are the first subcase's (one K per deck)
             m.spc_nodes, m.spc_masks = sn, sm
+            if si.nmpc:
+                mp = Mpc(ptr=np.empty(si.nmpc + 1, np.int32), node=np.empty(si.nmpc_terms, np.int32),
+                         dof=np.empty(si.nmpc_terms, np.int32), coeff=np.empty(si.nmpc_terms))
+                _check(lib.vbh_deck_subcase_mpc(handle, k, L.ip(mp.ptr), L.ip(mp.node),
+                                                L.ip(mp.dof), L.dp(mp.coeff)))
+                m.mpc = mp
         pl = None
         if si.npload4:
             # face grids as element-local corner indices (-1 = default face)
@@ -64,6 +70,11 @@ def _read(handle) -> Model:
         m.loads.append(LoadCase(nodal=nodal, pload4=pl, grav=grav))
         sub_ids.append(si.id)
     m.subcase_ids = sub_ids
+    cd = np.empty(nn, np.int32)
+    rot = np.empty((nn, 3, 3))
+    _check(lib.vbh_deck_frames(handle, L.ip(cd), L.dp(rot)))
+    if np.any(cd != 0):
+        m.cd, m.cd_rot = cd, rot
     m.meta["unsupported_cards"] = [lib.vbh_deck_unsupported(handle, i).decode()
                                    for i in range(sz.nunsupported)]
     return m

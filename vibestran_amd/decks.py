@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .model import CHEXA8, CPENTA6, CQUAD4, CTETRA4, LoadCase, Model, Pload4
+from .model import CHEXA8, CPENTA6, CQUAD4, CTETRA4, LoadCase, Model, Mpc, Pload4
 
 
 def _hex_grid(nx, ny, nz, h=1.0):
@@ -60,6 +60,30 @@ def hex_block(nx, ny, nz, h=1.0, E=200e3, nu=0.3, rho=7.85e-9, jitter=0.0, jitte
               solid_form=solid_form, spc_nodes=spc_nodes,
               spc_masks=np.full(len(spc_nodes), 0b000111, np.int32))
     m.meta.update(dict(nx=nx, ny=ny, nz=nz, h=h, ijk=ijk))
+    return m
+
+
+def hex_block_mpc(nx=120, ny=40, nz=40, seed=11, jitter=0.1):
+    """MPC workload on the CHEXA8 block (x = 0 clamped): the free end face
+    x = nx*h tied to its centre node in T1-T3 (u(g, d) - u(c, d) = 0, the
+    face moves as one translation), the y = 0 side face on inclined slides
+    a u_x + b u_y = 0 with |a| > |b| drawn per node, and a point FORCE -z at
+    the centre node."""
+    m = hex_block(nx, ny, nz, jitter=jitter, jitter_seed=seed)
+    ijk = m.meta["ijk"]
+    end = np.nonzero(ijk[:, 0] == nx)[0]
+    c = int(np.nonzero((ijk[:, 0] == nx) & (ijk[:, 1] == ny // 2) & (ijk[:, 2] == nz // 2))[0][0])
+    eqs = [[(int(g), d, 1.0), (c, d, -1.0)] for g in end if g != c for d in (1, 2, 3)]
+    rng = np.random.default_rng(seed)
+    side = np.nonzero((ijk[:, 1] == 0) & (ijk[:, 0] > 0) & (ijk[:, 0] < nx))[0]
+    a = rng.uniform(1.0, 2.0, size=len(side))
+    b = rng.uniform(-0.9, 0.9, size=len(side))
+    eqs += [[(int(g), 1, float(ai)), (int(g), 2, float(bi))] for g, ai, bi in zip(side, a, b)]
+    m.mpc = Mpc.from_equations(eqs)
+    nodal = np.zeros((m.nn, 6))
+    nodal[c, 2] = -1000.0
+    m.loads.append(LoadCase(nodal=nodal))
+    m.meta.update(dict(name=f"hex{nx}x{ny}x{nz}-mpc-tie-slide", centre=c))
     return m
 
 

@@ -9,6 +9,10 @@ Data layout (host side, handed to libvbh as flat arrays):
   E, nu, rho float64[ne]    per-element MAT1 data
   thickness float64[ne]     PSHELL T (shells only)
   elem_ids  int32[ne]       external element IDs (optional)
+  mpc       Mpc             MPC equations of the analysis (optional)
+  cd        int32[nn]       GRID CD frame per node, 0 = basic (optional)
+  cd_rot    float64[nn,3,3] rotation of the node's CD frame at the node:
+                            columns are the frame's unit vectors in basic
 """
 from __future__ import annotations
 
@@ -44,6 +48,40 @@ class LoadCase:
 
 
 @dataclass
+class Mpc:
+    """Multi-point constraint equations sum_t coeff_t u(node_t, dof_t) = 0:
+    equation k holds the terms ptr[k] .. ptr[k+1]-1 (node as a 0-based node
+    index, dof 1..6)."""
+
+    ptr: np.ndarray
+    node: np.ndarray
+    dof: np.ndarray
+    coeff: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.ptr) - 1
+
+    @staticmethod
+    def from_equations(eqs) -> "Mpc":
+        """From a list of equations, each a list of (node index, dof, coeff)."""
+        ptr = np.zeros(len(eqs) + 1, np.int32)
+        node, dof, coeff = [], [], []
+        for k, e in enumerate(eqs):
+            for g, d, c in e:
+                node.append(g)
+                dof.append(d)
+                coeff.append(c)
+            ptr[k + 1] = len(node)
+        return Mpc(ptr=ptr, node=np.asarray(node, np.int32), dof=np.asarray(dof, np.int32),
+                   coeff=np.asarray(coeff, np.float64))
+
+    def equations(self):
+        return [[(int(self.node[t]), int(self.dof[t]), float(self.coeff[t]))
+                 for t in range(self.ptr[k], self.ptr[k + 1])] for k in range(self.n)]
+
+
+@dataclass
 class Model:
     node_ids: np.ndarray
     xyz: np.ndarray
@@ -62,6 +100,9 @@ class Model:
     meta: dict = field(default_factory=dict)
     elem_ids: Optional[np.ndarray] = None  # external element IDs (default 1..ne)
     subcase_ids: Optional[List[int]] = None  # SUBCASE IDs of `loads` (default 1..)
+    mpc: Optional[Mpc] = None
+    cd: Optional[np.ndarray] = None
+    cd_rot: Optional[np.ndarray] = None
 
     def element_ids(self) -> np.ndarray:
         if self.elem_ids is not None:
@@ -96,11 +137,15 @@ def analysis_dof_map(model: Model):
     6 slots per node in sorted node-ID order; inactive slots (solid-only nodes
     lose R1-R3, mark_element_dofs 39-68) and SPC'd slots removed with the
     remaining indices shifted down. Equivalent closed form used here: eq is
-    the rank of each free (node ID, dof) pair in lexicographic order.
+    the rank of each free (node ID, dof) pair in lexicographic order. MPC
+    terms activate their DOFs (mark_explicit_mpc_dofs, 97-105); SPCs on nodes
+    with a CD frame are left to analysis_mpc_system (206-207).
 
     Returns (eq int32[nn,6] with -1 for constrained, n_free)."""
     nn = model.nn
     active = np.zeros((nn, 6), dtype=bool)
+    if model.mpc is not None and model.mpc.n:
+        active[model.mpc.node, model.mpc.dof - 1] = True
     for t in np.unique(model.elem_type):
         sel = np.nonzero(model.elem_type == t)[0]
         k = NODES_PER[int(t)]
@@ -110,17 +155,114 @@ def analysis_dof_map(model: Model):
             active[nodes, :] = True
         else:
             active[nodes, 0:3] = True
+    spc = _spc_flags(model)
+    if model.cd is not None:
+        spc[np.asarray(model.cd) != 0] = False
+    return _number(model, active & ~spc)
+
+
+def _spc_flags(model: Model) -> np.ndarray:
+    spc = np.zeros((model.nn, 6), dtype=bool)
     if len(model.spc_nodes):
         bits = (model.spc_masks[:, None] >> np.arange(6)[None, :]) & 1
-        spc = np.zeros((nn, 6), dtype=bool)
         np.logical_or.at(spc, model.spc_nodes, bits.astype(bool))
-        active &= ~spc
+    return spc
+
+
+def _number(model: Model, free: np.ndarray):
     order = np.argsort(model.node_ids, kind="stable")
-    flat = active[order].reshape(-1)
+    flat = free[order].reshape(-1)
     rank = np.cumsum(flat, dtype=np.int64) - 1
-    eq = np.full((nn, 6), -1, dtype=np.int32)
-    eq[order] = np.where(flat, rank, -1).reshape(nn, 6).astype(np.int32)
+    eq = np.full((model.nn, 6), -1, dtype=np.int32)
+    eq[order] = np.where(flat, rank, -1).reshape(model.nn, 6).astype(np.int32)
     return eq, int(flat.sum())
+
+
+def _reduce_3x3(node: int, T3: np.ndarray, cd_dofs, base: int, mpcs, direct):
+    """reduce_3x3_constraints (reference src/solver/analysis_support.cpp:107-171):
+    the constrained local axes of a CD frame (rows of A, basic components)
+    reduced by Gaussian elimination with column pivoting; a row left with one
+    nonzero is a direct constraint of that basic DOF, others become MPC
+    equations on the node's basic DOFs."""
+    count = len(cd_dofs)
+    if count == 0:
+        return
+    if count == 3:
+        direct.extend((node, base + j) for j in range(3))
+        return
+    A = np.zeros((3, 3))
+    perm = [0, 1, 2]
+    for i in range(count):
+        for j in range(3):
+            A[i, j] = T3[j, cd_dofs[i]]
+    for row in range(count):
+        best_col, best = row, abs(A[row, row])
+        for col in range(row + 1, 3):
+            if abs(A[row, col]) > best:
+                best, best_col = abs(A[row, col]), col
+        if best_col != row:
+            A[:count, [row, best_col]] = A[:count, [best_col, row]]
+            perm[row], perm[best_col] = perm[best_col], perm[row]
+        if abs(A[row, row]) < 1e-14:
+            continue
+        for i in range(row + 1, count):
+            factor = A[i, row] / A[row, row]
+            for j in range(row, 3):
+                A[i, j] -= factor * A[row, j]
+    for row in range(count):
+        if abs(A[row, row]) < 1e-14:
+            continue
+        cols = [c for c in range(row, 3) if abs(A[row, c]) > 1e-14]
+        if len(cols) == 1:
+            direct.append((node, base + perm[row]))
+            continue
+        mpcs.append([(node, base + perm[c] + 1, float(A[row, c])) for c in cols])
+
+
+def analysis_mpc_system(model: Model, eq: np.ndarray, n: int):
+    """build_analysis_mpc_system (reference src/solver/analysis_support.cpp:217-282)
+    up to MpcHandler::build: SPCs on nodes with a CD frame turn into direct
+    constraints of basic DOFs (removed from the numbering, 263) or MPC
+    equations, ahead of the model's explicit MPC equations (270-275).
+
+    Returns (eq, n, Mpc or None): the numbering MpcHandler::build starts from
+    and every equation it eliminates."""
+    eqs = []
+    direct = []
+    if model.cd is not None and np.any(np.asarray(model.cd) != 0):
+        spc = _spc_flags(model)
+        order = np.argsort(model.node_ids, kind="stable")
+        for i in order:
+            if model.cd[i] == 0 or not spc[i].any():
+                continue
+            T3 = np.asarray(model.cd_rot[i])
+            trans = [d for d in range(3) if spc[i, d] and eq[i, d] >= 0]
+            _reduce_3x3(int(i), T3, trans, 0, eqs, direct)
+            rot = [d for d in range(3) if spc[i, d + 3] and eq[i, d + 3] >= 0]
+            _reduce_3x3(int(i), T3, rot, 3, eqs, direct)
+    if direct:
+        free = eq >= 0
+        for g, d in direct:
+            free[g, d] = False
+        eq, n = _number(model, free)
+    if model.mpc is not None:
+        eqs.extend(model.mpc.equations())
+    return eq, n, (Mpc.from_equations(eqs) if eqs else None)
+
+
+def to_output_frames(model: Model, disp: np.ndarray) -> np.ndarray:
+    """Node displacements [nn, 6] from the basic frame into each node's CD
+    frame, translations and rotations alike (v_cd = T3^T v_basic, reference
+    src/solver/linear_static.cpp:591-612)."""
+    out = np.array(disp, dtype=np.float64, copy=True)
+    if model.cd is None:
+        return out
+    sel = np.nonzero(np.asarray(model.cd) != 0)[0]
+    if len(sel):
+        Tt = np.transpose(np.asarray(model.cd_rot)[sel], (0, 2, 1))
+        out[sel, 0:3] = np.einsum("nij,nj->ni", Tt, out[sel, 0:3])
+        out[sel, 3:6] = np.einsum("nij,nj->ni", Tt, out[sel, 3:6])
+    return out
 
 
 def element_gdofs(model: Model, eq: np.ndarray) -> np.ndarray:
