@@ -231,7 +231,7 @@ int vbh_write_elem_csv(const char* path, int append, int subcase_id, int nelem,
 
 /* ---- BDF input: BdfParser::parse_file (src/io/bdf_parser.cpp:78-633) for
  * the cards of this path (GRID, MAT1, PSOLID, PSHELL, CHEXA, CPENTA, CTETRA,
- * CQUAD4, FORCE, MOMENT, PLOAD4, GRAV, SPC, SPC1, MPC, MPCADD, CORD1R/C,
+ * CQUAD4, FORCE, MOMENT, PLOAD4, GRAV, SPC, SPC1, MPC, MPCADD, RBE2, RBE3, CORD1R/C,
  * CORD2R/C/S, PARAM SHELLFORM; case control SUBCASE, LOAD, SPC, MPC,
  * DISPLACEMENT, STRESS) in small, large and free field with continuations.
  * Host only, no device needed; parsed in parallel chunks. Nodes come out in
@@ -247,6 +247,7 @@ typedef struct {
   int solid_form;  /* VBH_SOLID_EAS / VBH_SOLID_SRI (PSOLID ISOP) */
   int shell_form;  /* VBH_SHELL_MITC4 / VBH_SHELL_MINDLIN (PARAM SHELLFORM) */
   int nunsupported;
+  int nrigid_mpc, nrigid_terms;  /* RBE2 / RBE3 expanded into MPC equations */
 } vbh_deck_sizes;
 typedef struct {
   int id, load_set, spc_set;
@@ -276,6 +277,14 @@ int vbh_deck_subcase_data(const vbh_deck* deck, int k, int* spc_nodes, int* spc_
  * nmpc + 1 offsets into term_node (node index), term_dof (1..6), term_coeff. */
 int vbh_deck_subcase_mpc(const vbh_deck* deck, int k, int* mpc_ptr, int* term_node, int* term_dof,
                          double* term_coeff);
+/* RBE2 / RBE3 elements as MPC equations (expand_rbe2 / expand_rbe3,
+ * src/elements/rbe_constraints.cpp:22-181; RBE2s then RBE3s, file order),
+ * applied in every subcase ahead of the MPC set: mpc_ptr nrigid_mpc + 1,
+ * terms as in vbh_deck_subcase_mpc; active[nnodes] the DOF mask (bit d =
+ * DOF d+1) the rigid elements activate (mark_rigid_element_dofs,
+ * src/solver/analysis_support.cpp:70-95). */
+int vbh_deck_rigid(const vbh_deck* deck, int* active, int* mpc_ptr, int* term_node, int* term_dof,
+                   double* term_coeff);
 /* Displacement frames: cd[nnodes] the GRID CD (0 = basic) and rot[nnodes x 9]
  * the frame's rotation matrix at the node, row-major, columns the local unit
  * vectors in basic (identity for CD = 0). Node positions are already basic. */

@@ -7,6 +7,9 @@ restatement of the reference's MpcHandler (oracle/mpc.py):
   differs: the reference transforms element by element, the device the
   assembled operator);
 - the solved and expanded u agrees with the oracle's eliminated solution;
+- models: random equations with a chained pair, tie + slide equations on a
+  hex block, the 6-fold ring deck (oblique MPCs), a CQUAD4 plate with RBE2
+  and RBE3 elements;
 - error paths: a dependency cycle, a bad term, a numbering mismatch;
 - at the 605k-DOF hex size (2.6k tie + 4.8k slide equations) the properties
   the elimination fixes: tied DOFs equal their master exactly, slide
@@ -61,10 +64,23 @@ def ring_full():
     return deck.read_deck(os.path.join(GOLD, "cquad4_ring_6fold_full.bdf"))
 
 
+def rbe_plate():
+    from vibestran_amd import deck
+    from tests.test_oracle_mpc import plate_deck
+    cards = lambda nid, nx, ny: [
+        "GRID,9999,,4.3,0.5,0.4", "GRID,9998,,4.2,0.45,0.3",
+        "RBE2,101,9999,123456," + ",".join(str(nid(nx, j)) for j in (0, 1)),
+        f"RBE3,201,,9998,123456,2.0,123,{nid(nx, 2)},{nid(nx - 1, 2)},0.5,123456,{nid(nx, 3)},"
+        f"{nid(nx - 1, 3)}"]
+    return deck.read_deck(plate_deck(cards).replace("ENDDATA", "FORCE,1,9998,0,20.,0.,1.,1.\nENDDATA"),
+                          is_text=True)
+
+
 MODELS = {
     "random_block": random_mpc_block,
     "tie_slide_block": lambda: decks.hex_block_mpc(5, 3, 4),
     "ring_full": ring_full,
+    "rbe_plate": rbe_plate,
 }
 
 
@@ -78,7 +94,9 @@ def test_mpc_matches_oracle(gpu_backend_factory, name):
     m = MODELS[name]()
     be = gpu_backend_factory(tolerance=1e-13, max_iters=100000)
     eq, n, n_red = analysis_numbering(be, m)
-    omp = MpcOracle(eq, n, m.mpc.equations())
+    eq0, n0 = M.analysis_dof_map(m)
+    _, _, mp = M.analysis_mpc_system(m, eq0, n0)
+    omp = MpcOracle(eq, n, mp.equations())
     assert n_red == omp.n_red < n
     st = mpc_stats(be)
     assert (st.n_full, st.n_reduced, st.n_eliminations) == (n, n_red, len(omp.eliminations))
@@ -100,8 +118,14 @@ def test_mpc_matches_oracle(gpu_backend_factory, name):
     F = assemble_rhs_device(be, m, eq, n_red, m.loads[0])
     Fo = nodal_rhs_mpc(m, eq, m.loads[0].nodal, omp)
     assert np.abs(F - Fo).max() <= 1e-12 * np.abs(Fo).max()
+    # the operator in the reduced numbering: SpMV and block solve
+    x = np.random.default_rng(0).standard_normal(n_red)
+    assert np.abs(be.spmv(x) - Ko @ x).max() <= 1e-12 * np.abs(Ko @ x).max()
+    U = be.solve_block(K, np.stack([F, -2.0 * F], axis=1))
     # solve, expand, recover
     u = be.solve(K, F)
+    assert np.abs(U[:, 0] - u).max() <= 1e-9 * np.abs(u).max()
+    assert np.abs(U[:, 1] + 2.0 * u).max() <= 2e-9 * np.abs(u).max()
     uo = omp.recover(np.linalg.solve(Ko, Fo))
     uf = mpc_expand(be, u, n)
     assert np.abs(uf - uo).max() <= 1e-9 * np.abs(uo).max()
@@ -153,6 +177,7 @@ def test_full_size_tie_and_slide(gpu_backend_factory):
     print(f"  K_red = T^T K T: {assembly_stats(be).mpc_ms:.1f} ms", flush=True)
     F = assemble_rhs_device(be, m, eq, n_red, m.loads[0])
     u = be.solve(K, F)
+    assert be.stats().format == 3     # node blocks kept: dependents are identity rows
     uf = mpc_expand(be, u, n)
     umax = np.abs(uf).max()
     c = m.meta["centre"]

@@ -196,3 +196,76 @@ def test_cd_frame_spc_equals_oblique_mpc():
     i8 = int(np.nonzero(cdm.node_ids == 8)[0][0])
     assert abs(out[i8, 1]) <= 1e-15 * np.abs(disp).max()
     assert abs(out[i8, 0] - np.hypot(disp[i8, 0], disp[i8, 1])) <= 1e-12 * np.abs(disp).max()
+
+
+def plate_deck(rigid_cards, nx=4, ny=3):
+    """CQUAD4 plate in the XY plane, x = 0 edge clamped, plus the given rigid
+    element cards and their extra grids; loads on grid 9999. Lever arms stay
+    below 1 so that every rigid equation keeps its unit-coefficient grid as
+    the dependent term (the largest |coefficient| is chosen)."""
+    lines = ["SOL 101", "CEND", "SUBCASE 1", "  LOAD = 1", "  SPC = 1", "BEGIN BULK",
+             "MAT1,1,70000.,,.3", "PSHELL,1,1,0.5"]
+    nid = lambda i, j: 1 + i + (nx + 1) * j
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            lines.append(f"GRID,{nid(i, j)},,{1.0 * i},{0.25 * j + 0.02 * i * j},0.")
+    e = 1
+    for j in range(ny):
+        for i in range(nx):
+            lines.append(f"CQUAD4,{e},1,{nid(i, j)},{nid(i + 1, j)},{nid(i + 1, j + 1)},{nid(i, j + 1)}")
+            e += 1
+    lines.append("SPC1,1,123456," + ",".join(str(nid(0, j)) for j in range(ny + 1)))
+    lines += rigid_cards(nid, nx, ny)
+    lines += ["FORCE,1,9999,0,100.,0.3,-0.2,1.", "MOMENT,1,9999,0,50.,1.,0.5,-0.4", "ENDDATA"]
+    return "\n".join(lines) + "\n"
+
+
+def solve_model(m):
+    eq, n = M.analysis_dof_map(m)
+    eq, n, mp = M.analysis_mpc_system(m, eq, n)
+    mpc = MpcOracle(eq, n, mp.equations())
+    K = assemble_model_mpc(m, eq, mpc).to_scipy().toarray()
+    u = mpc.recover(np.linalg.solve(K, nodal_rhs_mpc(m, eq, m.loads[0].nodal, mpc)))
+    disp = np.zeros((m.nn, 6))
+    disp[eq >= 0] = u[eq[eq >= 0]]
+    return disp, {int(g): i for i, g in enumerate(m.node_ids)}
+
+
+def test_rbe2_moves_its_grids_rigidly():
+    cards = lambda nid, nx, ny: [
+        "GRID,9999,,4.3,0.5,0.4",
+        "RBE2,101,9999,123456," + ",".join(str(nid(nx, j)) for j in range(ny + 1)) + ",1.-6"]
+    m = deck.read_deck(plate_deck(cards), is_text=True)
+    assert m.rigid_mpc is not None and m.rigid_mpc.n == 6 * 4
+    assert m.rigid_active[-1] == 63
+    disp, idx = solve_model(m)
+    h = idx[9999]
+    scale = np.abs(disp).max()
+    for j in range(4):
+        g = idx[1 + 4 + 5 * j]
+        r = m.xyz[g] - m.xyz[h]
+        rigid = disp[h, :3] + np.cross(disp[h, 3:], r)
+        assert np.abs(disp[g, :3] - rigid).max() <= 1e-10 * scale
+        assert np.abs(disp[g, 3:] - disp[h, 3:]).max() <= 1e-10 * scale
+    assert np.abs(disp[h]).max() > 0.0
+
+
+def test_rbe3_reference_grid_follows_the_weighted_average():
+    # group 1: weight 2, translations of two edge grids; group 2: weight 0.5,
+    # all six components of two other edge grids (lever terms omega_j x r_j)
+    cards = lambda nid, nx, ny: [
+        "GRID,9999,,4.2,0.45,0.3",
+        f"RBE3,201,,9999,123456,2.0,123,{nid(nx, 0)},{nid(nx, 1)},0.5,123456,{nid(nx, 2)},{nid(nx, 3)}"]
+    m = deck.read_deck(plate_deck(cards), is_text=True)
+    assert m.rigid_mpc.n == 6
+    disp, idx = solve_model(m)
+    h = idx[9999]
+    g1 = [idx[5], idx[10]]
+    g2 = [idx[15], idx[20]]
+    W = 2.0 * 2 + 0.5 * 2
+    avg = sum(2.0 * disp[g, :3] for g in g1) + sum(
+        0.5 * (disp[g, :3] + np.cross(disp[g, 3:], m.xyz[g] - m.xyz[h])) for g in g2)
+    scale = np.abs(disp).max()
+    assert np.abs(disp[h, :3] - avg / W).max() <= 1e-10 * scale
+    # rotations: only the group with rotational components, over W_rot
+    assert np.abs(disp[h, 3:] - sum(0.5 * disp[g, 3:] for g in g2) / (0.5 * 2)).max() <= 1e-10 * scale

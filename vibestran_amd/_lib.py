@@ -58,6 +58,8 @@ class vbh_deck_sizes(C.Structure):
         ("solid_form", C.c_int),
         ("shell_form", C.c_int),
         ("nunsupported", C.c_int),
+        ("nrigid_mpc", C.c_int),
+        ("nrigid_terms", C.c_int),
     ]
 
 
@@ -134,6 +136,7 @@ SIGNATURES = {
     "vbh_deck_subcase_data": (_I, [_P, _I, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
     "vbh_deck_subcase_mpc": (_I, [_P, _I, _IP, _IP, _IP, _DP]),
     "vbh_deck_frames": (_I, [_P, _IP, _DP]),
+    "vbh_deck_rigid": (_I, [_P, _IP, _IP, _IP, _IP, _DP]),
     "vbh_deck_unsupported": (C.c_char_p, [_P, _I]),
     "vbh_element_sensitivity": (_I, [_P, _I, _DP, _DP, _DP]),
     "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),

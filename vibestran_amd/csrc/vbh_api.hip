@@ -544,7 +544,11 @@ int vbh_set_csr(vbh_ctx* h, int n, int nnz, const int* row_ptr, const int* col_i
     return VBH_ERR_ARG;
   }
   Ctx* c = reinterpret_cast<Ctx*>(h);
-  return guarded([&] { c->upload_operator(n, nnz, row_ptr, col_ind, values, lower_only != 0); });
+  return guarded([&] {
+    c->upload_operator(n, nnz, row_ptr, col_ind, values, lower_only != 0);
+    c->op_padded = false;
+    c->n_api = c->n;
+  });
 }
 
 int vbh_get_csr_size(const vbh_ctx* h, int* n, int* nnz) {
@@ -557,8 +561,8 @@ int vbh_get_csr_size(const vbh_ctx* h, int* n, int* nnz) {
     vbh::set_error("no matrix set");
     return VBH_ERR_STATE;
   }
-  if (n) *n = c->n;
-  if (nnz) *nnz = c->l_nnz;
+  if (n) *n = c->n_api;
+  if (nnz) *nnz = c->op_padded ? c->l_nnz - (c->n - c->n_api) : c->l_nnz;
   return VBH_OK;
 }
 
@@ -574,9 +578,25 @@ int vbh_get_csr(const vbh_ctx* h, int* row_ptr, int* col_ind, double* values) {
   }
   return guarded([&] {
     c->fetch_lower_to_host();
-    if (row_ptr) std::copy(c->lower_rp.begin(), c->lower_rp.end(), row_ptr);
-    if (col_ind) std::copy(c->lower_ci.begin(), c->lower_ci.end(), col_ind);
-    if (values) std::copy(c->lower_v.begin(), c->lower_v.end(), values);
+    if (!c->op_padded) {
+      if (row_ptr) std::copy(c->lower_rp.begin(), c->lower_rp.end(), row_ptr);
+      if (col_ind) std::copy(c->lower_ci.begin(), c->lower_ci.end(), col_ind);
+      if (values) std::copy(c->lower_v.begin(), c->lower_v.end(), values);
+      return;
+    }
+    // the reduced CSR: dependent rows (a unit diagonal) dropped, the other
+    // rows and columns renumbered (no retained entry names a dependent column)
+    const std::vector<int>& api_of = c->op_api_host;
+    int r = 0, k = 0;
+    if (row_ptr) row_ptr[0] = 0;
+    for (int i = 0; i < c->n; ++i) {
+      if (api_of[static_cast<size_t>(i)] < 0) continue;
+      for (int e = c->lower_rp[static_cast<size_t>(i)]; e < c->lower_rp[static_cast<size_t>(i) + 1]; ++e, ++k) {
+        if (col_ind) col_ind[k] = api_of[static_cast<size_t>(c->lower_ci[static_cast<size_t>(e)])];
+        if (values) values[k] = c->lower_v[static_cast<size_t>(e)];
+      }
+      if (row_ptr) row_ptr[++r] = k;
+    }
   });
 }
 
@@ -587,14 +607,9 @@ static int solve_impl(Ctx* c, const double* F, double* u, bool device_ptrs, int*
     if (!F || !u) vbh::throw_error(VBH_ERR_ARG, "vbh_solve: null vector");
     c->select_stream_device();
     c->ensure_work(c->n_pad);
-    const size_t bytes = static_cast<size_t>(c->n) * sizeof(double);
-    VBH_HIP(hipMemcpyAsync(c->w_F.p, F, bytes,
-                           device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                           c->stream));
+    c->to_operator(F, device_ptrs, c->w_F.p);
     vbh::PcgScalars s = c->run_solve(c->precision);
-    VBH_HIP(hipMemcpyAsync(u, c->w_u.p, bytes,
-                           device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                           c->stream));
+    c->from_operator(c->w_u.p, u, device_ptrs);
     VBH_HIP(hipStreamSynchronize(c->stream));
     if (iters) *iters = s.iter_done;
     if (rel_res) *rel_res = c->stats.rel_res;
@@ -637,15 +652,14 @@ int vbh_spmv(vbh_ctx* h, const double* x, double* y) {
     if (!c->have_matrix) vbh::throw_error(VBH_ERR_STATE, "vbh_spmv: no matrix set");
     c->select_stream_device();
     c->ensure_work(c->n_pad);
-    const size_t bytes = static_cast<size_t>(c->n) * sizeof(double);
-    VBH_HIP(hipMemcpyAsync(c->w_r64.p, x, bytes, hipMemcpyHostToDevice, c->stream));
+    c->to_operator(x, false, c->w_r64.p);
     if (c->n_pad > c->n)
       vbh::launch_fill_f64(c->n_pad - c->n, c->w_r64.p + c->n, 0.0, c->stream);
     if (c->format == 0)
       vbh::launch_spmv_csr<double>(c->csr, c->w_r64.p, c->w_Ap.p, c->stream);
     else
       vbh::launch_spmv_bsr<double>(c->bsrd, c->w_r64.p, c->w_Ap.p, c->stream);
-    VBH_HIP(hipMemcpyAsync(y, c->w_Ap.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    c->from_operator(c->w_Ap.p, y, false);
     VBH_HIP(hipStreamSynchronize(c->stream));
   });
 }

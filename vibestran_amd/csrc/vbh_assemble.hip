@@ -1084,15 +1084,16 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   mesh.solid_form = solid_form;
   mesh.shell_form = shell_form;
   asm_stats.mpc_ms = 0.0;
+  op_padded = false;
+  n_api = n_;
   if (mpc.active) {
-    // K_red = T^T K_full T: node blocks no longer line up with the equations
+    // K_red = T^T K_full T in the full numbering (unit diagonal on dependents)
     const auto t0 = std::chrono::steady_clock::now();
     mesh.mpc.upload(mpc, n_, s);
     apply_mpc_to_lower();
     asm_stats.mpc_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    asm_stats.nnz = l_nnz;
-    block_hint = 0;
+    asm_stats.nnz = l_nnz - mesh.mpc.n_dep;
   } else {
     mesh.mpc.upload(mpc, n_, s);
   }

@@ -1218,22 +1218,17 @@ static void block_solve(Ctx& c, int k, const double* F, double* U, bool device_p
   ensure_block_operator(c, b);
   ensure_block_work(c, b, k, precision);
   hipStream_t s = c.stream;
-  const size_t bytes = static_cast<size_t>(c.n) * k * sizeof(double);
+  // column-major API block (n_api rows) <-> operator rows (n, mapped under MPCs)
   DBuf<double> stage;
-  const double* dF = F;
-  if (!device_ptrs) {
-    stage.resize(static_cast<size_t>(c.n) * k);
-    VBH_HIP(hipMemcpyAsync(stage.p, F, bytes, hipMemcpyHostToDevice, s));
-    dF = stage.p;
-  }
+  stage.resize(static_cast<size_t>(c.n) * k);
+  c.to_operator(F, device_ptrs, stage.p, k);
   const long long m = static_cast<long long>(b.npad) * b.ldk;
   hipLaunchKernelGGL(k_to_rows, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, s, c.n, k, b.npad, b.ldk,
-                     dF, b.F.p);
+                     stage.p, b.F.p);
   run_block_solve(c, b, precision);
-  double* dU = device_ptrs ? U : stage.p;
   hipLaunchKernelGGL(k_to_cols, dim3(grid_for(static_cast<long long>(c.n) * k, kBT * 4)), dim3(kBT),
-                     0, s, c.n, k, b.ldk, b.u.p, dU);
-  if (!device_ptrs) VBH_HIP(hipMemcpyAsync(U, stage.p, bytes, hipMemcpyDeviceToHost, s));
+                     0, s, c.n, k, b.ldk, b.u.p, stage.p);
+  c.from_operator(stage.p, U, device_ptrs, k);
   read_cols(c, b);
   int max_it = 0, refr = 0;
   double max_rel = 0.0;

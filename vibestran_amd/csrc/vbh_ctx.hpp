@@ -24,6 +24,8 @@ struct MpcSystem {
   bool active = false;
   int n_full = 0, n_red = 0, n_elim = 0, n_skipped = 0;
   std::vector<int> index_map;                // full -> reduced equation, -1 dependent
+  std::vector<int> red_to_full;              // reduced -> full equation
+  std::vector<int> deps;                     // dependent full equations, ascending
   std::vector<int> t_ptr, t_col, tt_ptr, tt_row;
   std::vector<double> t_val, tt_val;
   void build(int nnodes, const int* eq, int n_full, int nmpc, const int* ptr, const int* node,
@@ -33,8 +35,9 @@ struct MpcSystem {
 // T of the mesh's elimination on the device (inactive: identity, n_red = n_full).
 struct MpcDev {
   bool active = false;
-  int n_full = 0, n_red = 0;
+  int n_full = 0, n_red = 0, n_dep = 0;
   DBuf<int> t_ptr, t_col, tt_ptr, tt_row;
+  DBuf<int> t_colF, deps;                    // T's columns as full equations; dependents
   DBuf<double> t_val, tt_val;
   void upload(const MpcSystem& h, int n_mesh, hipStream_t s);
   void expand(const double* u_red, double* u_full, hipStream_t s) const;   // T u
@@ -74,6 +77,15 @@ struct Ctx {
   // operator
   bool have_matrix = false;
   int n = 0, n_pad = 0, format = 0;
+  // Under an MPC elimination the operator keeps the full equation numbering
+  // (dependent rows and columns an identity block, so node blocks stay
+  // aligned) while the API's vectors and CSR are the reduced system: n_api
+  // equations, op_api_of[i] the API index of operator row i (-1: dependent),
+  // op_full_of[j] the operator row of API index j.
+  bool op_padded = false;
+  int n_api = 0;
+  DBuf<int> op_api_of, op_full_of;
+  std::vector<int> op_api_host;  // op_api_of on the host
   long long operator_version = 0;
   std::vector<int> lower_rp, lower_ci;    // reference CsrData (lower) on the host
   std::vector<double> lower_v;
@@ -142,6 +154,9 @@ struct Ctx {
                 const double* thick, int solid_form, int shell_form);
   void finish_device_operator();
   void apply_mpc_to_lower();
+  // API vector (n_api, host or device) <-> operator vector (n, device)
+  void to_operator(const double* x, bool dev, double* dst, int k = 1);
+  void from_operator(const double* src, double* x, bool dev, int k = 1);
   const double* mesh_solution(const double* u, bool dev, DBuf<double>& up, DBuf<double>& full);
   void element_sensitivity(int param, const double* u, const double* v, double* out, bool dev);
   // full: F in node-DOF layout (6 per node, every DOF) instead of equations

@@ -17,7 +17,8 @@
 //                    CPENTA (1473-1532), FORCE / MOMENT (1663-1689), GRAV
 //                    (1717-1730), PLOAD4 (1835-1935), SPC / SPC1 (1937-1995),
 //                    PARAM SHELLFORM (574-586, 1997-), CORD2R/C/S (2008-2027),
-//                    CORD1R/C (2029-2043), MPC (2047-2068), MPCADD (2070-2095);
+//                    CORD1R/C (2029-2043), MPC (2047-2068), MPCADD (2070-2095),
+//                    RBE2 / RBE3 (2100-2183) expanded into MPC equations;
 //                    case control MPC = (242-248)
 //   coordinates      Model::resolve_coordinates (src/core/model.cpp:175-250):
 //                    CORD1x from basic node positions, CORD2x through their
@@ -38,8 +39,7 @@
 // Not covered (error, not silently wrong): GRAV in a CID frame, CHEXA20 /
 // CTETRA10 / CPENTA15, PLOAD4 direction vectors, mixed PSOLID ISOP within one
 // deck, SPC enforced values, a GRID CD naming an undefined frame. Other cards
-// (RBE2 / RBE3 among them) are counted as unsupported and skipped, as the
-// reference logs them.
+// are counted as unsupported and skipped, as the reference logs them.
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
@@ -59,6 +59,7 @@
 
 #include <array>
 #include <memory>
+#include <tuple>
 
 #include "vbh_error.hpp"
 
@@ -175,6 +176,12 @@ struct NodalRec { int sid, node, off, cid; double v[3]; };  // off 0 FORCE, 3 MO
 struct GravRec { int sid; double a[3]; };
 struct PloadRec { int sid, eid, g1, g34; double p[4]; int line; };
 struct SpcRec { int sid, node, mask; };
+struct RbeGroup { double w; int comp; std::vector<int> nodes; };
+struct RbeRec {  // RBE2: gn, cm, gm; RBE3: ref node, refc, weight groups
+  int kind, eid, line, node, comp;
+  std::vector<int> gm;
+  std::vector<RbeGroup> groups;
+};
 
 struct Chunk {
   std::vector<GridRec> grids;
@@ -187,6 +194,7 @@ struct Chunk {
   std::vector<SpcRec> spc;
   std::vector<CoordRec> coords;
   std::vector<MpcRec> mpcs;
+  std::vector<RbeRec> rbes;
   std::vector<std::pair<std::string, std::string>> params;
   std::set<std::string> unsupported;
   std::string error;
@@ -473,6 +481,61 @@ void process(const std::vector<std::string_view>& f, int line, Chunk& ck) {
       m.terms.push_back({g, c, parse_double(f[i + 2], line)});
     }
     if (!m.terms.empty()) ck.mpcs.push_back(std::move(m));
+  } else if (kw == "RBE2") {
+    // a field with '.', 'E' or 'e' is ALPHA and ends the dependent grid list
+    RbeRec r{};
+    r.kind = 2;
+    r.line = line;
+    r.eid = parse_int(fld(f, 1), line);
+    r.node = parse_int(fld(f, 2), line);
+    r.comp = dof_mask(parse_int(fld(f, 3), line));
+    for (size_t i = 4; i < f.size(); ++i) {
+      if (f[i].empty()) continue;
+      if (f[i].find_first_of(".Ee") != std::string_view::npos) break;
+      r.gm.push_back(parse_int(f[i], line));
+    }
+    ck.rbes.push_back(std::move(r));
+  } else if (kw == "RBE3") {
+    // groups: a weight, a component set, then grids up to the next real field
+    RbeRec r{};
+    r.kind = 3;
+    r.line = line;
+    r.eid = parse_int(fld(f, 1), line);
+    r.node = parse_int(fld(f, 3), line);
+    r.comp = dof_mask(parse_int(fld(f, 4), line));
+    size_t i = 5;
+    while (i < f.size()) {
+      if (f[i].empty()) {
+        ++i;
+        continue;
+      }
+      double w = 0.0;
+      try {
+        w = parse_double(f[i], line);
+      } catch (const DeckError&) {
+        ++i;
+        continue;
+      }
+      ++i;
+      if (i >= f.size() || f[i].empty()) break;
+      RbeGroup g{w, dof_mask(parse_int(f[i], line)), {}};
+      ++i;
+      while (i < f.size()) {
+        if (f[i].empty()) {
+          ++i;
+          continue;
+        }
+        if (f[i].find_first_of(".Ee") != std::string_view::npos) break;
+        try {
+          g.nodes.push_back(parse_int(f[i], line));
+        } catch (const DeckError&) {
+          break;
+        }
+        ++i;
+      }
+      if (!g.nodes.empty()) r.groups.push_back(std::move(g));
+    }
+    ck.rbes.push_back(std::move(r));
   } else if (kw == "MPCADD") {
     MpcRec m{};
     m.add = true;
@@ -551,6 +614,9 @@ struct Deck {
   };
   std::vector<Sub> subs;
   std::vector<std::string> unsupported;
+  // RBE2 / RBE3 as MPC equations (every subcase), and the DOFs they activate
+  std::vector<int> rigid_ptr{0}, rigid_node, rigid_dof, rigid_active;
+  std::vector<double> rigid_coeff;
 };
 
 namespace {
@@ -992,6 +1058,86 @@ void parse_text(std::string_view text, Deck& d) {
   bool any_pload = false;
   for (const auto& ck : chunks) any_pload = any_pload || !ck.pload.empty();
   if (any_pload) elem_ix.build(d.elem_ids.size(), [&](size_t i) { return d.elem_ids[i]; });
+  // RBE2 then RBE3 in file order as MPC equations (expand_rbe2 / expand_rbe3,
+  // src/elements/rbe_constraints.cpp:22-181) and the DOFs they activate
+  // (mark_rigid_element_dofs, src/solver/analysis_support.cpp:70-95)
+  d.rigid_active.assign(static_cast<size_t>(nn), 0);
+  {
+    auto add = [&](std::initializer_list<std::tuple<int, int, double>> terms) {
+      for (const auto& [g, c, v] : terms) {
+        d.rigid_node.push_back(g);
+        d.rigid_dof.push_back(c);
+        d.rigid_coeff.push_back(v);
+      }
+      d.rigid_ptr.push_back(static_cast<int>(d.rigid_node.size()));
+    };
+    auto pos = [&](int i) { return &d.xyz[3 * static_cast<size_t>(i)]; };
+    for (int kind : {2, 3})
+      for (const auto& ck : chunks)
+        for (const auto& r : ck.rbes) {
+          if (r.kind != kind) continue;
+          const int n0 = node_index(r.node, r.line);
+          const double* p0 = pos(n0);
+          if (kind == 2) {
+            d.rigid_active[static_cast<size_t>(n0)] |= r.comp;
+            for (int gid : r.gm) {
+              const int m = node_index(gid, r.line);
+              d.rigid_active[static_cast<size_t>(m)] |= r.comp;
+              const double* pm = pos(m);
+              const double rx = pm[0] - p0[0], ry = pm[1] - p0[1], rz = pm[2] - p0[2];
+              if (r.comp & 1) add({{m, 1, 1.0}, {n0, 1, -1.0}, {n0, 5, -rz}, {n0, 6, ry}});
+              if (r.comp & 2) add({{m, 2, 1.0}, {n0, 2, -1.0}, {n0, 4, rz}, {n0, 6, -rx}});
+              if (r.comp & 4) add({{m, 3, 1.0}, {n0, 3, -1.0}, {n0, 4, -ry}, {n0, 5, rx}});
+              for (int c = 4; c <= 6; ++c)
+                if (r.comp & (1 << (c - 1))) add({{m, c, 1.0}, {n0, c, -1.0}});
+            }
+            continue;
+          }
+          d.rigid_active[static_cast<size_t>(n0)] |= r.comp;
+          double wt = 0.0, wr = 0.0;
+          for (const auto& g : r.groups) {
+            for (int gid : g.nodes) d.rigid_active[static_cast<size_t>(node_index(gid, r.line))] |= g.comp;
+            const double w = g.w * static_cast<double>(g.nodes.size());
+            wt += w;
+            if (g.comp & 0x38) wr += w;
+          }
+          if (wt < 1e-30) continue;
+          for (int c = 1; c <= 6; ++c) {
+            if (!(r.comp & (1 << (c - 1)))) continue;
+            if (c >= 4 && wr < 1e-30) continue;
+            const size_t t0 = d.rigid_node.size();
+            auto term = [&](int g, int cc, double v) {
+              d.rigid_node.push_back(g);
+              d.rigid_dof.push_back(cc);
+              d.rigid_coeff.push_back(v);
+            };
+            term(n0, c, c <= 3 ? wt : wr);
+            for (const auto& g : r.groups) {
+              if (c >= 4 && !(g.comp & (1 << (c - 1)))) continue;
+              const bool rot = (g.comp & 0x38) != 0;
+              for (int gid : g.nodes) {
+                const int j = node_index(gid, r.line);
+                term(j, c, -g.w);
+                if (c >= 4 || !rot) continue;
+                const double* pj = pos(j);
+                const double rx = pj[0] - p0[0], ry = pj[1] - p0[1], rz = pj[2] - p0[2];
+                // lever terms of (omega_j x r_j)[c - 1]
+                if (c == 1) {
+                  if (g.comp & 16) term(j, 5, -g.w * rz);
+                  if (g.comp & 32) term(j, 6, g.w * ry);
+                } else if (c == 2) {
+                  if (g.comp & 32) term(j, 6, -g.w * rx);
+                  if (g.comp & 8) term(j, 4, g.w * rz);
+                } else {
+                  if (g.comp & 8) term(j, 4, -g.w * ry);
+                  if (g.comp & 16) term(j, 5, g.w * rx);
+                }
+              }
+            }
+            if (d.rigid_node.size() > t0) d.rigid_ptr.push_back(static_cast<int>(d.rigid_node.size()));
+          }
+        }
+  }
   // model.mpcs: MPC cards in file order, each MPCADD appending relabelled
   // copies of the equations listed so far (process_mpcadd, 2070-2095)
   std::vector<const MpcRec*> mpcs;
@@ -1119,6 +1265,8 @@ int vbh_deck_info(const vbh_deck* d, vbh_deck_sizes* s) {
   s->solid_form = d->solid_form;
   s->shell_form = d->shell_form;
   s->nunsupported = static_cast<int>(d->unsupported.size());
+  s->nrigid_mpc = static_cast<int>(d->rigid_ptr.size()) - 1;
+  s->nrigid_terms = static_cast<int>(d->rigid_node.size());
   return VBH_OK;
 }
 
@@ -1198,6 +1346,23 @@ int vbh_deck_subcase_mpc(const vbh_deck* d, int k, int* mpc_ptr, int* term_node,
   cp(term_node, u.mpc_node);
   cp(term_dof, u.mpc_dof);
   cp(term_coeff, u.mpc_coeff);
+  return VBH_OK;
+}
+
+int vbh_deck_rigid(const vbh_deck* d, int* active, int* mpc_ptr, int* term_node, int* term_dof,
+                   double* term_coeff) {
+  if (!d) {
+    vbh::set_error("vbh_deck_rigid: null deck");
+    return VBH_ERR_ARG;
+  }
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+  };
+  cp(active, d->rigid_active);
+  cp(mpc_ptr, d->rigid_ptr);
+  cp(term_node, d->rigid_node);
+  cp(term_dof, d->rigid_dof);
+  cp(term_coeff, d->rigid_coeff);
   return VBH_OK;
 }
 

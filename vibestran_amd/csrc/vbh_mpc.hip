@@ -10,7 +10,11 @@
 // sparse map T (n_full x n_reduced): row i of a retained DOF is one 1.0 at its
 // reduced index, row i of a dependent DOF holds its elimination terms.
 //
-// Device: K_red = T^T K_full T on the assembled lower CSR. Each stored entry
+// Device: K_red = T^T K_full T on the assembled lower CSR, kept in the full
+// equation numbering with a unit diagonal on the dependent equations, so the
+// operator's node blocks stay aligned (BSR / sliced ELL, mixed precision);
+// the solver entry points map the reduced vectors in and out and vbh_get_csr
+// returns the reduced CSR of the reference. Each stored entry
 // (i, j, v) of K_full expands into the products of the T rows of i and j
 // (c_a c_b v at (max(r_a, r_b), min(r_a, r_b)), doubled where both land on the
 // same diagonal entry); the expanded list is sorted by (row, col) and every run
@@ -129,6 +133,11 @@ void MpcSystem::build(int nnodes, const int* eq, int n_full_, int nmpc, const in
     else index_map[static_cast<size_t>(i)] = i - removed;
   }
   n_red = n_full - removed;
+  red_to_full.reserve(static_cast<size_t>(n_red));
+  for (int i = 0; i < n_full; ++i) {
+    if (index_map[static_cast<size_t>(i)] >= 0) red_to_full.push_back(i);
+    else deps.push_back(i);
+  }
   // the last elimination of a dependent DOF defines its row
   std::vector<int> elim_of(static_cast<size_t>(n_full), -1);
   for (size_t k = 0; k < elims.size(); ++k) elim_of[static_cast<size_t>(elims[k].dep)] = static_cast<int>(k);
@@ -253,6 +262,29 @@ __global__ void k_mpc_gather(int n, const int* __restrict__ ptr, const int* __re
   y[r] = s;
 }
 
+// Dependent rows of the padded operator: a unit diagonal, appended at off.
+__global__ void k_mpc_dep_diag(int nd, const int* __restrict__ deps, int bits, long long off,
+                               unsigned long long* __restrict__ keys, double* __restrict__ vals) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nd) return;
+  const unsigned long long d = static_cast<unsigned long long>(deps[k]);
+  keys[off + k] = (d << bits) | d;
+  vals[off + k] = 1.0;
+}
+
+// dst[i] = map[i] >= 0 ? src[map[i]] : 0 for i < n, k columns of strides
+// (ld_dst, ld_src).
+__global__ void k_mpc_permute(int n, int k, const int* __restrict__ map, const double* __restrict__ src,
+                              long long ld_src, double* __restrict__ dst, long long ld_dst) {
+  const long long m = static_cast<long long>(n) * k;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int c = static_cast<int>(e / n), i = static_cast<int>(e % n);
+    const int j = map[i];
+    dst[c * ld_dst + i] = j >= 0 ? src[c * ld_src + j] : 0.0;
+  }
+}
+
 }  // namespace
 
 void MpcDev::upload(const MpcSystem& h, int n_mesh, hipStream_t s) {
@@ -260,8 +292,13 @@ void MpcDev::upload(const MpcSystem& h, int n_mesh, hipStream_t s) {
   n_full = n_mesh;
   n_red = h.active ? h.n_red : n_mesh;
   if (!h.active) return;
+  n_dep = static_cast<int>(h.deps.size());
+  std::vector<int> colF(h.t_col.size());
+  for (size_t k = 0; k < colF.size(); ++k) colF[k] = h.red_to_full[static_cast<size_t>(h.t_col[k])];
   t_ptr.upload(h.t_ptr.data(), h.t_ptr.size(), s);
   t_col.upload(h.t_col.data(), h.t_col.size(), s);
+  t_colF.upload(colF.data(), colF.size(), s);
+  deps.upload(h.deps.data(), h.deps.size(), s);
   t_val.upload(h.t_val.data(), h.t_val.size(), s);
   tt_ptr.upload(h.tt_ptr.data(), h.tt_ptr.size(), s);
   tt_row.upload(h.tt_row.data(), h.tt_row.size(), s);
@@ -282,7 +319,8 @@ void MpcDev::reduce(const double* f_full, double* f_red, hipStream_t s) const {
 }
 
 // K_red = T^T K_full T on the device lower CSR (l_rp/l_ci/l_v/l_row, n_full
-// rows), replacing it.
+// rows), replacing it. The result keeps the full numbering: retained
+// equations carry K_red, dependent ones a unit diagonal and nothing else.
 void Ctx::apply_mpc_to_lower() {
   const MpcDev& T = mesh.mpc;
   hipStream_t s = stream;
@@ -308,8 +346,10 @@ void Ctx::apply_mpc_to_lower() {
     VBH_HIP(hipStreamSynchronize(s));
     m = a + b;
   }
+  const long long m_prod = m;
+  m += T.n_dep;
   if (m >= (1ll << 31)) throw_error(VBH_ERR_SOLVER, "MPC transform: expanded operator too large");
-  const int n_red = T.n_red;
+  const int n_op = T.n_full;
   DBuf<unsigned long long> keys, keys_s;
   DBuf<double> vals, vals_s;
   const size_t mm = static_cast<size_t>(std::max<long long>(m, 1));
@@ -318,10 +358,13 @@ void Ctx::apply_mpc_to_lower() {
   vals.resize(mm);
   vals_s.resize(mm);
   int bits = 1;
-  while ((1ll << bits) < static_cast<long long>(n_red) + 1) ++bits;
+  while ((1ll << bits) < static_cast<long long>(n_op) + 1) ++bits;
   if (nnz > 0)
     hipLaunchKernelGGL(k_mpc_emit, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, l_row.p, l_ci.p,
-                       l_v.p, T.t_ptr.p, T.t_col.p, T.t_val.p, off.p, bits, keys.p, vals.p);
+                       l_v.p, T.t_ptr.p, T.t_colF.p, T.t_val.p, off.p, bits, keys.p, vals.p);
+  if (T.n_dep > 0)
+    hipLaunchKernelGGL(k_mpc_dep_diag, dim3((T.n_dep + 255) / 256), dim3(256), 0, s, T.n_dep,
+                       T.deps.p, bits, m_prod, keys.p, vals.p);
   if (m > 0) {
     // stable radix sort of (row << bits | col): equal keys keep their emission order
     size_t tmp = 0;
@@ -360,12 +403,56 @@ void Ctx::apply_mpc_to_lower() {
   if (m > 0)
     hipLaunchKernelGGL(k_mpc_runs, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0, s, m,
                        keys_s.p, vals_s.p, head.p, pos.p, bits, l_row.p, l_ci.p, l_v.p);
-  l_rp.resize(static_cast<size_t>(n_red) + 1);
+  l_rp.resize(static_cast<size_t>(n_op) + 1);
   hipLaunchKernelGGL(k_mpc_row_ptr, dim3((nnz_red + 1 + 255) / 256), dim3(256), 0, s, nnz_red,
-                     n_red, l_row.p, l_rp.p);
+                     n_op, l_row.p, l_rp.p);
+  op_api_host = mpc.index_map;
+  op_api_of.upload(op_api_host.data(), op_api_host.size(), s);
+  op_full_of.upload(mpc.red_to_full.data(), mpc.red_to_full.size(), s);
   VBH_HIP(hipStreamSynchronize(s));
-  n = n_red;
+  n = n_op;
   l_nnz = nnz_red;
+  op_padded = true;
+  n_api = T.n_red;
+}
+
+void Ctx::to_operator(const double* x, bool dev, double* dst, int k) {
+  if (!op_padded) {
+    VBH_HIP(hipMemcpyAsync(dst, x, sizeof(double) * n * k,
+                           dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+    return;
+  }
+  DBuf<double> up;
+  const double* src = x;
+  if (!dev) {
+    up.upload(x, static_cast<size_t>(n_api) * k, stream);
+    src = up.p;
+  }
+  hipLaunchKernelGGL(k_mpc_permute, dim3(grid_for(static_cast<long long>(n) * k, 1024)), dim3(256), 0,
+                     stream, n, k, op_api_of.p, src, static_cast<long long>(n_api), dst,
+                     static_cast<long long>(n));
+  if (!dev) VBH_HIP(hipStreamSynchronize(stream));
+}
+
+void Ctx::from_operator(const double* src, double* x, bool dev, int k) {
+  if (!op_padded) {
+    VBH_HIP(hipMemcpyAsync(x, src, sizeof(double) * n * k,
+                           dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream));
+    return;
+  }
+  DBuf<double> down;
+  double* dst = x;
+  if (!dev) {
+    down.resize(static_cast<size_t>(n_api) * k);
+    dst = down.p;
+  }
+  hipLaunchKernelGGL(k_mpc_permute, dim3(grid_for(static_cast<long long>(n_api) * k, 1024)),
+                     dim3(256), 0, stream, n_api, k, op_full_of.p, src, static_cast<long long>(n),
+                     dst, static_cast<long long>(n_api));
+  if (!dev) {
+    VBH_HIP(hipMemcpyAsync(x, down.p, sizeof(double) * n_api * k, hipMemcpyDeviceToHost, stream));
+    VBH_HIP(hipStreamSynchronize(stream));
+  }
 }
 
 // u in the operator's numbering (host or device) as a device vector in the

@@ -70,6 +70,13 @@ def _read(handle) -> Model:
         m.loads.append(LoadCase(nodal=nodal, pload4=pl, grav=grav))
         sub_ids.append(si.id)
     m.subcase_ids = sub_ids
+    if sz.nrigid_mpc:
+        act = np.empty(nn, np.int32)
+        rg = Mpc(ptr=np.empty(sz.nrigid_mpc + 1, np.int32), node=np.empty(sz.nrigid_terms, np.int32),
+                 dof=np.empty(sz.nrigid_terms, np.int32), coeff=np.empty(sz.nrigid_terms))
+        _check(lib.vbh_deck_rigid(handle, L.ip(act), L.ip(rg.ptr), L.ip(rg.node), L.ip(rg.dof),
+                                  L.dp(rg.coeff)))
+        m.rigid_mpc, m.rigid_active = rg, act
     cd = np.empty(nn, np.int32)
     rot = np.empty((nn, 3, 3))
     _check(lib.vbh_deck_frames(handle, L.ip(cd), L.dp(rot)))

@@ -9,7 +9,9 @@ Data layout (host side, handed to libvbh as flat arrays):
   E, nu, rho float64[ne]    per-element MAT1 data
   thickness float64[ne]     PSHELL T (shells only)
   elem_ids  int32[ne]       external element IDs (optional)
-  mpc       Mpc             MPC equations of the analysis (optional)
+  mpc       Mpc             MPC equations of the analysis's MPC set (optional)
+  rigid_mpc Mpc             RBE2 / RBE3 elements as MPC equations (optional)
+  rigid_active int32[nn]    DOF mask the rigid elements activate (optional)
   cd        int32[nn]       GRID CD frame per node, 0 = basic (optional)
   cd_rot    float64[nn,3,3] rotation of the node's CD frame at the node:
                             columns are the frame's unit vectors in basic
@@ -101,6 +103,8 @@ class Model:
     elem_ids: Optional[np.ndarray] = None  # external element IDs (default 1..ne)
     subcase_ids: Optional[List[int]] = None  # SUBCASE IDs of `loads` (default 1..)
     mpc: Optional[Mpc] = None
+    rigid_mpc: Optional[Mpc] = None
+    rigid_active: Optional[np.ndarray] = None
     cd: Optional[np.ndarray] = None
     cd_rot: Optional[np.ndarray] = None
 
@@ -137,13 +141,16 @@ def analysis_dof_map(model: Model):
     6 slots per node in sorted node-ID order; inactive slots (solid-only nodes
     lose R1-R3, mark_element_dofs 39-68) and SPC'd slots removed with the
     remaining indices shifted down. Equivalent closed form used here: eq is
-    the rank of each free (node ID, dof) pair in lexicographic order. MPC
-    terms activate their DOFs (mark_explicit_mpc_dofs, 97-105); SPCs on nodes
-    with a CD frame are left to analysis_mpc_system (206-207).
+    the rank of each free (node ID, dof) pair in lexicographic order. Rigid
+    elements activate their components (mark_rigid_element_dofs, 70-95), MPC
+    terms their DOFs (mark_explicit_mpc_dofs, 97-105); SPCs on nodes with a
+    CD frame are left to analysis_mpc_system (206-207).
 
     Returns (eq int32[nn,6] with -1 for constrained, n_free)."""
     nn = model.nn
     active = np.zeros((nn, 6), dtype=bool)
+    if model.rigid_active is not None:
+        active |= ((np.asarray(model.rigid_active)[:, None] >> np.arange(6)[None, :]) & 1).astype(bool)
     if model.mpc is not None and model.mpc.n:
         active[model.mpc.node, model.mpc.dof - 1] = True
     for t in np.unique(model.elem_type):
@@ -223,7 +230,8 @@ def analysis_mpc_system(model: Model, eq: np.ndarray, n: int):
     """build_analysis_mpc_system (reference src/solver/analysis_support.cpp:217-282)
     up to MpcHandler::build: SPCs on nodes with a CD frame turn into direct
     constraints of basic DOFs (removed from the numbering, 263) or MPC
-    equations, ahead of the model's explicit MPC equations (270-275).
+    equations, ahead of the rigid elements' equations (266-269) and the
+    model's explicit MPC equations (270-275).
 
     Returns (eq, n, Mpc or None): the numbering MpcHandler::build starts from
     and every equation it eliminates."""
@@ -245,6 +253,8 @@ def analysis_mpc_system(model: Model, eq: np.ndarray, n: int):
         for g, d in direct:
             free[g, d] = False
         eq, n = _number(model, free)
+    if model.rigid_mpc is not None:
+        eqs.extend(model.rigid_mpc.equations())
     if model.mpc is not None:
         eqs.extend(model.mpc.equations())
     return eq, n, (Mpc.from_equations(eqs) if eqs else None)
