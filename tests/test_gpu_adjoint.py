@@ -1,7 +1,10 @@
 """GPU adjoint sensitivities through the C ABI: the element contraction kernel
 against the FP64 CPU adjoint (oracle) on random vectors, and compliance /
 tip-displacement gradients through a torch.autograd.Function checked by
-central finite differences (h = 1e-6) on random elements."""
+central finite differences on random elements (h = 1e-4 E for the SIMP block:
+truncation O(h^2) ~ 1e-8, while the assembly's atomic summation order moves
+each re-assembled K in its last bits, noise that 1e-6 E amplified to the
+tolerance on this ill-conditioned block)."""
 import numpy as np
 import pytest
 
@@ -98,7 +101,7 @@ def test_simp_tip_displacement_gradient_fd(gpu_backend_factory):
     assert abs(C - (-1000.0) * ut) <= 1e-12 * abs(C)  # single point load: C = F_tip u_tip
     assert np.abs(dC - (-1000.0) * dut).max() <= 1e-8 * np.abs(dC).max()
     for e in np.random.default_rng(6).choice(m.ne, 8, replace=False):
-        h = 1e-6 * m.E[e]
+        h = 1e-4 * m.E[e]
         Ep, Em = m.E.copy(), m.E.copy()
         Ep[e] += h
         Em[e] -= h
