@@ -162,3 +162,36 @@ def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
     U2 = be2.solve_block(csr_of(oc2), G)
     for j in (0, 7, 31):
         assert O.true_relative_residual(oc2, U2[:, j], G[:, j]) < 1e-8
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_block_column_compaction(gpu_backend_factory, c1, monkeypatch, mixed):
+    """32 load cases of which 20 are zero (done at once) and the rest converge
+    at different iterations: the running columns move into 16-, then 8-wide
+    layouts. Each column must match its own single-case solve and the
+    full-width block run (VBH_BLOCK_COMPACT=0); zero columns stay zero."""
+    m, eq, n, oc, F = c1
+    rng = np.random.default_rng(5)
+    G = np.zeros((n, 32))
+    live = rng.choice(32, 12, replace=False)
+    for i, j in enumerate(live):
+        G[:, j] = F[:, i % 6] * (1.0 + i) if i < 6 else rng.standard_normal(n) * 10.0 ** (i - 8)
+    tol = 1e-8 if mixed else 1e-10
+    runs = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("VBH_BLOCK_COMPACT", flag)
+        be = gpu_backend_factory(mixed=mixed, tolerance=tol, max_iters=20000)
+        U = be.solve_block(csr_of(oc), G)
+        runs[flag] = (U, np.array(be.last_block_iterations), np.array(be.last_block_residuals))
+    (U, its, rels), (U0, its0, rels0) = runs["1"], runs["0"]
+    dead = np.setdiff1d(np.arange(32), live)
+    assert np.all(U[:, dead] == 0.0) and np.all(its[dead] == 0) and np.all(rels[dead] == 0.0)
+    for j in live:
+        rel = O.true_relative_residual(oc, U[:, j], G[:, j])
+        assert rel < tol * (1.0 if mixed else 10.0), (j, rel)
+        assert abs(int(its[j]) - int(its0[j])) <= max(3, 0.02 * its0[j]), (j, its[j], its0[j])
+        scale = np.abs(U0[:, j]).max()
+        assert np.abs(U[:, j] - U0[:, j]).max() <= (1e-6 if mixed else 1e-8) * scale, j
+        if not mixed:
+            uo, it_o, _ = O.pcg_jacobi(oc, G[:, j], tol=tol, max_iters=20000)
+            assert abs(int(its[j]) - it_o) <= 2, (j, its[j], it_o)

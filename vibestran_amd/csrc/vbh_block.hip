@@ -25,7 +25,13 @@
 //     scalar kernel (fixed order, deterministic), which also owns that
 //     column's state, so no kernel reads what another block of it writes;
 //   * iterations are replayed from a captured hipGraph; the host polls the
-//     column states between replays.
+//     column states between replays;
+//   * when at most half of the columns are still running, the running ones
+//     are moved into a layout half as wide (or narrower) and continue there:
+//     a frozen column no longer costs its share of the gathers, the vector
+//     streams and the stores of every later iteration. Each column's vectors
+//     and scalar state move unchanged, so its iterates are the same; the
+//     finished columns' solutions wait in the full-width buffers.
 // Mixed precision (FP32 r, p, Ap, Jacobi and SpMM, FP64 solution update)
 // pauses a column when its recursive residual fell by refresh_delta since its
 // last refresh (or below tol); the host then recomputes r64 = F - K u in FP64
@@ -95,6 +101,15 @@ struct BlockState {
   // autotune cache for the current operator: (ldk, precision) -> config
   std::map<std::pair<int, int>, BlkTune> tuned;
   BlkTune cur;
+  // full-width work space kept aside while compacted layouts run
+  struct Wide {
+    bool on = false;
+    int ldk = 0, lg2 = 0, k = 0, nvec = 1;
+    DBuf<double> F, u, r64, rd, pd, Apd, partA, partB, partR, coefA, coefB;
+    DBuf<float> rf, pf, Apf;
+    DBuf<int> actD, actU, reload, ones;
+    DBuf<BlkCol> cols;
+  } wide;
   // captured iteration graph
   hipGraphExec_t graph = nullptr;
   int g_prec = -1, g_ipg = 0, g_max_iters = 0, g_cw = 0, g_lg = 0, g_nrb = 0, g_xcd = 0, g_stage = 0;
@@ -668,6 +683,33 @@ __global__ void k_to_cols(int n, int k, int ldk, const double* __restrict__ b,
   }
 }
 
+// Column moves between row-major layouts (npad rows): gather dst[row][j] =
+// src[row][map[j]] (0 where map[j] < 0) into a narrower layout; scatter
+// dst[row][map[j]] = src[row][j] where map[j] >= 0 back into a wider one.
+template <typename T>
+__global__ void k_blk_gather(long long npad, int ldk_src, int lg2_dst, const int* __restrict__ map,
+                             const T* __restrict__ src, T* __restrict__ dst) {
+  const long long m = npad << lg2_dst;
+  const int mask = (1 << lg2_dst) - 1;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int j = map[static_cast<int>(e & mask)];
+    dst[e] = j >= 0 ? src[(e >> lg2_dst) * ldk_src + j] : T(0);
+  }
+}
+
+template <typename T>
+__global__ void k_blk_scatter(long long npad, int lg2_src, int ldk_dst, const int* __restrict__ map,
+                              const T* __restrict__ src, T* __restrict__ dst) {
+  const long long m = npad << lg2_src;
+  const int mask = (1 << lg2_src) - 1;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int j = map[static_cast<int>(e & mask)];
+    if (j >= 0) dst[(e >> lg2_src) * ldk_dst + j] = src[e];
+  }
+}
+
 // Entry-major BSR (vbh BsrDev) -> array-of-blocks, each block column-major
 // (entry (r, c) at c * B + r), so the SpMM reads one block column per row of X.
 template <int B>
@@ -1150,13 +1192,195 @@ static void read_cols(Ctx& c, BlockState& b) {
   VBH_HIP(hipStreamSynchronize(c.stream));
 }
 
+// ---- column compaction -------------------------------------------------------------------
+static void swap_work(BlockState& b, BlockState::Wide& w) {
+  b.F.swap(w.F);
+  b.u.swap(w.u);
+  b.r64.swap(w.r64);
+  b.rd.swap(w.rd);
+  b.pd.swap(w.pd);
+  b.Apd.swap(w.Apd);
+  b.partA.swap(w.partA);
+  b.partB.swap(w.partB);
+  b.partR.swap(w.partR);
+  b.coefA.swap(w.coefA);
+  b.coefB.swap(w.coefB);
+  b.rf.swap(w.rf);
+  b.pf.swap(w.pf);
+  b.Apf.swap(w.Apf);
+  b.actD.swap(w.actD);
+  b.actU.swap(w.actU);
+  b.reload.swap(w.reload);
+  b.ones.swap(w.ones);
+  b.cols.swap(w.cols);
+  std::swap(b.ldk, w.ldk);
+  std::swap(b.lg2, w.lg2);
+  std::swap(b.k, w.k);
+  std::swap(b.nvec, w.nvec);
+}
+
+// Finished slots of the current layout: final state into res, solution into
+// the full-width buffer (the layout in use after the first compaction).
+static void retire_columns(Ctx& c, BlockState& b, const std::vector<int>& orig,
+                           std::vector<BlkCol>& res, bool all) {
+  std::vector<int> ret(static_cast<size_t>(b.ldk), -1);
+  for (int j = 0; j < b.ldk; ++j)
+    if ((all || b.cols_host[j].state != kColActive) && orig[static_cast<size_t>(j)] >= 0) {
+      res[static_cast<size_t>(orig[static_cast<size_t>(j)])] = b.cols_host[j];
+      ret[static_cast<size_t>(j)] = orig[static_cast<size_t>(j)];
+    }
+  if (!b.wide.on) return;  // the full-width layout is the current one
+  DBuf<int> dret;
+  dret.upload(ret.data(), ret.size(), c.stream);
+  const long long m = static_cast<long long>(b.npad) * b.ldk;
+  hipLaunchKernelGGL(k_blk_scatter<double>, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, c.stream,
+                     static_cast<long long>(b.npad), b.lg2, b.wide.ldk, dret.p, b.u.p, b.wide.u.p);
+  VBH_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Moves the running columns (all active, none paused) into a layout of
+// pow2_at_least(count) columns. orig[j] is the caller's column of slot j.
+static void compact_columns(Ctx& c, BlockState& b, int precision, std::vector<int>& orig,
+                            std::vector<BlkCol>& res) {
+  hipStream_t s = c.stream;
+  std::vector<int> live;
+  for (int j = 0; j < b.ldk; ++j)
+    if (b.cols_host[j].state == kColActive) live.push_back(j);
+  const int k2 = static_cast<int>(live.size());
+  const int ldk2 = pow2_at_least(k2);
+  retire_columns(c, b, orig, res, false);
+  std::vector<double> cb(static_cast<size_t>(b.ldk));
+  VBH_HIP(hipMemcpyAsync(cb.data(), b.coefB.p, sizeof(double) * b.ldk, hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  // the source layout: the full-width buffers on the first compaction (kept
+  // aside for the results), otherwise the current narrow ones (freed after)
+  BlockState::Wide cur;
+  const bool first = !b.wide.on;
+  if (first) {
+    swap_work(b, b.wide);
+    b.wide.on = true;
+  } else {
+    swap_work(b, cur);
+  }
+  const BlockState::Wide& src = first ? b.wide : cur;
+  const size_t m2 = static_cast<size_t>(b.npad) * ldk2;
+  {
+    BlockState::Wide nb;
+    nb.ldk = ldk2;
+    nb.lg2 = log2_of(ldk2);
+    nb.k = k2;
+    nb.nvec = grid_for(static_cast<long long>(m2), kBT * 8);
+    nb.F.resize(m2);
+    nb.u.resize(m2);
+    nb.r64.resize(m2);
+    if (precision == VBH_PREC_MIXED) {
+      nb.rf.resize(m2);
+      nb.pf.resize(m2);
+      nb.Apf.resize(m2);
+    } else {
+      nb.rd.resize(m2);
+      nb.pd.resize(m2);
+      nb.Apd.resize(m2);
+    }
+    nb.partA.resize(static_cast<size_t>(ldk2) * kMaxGrid);
+    nb.partB.resize(static_cast<size_t>(3) * ldk2 * kMaxGrid);
+    nb.partR.resize(static_cast<size_t>(ldk2) * kMaxGrid);
+    swap_work(b, nb);  // b.* was emptied above
+  }
+  std::vector<int> g(static_cast<size_t>(ldk2), -1);
+  for (int j = 0; j < k2; ++j) g[static_cast<size_t>(j)] = live[static_cast<size_t>(j)];
+  DBuf<int> dg;
+  dg.upload(g.data(), g.size(), s);
+  const dim3 grid(grid_for(static_cast<long long>(m2), kBT * 4));
+  auto move64 = [&](DBuf<double>& dst, const DBuf<double>& from) {
+    hipLaunchKernelGGL(k_blk_gather<double>, grid, dim3(kBT), 0, s, static_cast<long long>(b.npad),
+                       src.ldk, b.lg2, dg.p, from.p, dst.p);
+  };
+  auto move32 = [&](DBuf<float>& dst, const DBuf<float>& from) {
+    hipLaunchKernelGGL(k_blk_gather<float>, grid, dim3(kBT), 0, s, static_cast<long long>(b.npad),
+                       src.ldk, b.lg2, dg.p, from.p, dst.p);
+  };
+  // per-column state: the running columns' as they are, padding columns done
+  std::vector<double> cb2(static_cast<size_t>(ldk2), 0.0), zero64(static_cast<size_t>(ldk2), 0.0);
+  std::vector<int> on(static_cast<size_t>(ldk2), 0), zero(static_cast<size_t>(ldk2), 0),
+      one(static_cast<size_t>(ldk2), 1);
+  std::vector<BlkCol> cols2(static_cast<size_t>(ldk2));
+  std::vector<int> orig2(static_cast<size_t>(ldk2), -1);
+  for (int j = 0; j < ldk2; ++j) {
+    BlkCol& cj = cols2[static_cast<size_t>(j)];
+    if (j < k2) {
+      const int q = live[static_cast<size_t>(j)];
+      cj = b.cols_host[q];
+      cb2[static_cast<size_t>(j)] = cb[static_cast<size_t>(q)];
+      on[static_cast<size_t>(j)] = 1;
+      orig2[static_cast<size_t>(j)] = orig[static_cast<size_t>(q)];
+    } else {
+      std::memset(&cj, 0, sizeof cj);
+      cj.state = kColDone;
+      cj.flags = kFlagZeroRhs;
+      cj.norm_b = 1.0;
+    }
+  }
+  b.ones.upload(one.data(), one.size(), s);
+  move64(b.F, src.F);
+  // the SpMM shape for this width: tuning (first time only) runs its
+  // candidates on these buffers and overwrites u, p, Ap and the partials, so
+  // the running columns' vectors are moved in after it
+  b.drop_graph();
+  b.cur = tune(c, b, precision);
+  move64(b.u, src.u);
+  if (precision == VBH_PREC_MIXED) {
+    move32(b.rf, src.rf);
+    move32(b.pf, src.pf);
+  } else {
+    move64(b.rd, src.rd);
+    move64(b.pd, src.pd);
+  }
+  b.coefA.upload(zero64.data(), zero64.size(), s);
+  b.coefB.upload(cb2.data(), cb2.size(), s);
+  b.actD.upload(on.data(), on.size(), s);
+  b.actU.upload(zero.data(), zero.size(), s);
+  b.reload.upload(zero.data(), zero.size(), s);
+  b.cols.upload(cols2.data(), cols2.size(), s);
+  VBH_HIP(hipStreamSynchronize(s));
+  orig = orig2;
+}
+
+// After the solve: every column's state and solution back in the full-width
+// layout the caller reads.
+static void finish_compaction(Ctx& c, BlockState& b, int precision, std::vector<int>& orig,
+                              std::vector<BlkCol>& res) {
+  if (!b.wide.on) return;
+  read_cols(c, b);
+  retire_columns(c, b, orig, res, true);
+  BlockState::Wide cur;
+  swap_work(b, cur);
+  swap_work(b, b.wide);
+  b.wide.on = false;
+  b.cols.upload(res.data(), res.size(), c.stream);
+  VBH_HIP(hipStreamSynchronize(c.stream));
+  b.drop_graph();
+  b.cur = tune(c, b, precision);  // the full-width shape (cached), as reported
+}
+
 // Runs the block solve on b.F (row-major) into b.u.
 static void run_block_solve(Ctx& c, BlockState& b, int precision) {
   hipStream_t s = c.stream;
-  const long long m = static_cast<long long>(b.npad) * b.ldk;
+  long long m = static_cast<long long>(b.npad) * b.ldk;
   const PcgParams prm = c.params();
   const int mixed = precision == VBH_PREC_MIXED;
   b.cur = tune(c, b, precision);
+  // VBH_BLOCK_COMPACT=0 keeps the full width; so does a forced SpMM shape
+  // (VBH_BLOCK_TUNE=cw,lg,...), which exists for this width only
+  const char* cp_env = std::getenv("VBH_BLOCK_COMPACT");
+  const char* tn_env = std::getenv("VBH_BLOCK_TUNE");
+  const bool compact = !(cp_env && cp_env[0] == '0') && !(tn_env && std::strchr(tn_env, ','));
+  std::vector<int> orig(static_cast<size_t>(b.ldk));
+  for (int j = 0; j < b.ldk; ++j) orig[static_cast<size_t>(j)] = j;
+  std::vector<BlkCol> res(static_cast<size_t>(b.ldk));
+  const char* lg_env = std::getenv("VBH_BLOCK_LOG");
+  const bool blog = lg_env && lg_env[0] == '1';
+  long long n_refresh_passes = 0, n_launches = 0;
   VBH_HIP(hipEventRecord(c.ev_start, s));
   if (!mixed)
     hipLaunchKernelGGL(k_blk_init<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
@@ -1181,7 +1405,16 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
     }
     if (!active && !paused) break;
     if (++passes > cap) throw_error(VBH_ERR_HIP, "HIP block PCG: device state stopped advancing");
+    m = static_cast<long long>(b.npad) * b.ldk;
+    if (!paused && compact && b.ldk >= 16 && active <= b.ldk / 2) {
+      if (blog) std::fprintf(stderr, "vbh block: launch %lld: %d running columns -> width %d\n",
+                             n_launches, active, pow2_at_least(active));
+      compact_columns(c, b, precision, orig, res);
+      ge = use_graph ? block_graph(c, b, precision) : nullptr;
+      continue;
+    }
     if (paused) {
+      ++n_refresh_passes;
       spmm<double, 1>(b.cur, b.op, b.ldk, b.u.p, b.r64.p, b.F.p, b.ones.p, b.partR.p, s);
       hipLaunchKernelGGL(k_blk_scal_refresh, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partR.p,
                          b.cur.nrb, b.cols.p, b.reload.p, prm);
@@ -1191,9 +1424,14 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
                          b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
       continue;
     }
+    ++n_launches;
     if (use_graph) VBH_HIP(hipGraphLaunch(ge, s));
     else enqueue_block_iterations(c, b, precision, c.iters_per_graph);
   }
+  finish_compaction(c, b, precision, orig, res);
+  if (blog)
+    std::fprintf(stderr, "vbh block: %lld iteration launches, %lld refresh passes\n", n_launches,
+                 n_refresh_passes);
   VBH_HIP(hipEventRecord(c.ev_stop, s));
   VBH_HIP(hipEventSynchronize(c.ev_stop));
   float ms = 0.0f;
