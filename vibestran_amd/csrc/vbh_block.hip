@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm(BlkOp A, int ldk, int xcd,
   const int lane = threadIdx.x & 63;
   const int lg = lane % LG, g = lane / LG;
   const int c0 = blockIdx.y * KT + lg * CW;
-  if (MODE == 0) {
+  {  // a column tile with no column to compute leaves at once
     int on = 0;
 #pragma unroll
     for (int w = 0; w < CW; ++w) on |= act[c0 + w];
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
   const int lane = threadIdx.x & 63;
   const int lg = lane % LG, g = lane / LG;
   const int c0 = blockIdx.y * KT + lg * CW;
-  if (MODE == 0) {
+  {  // a column tile with no column to compute leaves at once
     int on = 0;
 #pragma unroll
     for (int w = 0; w < CW; ++w) on |= act[c0 + w];
@@ -1381,6 +1381,7 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
   const char* lg_env = std::getenv("VBH_BLOCK_LOG");
   const bool blog = lg_env && lg_env[0] == '1';
   long long n_refresh_passes = 0, n_launches = 0;
+  DBuf<int> pmask;
   VBH_HIP(hipEventRecord(c.ev_start, s));
   if (!mixed)
     hipLaunchKernelGGL(k_blk_init<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
@@ -1415,9 +1416,27 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
     }
     if (paused) {
       ++n_refresh_passes;
-      spmm<double, 1>(b.cur, b.op, b.ldk, b.u.p, b.r64.p, b.F.p, b.ones.p, b.partR.p, s);
+      // r = F - K u for the paused columns: tiles of 8 columns, tiles without
+      // a paused column skipped, when that streams the matrix fewer times
+      // than the iteration's own tile width would
+      std::vector<int> pm(static_cast<size_t>(b.ldk), 0);
+      int tiles = 0;
+      for (int j = 0; j < b.ldk; ++j) pm[static_cast<size_t>(j)] = b.cols_host[j].state == kColPaused;
+      for (int t0 = 0; t0 < b.ldk; t0 += 8) {
+        int any = 0;
+        for (int j = t0; j < t0 + 8; ++j) any |= pm[static_cast<size_t>(j)];
+        tiles += any;
+      }
+      BlkTune rt = b.cur;
+      if (b.op.B > 1 && tiles * 8 < b.cur.cw * b.cur.lg) {
+        rt.cw = 1;
+        rt.lg = 8;
+        rt.stage = 0;
+      }
+      pmask.upload(pm.data(), pm.size(), s);
+      spmm<double, 1>(rt, b.op, b.ldk, b.u.p, b.r64.p, b.F.p, pmask.p, b.partR.p, s);
       hipLaunchKernelGGL(k_blk_scal_refresh, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partR.p,
-                         b.cur.nrb, b.cols.p, b.reload.p, prm);
+                         rt.nrb, b.cols.p, b.reload.p, prm);
       hipLaunchKernelGGL(k_blk_reload, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, c.diag32.p,
                          b.r64.p, b.rf.p, b.reload.p, b.partB.p);
       hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 2, b.ldk, b.k, mixed,
