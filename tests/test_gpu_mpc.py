@@ -81,6 +81,9 @@ MODELS = {
     "tie_slide_block": lambda: decks.hex_block_mpc(5, 3, 4),
     "ring_full": ring_full,
     "rbe_plate": rbe_plate,
+    # the tied end face makes the centre grid's block row ~240 blocks long:
+    # past the sliced-ELL cap, so the overflow pass carries the rest
+    "long_row_block": lambda: decks.hex_block_mpc(8, 10, 10),
 }
 
 
@@ -199,3 +202,23 @@ def test_full_size_tie_and_slide(gpu_backend_factory):
     T = sp.csr_matrix((tv, tc, tp), shape=(n, n_red))
     r = T.T @ (Ku - Ffull)
     assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(F), np.linalg.norm(r) / np.linalg.norm(F)
+
+
+def test_long_rows_mixed_precision(gpu_backend_factory):
+    """A block row longer than the sliced-ELL cap through the mixed solver
+    (FP32 SpMV overflow pass and FP64 refresh overflow pass): the true FP64
+    residual of K_red meets the tolerance and the solution matches FP64's."""
+    m = decks.hex_block_mpc(8, 10, 10)
+    sols = {}
+    for mixed in (False, True):
+        be = gpu_backend_factory(mixed=mixed, tolerance=1e-10, max_iters=100000)
+        eq, n, n_red = analysis_numbering(be, m)
+        K = assemble_model(be, m, eq, n)
+        F = assemble_rhs_device(be, m, eq, n_red, m.loads[0])
+        u = be.solve(K, F)
+        K.fetch()
+        Kd = dense_lower_sym(K.row_ptr, K.col_ind, K.values, n_red)
+        assert np.linalg.norm(F - Kd @ u) <= 1e-9 * np.linalg.norm(F)
+        assert be.stats().format == 3
+        sols[mixed] = u
+    assert np.abs(sols[True] - sols[False]).max() <= 1e-7 * np.abs(sols[False]).max()

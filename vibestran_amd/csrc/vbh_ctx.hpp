@@ -106,6 +106,7 @@ struct Ctx {
   DBuf<float> sell_v32;
   DBuf<double> sell_dblk;   // FP64 corrected diagonal node blocks (row-sum preserving FP32)
   DBuf<int> sell_kdiag;     // slot of the diagonal block in each block row
+  DBuf<int> sell_lrows;     // block rows longer than kSellCap blocks
   int rowsum_fp32 = 1;      // split the FP32 stream as above (VBH_ROWSUM=0 disables)
   long long sell_padded_blocks = 0;
   DBuf<double> diag64;

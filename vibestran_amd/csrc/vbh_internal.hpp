@@ -148,9 +148,16 @@ struct BsrDev {
   // the FP64 operator's; FP32 SpMVs sum the off-diagonal blocks in FP32 and
   // add the diagonal block's product in FP64.
   const double* dblk = nullptr;
+  // Long block rows (more than kSellCap blocks, e.g. an MPC master coupled to
+  // a whole face): the sliced ELL holds their first kSellCap blocks only, so
+  // one row does not widen its slice, and a separate pass adds the rest
+  // (blocks kSellCap.. of the row in the BSR arrays above) after the SpMV.
+  int nlong = 0;
+  const int* lrows = nullptr;  // block rows with more than kSellCap blocks
 };
 
 constexpr int kSliceRows = 64;
+constexpr int kSellCap = 128;  // block slots per row kept in the sliced ELL
 __host__ __device__ constexpr int sell_nps(int B) { return kSliceRows / B + 2; }
 
 // ---- kernel launchers (vbh_kernels.hip) ------------------------------------------

@@ -19,6 +19,7 @@
 
 #include "vbh_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace vbh {
@@ -690,6 +691,70 @@ __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* 
   }
 }
 
+// ---- long rows: the blocks past kSellCap -------------------------------------------------
+// One workgroup per long block row (grid-stride over the rows): the row's
+// remaining blocks (read from the BSR arrays, FP64 sums) are reduced over the
+// workgroup and added to what the sliced-ELL pass wrote for the row's B
+// scalar rows. MODE 0: y += rest. MODE 1 (PCG SpMV): y += rest, and the
+// change of p.Ap, p_g (y_new - y_old), is added to partial slot blockIdx.x,
+// which the SpMV pass wrote (grid <= its grid; one writer per slot, so the
+// sums stay deterministic). MODE 2 (refresh, FP64): r -= rest and the change
+// of r.r into the refresh's slot. With the split FP32 stream the diagonal
+// block is skipped (the SpMV added its FP64 form).
+template <typename T, int B, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __restrict__ x,
+                                                          T* __restrict__ y,
+                                                          double* __restrict__ part,
+                                                          const PcgScalars* st) {
+  __shared__ double lds[4 * 8];
+  if (MODE == 1) {
+    __shared__ int dn;
+    if (threadIdx.x == 0) dn = st->done;
+    __syncthreads();
+    if (dn) return;
+  }
+  const bool split = sizeof(T) == 4 && A.dblk != nullptr;
+  const T* __restrict__ val = sizeof(T) == 4 ? reinterpret_cast<const T*>(A.v32)
+                                             : reinterpret_cast<const T*>(A.v64);
+  double corr = 0.0;
+  for (int li = blockIdx.x; li < A.nlong; li += gridDim.x) {
+    const int I = A.lrows[li];
+    const int o = A.brp[I], nb = A.brp[I + 1] - o;
+    double acc[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) acc[r] = 0.0;
+    for (int k = kSellCap + threadIdx.x; k < nb; k += blockDim.x) {
+      const int J = A.bci[o + k];
+      if (split && J == I) continue;
+      const T* vb = val + static_cast<size_t>(B * B) * o + k;
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        const double xc = static_cast<double>(x[B * J + c]);
+#pragma unroll
+        for (int r = 0; r < B; ++r) acc[r] += static_cast<double>(vb[static_cast<size_t>(r * B + c) * nb]) * xc;
+      }
+    }
+    block_sum<B>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        const int g = B * I + r;
+        const double y0 = static_cast<double>(y[g]);
+        if (MODE == 2) {
+          const double rn = y0 - acc[r];
+          y[g] = static_cast<T>(rn);
+          corr += rn * rn - y0 * y0;
+        } else {
+          const T yn = static_cast<T>(y0 + acc[r]);
+          y[g] = yn;
+          if (MODE == 1) corr += static_cast<double>(x[g]) * (static_cast<double>(yn) - y0);
+        }
+      }
+    }
+  }
+  if (MODE != 0 && threadIdx.x == 0) part[blockIdx.x] += corr;
+}
+
 __global__ void k_cast_f64_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     b[i] = static_cast<float>(a[i]);
@@ -714,6 +779,18 @@ constexpr int kCsrG = 8;  // lanes per scalar CSR row
 }  // namespace
 
 // ---- launchers ------------------------------------------------------------------------
+// The overflow pass of long rows after a sliced-ELL pass (see k_sell_overflow).
+template <typename T, int MODE>
+static void launch_overflow(const BsrDev& A, const T* x, T* y, double* part, int nprod,
+                            const PcgScalars* st, hipStream_t s) {
+  if (A.nlong == 0) return;
+  const int g = MODE == 0 ? std::min(A.nlong, kMaxGrid) : std::min(A.nlong, nprod);
+  if (A.B == 3)
+    hipLaunchKernelGGL((k_sell_overflow<T, 3, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, part, st);
+  else
+    hipLaunchKernelGGL((k_sell_overflow<T, 6, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, part, st);
+}
+
 template <typename T>
 void launch_spmv_csr(const CsrDev& A, const T* x, T* y, hipStream_t s) {
   hipLaunchKernelGGL((k_spmv_csr<T, kCsrG>), dim3(grid_for(A.n, kBlock / kCsrG)), dim3(kBlock), 0,
@@ -734,6 +811,7 @@ void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s) {
     hipLaunchKernelGGL((k_spmv_bsr<T, 6, double>), g, dim3(kBlock), 0, s, A, x, y);
   else
     hipLaunchKernelGGL((k_spmv_bsr<T, 6>), g, dim3(kBlock), 0, s, A, x, y);
+  launch_overflow<T, 0>(A, x, y, nullptr, 0, nullptr, s);
 }
 
 template <typename T>
@@ -766,6 +844,7 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red);
   else
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red);
+  if (format != 0) launch_overflow<T, 1>(Ab, b.p, b.Ap, red.partA, red.nA, st, s);
 }
 
 // partial sums per thread for a producer grid of nprod blocks
@@ -813,6 +892,7 @@ void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, cons
   else
     hipLaunchKernelGGL((k_refresh_residual_bsr<6>), dim3(red.nR), dim3(kBlock), 0, s, Ab, F, u64,
                        r64, st, red);
+  if (format != 0) launch_overflow<double, 2>(Ab, u64, r64, red.partR, red.nR, st, s);
 }
 
 void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,

@@ -5,6 +5,7 @@
 // contiguous and the lanes of a wave read one contiguous span per block slot,
 // with no cross-lane reduction. Built on the device from the BSR operator.
 #include <cstring>
+#include <vector>
 
 #include <rocprim/rocprim.hpp>
 
@@ -22,7 +23,7 @@ __global__ void k_sell_kmax(int nslices, int nbr, int B, const int* __restrict__
   if (n1 > nbr - 1) n1 = nbr - 1;
   int km = 0;
   for (int I = n0; I <= n1; ++I) {
-    const int nb = brp[I + 1] - brp[I];
+    const int nb = min(brp[I + 1] - brp[I], kSellCap);  // longer rows: the rest overflows
     km = nb > km ? nb : km;
   }
   kcnt[s] = km;
@@ -103,7 +104,7 @@ __global__ void k_sell_zero_diag(int nslices, int nbr, const int* __restrict__ k
   const int g = kSliceRows * s + l;
   if (g >= nbr * B) return;
   const int kd = kdiag[g / B];
-  if (kd < 0) return;
+  if (kd < 0 || kd >= koff[s + 1] - koff[s]) return;  // none, or in a long row's overflow
   for (int c = 0; c < B; ++c) s32[(static_cast<size_t>(koff[s] + kd) * kSliceRows + l) * B + c] = 0.0f;
 }
 
@@ -171,6 +172,17 @@ void build_sell(Ctx& c) {
   } else {
     c.sell_dblk.release();
     c.sell_kdiag.release();
+  }
+  // long rows: their blocks past kSellCap are added by the overflow pass
+  {
+    std::vector<int> brp(static_cast<size_t>(A.nbr) + 1), lr;
+    VBH_HIP(hipMemcpyAsync(brp.data(), A.brp, sizeof(int) * brp.size(), hipMemcpyDeviceToHost, s));
+    VBH_HIP(hipStreamSynchronize(s));
+    for (int I = 0; I < A.nbr; ++I)
+      if (brp[static_cast<size_t>(I) + 1] - brp[static_cast<size_t>(I)] > kSellCap) lr.push_back(I);
+    c.sell_lrows.upload(lr.data(), lr.size(), s);
+    A.nlong = static_cast<int>(lr.size());
+    A.lrows = lr.empty() ? nullptr : c.sell_lrows.p;
   }
   VBH_HIP(hipStreamSynchronize(s));
   A.nslices = nslices;
