@@ -109,6 +109,14 @@ struct BlockState {
     DBuf<float> rf, pf, Apf;
     DBuf<int> actD, actU, reload, ones;
     DBuf<BlkCol> cols;
+    void reset() {  // after a solve that did not finish its compaction
+      for (DBuf<double>* d : {&F, &u, &r64, &rd, &pd, &Apd, &partA, &partB, &partR, &coefA, &coefB})
+        d->release();
+      for (DBuf<float>* f : {&rf, &pf, &Apf}) f->release();
+      for (DBuf<int>* i : {&actD, &actU, &reload, &ones}) i->release();
+      cols.release();
+      on = false;
+    }
   } wide;
   // captured iteration graph
   hipGraphExec_t graph = nullptr;
@@ -1024,6 +1032,10 @@ static void ensure_block_operator(Ctx& c, BlockState& b) {
 }
 
 static void ensure_block_work(Ctx& c, BlockState& b, int k, int precision) {
+  if (b.wide.on) {  // an earlier solve stopped inside a compacted layout
+    b.wide.reset();
+    b.ldk = -1;
+  }
   const int ldk = pow2_at_least(k);
   const int npad = c.n_pad;
   if (!(b.ldk == ldk && b.npad == npad && b.work_version == c.operator_version)) {
