@@ -91,7 +91,9 @@ struct BlockState {
   int ldk = 0, lg2 = 0, npad = 0, k = 0;
   long long work_version = -1;
   DBuf<double> F, u, r64, rd, pd, Apd;
-  DBuf<float> rf, pf, Apf;
+  // mixed precision: r, p, Ap in FP32, and the solution's increments since the
+  // last fold into u (FP32, folded once per batch of iterations)
+  DBuf<float> rf, pf, Apf, duf;
   DBuf<double> partA, partB, partR;
   DBuf<double> coefA, coefB;
   DBuf<int> actD, actU, reload, ones;
@@ -106,13 +108,13 @@ struct BlockState {
     bool on = false;
     int ldk = 0, lg2 = 0, k = 0, nvec = 1;
     DBuf<double> F, u, r64, rd, pd, Apd, partA, partB, partR, coefA, coefB;
-    DBuf<float> rf, pf, Apf;
+    DBuf<float> rf, pf, Apf, duf;
     DBuf<int> actD, actU, reload, ones;
     DBuf<BlkCol> cols;
     void reset() {  // after a solve that did not finish its compaction
       for (DBuf<double>* d : {&F, &u, &r64, &rd, &pd, &Apd, &partA, &partB, &partR, &coefA, &coefB})
         d->release();
-      for (DBuf<float>* f : {&rf, &pf, &Apf}) f->release();
+      for (DBuf<float>* f : {&rf, &pf, &Apf, &duf}) f->release();
       for (DBuf<int>* i : {&actD, &actU, &reload, &ones}) i->release();
       cols.release();
       on = false;
@@ -274,11 +276,13 @@ __global__ __launch_bounds__(kBT) void k_blk_direction(long long m, int ldk, int
   }
 }
 
-// u += alpha p (FP64); r -= alpha Ap; partials r.z, r.r per column.
-template <typename T>
+// u += alpha p; r -= alpha Ap; partials r.z, r.r per column. U = double:
+// the FP64 solution itself; U = float (mixed precision): the FP32 increments
+// since the last fold, each step computed in FP64 and rounded once.
+template <typename T, typename U>
 __global__ __launch_bounds__(kBT) void k_blk_update(long long m, int ldk, int lg2,
                                                     const T* __restrict__ diag,
-                                                    double* __restrict__ u, T* __restrict__ r,
+                                                    U* __restrict__ u, T* __restrict__ r,
                                                     const T* __restrict__ p,
                                                     const T* __restrict__ Ap,
                                                     const double* __restrict__ coefA,
@@ -300,26 +304,51 @@ __global__ __launch_bounds__(kBT) void k_blk_update(long long m, int ldk, int lg
   double acc[2][V] = {};
   VBH_BLK_LOOP(e) {
     const T d = diag[e >> lg2];
-    double uv[V];
+    U uv[V];
     T rv[V], pv[V], qv[V];
-    vload<double, V>(u + e, uv);
+    vload<U, V>(u + e, uv);
     vload<T, V>(r + e, rv);
     vload<T, V>(p + e, pv);
     vload<T, V>(Ap + e, qv);
 #pragma unroll
     for (int w = 0; w < V; ++w) {
       if (on[w]) {
-        uv[w] += al[w] * static_cast<double>(pv[w]);
+        uv[w] = static_cast<U>(static_cast<double>(uv[w]) + al[w] * static_cast<double>(pv[w]));
         rv[w] = rv[w] - a[w] * qv[w];
       }
       const T zi = rv[w] / d;
       acc[0][w] += static_cast<double>(rv[w]) * static_cast<double>(zi);
       acc[1][w] += static_cast<double>(rv[w]) * static_cast<double>(rv[w]);
     }
-    vstore<double, V>(u + e, uv);
+    vstore<U, V>(u + e, uv);
     vstore<T, V>(r + e, rv);
   }
   column_partials<V, 2>(acc, ldk, lds, part);
+}
+
+// u += du (FP64), du = 0: the FP32 increments of a batch of mixed-precision
+// iterations folded into the solution; every column, so that u is current
+// wherever the host looks (refresh, compaction, the end of the solve).
+// add = 0 only clears du.
+__global__ __launch_bounds__(kBT) void k_blk_fold(long long m, int add, double* __restrict__ u,
+                                                  float* __restrict__ du) {
+  constexpr int V = 4;
+  for (long long e = (blockIdx.x * static_cast<long long>(kBT) + threadIdx.x) * V; e < m;
+       e += static_cast<long long>(gridDim.x) * kBT * V) {
+    float zero[V];
+#pragma unroll
+    for (int w = 0; w < V; ++w) zero[w] = 0.0f;
+    if (add) {
+      float dv[V];
+      double uv[V];
+      vload<float, V>(du + e, dv);
+      vload<double, V>(u + e, uv);
+#pragma unroll
+      for (int w = 0; w < V; ++w) uv[w] += static_cast<double>(dv[w]);
+      vstore<double, V>(u + e, uv);
+    }
+    vstore<float, V>(du + e, zero);
+  }
 }
 
 // Mixed refresh: reload r = (float) r64 in the columns the scalar kernel
@@ -1049,6 +1078,7 @@ static void ensure_block_work(Ctx& c, BlockState& b, int k, int precision) {
     b.rf.release();
     b.pf.release();
     b.Apf.release();
+    b.duf.release();
     b.partA.resize(static_cast<size_t>(ldk) * kMaxGrid);
     b.partB.resize(static_cast<size_t>(3) * ldk * kMaxGrid);
     b.partR.resize(static_cast<size_t>(ldk) * kMaxGrid);
@@ -1081,6 +1111,7 @@ static void ensure_block_work(Ctx& c, BlockState& b, int k, int precision) {
     b.rf.resize(m);
     b.pf.resize(m);
     b.Apf.resize(m);
+    b.duf.resize(m);
   }
   b.k = k;
 }
@@ -1160,16 +1191,19 @@ static void enqueue_block_iterations(Ctx& c, BlockState& b, int precision, int c
     hipLaunchKernelGGL(k_blk_scal_alpha, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partA.p,
                        b.cur.nrb, b.cols.p, b.coefA.p, b.actU.p, b.actD.p);
     if (!mixed)
-      hipLaunchKernelGGL(k_blk_update<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
-                         c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.Apd.p, b.coefA.p, b.actU.p,
+      hipLaunchKernelGGL((k_blk_update<double, double>), dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk,
+                         b.lg2, c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.Apd.p, b.coefA.p, b.actU.p,
                          b.partB.p);
     else
-      hipLaunchKernelGGL(k_blk_update<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
-                         c.diag32.p, b.u.p, b.rf.p, b.pf.p, b.Apf.p, b.coefA.p, b.actU.p,
+      hipLaunchKernelGGL((k_blk_update<float, float>), dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk,
+                         b.lg2, c.diag32.p, b.duf.p, b.rf.p, b.pf.p, b.Apf.p, b.coefA.p, b.actU.p,
                          b.partB.p);
     hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 1, b.ldk, b.k, mixed,
                        b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
   }
+  if (mixed)  // the batch's increments into u (the iterations above only touched du)
+    hipLaunchKernelGGL(k_blk_fold, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, s, m, 1, b.u.p,
+                       b.duf.p);
 }
 
 static hipGraphExec_t block_graph(Ctx& c, BlockState& b, int precision) {
@@ -1220,6 +1254,7 @@ static void swap_work(BlockState& b, BlockState::Wide& w) {
   b.rf.swap(w.rf);
   b.pf.swap(w.pf);
   b.Apf.swap(w.Apf);
+  b.duf.swap(w.duf);
   b.actD.swap(w.actD);
   b.actU.swap(w.actU);
   b.reload.swap(w.reload);
@@ -1289,6 +1324,7 @@ static void compact_columns(Ctx& c, BlockState& b, int precision, std::vector<in
       nb.rf.resize(m2);
       nb.pf.resize(m2);
       nb.Apf.resize(m2);
+      nb.duf.resize(m2);
     } else {
       nb.rd.resize(m2);
       nb.pd.resize(m2);
@@ -1344,6 +1380,9 @@ static void compact_columns(Ctx& c, BlockState& b, int precision, std::vector<in
   if (precision == VBH_PREC_MIXED) {
     move32(b.rf, src.rf);
     move32(b.pf, src.pf);
+    // the increments were folded into u at the end of the last batch
+    hipLaunchKernelGGL(k_blk_fold, dim3(grid_for(static_cast<long long>(m2), kBT * 4)), dim3(kBT),
+                       0, s, static_cast<long long>(m2), 0, b.u.p, b.duf.p);
   } else {
     move64(b.rd, src.rd);
     move64(b.pd, src.pd);
@@ -1398,9 +1437,12 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
   if (!mixed)
     hipLaunchKernelGGL(k_blk_init<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
                        c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.partB.p);
-  else
+  else {
     hipLaunchKernelGGL(k_blk_init<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
                        c.diag32.p, b.u.p, b.rf.p, b.pf.p, b.partB.p);
+    hipLaunchKernelGGL(k_blk_fold, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, s, m, 0, b.u.p,
+                       b.duf.p);
+  }
   hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 0, b.ldk, b.k, mixed,
                      b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
   const char* ge_env = std::getenv("VBH_USE_GRAPH");
