@@ -382,7 +382,7 @@ def bench_tube(args):
                       "spmm": {"cols_per_lane": cfg.cols_per_lane,
                                "lanes_per_group": cfg.lanes_per_group,
                                "row_blocks": cfg.row_blocks, "xcd_rows": cfg.xcd_rows,
-                               "lds_staged": cfg.lds_staged, "ms": round(cfg.spmm_ms, 4),
+                               "value_path": cfg.value_path, "ms": round(cfg.spmm_ms, 4),
                                "algorithmic_GBs": round(spmm_bytes / (cfg.spmm_ms * 1e-3) / 1e9, 1)},
                       "ms_per_iteration": round(t_block / int(its.max()) * 1e3, 4)}
         log(f"[bench] tube {label}: block {t_block:.2f} s, 32 sequential ~{t_seq32:.2f} s "
@@ -423,7 +423,7 @@ def profile_tube(args, m):
             pass  # stopped at the 200-iteration limit, as intended
         cfg = be.block_config()
         log(f"[bench] tube profile prec={prec}: {be.last_block_iterations.max()} iterations, "
-            f"SpMM cw={cfg.cols_per_lane} lg={cfg.lanes_per_group} stage={cfg.lds_staged} "
+            f"SpMM cw={cfg.cols_per_lane} lg={cfg.lanes_per_group} values={cfg.value_path} "
             f"{cfg.spmm_ms*1e3:.1f} us")
         ms = C.c_double(0.0)
         lib.vbh_bench_spmv(be._h, 6, 5, C.byref(ms))
@@ -660,7 +660,7 @@ def bench_mixed(args):
            "true_rel_res_max": max(true),
            "spmm": {"cols_per_lane": cfg.cols_per_lane, "lanes_per_group": cfg.lanes_per_group,
                     "row_blocks": cfg.row_blocks, "xcd_rows": cfg.xcd_rows,
-                    "lds_staged": cfg.lds_staged, "ms": round(cfg.spmm_ms, 4)},
+                    "value_path": cfg.value_path, "ms": round(cfg.spmm_ms, 4)},
            "adjoint_dC_dE_s": round(t_adj, 4), "cpu_adjoint_s": round(t_cpu_adj, 2),
            "adjoint_max_rel_diff_vs_cpu": err}
     print(json.dumps(out), flush=True)

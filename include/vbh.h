@@ -308,15 +308,17 @@ int vbh_solve_block_device(vbh_ctx* ctx, int k, const double* dF, double* dU, in
  * solve (no reference counterpart): column tile (load cases per pass over K),
  * load cases per lane, lanes per block-row group (64 / lanes_per_group block
  * rows per wave), grid rows, whether block rows are split into one
- * contiguous range per XCD, whether block values are staged through LDS,
- * padded column count and the measured mean SpMM launch in ms. */
+ * contiguous range per XCD, how each lane gets the block values
+ * (value_path: 0 direct loads, 1 staged through LDS, 2 broadcast across
+ * the 16-lane DPP row), padded column count and the measured mean SpMM
+ * launch in ms. */
 typedef struct {
   int col_tile;
   int cols_per_lane;
   int lanes_per_group;
   int row_blocks;
   int xcd_rows;
-  int lds_staged;
+  int value_path;
   int ldk;
   double spmm_ms;
 } vbh_block_config;

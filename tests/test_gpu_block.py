@@ -133,7 +133,8 @@ def _variants():
     out = []
     for cw, lg in [(2, 16), (4, 8), (8, 4), (1, 32)]:
         for xcd in (0, 1):
-            for stage in (0, 1):
+            # block values: 0 direct loads, 1 through LDS, 2 DPP row broadcast
+            for stage in ((0, 1, 2) if lg == 16 else (0, 1)):
                 out.append(f"{cw},{lg},1024,{xcd},{stage}")
     return out
 
@@ -141,7 +142,7 @@ def _variants():
 @pytest.mark.parametrize("cfg", _variants())
 def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
     """Each SpMM shape the autotuner can pick, forced, on a BSR6 shell
-    operator (FP64) and a BSR3 solid operator (mixed), k = 32."""
+    operator (FP64 and mixed) and a BSR3 solid operator (mixed), k = 32."""
     monkeypatch.setenv("VBH_BLOCK_TUNE", cfg)
     m = decks.cquad4_tube(n_circ=16, n_axial=24, n_cases=32)
     be = gpu_backend_factory(tolerance=1e-10, max_iters=50000)
@@ -150,10 +151,15 @@ def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
     F = np.asfortranarray(np.stack([M.assemble_rhs(m, eq, n, lc) for lc in m.loads], axis=1))
     U = be.solve_block(K, F)
     cfg_used = be.block_config()
-    assert (cfg_used.cols_per_lane, cfg_used.lanes_per_group) == tuple(map(int, cfg.split(",")[:2]))
+    assert (cfg_used.cols_per_lane, cfg_used.lanes_per_group, cfg_used.value_path) == \
+        tuple(int(v) for v in (cfg.split(",")[:2] + cfg.split(",")[4:]))
     oc = O.assemble_model(n, m, eq)
     for j in (0, 13, 31):
         assert O.true_relative_residual(oc, U[:, j], F[:, j]) < 2e-10
+    bm = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=50000)
+    Um = bm.solve_block(csr_of(oc), F)
+    for j in (0, 13, 31):
+        assert O.true_relative_residual(oc, Um[:, j], F[:, j]) < 1e-8
     hm = decks.config2_hex_jitter(nx=10, ny=4, nz=4, seed=7)
     eq2, n2 = M.analysis_dof_map(hm)
     oc2 = O.assemble_model(n2, hm, eq2)

@@ -43,7 +43,7 @@ class vbh_block_config(C.Structure):
         ("lanes_per_group", C.c_int),
         ("row_blocks", C.c_int),
         ("xcd_rows", C.c_int),
-        ("lds_staged", C.c_int),
+        ("value_path", C.c_int),
         ("ldk", C.c_int),
         ("spmm_ms", C.c_double),
     ]

@@ -76,7 +76,8 @@ struct BlkOp {
 struct BlkTune {
   int cw = 0, lg = 0, nrb = 0;  // columns per lane, lanes per group, grid rows
   int xcd = 0;                  // XCD-contiguous row ranges
-  int stage = 0;                // 1: block values staged through LDS
+  int stage = 0;                // block values: 0 direct loads, 1 staged through LDS,
+                                // 2 DPP row broadcast (lg = 16)
   double ms = 0.0;
 };
 
@@ -926,14 +927,211 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
   }
 }
 
+// DPP variant (lanes per group = 16 = one DPP row): the 16 lanes of a group
+// load their block's B*B values with one coalesced load per 16 values (lane l
+// holds values l, l + 16, ...), and every product takes its block value from
+// the owning lane with a row broadcast (DPP "row_newbcast:n", gfx90a and
+// later: lane n of each 16-lane row to the whole row), which the compiler
+// folds into the instruction that consumes it (the FP32 -> FP64 conversion
+// of the shell kernels). Neither the address unit nor LDS sees the broadcast.
+constexpr int kDppRowNewBcast = 0x150;  // + n: row_newbcast:n
+
+template <int N>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), kDppRowNewBcast + N, 0xF, 0xF, false));
+}
+
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), kDppRowNewBcast + N, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), kDppRowNewBcast + N,
+                                             0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+template <typename Fn, int... I>
+__device__ __forceinline__ void static_for_seq(Fn&& fn, std::integer_sequence<int, I...>) {
+  (fn(std::integral_constant<int, I>{}), ...);
+}
+
+template <int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  static_for_seq(fn, std::make_integer_sequence<int, N>{});
+}
+
+template <typename T, int B, int CW, int MODE>
+__global__ __launch_bounds__(kBT) void k_blk_spmm_dpp(BlkOp A, int ldk, int xcd,
+                                                      const T* __restrict__ X, T* __restrict__ Y,
+                                                      const double* __restrict__ F,
+                                                      const int* __restrict__ act,
+                                                      double* __restrict__ part) {
+  constexpr int LG = 16, KT = CW * LG, GR = 64 / LG, NB2 = B * B, NL = (NB2 + LG - 1) / LG;
+  __shared__ double lds[kBT * CW];
+  const int lane = threadIdx.x & 63;
+  const int lg = lane % LG, g = lane / LG;
+  const int c0 = blockIdx.y * KT + lg * CW;
+  {  // a column tile with no column to compute leaves at once
+    int on = 0;
+#pragma unroll
+    for (int w = 0; w < CW; ++w) on |= act[c0 + w];
+    if (!__syncthreads_or(on)) return;
+  }
+  // FP64 sums for the FP32 6x6 (shell) blocks, as in k_blk_spmm
+  using Acc = typename std::conditional<(MODE == 0 && sizeof(T) == 4 && B == 6), double, T>::type;
+  const T* __restrict__ val = op_vals<T>(A);
+  const double* __restrict__ dblk = (MODE == 0 && sizeof(T) == 4) ? A.dblk : nullptr;
+  const int wave = threadIdx.x >> 6;
+  double dot[CW];
+#pragma unroll
+  for (int w = 0; w < CW; ++w) dot[w] = 0.0;
+  int rb = blockIdx.x, nrbx = gridDim.x, row_lo = 0, row_hi = A.nbr;
+  if (xcd) {
+    const int x = blockIdx.x & 7, per = (A.nbr + 7) >> 3;
+    rb = blockIdx.x >> 3;
+    nrbx = gridDim.x >> 3;
+    row_lo = min(A.nbr, x * per);
+    row_hi = min(A.nbr, row_lo + per);
+  }
+  // every lane of a group follows the same block row and block sequence, so
+  // the broadcast source lane is always active when its row is
+  for (int I0 = row_lo + (rb * (kBT / 64) + wave) * GR; I0 < row_hi;
+       I0 += nrbx * (kBT / 64) * GR) {
+    const int I = I0 + g;
+    if (I >= row_hi) continue;
+    Acc acc[B][CW];
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int w = 0; w < CW; ++w) acc[r][w] = Acc(0);
+    const int o = A.brp[I], e = A.brp[I + 1];
+    // the row's block columns 16 at a time (lane l holds the l-th); blocks in
+    // pairs with two register sets, so the next block's values and X rows
+    // load while the current one is summed (the diagonal block of an FP32
+    // operator is neither loaded nor summed: FP64 epilogue below)
+    T m0[NL], x0[B][CW], m1[NL], x1[B][CW];
+    auto fetch = [&](T (&mv)[NL], T (&xv)[B][CW], int q, int J) {
+      const T* __restrict__ v = val + static_cast<size_t>(q) * NB2;
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        const int f = lg + LG * j;
+        mv[j] = (f < NB2) ? v[f] : T(0);
+      }
+      const T* __restrict__ xj = X + static_cast<size_t>(B * J) * ldk + c0;
+#pragma unroll
+      for (int s = 0; s < B; ++s) vload<T, CW>(xj + static_cast<size_t>(s) * ldk, xv[s]);
+    };
+    auto sum = [&](const T (&mv)[NL], const T (&xv)[B][CW]) {
+      // value i of the block is entry (r, s) = (i mod B, i / B) (column-major)
+      static_for<NB2>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int r = i % B, s = i / B;
+        const Acc a = static_cast<Acc>(row_bcast<i % LG>(mv[i / LG]));
+#pragma unroll
+        for (int w = 0; w < CW; ++w) acc[r][w] += a * static_cast<Acc>(xv[s][w]);
+      });
+    };
+    for (int q0 = o; q0 < e; q0 += LG) {
+      const int qn = min(LG, e - q0);
+      const int jl = (lg < qn) ? A.bci[q0 + lg] : 0;
+      int J = __shfl(jl, 0, LG);
+      bool use0 = !(dblk && J == I), use1 = false;
+      if (use0) fetch(m0, x0, q0, J);
+      for (int t = 0; t < qn; t += 2) {
+        if (t + 1 < qn) {
+          J = __shfl(jl, t + 1, LG);
+          use1 = !(dblk && J == I);
+          if (use1) fetch(m1, x1, q0 + t + 1, J);
+        }
+        if (use0) sum(m0, x0);
+        if (t + 1 >= qn) break;
+        if (t + 2 < qn) {
+          J = __shfl(jl, t + 2, LG);
+          use0 = !(dblk && J == I);
+          if (use0) fetch(m0, x0, q0 + t + 2, J);
+        }
+        if (use1) sum(m1, x1);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
+      if constexpr (MODE == 0) {
+        T xo[CW], yo[CW];
+        vload<T, CW>(X + idx, xo);
+        for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(acc[r][w]);
+        if (dblk) {  // FP64 corrected diagonal block: y = acc + D_II x_I in FP64
+          double dsum[CW];
+#pragma unroll
+          for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
+#pragma unroll
+          for (int s = 0; s < B; ++s) {
+            T xi[CW];
+            vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xi);
+            const double dv = dblk[(static_cast<size_t>(I) * B + r) * B + s];
+#pragma unroll
+            for (int w = 0; w < CW; ++w) dsum[w] += dv * static_cast<double>(xi[w]);
+          }
+#pragma unroll
+          for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(dsum[w]);
+        }
+        vstore<T, CW>(Y + idx, yo);
+#pragma unroll
+        for (int w = 0; w < CW; ++w)
+          dot[w] += static_cast<double>(xo[w]) * static_cast<double>(yo[w]);
+      } else {
+        double f[CW];
+        T y[CW];
+        vload<double, CW>(F + idx, f);
+#pragma unroll
+        for (int w = 0; w < CW; ++w) {
+          const double yy = f[w] - static_cast<double>(acc[r][w]);
+          y[w] = static_cast<T>(yy);
+          dot[w] += yy * yy;
+        }
+        vstore<T, CW>(Y + idx, y);
+      }
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < CW; ++w) lds[threadIdx.x * CW + w] = dot[w];
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < KT) {
+    const int t = threadIdx.x, w = t % CW;
+    double sum = 0.0;
+    for (int th = t / CW; th < kBT; th += LG) sum += lds[th * CW + w];
+    part[static_cast<size_t>(blockIdx.y * KT + t) * kMaxGrid + blockIdx.x] = sum;
+  }
+}
+
 // ---- SpMM dispatch ----------------------------------------------------------------------------
 template <typename T, int B, int MODE>
 void spmm_b(const BlkTune& t, const BlkOp& A, int ldk, const T* X, T* Y, const double* F,
             const int* act, double* part, hipStream_t s) {
   const dim3 grid(t.nrb, ldk / (t.cw * t.lg));
+  if (t.stage == 2) {  // DPP row broadcast: 16 lanes per group
+    if constexpr (B > 1) {
+      if (t.lg == 16) {
+#define VBH_SPMM_DPP(CW_)                                                                       \
+  if (t.cw == CW_) {                                                                            \
+    hipLaunchKernelGGL((k_blk_spmm_dpp<T, B, CW_, MODE>), grid, dim3(kBT), 0, s, A, ldk, t.xcd, \
+                       X, Y, F, act, part);                                                     \
+    return;                                                                                     \
+  }
+        VBH_SPMM_DPP(1)
+        VBH_SPMM_DPP(2)
+        VBH_SPMM_DPP(4)
+#undef VBH_SPMM_DPP
+      }
+    }
+    throw_error(VBH_ERR_HIP, "block SpMM: no DPP kernel for the tuned configuration");
+  }
 #define VBH_SPMM_CASE(CW_, LG_)                                                                 \
   if (t.cw == CW_ && t.lg == LG_) {                                                             \
-    if (t.stage)                                                                                \
+    if (t.stage == 1)                                                                           \
       hipLaunchKernelGGL((k_blk_spmm_lds<T, B, CW_, LG_, MODE>), grid, dim3(kBT), 0, s, A, ldk, \
                          t.xcd, X, Y, F, act, part);                                            \
     else                                                                                        \
@@ -989,7 +1187,8 @@ std::vector<BlkTune> candidates(int ldk, int B) {
   const int kt = std::min(ldk, 64);
   std::vector<BlkTune> c;
   auto add = [&](int cw, int lg) {
-    for (int stage : {1, 0})
+    for (int stage : {2, 1, 0}) {
+      if (stage == 2 && (lg != 16 || B == 1)) continue;  // DPP rows are 16 lanes
       for (int xcd : {1, 0})
         for (int nrb : {2048, 1024}) {
           BlkTune t;
@@ -1000,6 +1199,7 @@ std::vector<BlkTune> candidates(int ldk, int B) {
           t.stage = stage;
           c.push_back(t);
         }
+    }
   };
   for (int cw : {2, 4, 8, 1}) {
     if (B == 1 && cw > 1) continue;
@@ -1130,7 +1330,7 @@ static BlkTune tune(Ctx& c, BlockState& b, int precision) {
     BlkTune f;
     if (std::sscanf(env, "%d,%d,%d,%d,%d", &f.cw, &f.lg, &f.nrb, &f.xcd, &f.stage) >= 3) {
       bool known = false;
-      for (const auto& t : cand) known |= (t.cw == f.cw && t.lg == f.lg);
+      for (const auto& t : cand) known |= (t.cw == f.cw && t.lg == f.lg && t.stage == f.stage);
       if (!known || f.nrb < 8 || f.nrb > kMaxGrid || f.nrb % 8)
         throw_error(VBH_ERR_ARG, "VBH_BLOCK_TUNE: no such SpMM configuration for this width");
       cand.assign(1, f);
@@ -1611,7 +1811,7 @@ int vbh_last_block_config(const vbh_ctx* h, vbh_block_config* out) {
   out->lanes_per_group = c->blk->cur.lg;
   out->row_blocks = c->blk->cur.nrb;
   out->xcd_rows = c->blk->cur.xcd;
-  out->lds_staged = c->blk->cur.stage;
+  out->value_path = c->blk->cur.stage;
   out->ldk = c->blk->ldk;
   out->spmm_ms = c->blk->cur.ms;
   return VBH_OK;
