@@ -963,8 +963,17 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
   static_for_seq(fn, std::make_integer_sequence<int, N>{});
 }
 
+// values per lane of the FP64 diagonal block a 16-lane group holds (only the
+// FP32 plain product adds it)
+template <typename T, int B, int MODE>
+constexpr int dblk_lanes() {
+  return (MODE == 0 && sizeof(T) == 4) ? (B * B + 15) / 16 : 0;
+}
+
+// FP32 values with at most 2 load cases per lane: at least 4 waves per SIMD
+// (<= 128 VGPRs, which those shapes fit without spilling)
 template <typename T, int B, int CW, int MODE>
-__global__ __launch_bounds__(kBT) void k_blk_spmm_dpp(BlkOp A, int ldk, int xcd,
+__global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_blk_spmm_dpp(BlkOp A, int ldk, int xcd,
                                                       const T* __restrict__ X, T* __restrict__ Y,
                                                       const double* __restrict__ F,
                                                       const int* __restrict__ act,
@@ -1056,33 +1065,52 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_dpp(BlkOp A, int ldk, int xcd,
         if (use1) sum(m1, x1);
       }
     }
+    if constexpr (MODE == 0) {
+      // the row's own X rows (for p.Ap, and the diagonal block)
+      T xI[B][CW];
 #pragma unroll
-    for (int r = 0; r < B; ++r) {
-      const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
-      if constexpr (MODE == 0) {
-        T xo[CW], yo[CW];
-        vload<T, CW>(X + idx, xo);
+      for (int s = 0; s < B; ++s)
+        vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xI[s]);
+      // FP64 corrected diagonal block (row-major B*B per block row), held
+      // like the block values: lane l of the group has entries l, l + 16, ...
+      constexpr int ND = dblk_lanes<T, B, MODE>();
+      double dm[ND > 0 ? ND : 1];
+      if (ND > 0 && dblk) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+          const int f = lg + LG * j;
+          dm[j] = (f < NB2) ? dblk[static_cast<size_t>(I) * NB2 + f] : 0.0;
+        }
+      }
+      static_for<B>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
+        T yo[CW];
+#pragma unroll
         for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(acc[r][w]);
-        if (dblk) {  // FP64 corrected diagonal block: y = acc + D_II x_I in FP64
+        if (ND > 0 && dblk) {  // y = acc + D_II x_I in FP64
           double dsum[CW];
 #pragma unroll
           for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
+          static_for<B>([&](auto sc) {
+            constexpr int i = r * B + decltype(sc)::value;
+            const double dv = row_bcast<i % LG>(dm[i / LG]);
 #pragma unroll
-          for (int s = 0; s < B; ++s) {
-            T xi[CW];
-            vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xi);
-            const double dv = dblk[(static_cast<size_t>(I) * B + r) * B + s];
-#pragma unroll
-            for (int w = 0; w < CW; ++w) dsum[w] += dv * static_cast<double>(xi[w]);
-          }
+            for (int w = 0; w < CW; ++w)
+              dsum[w] += dv * static_cast<double>(xI[decltype(sc)::value][w]);
+          });
 #pragma unroll
           for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(dsum[w]);
         }
         vstore<T, CW>(Y + idx, yo);
 #pragma unroll
         for (int w = 0; w < CW; ++w)
-          dot[w] += static_cast<double>(xo[w]) * static_cast<double>(yo[w]);
-      } else {
+          dot[w] += static_cast<double>(xI[r][w]) * static_cast<double>(yo[w]);
+      });
+    } else {
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        const size_t idx = static_cast<size_t>(B * I + r) * ldk + c0;
         double f[CW];
         T y[CW];
         vload<double, CW>(F + idx, f);
