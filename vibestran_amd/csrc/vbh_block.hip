@@ -246,46 +246,66 @@ __global__ __launch_bounds__(kBT) void k_blk_init(long long m, int ldk, int lg2,
   column_partials<V, 3>(acc, ldk, lds, part);
 }
 
-// p = z + beta p with z = r / diag, for active columns.
-template <typename T>
+// p = z + beta p with z = r / diag, for the columns still running, after
+// adding the previous iteration's alpha p to the solution of the columns it
+// updated (actU): u (U = double, FP64) or the FP32 increments du (U = float,
+// mixed precision; each step summed in FP64 and rounded once). p is read
+// here anyway, so the update kernel need not read it again.
+template <typename T, typename U>
 __global__ __launch_bounds__(kBT) void k_blk_direction(long long m, int ldk, int lg2,
                                                        const T* __restrict__ diag,
                                                        const T* __restrict__ r, T* __restrict__ p,
                                                        const double* __restrict__ coefB,
-                                                       const int* __restrict__ actD) {
+                                                       const int* __restrict__ actD,
+                                                       U* __restrict__ u,
+                                                       const double* __restrict__ coefA,
+                                                       const int* __restrict__ actU) {
   constexpr int V = vwidth<T>();
-  if (!any_active(actD, ldk)) return;
+  if (!any_active(actD, ldk) && !any_active(actU, ldk)) return;
   const int c0 = (threadIdx.x * V) & (ldk - 1);
-  bool on[V];
+  bool on[V], up[V];
   T bt[V];
-  int any = 0;
+  double al[V];
+  int any_on = 0, any_up = 0;
 #pragma unroll
   for (int w = 0; w < V; ++w) {
     on[w] = actD[c0 + w] != 0;
-    any |= on[w];
+    up[w] = actU[c0 + w] != 0;
+    any_on |= on[w];
+    any_up |= up[w];
     bt[w] = static_cast<T>(coefB[c0 + w]);
+    al[w] = coefA[c0 + w];
   }
-  if (!any) return;
+  if (!any_on && !any_up) return;
   VBH_BLK_LOOP(e) {
-    const T d = diag[e >> lg2];
-    T rv[V], pv[V];
-    vload<T, V>(r + e, rv);
+    T pv[V];
     vload<T, V>(p + e, pv);
+    if (any_up) {
+      U uv[V];
+      vload<U, V>(u + e, uv);
 #pragma unroll
-    for (int w = 0; w < V; ++w) pv[w] = on[w] ? rv[w] / d + bt[w] * pv[w] : pv[w];
-    vstore<T, V>(p + e, pv);
+      for (int w = 0; w < V; ++w)
+        if (up[w]) uv[w] = static_cast<U>(static_cast<double>(uv[w]) + al[w] * static_cast<double>(pv[w]));
+      vstore<U, V>(u + e, uv);
+    }
+    if (any_on) {
+      const T d = diag[e >> lg2];
+      T rv[V];
+      vload<T, V>(r + e, rv);
+#pragma unroll
+      for (int w = 0; w < V; ++w) pv[w] = on[w] ? rv[w] / d + bt[w] * pv[w] : pv[w];
+      vstore<T, V>(p + e, pv);
+    }
   }
 }
 
-// u += alpha p; r -= alpha Ap; partials r.z, r.r per column. U = double:
-// the FP64 solution itself; U = float (mixed precision): the FP32 increments
-// since the last fold, each step computed in FP64 and rounded once.
-template <typename T, typename U>
+// r -= alpha Ap for the columns updated this iteration; partials r.z, r.r
+// per column (the solution's alpha p is added by the next direction kernel,
+// or by k_blk_settle at the end of the batch).
+template <typename T>
 __global__ __launch_bounds__(kBT) void k_blk_update(long long m, int ldk, int lg2,
                                                     const T* __restrict__ diag,
-                                                    U* __restrict__ u, T* __restrict__ r,
-                                                    const T* __restrict__ p,
-                                                    const T* __restrict__ Ap,
+                                                    T* __restrict__ r, const T* __restrict__ Ap,
                                                     const double* __restrict__ coefA,
                                                     const int* __restrict__ actU,
                                                     double* __restrict__ part) {
@@ -294,61 +314,73 @@ __global__ __launch_bounds__(kBT) void k_blk_update(long long m, int ldk, int lg
   if (!any_active(actU, ldk)) return;
   const int c0 = (threadIdx.x * V) & (ldk - 1);
   bool on[V];
-  double al[V];
   T a[V];
 #pragma unroll
   for (int w = 0; w < V; ++w) {
     on[w] = actU[c0 + w] != 0;
-    al[w] = coefA[c0 + w];
-    a[w] = static_cast<T>(al[w]);
+    a[w] = static_cast<T>(coefA[c0 + w]);
   }
   double acc[2][V] = {};
   VBH_BLK_LOOP(e) {
     const T d = diag[e >> lg2];
-    U uv[V];
-    T rv[V], pv[V], qv[V];
-    vload<U, V>(u + e, uv);
+    T rv[V], qv[V];
     vload<T, V>(r + e, rv);
-    vload<T, V>(p + e, pv);
     vload<T, V>(Ap + e, qv);
 #pragma unroll
     for (int w = 0; w < V; ++w) {
-      if (on[w]) {
-        uv[w] = static_cast<U>(static_cast<double>(uv[w]) + al[w] * static_cast<double>(pv[w]));
-        rv[w] = rv[w] - a[w] * qv[w];
-      }
+      if (on[w]) rv[w] = rv[w] - a[w] * qv[w];
       const T zi = rv[w] / d;
       acc[0][w] += static_cast<double>(rv[w]) * static_cast<double>(zi);
       acc[1][w] += static_cast<double>(rv[w]) * static_cast<double>(rv[w]);
     }
-    vstore<U, V>(u + e, uv);
     vstore<T, V>(r + e, rv);
   }
   column_partials<V, 2>(acc, ldk, lds, part);
 }
 
-// u += du (FP64), du = 0: the FP32 increments of a batch of mixed-precision
-// iterations folded into the solution; every column, so that u is current
-// wherever the host looks (refresh, compaction, the end of the solve).
-// add = 0 only clears du.
-__global__ __launch_bounds__(kBT) void k_blk_fold(long long m, int add, double* __restrict__ u,
-                                                  float* __restrict__ du) {
+// End of a batch of iterations: u += du + alpha p for the columns whose last
+// update is still pending (actU), du = 0 (du == nullptr: FP64, no
+// increments), so that u is current wherever the host looks (refresh,
+// compaction, the end of the solve); actU is cleared by the next launch.
+// settle = 0 only clears du.
+template <typename T>
+__global__ __launch_bounds__(kBT) void k_blk_settle(long long m, int ldk, int settle,
+                                                    double* __restrict__ u, float* __restrict__ du,
+                                                    const T* __restrict__ p,
+                                                    const double* __restrict__ coefA,
+                                                    const int* __restrict__ actU) {
   constexpr int V = 4;
+  const int c0 = (threadIdx.x * V) & (ldk - 1);
+  bool up[V];
+  double al[V];
+  int any_up = 0;
+#pragma unroll
+  for (int w = 0; w < V; ++w) {
+    up[w] = settle && actU[c0 + w] != 0;
+    any_up |= up[w];
+    al[w] = coefA[c0 + w];
+  }
   for (long long e = (blockIdx.x * static_cast<long long>(kBT) + threadIdx.x) * V; e < m;
        e += static_cast<long long>(gridDim.x) * kBT * V) {
-    float zero[V];
+    float dv[V];
+    if (du) {
+      if (settle) vload<float, V>(du + e, dv);
+      float zero[V];
 #pragma unroll
-    for (int w = 0; w < V; ++w) zero[w] = 0.0f;
-    if (add) {
-      float dv[V];
-      double uv[V];
-      vload<float, V>(du + e, dv);
-      vload<double, V>(u + e, uv);
-#pragma unroll
-      for (int w = 0; w < V; ++w) uv[w] += static_cast<double>(dv[w]);
-      vstore<double, V>(u + e, uv);
+      for (int w = 0; w < V; ++w) zero[w] = 0.0f;
+      vstore<float, V>(du + e, zero);
     }
-    vstore<float, V>(du + e, zero);
+    if (!settle || (!du && !any_up)) continue;
+    double uv[V];
+    vload<double, V>(u + e, uv);
+    T pv[V];
+    if (any_up) vload<T, V>(p + e, pv);
+#pragma unroll
+    for (int w = 0; w < V; ++w) {
+      if (du) uv[w] += static_cast<double>(dv[w]);
+      if (up[w]) uv[w] += al[w] * static_cast<double>(pv[w]);
+    }
+    vstore<double, V>(u + e, uv);
   }
 }
 
@@ -1401,37 +1433,53 @@ static BlkTune tune(Ctx& c, BlockState& b, int precision) {
   return best;
 }
 
+// settle = 1: the last update of each column (still pending in actU) and
+// the FP32 increments into u, then actU cleared; settle = 0: only du (mixed)
+// and actU cleared, at the start of a solve or of a compacted layout.
+static void settle_batch(Ctx& c, BlockState& b, int precision, int settle) {
+  hipStream_t s = c.stream;
+  const long long m = static_cast<long long>(b.npad) * b.ldk;
+  const dim3 grid(grid_for(m, kBT * 4));
+  if (precision == VBH_PREC_MIXED)
+    hipLaunchKernelGGL(k_blk_settle<float>, grid, dim3(kBT), 0, s, m, b.ldk, settle, b.u.p,
+                       b.duf.p, b.pf.p, b.coefA.p, b.actU.p);
+  else if (settle)
+    hipLaunchKernelGGL(k_blk_settle<double>, grid, dim3(kBT), 0, s, m, b.ldk, settle, b.u.p,
+                       static_cast<float*>(nullptr), b.pd.p, b.coefA.p, b.actU.p);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, b.ldk, b.actU.p, 0);
+}
+
 static void enqueue_block_iterations(Ctx& c, BlockState& b, int precision, int count) {
   hipStream_t s = c.stream;
   const long long m = static_cast<long long>(b.npad) * b.ldk;
   const PcgParams prm = c.params();
   const int mixed = precision == VBH_PREC_MIXED;
   for (int i = 0; i < count; ++i) {
+    // the direction kernel also adds the previous iteration's alpha p to the
+    // solution (u in FP64, the FP32 increments du in mixed precision)
     if (!mixed) {
-      hipLaunchKernelGGL(k_blk_direction<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
-                         c.diag64.p, b.rd.p, b.pd.p, b.coefB.p, b.actD.p);
+      hipLaunchKernelGGL((k_blk_direction<double, double>), dim3(b.nvec), dim3(kBT), 0, s, m,
+                         b.ldk, b.lg2, c.diag64.p, b.rd.p, b.pd.p, b.coefB.p, b.actD.p, b.u.p,
+                         b.coefA.p, b.actU.p);
       spmm<double, 0>(b.cur, b.op, b.ldk, b.pd.p, b.Apd.p, nullptr, b.actD.p, b.partA.p, s);
     } else {
-      hipLaunchKernelGGL(k_blk_direction<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
-                         c.diag32.p, b.rf.p, b.pf.p, b.coefB.p, b.actD.p);
+      hipLaunchKernelGGL((k_blk_direction<float, float>), dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk,
+                         b.lg2, c.diag32.p, b.rf.p, b.pf.p, b.coefB.p, b.actD.p, b.duf.p,
+                         b.coefA.p, b.actU.p);
       spmm<float, 0>(b.cur, b.op, b.ldk, b.pf.p, b.Apf.p, nullptr, b.actD.p, b.partA.p, s);
     }
     hipLaunchKernelGGL(k_blk_scal_alpha, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partA.p,
                        b.cur.nrb, b.cols.p, b.coefA.p, b.actU.p, b.actD.p);
     if (!mixed)
-      hipLaunchKernelGGL((k_blk_update<double, double>), dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk,
-                         b.lg2, c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.Apd.p, b.coefA.p, b.actU.p,
-                         b.partB.p);
+      hipLaunchKernelGGL(k_blk_update<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
+                         c.diag64.p, b.rd.p, b.Apd.p, b.coefA.p, b.actU.p, b.partB.p);
     else
-      hipLaunchKernelGGL((k_blk_update<float, float>), dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk,
-                         b.lg2, c.diag32.p, b.duf.p, b.rf.p, b.pf.p, b.Apf.p, b.coefA.p, b.actU.p,
-                         b.partB.p);
+      hipLaunchKernelGGL(k_blk_update<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2,
+                         c.diag32.p, b.rf.p, b.Apf.p, b.coefA.p, b.actU.p, b.partB.p);
     hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 1, b.ldk, b.k, mixed,
                        b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
   }
-  if (mixed)  // the batch's increments into u (the iterations above only touched du)
-    hipLaunchKernelGGL(k_blk_fold, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, s, m, 1, b.u.p,
-                       b.duf.p);
+  settle_batch(c, b, precision, 1);
 }
 
 static hipGraphExec_t block_graph(Ctx& c, BlockState& b, int precision) {
@@ -1608,9 +1656,6 @@ static void compact_columns(Ctx& c, BlockState& b, int precision, std::vector<in
   if (precision == VBH_PREC_MIXED) {
     move32(b.rf, src.rf);
     move32(b.pf, src.pf);
-    // the increments were folded into u at the end of the last batch
-    hipLaunchKernelGGL(k_blk_fold, dim3(grid_for(static_cast<long long>(m2), kBT * 4)), dim3(kBT),
-                       0, s, static_cast<long long>(m2), 0, b.u.p, b.duf.p);
   } else {
     move64(b.rd, src.rd);
     move64(b.pd, src.pd);
@@ -1619,6 +1664,8 @@ static void compact_columns(Ctx& c, BlockState& b, int precision, std::vector<in
   b.coefB.upload(cb2.data(), cb2.size(), s);
   b.actD.upload(on.data(), on.size(), s);
   b.actU.upload(zero.data(), zero.size(), s);
+  // the last batch settled every pending update into u: the new du starts clear
+  settle_batch(c, b, precision, 0);
   b.reload.upload(zero.data(), zero.size(), s);
   b.cols.upload(cols2.data(), cols2.size(), s);
   VBH_HIP(hipStreamSynchronize(s));
@@ -1665,12 +1712,10 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
   if (!mixed)
     hipLaunchKernelGGL(k_blk_init<double>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
                        c.diag64.p, b.u.p, b.rd.p, b.pd.p, b.partB.p);
-  else {
+  else
     hipLaunchKernelGGL(k_blk_init<float>, dim3(b.nvec), dim3(kBT), 0, s, m, b.ldk, b.lg2, b.F.p,
                        c.diag32.p, b.u.p, b.rf.p, b.pf.p, b.partB.p);
-    hipLaunchKernelGGL(k_blk_fold, dim3(grid_for(m, kBT * 4)), dim3(kBT), 0, s, m, 0, b.u.p,
-                       b.duf.p);
-  }
+  settle_batch(c, b, precision, 0);  // no update pending, du clear
   hipLaunchKernelGGL(k_blk_scal_beta, dim3(b.ldk), dim3(kBT), 0, s, 0, b.ldk, b.k, mixed,
                      b.partB.p, b.nvec, b.cols.p, b.coefB.p, b.actD.p, b.reload.p, prm);
   const char* ge_env = std::getenv("VBH_USE_GRAPH");
