@@ -133,8 +133,9 @@ def _variants():
     out = []
     for cw, lg in [(2, 16), (4, 8), (8, 4), (1, 32)]:
         for xcd in (0, 1):
-            # block values: 0 direct loads, 1 through LDS, 2 DPP row broadcast
-            for stage in ((0, 1, 2) if lg == 16 else (0, 1)):
+            # block values: 0 direct loads, 1 through LDS, 2 broadcast across
+            # the group's lanes (groups of 4, 8 or 16 lanes)
+            for stage in ((0, 1, 2) if lg in (4, 8, 16) else (0, 1)):
                 out.append(f"{cw},{lg},1024,{xcd},{stage}")
     return out
 

@@ -77,7 +77,7 @@ struct BlkTune {
   int cw = 0, lg = 0, nrb = 0;  // columns per lane, lanes per group, grid rows
   int xcd = 0;                  // XCD-contiguous row ranges
   int stage = 0;                // block values: 0 direct loads, 1 staged through LDS,
-                                // 2 DPP row broadcast (lg = 16)
+                                // 2 broadcast across the group's lanes (lg = 4, 8, 16)
   double ms = 0.0;
 };
 
@@ -959,31 +959,47 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
   }
 }
 
-// DPP variant (lanes per group = 16 = one DPP row): the 16 lanes of a group
-// load their block's B*B values with one coalesced load per 16 values (lane l
-// holds values l, l + 16, ...), and every product takes its block value from
-// the owning lane with a row broadcast (DPP "row_newbcast:n", gfx90a and
-// later: lane n of each 16-lane row to the whole row), which the compiler
-// folds into the instruction that consumes it (the FP32 -> FP64 conversion
-// of the shell kernels). Neither the address unit nor LDS sees the broadcast.
+// Register-broadcast variants (lanes per group LG = 4, 8 or 16): the LG lanes
+// of a group load their block's B*B values with one coalesced load per LG
+// values (lane l holds values l, l + LG, ...), and every product takes its
+// block value from the owning lane with a broadcast inside the group, so the
+// address unit sees each block value once:
+//   LG = 16  DPP row_newbcast:n (gfx90a and later: lane n of each 16-lane row
+//            to the whole row), folded by the compiler into the instruction
+//            that consumes it (the FP32 -> FP64 conversion of the shell sums);
+//   LG = 4   DPP quad_perm [n,n,n,n], folded the same way;
+//   LG = 8   ds_swizzle in bit-mask mode (lane (l & 0x18) | n of each 32),
+//            which runs in the LDS unit without touching LDS memory.
 constexpr int kDppRowNewBcast = 0x150;  // + n: row_newbcast:n
 
-template <int N>
-__device__ __forceinline__ float row_bcast(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), kDppRowNewBcast + N, 0xF, 0xF, false));
+template <int LG, int N>
+__device__ __forceinline__ int group_bcast_i32(int v) {
+  static_assert(N >= 0 && N < LG, "source lane outside the group");
+  if constexpr (LG == 16) {
+    return __builtin_amdgcn_update_dpp(0, v, kDppRowNewBcast + N, 0xF, 0xF, false);
+  } else if constexpr (LG == 4) {
+    return __builtin_amdgcn_update_dpp(0, v, N | (N << 2) | (N << 4) | (N << 6), 0xF, 0xF, false);
+  } else {
+    static_assert(LG == 8, "group broadcast for 4, 8 or 16 lanes");
+    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (N << 5));
+  }
 }
 
-template <int N>
-__device__ __forceinline__ double row_bcast(double v) {
+template <int LG, int N>
+__device__ __forceinline__ float group_bcast(float v) {
+  return __int_as_float(group_bcast_i32<LG, N>(__float_as_int(v)));
+}
+
+template <int LG, int N>
+__device__ __forceinline__ double group_bcast(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), kDppRowNewBcast + N, 0xF,
-                                             0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), kDppRowNewBcast + N,
-                                             0xF, 0xF, false);
+  const int lo = group_bcast_i32<LG, N>(static_cast<int>(b));
+  const int hi = group_bcast_i32<LG, N>(static_cast<int>(b >> 32));
   return __longlong_as_double((static_cast<long long>(hi) << 32) |
                               static_cast<unsigned int>(lo));
 }
+
+constexpr bool has_group_bcast(int lg) { return lg == 4 || lg == 8 || lg == 16; }
 
 template <typename Fn, int... I>
 __device__ __forceinline__ void static_for_seq(Fn&& fn, std::integer_sequence<int, I...>) {
@@ -995,22 +1011,22 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
   static_for_seq(fn, std::make_integer_sequence<int, N>{});
 }
 
-// values per lane of the FP64 diagonal block a 16-lane group holds (only the
-// FP32 plain product adds it)
-template <typename T, int B, int MODE>
+// values per lane of the FP64 diagonal block a group of LG lanes holds (only
+// the FP32 plain product adds it)
+template <typename T, int B, int LG, int MODE>
 constexpr int dblk_lanes() {
-  return (MODE == 0 && sizeof(T) == 4) ? (B * B + 15) / 16 : 0;
+  return (MODE == 0 && sizeof(T) == 4) ? (B * B + LG - 1) / LG : 0;
 }
 
 // FP32 values with at most 2 load cases per lane: at least 4 waves per SIMD
 // (<= 128 VGPRs, which those shapes fit without spilling)
-template <typename T, int B, int CW, int MODE>
-__global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_blk_spmm_dpp(BlkOp A, int ldk, int xcd,
+template <typename T, int B, int CW, int LG, int MODE>
+__global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_blk_spmm_bcast(BlkOp A, int ldk, int xcd,
                                                       const T* __restrict__ X, T* __restrict__ Y,
                                                       const double* __restrict__ F,
                                                       const int* __restrict__ act,
                                                       double* __restrict__ part) {
-  constexpr int LG = 16, KT = CW * LG, GR = 64 / LG, NB2 = B * B, NL = (NB2 + LG - 1) / LG;
+  constexpr int KT = CW * LG, GR = 64 / LG, NB2 = B * B, NL = (NB2 + LG - 1) / LG;
   __shared__ double lds[kBT * CW];
   const int lane = threadIdx.x & 63;
   const int lg = lane % LG, g = lane / LG;
@@ -1049,7 +1065,7 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
 #pragma unroll
       for (int w = 0; w < CW; ++w) acc[r][w] = Acc(0);
     const int o = A.brp[I], e = A.brp[I + 1];
-    // the row's block columns 16 at a time (lane l holds the l-th); blocks in
+    // the row's block columns LG at a time (lane l holds the l-th); blocks in
     // pairs with two register sets, so the next block's values and X rows
     // load while the current one is summed (the diagonal block of an FP32
     // operator is neither loaded nor summed: FP64 epilogue below)
@@ -1070,7 +1086,7 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
       static_for<NB2>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         constexpr int r = i % B, s = i / B;
-        const Acc a = static_cast<Acc>(row_bcast<i % LG>(mv[i / LG]));
+        const Acc a = static_cast<Acc>(group_bcast<LG, i % LG>(mv[i / LG]));
 #pragma unroll
         for (int w = 0; w < CW; ++w) acc[r][w] += a * static_cast<Acc>(xv[s][w]);
       });
@@ -1105,7 +1121,7 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
         vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xI[s]);
       // FP64 corrected diagonal block (row-major B*B per block row), held
       // like the block values: lane l of the group has entries l, l + 16, ...
-      constexpr int ND = dblk_lanes<T, B, MODE>();
+      constexpr int ND = dblk_lanes<T, B, LG, MODE>();
       double dm[ND > 0 ? ND : 1];
       if (ND > 0 && dblk) {
 #pragma unroll
@@ -1126,7 +1142,7 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
           for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
           static_for<B>([&](auto sc) {
             constexpr int i = r * B + decltype(sc)::value;
-            const double dv = row_bcast<i % LG>(dm[i / LG]);
+            const double dv = group_bcast<LG, i % LG>(dm[i / LG]);
 #pragma unroll
             for (int w = 0; w < CW; ++w)
               dsum[w] += dv * static_cast<double>(xI[decltype(sc)::value][w]);
@@ -1172,25 +1188,16 @@ template <typename T, int B, int MODE>
 void spmm_b(const BlkTune& t, const BlkOp& A, int ldk, const T* X, T* Y, const double* F,
             const int* act, double* part, hipStream_t s) {
   const dim3 grid(t.nrb, ldk / (t.cw * t.lg));
-  if (t.stage == 2) {  // DPP row broadcast: 16 lanes per group
-    if constexpr (B > 1) {
-      if (t.lg == 16) {
-#define VBH_SPMM_DPP(CW_)                                                                       \
-  if (t.cw == CW_) {                                                                            \
-    hipLaunchKernelGGL((k_blk_spmm_dpp<T, B, CW_, MODE>), grid, dim3(kBT), 0, s, A, ldk, t.xcd, \
-                       X, Y, F, act, part);                                                     \
-    return;                                                                                     \
-  }
-        VBH_SPMM_DPP(1)
-        VBH_SPMM_DPP(2)
-        VBH_SPMM_DPP(4)
-#undef VBH_SPMM_DPP
-      }
-    }
-    throw_error(VBH_ERR_HIP, "block SpMM: no DPP kernel for the tuned configuration");
-  }
 #define VBH_SPMM_CASE(CW_, LG_)                                                                 \
   if (t.cw == CW_ && t.lg == LG_) {                                                             \
+    if (t.stage == 2) {                                                                         \
+      if constexpr (B > 1 && has_group_bcast(LG_)) {                                            \
+        hipLaunchKernelGGL((k_blk_spmm_bcast<T, B, CW_, LG_, MODE>), grid, dim3(kBT), 0, s, A,  \
+                           ldk, t.xcd, X, Y, F, act, part);                                     \
+        return;                                                                                 \
+      }                                                                                         \
+      throw_error(VBH_ERR_HIP, "block SpMM: no broadcast kernel for the tuned configuration");  \
+    }                                                                                           \
     if (t.stage == 1)                                                                           \
       hipLaunchKernelGGL((k_blk_spmm_lds<T, B, CW_, LG_, MODE>), grid, dim3(kBT), 0, s, A, ldk, \
                          t.xcd, X, Y, F, act, part);                                            \
@@ -1248,7 +1255,7 @@ std::vector<BlkTune> candidates(int ldk, int B) {
   std::vector<BlkTune> c;
   auto add = [&](int cw, int lg) {
     for (int stage : {2, 1, 0}) {
-      if (stage == 2 && (lg != 16 || B == 1)) continue;  // DPP rows are 16 lanes
+      if (stage == 2 && (!has_group_bcast(lg) || B == 1)) continue;
       for (int xcd : {1, 0})
         for (int nrb : {2048, 1024}) {
           BlkTune t;
