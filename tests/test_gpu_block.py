@@ -171,6 +171,40 @@ def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
         assert O.true_relative_residual(oc2, U2[:, j], G[:, j]) < 1e-8
 
 
+@pytest.mark.parametrize("k,cfg", [(8, "1,8,1024,0,2"), (8, "2,4,1024,1,2"), (16, "2,8,1024,0,2"),
+                                   (16, "1,16,1024,1,2"), (64, "4,16,1024,0,2"),
+                                   (64, "8,8,1024,1,2"), (5, "2,4,1024,0,2")])
+def test_block_broadcast_widths(gpu_backend_factory, monkeypatch, k, cfg):
+    """The register-broadcast SpMM at the other column widths (groups of 4,
+    8 and 16 lanes; k = 5 pads to 8 columns): the shell tube (BSR6) in FP64
+    and mixed precision and a jittered hex (BSR3) in mixed precision, each
+    column to the tolerance against the FP64 oracle operator."""
+    monkeypatch.setenv("VBH_BLOCK_TUNE", cfg)
+    m = decks.cquad4_tube(n_circ=16, n_axial=24, n_cases=k)
+    be = gpu_backend_factory(tolerance=1e-10, max_iters=50000)
+    eq, n = dof_map_device(be, m)
+    K = assemble_model(be, m, eq, n)
+    F = np.asfortranarray(np.stack([M.assemble_rhs(m, eq, n, lc) for lc in m.loads], axis=1))
+    U = be.solve_block(K, F)
+    used = be.block_config()
+    assert (used.cols_per_lane, used.lanes_per_group, used.value_path) == \
+        tuple(int(v) for v in (cfg.split(",")[:2] + cfg.split(",")[4:]))
+    oc = O.assemble_model(n, m, eq)
+    for j in sorted({0, k // 2, k - 1}):
+        assert O.true_relative_residual(oc, U[:, j], F[:, j]) < 2e-10
+    bm = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=50000)
+    Um = bm.solve_block(csr_of(oc), F)
+    for j in sorted({0, k // 2, k - 1}):
+        assert O.true_relative_residual(oc, Um[:, j], F[:, j]) < 1e-8
+    hm = decks.config2_hex_jitter(nx=10, ny=4, nz=4, seed=7)
+    eq2, n2 = M.analysis_dof_map(hm)
+    oc2 = O.assemble_model(n2, hm, eq2)
+    G = np.random.default_rng(3).standard_normal((n2, k))
+    U2 = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=50000).solve_block(csr_of(oc2), G)
+    for j in sorted({0, k // 2, k - 1}):
+        assert O.true_relative_residual(oc2, U2[:, j], G[:, j]) < 1e-8
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_block_column_compaction(gpu_backend_factory, c1, monkeypatch, mixed):
     """32 load cases of which 20 are zero (done at once) and the rest converge
