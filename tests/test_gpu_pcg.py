@@ -171,3 +171,38 @@ def test_mixed_tridiagonal(gpu_backend_factory):  # 412-425 (reference bound 5e-
     F = np.ones(200)
     u = be.solve(csr_of(oc), F)
     assert np.allclose(u, direct(oc, F), atol=5e-3) and be.last_estimated_error() < 1e-4
+
+
+def _sell_cases():
+    """C1 (BSR3), a shell tube (BSR6), and a jittered hex of 40,625 nodes
+    whose node IDs are shuffled, so that block columns lie more than 32,767
+    nodes apart and the sliced ELL keeps 32-bit block columns."""
+    from vibestran_amd.assembly import assemble_model, dof_map_device
+    hx = decks.config2_hex_jitter(nx=64, ny=24, nz=24, seed=3)
+    hx.node_ids = np.random.default_rng(9).permutation(hx.node_ids).astype(np.int32)
+    out = []
+    for m in (decks.config1_hex_cantilever(), decks.cquad4_tube(n_circ=16, n_axial=24, n_cases=1), hx):
+        from vibestran_amd import HipPcgSolverBackend
+        be = HipPcgSolverBackend.try_create()
+        eq, n = dof_map_device(be, m)
+        K = assemble_model(be, m, eq, n).fetch()
+        csr = CsrData.from_arrays(K.row_ptr.copy(), K.col_ind.copy(), K.values.copy(), lower_only=True)
+        out.append((csr, M.assemble_rhs(m, eq, n, m.loads[0])))
+        be.close()
+    return out
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_sell_16bit_block_columns_match_32bit(gpu_backend_factory, monkeypatch, mixed):
+    """The sliced ELL's 16-bit block-column offsets (node0-relative) and its
+    32-bit block columns (VBH_SELL_IDX16=0) give bit-identical solves and
+    iteration counts on the same operator."""
+    for csr, F in _sell_cases():
+        res = []
+        for flag in ("1", "0"):
+            monkeypatch.setenv("VBH_SELL_IDX16", flag)
+            be = gpu_backend_factory(mixed=mixed, tolerance=1e-8, max_iters=50000)
+            u = be.solve(csr, F)
+            res.append((u, be.last_iteration_count()))
+        (u16, it16), (u32, it32) = res
+        assert it16 == it32 and np.array_equal(u16, u32)

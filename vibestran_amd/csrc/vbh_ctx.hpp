@@ -102,6 +102,7 @@ struct Ctx {
   DBuf<double> bsr_v64;
   DBuf<float> bsr_v32;
   DBuf<int> sell_koff, sell_idx;
+  DBuf<short> sell_idx16;
   DBuf<double> sell_v64;
   DBuf<float> sell_v32;
   DBuf<double> sell_dblk;   // FP64 corrected diagonal node blocks (row-sum preserving FP32)

@@ -140,6 +140,10 @@ struct BsrDev {
   int nslices = 0;
   const int* koff = nullptr;
   const int* sidx = nullptr;
+  // The same block columns as 16-bit offsets from node0(s), when every one
+  // fits (banded numberings: a structured hex's neighbours lie within
+  // +-(ny+1)(nz+1) nodes); the SpMVs then read these instead of sidx.
+  const short* sidx16 = nullptr;
   const double* s64 = nullptr;
   const float* s32 = nullptr;
   // Row-sum preserving FP32 stream: when dblk is set, s32 holds zeros in

@@ -140,6 +140,22 @@ __device__ __forceinline__ const double* svals<double>(const BsrDev& A) { return
 template <>
 __device__ __forceinline__ const float* svals<float>(const BsrDev& A) { return A.s32; }
 
+// Slots 0..km-1 of one lane: values at v + k * 64 * B, block column j0 + ix[k * NPS].
+template <typename T, int B, typename Acc, typename IX>
+__device__ __forceinline__ Acc sell_row_ix(const T* __restrict__ v, const T* __restrict__ x,
+                                           const IX* __restrict__ ix, int j0, int km) {
+  constexpr int NPS = sell_nps(B);
+  Acc acc = Acc(0);
+#pragma unroll 4
+  for (int k = 0; k < km; ++k) {
+    const int J = j0 + static_cast<int>(ix[k * NPS]);
+    const T* vk = v + static_cast<size_t>(k) * kSliceRows * B;
+#pragma unroll
+    for (int c = 0; c < B; ++c) acc += static_cast<Acc>(vk[c]) * static_cast<Acc>(x[B * J + c]);
+  }
+  return acc;
+}
+
 // Row g = 64 s + l of the sliced-ELL operator times x (one lane, no reduction),
 // accumulated in Acc.
 template <typename T, int B, typename Acc = T>
@@ -150,16 +166,11 @@ __device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ v
   const int node = (kSliceRows * s + l) / B;
   const int node0 = (kSliceRows * s) / B;
   const T* v = val + (static_cast<size_t>(k0) * kSliceRows + l) * B;
-  const int* ix = A.sidx + static_cast<size_t>(k0) * NPS + (node - node0);
-  Acc acc = Acc(0);
-#pragma unroll 4
-  for (int k = 0; k < km; ++k) {
-    const int J = ix[k * NPS];
-    const T* vk = v + static_cast<size_t>(k) * kSliceRows * B;
-#pragma unroll
-    for (int c = 0; c < B; ++c) acc += static_cast<Acc>(vk[c]) * static_cast<Acc>(x[B * J + c]);
-  }
-  return acc;
+  if (A.sidx16)  // 16-bit offsets from node0
+    return sell_row_ix<T, B, Acc>(v, x, A.sidx16 + static_cast<size_t>(k0) * NPS + (node - node0),
+                                  node0, km);
+  return sell_row_ix<T, B, Acc>(v, x, A.sidx + static_cast<size_t>(k0) * NPS + (node - node0), 0,
+                                km);
 }
 
 // FP64 diagonal node block of row g (row-sum preserving FP32 stream, see

@@ -61,6 +61,30 @@ __global__ void k_sell_fill(int nslices, int nbr, const int* __restrict__ brp,
   }
 }
 
+// 16-bit form of the slot block columns: J - node0(s); entries of rows past
+// the last node (padding, zero values) become 0. *bad = 1 when an offset does
+// not fit.
+template <int B>
+__global__ void k_sell_idx16(int nslices, int nbr, const int* __restrict__ koff,
+                             const int* __restrict__ sidx, short* __restrict__ idx16,
+                             int* __restrict__ bad) {
+  const int s = blockIdx.x;
+  const int l = threadIdx.x;
+  constexpr int NPS = sell_nps(B);
+  if (s >= nslices || l >= NPS) return;
+  const int node0 = (kSliceRows * s) / B;
+  const int I = node0 + l;
+  for (int k = koff[s]; k < koff[s + 1]; ++k) {
+    const size_t at = static_cast<size_t>(k) * NPS + l;
+    int d = (I < nbr) ? sidx[at] - node0 : 0;
+    if (d < -32768 || d > 32767) {
+      *bad = 1;
+      d = 0;
+    }
+    idx16[at] = static_cast<short>(d);
+  }
+}
+
 // Diagonal node block of block row I in FP64 plus the symmetrised correction
 // (e + e^T)/2, e[r][c] = sum over the off-diagonal blocks of the row of
 // (v - (double)(float)v) at entry (r, c): with FP32 off-diagonal blocks and
@@ -183,6 +207,28 @@ void build_sell(Ctx& c) {
     c.sell_lrows.upload(lr.data(), lr.size(), s);
     A.nlong = static_cast<int>(lr.size());
     A.lrows = lr.empty() ? nullptr : c.sell_lrows.p;
+  }
+  // 16-bit block columns when every offset fits (VBH_SELL_IDX16=0 keeps the
+  // 32-bit ones)
+  A.sidx16 = nullptr;
+  c.sell_idx16.release();
+  const char* i16_env = std::getenv("VBH_SELL_IDX16");
+  if (!(i16_env && i16_env[0] == '0')) {
+    c.sell_idx16.resize(static_cast<size_t>(ktot) * sell_nps(B) + 1);
+    DBuf<int> bad;
+    bad.resize(1);
+    VBH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    if (B == 3)
+      hipLaunchKernelGGL(k_sell_idx16<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_idx.p, c.sell_idx16.p, bad.p);
+    else
+      hipLaunchKernelGGL(k_sell_idx16<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_idx.p, c.sell_idx16.p, bad.p);
+    int h_bad = 0;
+    VBH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    VBH_HIP(hipStreamSynchronize(s));
+    if (h_bad) c.sell_idx16.release();
+    else A.sidx16 = c.sell_idx16.p;
   }
   VBH_HIP(hipStreamSynchronize(s));
   A.nslices = nslices;
