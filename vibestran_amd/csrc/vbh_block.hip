@@ -1123,11 +1123,13 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
       // like the block values: lane l of the group has entries l, l + 16, ...
       constexpr int ND = dblk_lanes<T, B, LG, MODE>();
       double dm[ND > 0 ? ND : 1];
-      if (ND > 0 && dblk) {
+      if constexpr (ND > 0) {
+        if (dblk) {
 #pragma unroll
-        for (int j = 0; j < ND; ++j) {
-          const int f = lg + LG * j;
-          dm[j] = (f < NB2) ? dblk[static_cast<size_t>(I) * NB2 + f] : 0.0;
+          for (int j = 0; j < ND; ++j) {
+            const int f = lg + LG * j;
+            dm[j] = (f < NB2) ? dblk[static_cast<size_t>(I) * NB2 + f] : 0.0;
+          }
         }
       }
       static_for<B>([&](auto rc) {
@@ -1136,19 +1138,21 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
         T yo[CW];
 #pragma unroll
         for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(acc[r][w]);
-        if (ND > 0 && dblk) {  // y = acc + D_II x_I in FP64
-          double dsum[CW];
+        if constexpr (ND > 0) {
+          if (dblk) {  // y = acc + D_II x_I in FP64
+            double dsum[CW];
 #pragma unroll
-          for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
-          static_for<B>([&](auto sc) {
-            constexpr int i = r * B + decltype(sc)::value;
-            const double dv = group_bcast<LG, i % LG>(dm[i / LG]);
+            for (int w = 0; w < CW; ++w) dsum[w] = static_cast<double>(acc[r][w]);
+            static_for<B>([&](auto sc) {
+              constexpr int i = r * B + decltype(sc)::value;
+              const double dv = group_bcast<LG, i % LG>(dm[i / LG]);
 #pragma unroll
-            for (int w = 0; w < CW; ++w)
-              dsum[w] += dv * static_cast<double>(xI[decltype(sc)::value][w]);
-          });
+              for (int w = 0; w < CW; ++w)
+                dsum[w] += dv * static_cast<double>(xI[decltype(sc)::value][w]);
+            });
 #pragma unroll
-          for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(dsum[w]);
+            for (int w = 0; w < CW; ++w) yo[w] = static_cast<T>(dsum[w]);
+          }
         }
         vstore<T, CW>(Y + idx, yo);
 #pragma unroll
