@@ -1,10 +1,12 @@
 """GPU adjoint sensitivities through the C ABI: the element contraction kernel
 against the FP64 CPU adjoint (oracle) on random vectors, and compliance /
 tip-displacement gradients through a torch.autograd.Function checked by
-central finite differences on random elements (h = 1e-4 E for the SIMP block:
-truncation O(h^2) ~ 1e-8, while the assembly's atomic summation order moves
-each re-assembled K in its last bits, noise that 1e-6 E amplified to the
-tolerance on this ill-conditioned block)."""
+central finite differences on random elements. The device assembly's atomic
+summation order moves each re-assembled K in its last bits, and this
+ill-conditioned SIMP block (stiffness contrast 2e8) amplifies that into the
+difference quotients: the density FD (h = 1e-6) therefore evaluates C(rho +- h)
+on the oracle's deterministic FP64 assembly (solved on the GPU), and the
+modulus FDs use h = 1e-4 E (truncation O(h^2) ~ 1e-8)."""
 import numpy as np
 import pytest
 
@@ -14,6 +16,7 @@ from vibestran_amd import model as M
 from vibestran_amd.assembly import (assemble_model, compliance_gradients, dof_map_device,
                                     element_sensitivity)
 from tests.test_gpu_assembly import mixed_mesh
+from tests.test_gpu_pcg import csr_of
 from tests.test_gpu_shell import warped_mesh
 
 pytestmark = pytest.mark.gpu
@@ -83,12 +86,19 @@ def test_simp_compliance_autograd_vs_fd(gpu_backend_factory):
     go = O.element_sensitivity(n, mm, eq, 0, u, u) * 3.0 * m.meta["density"] ** 2 * (m.meta["E0"] - m.meta["Emin"])
     assert np.abs(grad.numpy() - go).max() <= 1e-10 * np.abs(go).max()
     h = 1e-6
+
+    def compliance(r):  # C(rho) on the deterministic oracle assembly, solved on the GPU
+        mr = M.Model(**{**m.__dict__, "E": m.meta["Emin"] + r ** 3 * (m.meta["E0"] - m.meta["Emin"])})
+        return float(F @ be.solve(csr_of(O.assemble_model(n, mr, eq)), F))
+
+    r0 = m.meta["density"]
+    assert abs(compliance(r0) - C.item()) <= 1e-9 * abs(C.item())
     for e in np.random.default_rng(5).choice(m.ne, 32, replace=False):
-        rp, rm = rho.detach().clone(), rho.detach().clone()
+        rp, rm = r0.copy(), r0.copy()
         rp[e] += h
         rm[e] -= h
-        fd = (SimpCompliance.apply(rp, be, m, eq, n, F) - SimpCompliance.apply(rm, be, m, eq, n, F)) / (2 * h)
-        assert abs(grad[e].item() - fd.item()) <= 1e-4 * abs(fd.item()) + 1e-7 * grad.abs().max().item(), e
+        fd = (compliance(rp) - compliance(rm)) / (2 * h)
+        assert abs(grad[e].item() - fd) <= 1e-4 * abs(fd) + 1e-7 * grad.abs().max().item(), e
 
 
 def test_simp_tip_displacement_gradient_fd(gpu_backend_factory):
