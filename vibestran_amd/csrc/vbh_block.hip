@@ -1021,11 +1021,9 @@ constexpr int dblk_lanes() {
 // FP32 values with at most 2 load cases per lane: at least 4 waves per SIMD
 // (<= 128 VGPRs, which those shapes fit without spilling)
 template <typename T, int B, int CW, int LG, int MODE>
-__global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_blk_spmm_bcast(BlkOp A, int ldk, int xcd,
-                                                      const T* __restrict__ X, T* __restrict__ Y,
-                                                      const double* __restrict__ F,
-                                                      const int* __restrict__ act,
-                                                      double* __restrict__ part) {
+__global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_blk_spmm_bcast(
+    BlkOp A, int ldk, int xcd, const T* __restrict__ X, T* __restrict__ Y,
+    const double* __restrict__ F, const int* __restrict__ act, double* __restrict__ part) {
   constexpr int KT = CW * LG, GR = 64 / LG, NB2 = B * B, NL = (NB2 + LG - 1) / LG;
   __shared__ double lds[kBT * CW];
   const int lane = threadIdx.x & 63;
