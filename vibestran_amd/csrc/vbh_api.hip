@@ -307,7 +307,7 @@ PcgParams Ctx::params() const {
   PcgParams p;
   p.tol = tol;
   p.max_iters = max_iters;
-  p.refresh_delta = refresh_delta;
+  p.refresh_delta = delta();
   return p;
 }
 

@@ -1493,7 +1493,7 @@ static void enqueue_block_iterations(Ctx& c, BlockState& b, int precision, int c
 
 static hipGraphExec_t block_graph(Ctx& c, BlockState& b, int precision) {
   if (b.graph && b.g_prec == precision && b.g_ipg == c.iters_per_graph && b.g_tol == c.tol &&
-      b.g_max_iters == c.max_iters && b.g_delta == c.refresh_delta && b.g_cw == b.cur.cw &&
+      b.g_max_iters == c.max_iters && b.g_delta == c.delta() && b.g_cw == b.cur.cw &&
       b.g_lg == b.cur.lg && b.g_nrb == b.cur.nrb && b.g_xcd == b.cur.xcd &&
       b.g_stage == b.cur.stage)
     return b.graph;
@@ -1508,7 +1508,7 @@ static hipGraphExec_t block_graph(Ctx& c, BlockState& b, int precision) {
   b.g_ipg = c.iters_per_graph;
   b.g_tol = c.tol;
   b.g_max_iters = c.max_iters;
-  b.g_delta = c.refresh_delta;
+  b.g_delta = c.delta();
   b.g_cw = b.cur.cw;
   b.g_lg = b.cur.lg;
   b.g_nrb = b.cur.nrb;
