@@ -160,3 +160,133 @@ def test_e2e_solid_axial_decks(deck):
             assert abs(val - c["expected"]) <= c["abs_tol"]
         nchk += 1
     assert nchk >= 1
+
+
+# ---- pins of the two EAS formulations ----------------------------------------------
+# Pure bending. On a rectangular prism the trilinear field plus the nine
+# enhanced modes contains the exact 3D pure-bending field (sigma_xx = E k z,
+# every other stress 0):
+#     u = k x z,   v = -nu k y z,   w = -k/2 (x^2 + nu (z^2 - y^2)),
+# whose quadratic parts are exactly the incompatible bubbles the modes stand
+# for (reference solid_elements.cpp:1095-1163: modes xi, eta, zeta in each of
+# x, y, z). The exact stress is orthogonal to every enhanced mode, so the
+# condensed Ke applied to the nodal values must return the consistent nodal
+# forces of the end tractions +-sigma_xx on the x faces -- exactly. SRI (no
+# enhancement), a mode set missing one direction, or a wrong condensation
+# (Kuu - Kua Kaa^-1 Kau with a sign or transpose slip) leaves an O(1) residual.
+
+
+def _bending_field(X, kap, nu):
+    x, y, z = X.T
+    return np.stack([kap * x * z, -nu * kap * y * z,
+                     -0.5 * kap * (x ** 2 + nu * (z ** 2 - y ** 2))], axis=1).reshape(-1)
+
+
+def _box_end_forces(X, a, b, c, E, kap):
+    """Consistent nodal forces of sigma_xx = E k z on the faces x = +-a/2 of the
+    box [-a/2, a/2] x [-b/2, b/2] x [-c/2, c/2] (2x2 Gauss on each face, exact for
+    the bilinear x linear integrand)."""
+    f = np.zeros((len(X), 3))
+    g = np.array([-1.0, 1.0]) / np.sqrt(3.0)
+    for sgn in (-1.0, 1.0):
+        face = [i for i in range(len(X)) if np.isclose(X[i, 0], sgn * a / 2)]
+        for s in g:
+            for t in g:
+                z = t * c / 2
+                for i in face:
+                    Ny = 0.5 * (1 + np.sign(X[i, 1]) * s)
+                    Nz = 0.5 * (1 + np.sign(X[i, 2]) * t)
+                    f[i, 0] += sgn * E * kap * z * Ny * Nz * (b / 2) * (c / 2)
+    return f.reshape(-1)
+
+
+@pytest.mark.parametrize("nu", [0.0, 0.3, 0.45])
+def test_hexa_eas_pure_bending_exact(nu):
+    E, kap, (a, b, c) = 2.1e5, 1e-3, (2.0, 1.5, 1.0)
+    X = (2 * CUBE - 1) * np.array([a / 2, b / 2, c / 2])  # centred box, CHEXA8 order
+    f = _box_end_forces(X, a, b, c, E, kap)
+    u = _bending_field(X, kap, nu)
+    r_eas = O.hexa8_ke(X, E, nu, "EAS") @ u - f
+    assert np.abs(r_eas).max() < 1e-12 * np.abs(f).max()
+    # the check has teeth: SRI (no enhanced modes) misses the bending field by O(1)
+    r_sri = O.hexa8_ke(X, E, nu, "SRI") @ u - f
+    assert np.abs(r_sri).max() > 0.2 * np.abs(f).max()
+
+
+def test_penta_eas_pure_bending_exact():
+    # Wedge of the unit right triangle x unit height, bending about y with
+    # sigma_zz = E k x (nu = 0): u = -k z^2 / 2, w = k x z. The z^2 term is the
+    # zeta bubble; the in-plane field is linear, so the EAS wedge (modes
+    # L1 - 1/3, L2 - 1/3, zeta; solid_elements.cpp:1333-1467) is exact.
+    E, kap = 7.0e4, 2e-3
+    X = WEDGE.copy()
+    x, z = X[:, 0], X[:, 2]
+    u = np.stack([-0.5 * kap * z ** 2, 0 * x, kap * x * z], axis=1).reshape(-1)
+    # consistent forces of +-sigma_zz on the two triangles (3-point midpoint
+    # rule: exact for L_a * x, a quadratic)
+    f = np.zeros((6, 3))
+    mids = [(0.5, 0.5, 0.0), (0.0, 0.5, 0.5), (0.5, 0.0, 0.5)]  # (L_node0, L_node1, L_node2)
+    tri = X[:3, :2]
+    for top, sgn in ((0, -1.0), (3, 1.0)):
+        for L in mids:
+            xq = sum(L[k] * tri[k, 0] for k in range(3))
+            for k in range(3):
+                f[top + k, 2] += sgn * E * kap * xq * L[k] * 0.5 / 3
+    f = f.reshape(-1)
+    r = O.penta6_ke(X, E, 0.0) @ u - f
+    assert np.abs(r).max() < 1e-12 * np.abs(f).max()
+
+
+# Frame objectivity. A formulation whose enhanced-strain directions are the
+# physical gradients of the parametric coordinates (the COLUMNS of J0^-1,
+# d xi_k / dx) is rotation covariant: Ke(R X) = R_b Ke(X) R_b^T with R_b the
+# block-diagonal node rotation. The reference's CHEXA8 EAS takes the ROWS of
+# J0^-1 instead (solid_elements.cpp:1131-1133), which rotate as R J0^-1 only
+# row-mixed: under a general rotation of a cube the mode space changes, while
+# under a symmetric rotation (180 degrees about any axis, R = R^T) rows and
+# columns coincide and covariance returns. CPENTA6 EAS uses the columns
+# (1333-1467) and is covariant on any wedge. Swapping rows and columns in
+# either element fails one of the assertions below.
+
+def _rot_generic():
+    w = np.array([0.3, -0.5, 0.7])
+    th = np.linalg.norm(w)
+    k = w / th
+    Kx = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(th) * Kx + (1 - np.cos(th)) * Kx @ Kx
+
+
+def _rot_half_turn():
+    n = np.array([1.0, 2.0, -0.5])
+    n /= np.linalg.norm(n)
+    return 2 * np.outer(n, n) - np.eye(3)
+
+
+def _covariance_defect(fn, X, R):
+    K = fn(X)
+    Kr = fn(X @ R.T)
+    Rb = np.kron(np.eye(len(X)), R)
+    return np.abs(Kr - Rb @ K @ Rb.T).max() / np.abs(K).max()
+
+
+SHEAR = np.array([[1.0, 0.3, 0.2], [0.0, 1.0, 0.25], [0.0, 0.0, 1.0]])
+
+
+def test_hexa_eas_uses_rows_of_inverse_centroidal_jacobian():
+    hexa = lambda X: O.hexa8_ke(X, 2.1e5, 0.3, "EAS")  # noqa: E731
+    cube = 2 * CUBE - 1
+    assert _covariance_defect(hexa, cube, _rot_half_turn()) < 1e-12
+    assert _covariance_defect(hexa, cube, _rot_generic()) > 1e-2
+    assert _covariance_defect(hexa, cube @ SHEAR.T, _rot_generic()) > 1e-2
+    # SRI has no enhanced modes: covariant on the same shapes
+    sri = lambda X: O.hexa8_ke(X, 2.1e5, 0.3, "SRI")  # noqa: E731
+    assert _covariance_defect(sri, cube @ SHEAR.T, _rot_generic()) < 1e-12
+
+
+@pytest.mark.parametrize("shape", ["right", "sheared", "distorted"])
+def test_penta_eas_is_rotation_covariant(shape):
+    X = {"right": WEDGE, "sheared": WEDGE @ SHEAR.T,
+         "distorted": distorted(WEDGE, 11, 0.08)}[shape]
+    penta = lambda Y: O.penta6_ke(Y, 2.1e5, 0.3)  # noqa: E731
+    assert _covariance_defect(penta, X, _rot_generic()) < 1e-12
+    assert _covariance_defect(penta, X, _rot_half_turn()) < 1e-12
