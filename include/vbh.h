@@ -342,6 +342,14 @@ int vbh_spmv(vbh_ctx* ctx, const double* x, double* y);
  * back between one event pair (kernel plus dispatch gap, no per-launch event
  * cost). 2..5 and 7..14 use the F of the last solve. */
 int vbh_bench_spmv(vbh_ctx* ctx, int which, int reps, double* ms_per_launch);
+/* Bytes one launch of the single-load-case solver SpMV reads from the device
+ * operator in the layout it actually streams (no reference counterpart;
+ * measurement): precision VBH_PREC_FP64 or VBH_PREC_MIXED (the FP32 stream).
+ * Sliced ELL: padded block values, block columns (16- or 32-bit), slice
+ * offsets and, for the split FP32 stream, the FP64 diagonal node blocks;
+ * scalar CSR: values, column indices and row pointers. Vector streams (p
+ * gathered, Ap written) are not included. */
+int vbh_operator_stream_bytes(const vbh_ctx* ctx, int precision, long long* bytes);
 
 /* ---- device memory helpers for device-resident workflows ---------------- */
 int vbh_device_alloc(vbh_ctx* ctx, long long bytes, void** dptr);

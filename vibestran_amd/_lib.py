@@ -148,6 +148,7 @@ SIGNATURES = {
     "vbh_last_stats": (_I, [_P, C.POINTER(vbh_solve_stats)]),
     "vbh_spmv": (_I, [_P, _DP, _DP]),
     "vbh_bench_spmv": (_I, [_P, _I, _I, _DP]),
+    "vbh_operator_stream_bytes": (_I, [_P, _I, C.POINTER(_LL)]),
     "vbh_device_alloc": (_I, [_P, _LL, C.POINTER(_P)]),
     "vbh_device_free": (_I, [_P, _P]),
     "vbh_memcpy_h2d": (_I, [_P, _P, _P, _LL]),

@@ -816,6 +816,32 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
   });
 }
 
+int vbh_operator_stream_bytes(const vbh_ctx* h, int precision, long long* bytes) {
+  if (!h || !bytes || (precision != VBH_PREC_FP64 && precision != VBH_PREC_MIXED)) {
+    vbh::set_error("vbh_operator_stream_bytes: bad argument");
+    return VBH_ERR_ARG;
+  }
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c->have_matrix) {
+    vbh::set_error("no matrix set");
+    return VBH_ERR_STATE;
+  }
+  const long long sv = precision == VBH_PREC_FP64 ? 8 : 4;
+  if (c->format == 0) {
+    *bytes = static_cast<long long>(c->csr.nnz) * (sv + 4) + 4ll * (c->csr.n + 1);
+    return VBH_OK;
+  }
+  const vbh::BsrDev& A = c->bsrd;
+  const long long B = A.B;
+  const long long slots = c->sell_slots;
+  long long b = slots * vbh::kSliceRows * B * sv;                        // values
+  b += slots * vbh::sell_nps(A.B) * (A.sidx16 ? 2 : 4);                 // block columns
+  b += 4ll * (A.nslices + 1);                                           // slice offsets
+  if (precision == VBH_PREC_MIXED && A.dblk) b += static_cast<long long>(A.nbr) * B * B * 8;
+  *bytes = b;
+  return VBH_OK;
+}
+
 int vbh_device_alloc(vbh_ctx* h, long long bytes, void** dptr) {
   if (!h || !dptr || bytes < 0) {
     vbh::set_error("vbh_device_alloc: bad argument");

@@ -116,6 +116,7 @@ struct Ctx {
   DBuf<int> sell_lrows;     // block rows longer than kSellCap blocks
   int rowsum_fp32 = 1;      // split the FP32 stream as above (VBH_ROWSUM=0 disables)
   long long sell_padded_blocks = 0;
+  long long sell_slots = 0;  // block slots of all slices (sum of the slice widths)
   DBuf<double> diag64;
   DBuf<float> diag32;
   CsrDev csr;

@@ -237,6 +237,7 @@ void build_sell(Ctx& c) {
   A.s64 = c.sell_v64.p;
   A.s32 = c.sell_v32.p;
   c.sell_padded_blocks = static_cast<long long>(ktot) * kSliceRows / B;
+  c.sell_slots = ktot;
 }
 
 }  // namespace vbh
