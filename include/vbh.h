@@ -297,6 +297,21 @@ const char* vbh_deck_unsupported(const vbh_deck* deck, int i);
 int vbh_solve(vbh_ctx* ctx, const double* F, double* u, int* iters, double* rel_res);
 /* Same with device-resident F and u (HBM pointers); no host copies. */
 int vbh_solve_device(vbh_ctx* ctx, const double* dF, double* du, int* iters, double* rel_res);
+/* vbh_solve_device without any device allocation. No reference
+ * counterpart: it is for callers that cannot allocate device memory during a
+ * solve (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, results and errors as
+ * vbh_solve_device on the operator set up before (vbh_set_csr /
+ * vbh_assemble, which may allocate); the solver's work buffers live in one
+ * static device array of 96 MiB per GPU, taken at vbh_ctx_create, and the
+ * iterations are launched directly (no graph). Operators of more than
+ * about 2,093,000 rows in FP64 (2,511,000 in mixed precision: 48 / 40 bytes
+ * per row after 197 KiB of state and partial sums) are rejected with
+ * VBH_ERR_ARG. Every call on one GPU shares that
+ * GPU's array and keeps the last call's contents: calls that overlap on one
+ * GPU (other threads or contexts) race. */
+int vbh_solve_device_noalloc(vbh_ctx* ctx, const double* dF, double* du, int* iters,
+                             double* rel_res);
 /* Many load cases sharing K in one fused block-PCG (no reference counterpart:
  * the reference re-assembles and re-solves per SUBCASE, linear_static.cpp:202-320).
  * F and U are column-major n x k host arrays; iters/rel_res have length k. */

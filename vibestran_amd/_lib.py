@@ -142,6 +142,7 @@ SIGNATURES = {
     "vbh_element_sensitivity_device": (_I, [_P, _I, _P, _P, _P]),
     "vbh_solve": (_I, [_P, _DP, _DP, _IP, _DP]),
     "vbh_solve_device": (_I, [_P, _P, _P, _IP, _DP]),
+    "vbh_solve_device_noalloc": (_I, [_P, _P, _P, _IP, _DP]),
     "vbh_solve_block": (_I, [_P, _I, _DP, _DP, _IP, _DP]),
     "vbh_solve_block_device": (_I, [_P, _I, _P, _P, _IP, _DP]),
     "vbh_last_block_config": (_I, [_P, C.POINTER(vbh_block_config)]),

@@ -495,6 +495,10 @@ int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
     VBH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     VBH_HIP(hipEventCreate(&c->ev_start));
     VBH_HIP(hipEventCreate(&c->ev_stop));
+    // set-up for the allocation-free solve: the pinned state word buffer and
+    // the device address of this device's solver scratch array
+    VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->st_host), sizeof(vbh::PcgScalars), 0));
+    vbh::scratch_init(device);
     *out = reinterpret_cast<vbh_ctx*>(c.release());
   });
 }

@@ -196,6 +196,10 @@ void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, cons
 void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
                            PcgScalars* st, const CgRed& red, const PcgParams& prm, hipStream_t s);
 
+// Device address of the allocation-free solver's scratch array on `device`
+// (current), taken once per device (vbh_noalloc.hip).
+void scratch_init(int device);
+
 // Builds the sliced-ELL arrays of a device BSR (vbh_sell.hip).
 struct Ctx;
 void build_sell(Ctx& c);
