@@ -9,7 +9,8 @@ import pytest
 import oracle as O
 from vibestran_amd import bdf, decks
 from vibestran_amd import model as M
-from vibestran_amd.assembly import assemble_model, assembly_stats, dof_map_device
+from vibestran_amd.assembly import (assemble_model, assemble_rhs_device, assembly_stats,
+                                    dof_map_device)
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "e2e")
@@ -143,3 +144,30 @@ def test_assembled_solve_matches_oracle(gpu_backend_factory):
     u = be.solve(K, F)
     ud = np.load(os.path.join(os.path.dirname(GOLD), "c1_direct.npz"))["u"]
     assert np.abs(u - ud).max() / np.abs(ud).max() < 1e-8
+
+
+@pytest.mark.parametrize("mesh", ["hex-jitter", "tube", "mixed"])
+def test_assembly_is_deterministic(gpu_backend_factory, mesh):
+    """Every CSR slot is summed in element order (no atomics), so two
+    assemblies of the same mesh -- in fresh contexts -- are bit-identical, and
+    so are the mixed-precision solves on them (iterations and u)."""
+    m = {"hex-jitter": lambda: decks.config2_hex_jitter(nx=24, ny=10, nz=10, seed=8),
+         "tube": lambda: decks.cquad4_tube(n_circ=24, n_axial=30, n_cases=1),
+         "mixed": lambda: decks.mixed_block(nx=6, ny=4, nz=4, n_cases=1)}[mesh]()
+    runs = []
+    F = None
+    for _ in range(2):
+        be = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=100000)
+        eq, n = dof_map_device(be, m)
+        K = assemble_model(be, m, eq, n)
+        Kh = K.fetch()
+        if F is None:  # one load vector for both solves (GRAV is assembled on the device)
+            F = assemble_rhs_device(be, m, eq, n, m.loads[0])
+        u = be.solve(K, F)
+        runs.append((Kh.row_ptr.copy(), Kh.col_ind.copy(), Kh.values.copy(), u,
+                     be.last_iteration_count()))
+        be.close()
+    (rp0, ci0, v0, u0, it0), (rp1, ci1, v1, u1, it1) = runs
+    assert np.array_equal(rp0, rp1) and np.array_equal(ci0, ci1)
+    assert np.array_equal(v0.view(np.int64), v1.view(np.int64))
+    assert it0 == it1 and np.array_equal(u0, u1)

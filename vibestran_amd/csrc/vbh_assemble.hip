@@ -10,20 +10,29 @@
 //   1. element DOF lists           (solid_global_dof_indices, solid_elements.cpp:29-42)
 //   2. topological lower pattern   every (max(gi,gj), min(gi,gj)) pair of every
 //                                  element, radix-sorted and de-duplicated
-//   3. element matrices + scatter  one wave64 per element; each lane owns a few
+//   3. element matrices           one wave64 per element; each lane owns a few
 //                                  lower entries, accumulated with the closed
 //                                  isotropic form lam*a_p*b_q + mu*a_q*b_p +
 //                                  mu*delta_pq*(a.b) (== B^T D B for the
 //                                  isotropic D of solid_elements.cpp:17-27);
 //                                  EAS modes condensed through a 9x9 Cholesky in
-//                                  LDS (solid_elements.cpp:1201-1204). Entries
-//                                  that are exactly 0.0 are not added, and each
-//                                  CSR slot records whether any element gave it
-//                                  a nonzero (sparse_matrix.cpp:30-33).
-//   4. compaction                  slots nobody touched with a nonzero vanish,
-//                                  so the pattern equals the reference's.
-// Summation order inside a slot follows atomic arrival (the reference's TBB
-// merge order is also thread-dependent, assembly_parallel.hpp:84-100).
+//                                  LDS (solid_elements.cpp:1201-1204). Every
+//                                  entry is stored at its position in the pair
+//                                  list of pass 2 (coalesced, no atomics).
+//   3b. slot sums                  the pair keys were sorted together with
+//                                  their positions (stable radix sort), so each
+//                                  CSR slot's contributions form one run in
+//                                  element order; one thread per slot sums the
+//                                  run's nonzero entries in that order. Entries
+//                                  that are exactly 0.0 are not added, and a
+//                                  slot without any nonzero is marked empty
+//                                  (sparse_matrix.cpp:30-33).
+//   4. compaction                  empty slots vanish, so the pattern equals
+//                                  the reference's.
+// K is therefore bit-identical from run to run: every slot is summed in
+// element order (element kind groups in turn: CHEXA8, CPENTA6, CTETRA4,
+// CQUAD4), one fixed order of the serial assembly (the reference's TBB merge
+// order is thread-dependent, assembly_parallel.hpp:84-100).
 #include <cstring>
 #include <numeric>
 
@@ -92,12 +101,16 @@ __global__ void k_gdofs(int nelem, const int* __restrict__ etype, const int* __r
 }
 
 // ---- 2. topological pattern keys ----------------------------------------------------------
+// keys[idx] = (row << bits) | col of the element's lower pair, src[idx] =
+// off + idx: the pair's position in the concatenated list of all kinds.
 template <int ND>
 __global__ void k_pair_keys(int nsel, const int* __restrict__ sel, const int* __restrict__ gd,
-                            unsigned long long* __restrict__ keys, int bits) {
+                            unsigned long long* __restrict__ keys, int bits,
+                            unsigned* __restrict__ src = nullptr, unsigned off = 0) {
   constexpr int NL = ND * (ND + 1) / 2;
   const long long idx = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (idx >= static_cast<long long>(nsel) * NL) return;
+  if (src) src[idx] = off + static_cast<unsigned>(idx);
   const int s = static_cast<int>(idx / NL), t = static_cast<int>(idx % NL);
   const unsigned short pr = pair_at<ND>(t);
   const int e = sel[s];
@@ -122,13 +135,48 @@ __global__ void k_mark_unique(long long m, const unsigned long long* __restrict_
 
 __global__ void k_compact_keys(long long m, const unsigned long long* __restrict__ keys,
                                const int* __restrict__ head, const int* __restrict__ pos,
-                               int bits, int* __restrict__ row_of, int* __restrict__ col) {
+                               int bits, int* __restrict__ row_of, int* __restrict__ col,
+                               long long* __restrict__ seg = nullptr) {
   const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (i >= m || !head[i]) return;
   const unsigned long long k = keys[i];
   const int p = pos[i];
   row_of[p] = static_cast<int>(k >> bits);
   col[p] = static_cast<int>(k & ((1ull << bits) - 1ull));
+  if (seg) seg[p] = i;
+}
+
+// seg[cnt] = number of keys before the first sentinel (sorted keys).
+__global__ void k_seg_end(long long m, const unsigned long long* __restrict__ keys,
+                          unsigned long long sentinel, int cnt, long long* __restrict__ seg) {
+  long long lo = 0, hi = m;  // first index with keys[i] == sentinel
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (keys[mid] < sentinel) lo = mid + 1;
+    else hi = mid;
+  }
+  seg[cnt] = lo;
+}
+
+// Slot sums in element order: slot p's contributions are the sorted pair
+// positions seg[p] .. seg[p+1]-1, whose entries the element kernels stored at
+// ke[src[j]]. Exact zeros are skipped; touched[p] = 1 if any entry was nonzero.
+__global__ void k_slot_sums(int nslots, const long long* __restrict__ seg,
+                            const unsigned* __restrict__ src, const double* __restrict__ ke,
+                            double* __restrict__ val, int* __restrict__ touched) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nslots) return;
+  double acc = 0.0;
+  int any = 0;
+  for (long long j = seg[p]; j < seg[p + 1]; ++j) {
+    const double v = ke[src[j]];
+    if (v != 0.0) {
+      acc += v;
+      any = 1;
+    }
+  }
+  val[p] = acc;
+  touched[p] = any;
 }
 
 // row_ptr from a row-sorted entry list (handles empty rows).
@@ -587,25 +635,13 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
         atomicOr(&err[0], kErrJacobian);
         atomicMin(&err[1], e);
       }
-      const int* g = gd + static_cast<size_t>(e) * kGdStride;
+      // entry t of element s at val[s * NL + t] (its pair position; the
+      // slot sums read only pairs whose DOFs are both free)
+      double* dst = val + static_cast<size_t>(s) * NL;
 #pragma unroll
       for (int q = 0; q < KD::SU; ++q) {
         const int t = lane + 64 * q;
-        if (t < NL) {
-          const unsigned short pr = pair_at<ND>(t);
-          const int gi = g[pr >> 8], gj = g[pr & 0xff];
-          const double v = kuu[q];
-          if (gi >= 0 && gj >= 0 && v != 0.0) {
-            const int row = gi > gj ? gi : gj, col = gi > gj ? gj : gi;
-            const int slot = find_slot(rp, ci, row, col);
-            if (slot < 0) {
-              atomicOr(&err[0], kErrSlot);
-            } else {
-              atomicAdd(&val[slot], v);
-              touched[slot] = 1;
-            }
-          }
-        }
+        if (t < NL) dst[t] = kuu[q];
       }
     }
   }
@@ -788,6 +824,20 @@ void sort_keys(DBuf<unsigned long long>& in, DBuf<unsigned long long>& out, size
   VBH_HIP(hipStreamSynchronize(s));
 }
 
+// Stable radix sort of (key, position) pairs: equal keys keep their positions
+// in increasing order.
+void sort_pairs(DBuf<unsigned long long>& kin, DBuf<unsigned long long>& kout, DBuf<unsigned>& vin,
+                DBuf<unsigned>& vout, size_t m, int end_bit, hipStream_t s) {
+  size_t tmp = 0;
+  VBH_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin.p, kout.p, vin.p, vout.p, m, 0u,
+                                    static_cast<unsigned>(end_bit), s));
+  DBuf<char> t;
+  t.resize(std::max<size_t>(tmp, 1));
+  VBH_HIP(rocprim::radix_sort_pairs(t.p, tmp, kin.p, kout.p, vin.p, vout.p, m, 0u,
+                                    static_cast<unsigned>(end_bit), s));
+  VBH_HIP(hipStreamSynchronize(s));
+}
+
 void excl_scan(const int* in, int* out, size_t m, hipStream_t s) {
   size_t tmp = 0;
   VBH_HIP(rocprim::exclusive_scan(nullptr, tmp, in, out, 0, m, rocprim::plus<int>(), s));
@@ -808,9 +858,11 @@ int read_last_plus(const int* scan, const int* flags, size_t m, hipStream_t s) {
 
 }  // namespace
 
-// Unique sorted (row,col) list from sorted keys: fills row_of/col, returns count.
+// Unique sorted (row,col) list from sorted keys: fills row_of/col, returns
+// count; with seg, also each pair's first position in the sorted keys
+// (seg[count] = position of the first sentinel).
 static int unique_pairs(DBuf<unsigned long long>& keys, size_t m, int bits, DBuf<int>& row_of,
-                        DBuf<int>& col, hipStream_t s) {
+                        DBuf<int>& col, hipStream_t s, DBuf<long long>* seg = nullptr) {
   DBuf<int> head, pos;
   head.resize(std::max<size_t>(m, 1));
   pos.resize(std::max<size_t>(m, 1));
@@ -821,8 +873,13 @@ static int unique_pairs(DBuf<unsigned long long>& keys, size_t m, int bits, DBuf
   const int cnt = read_last_plus(pos.p, head.p, m, s);
   row_of.resize(std::max(cnt, 1));
   col.resize(std::max(cnt, 1));
+  if (seg) seg->resize(static_cast<size_t>(cnt) + 1);
   hipLaunchKernelGGL(k_compact_keys, dim3((m + 255) / 256), dim3(256), 0, s,
-                     static_cast<long long>(m), keys.p, head.p, pos.p, bits, row_of.p, col.p);
+                     static_cast<long long>(m), keys.p, head.p, pos.p, bits, row_of.p, col.p,
+                     seg ? seg->p : nullptr);
+  if (seg)
+    hipLaunchKernelGGL(k_seg_end, dim3(1), dim3(1), 0, s, static_cast<long long>(m), keys.p,
+                       sentinel, cnt, seg->p);
   VBH_HIP(hipStreamSynchronize(s));
   return cnt;
 }
@@ -952,68 +1009,82 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   size_t m = 0;
   const int ndk[5] = {24, 24, 18, 12, 24};
   for (int k = 0; k < 5; ++k) m += groups[k].size() * static_cast<size_t>(ndk[k] * (ndk[k] + 1) / 2);
+  if (m >= (1ull << 32))
+    throw_error(VBH_ERR_ARG, "vbh_assemble: more than 2^32 element matrix entries");
   DBuf<int> t_row, t_col;
+  DBuf<long long> seg;       // slot p's run in the sorted pair list
+  DBuf<unsigned> src_sorted;  // pair positions in sorted key order
+  size_t off_k[5] = {0, 0, 0, 0, 0};
   int nnz_topo = 0;
   {
     DBuf<unsigned long long> keys, sorted;
+    DBuf<unsigned> src;
     keys.resize(m);
     sorted.resize(m);
+    src.resize(m);
+    src_sorted.resize(m);
     size_t off = 0;
     for (int k = 0; k < 5; ++k) {
+      off_k[k] = off;
       const int ns = static_cast<int>(groups[k].size());
       if (!ns) continue;
       const int NL = ndk[k] * (ndk[k] + 1) / 2;
       const long long cnt = static_cast<long long>(ns) * NL;
       const dim3 g(static_cast<unsigned>((cnt + 255) / 256));
+      const unsigned o32 = static_cast<unsigned>(off);
       if (ndk[k] == 24)
-        hipLaunchKernelGGL(k_pair_keys<24>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p, keys.p + off, bits);
+        hipLaunchKernelGGL(k_pair_keys<24>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
+                           keys.p + off, bits, src.p + off, o32);
       else if (ndk[k] == 18)
-        hipLaunchKernelGGL(k_pair_keys<18>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p, keys.p + off, bits);
+        hipLaunchKernelGGL(k_pair_keys<18>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
+                           keys.p + off, bits, src.p + off, o32);
       else
-        hipLaunchKernelGGL(k_pair_keys<12>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p, keys.p + off, bits);
+        hipLaunchKernelGGL(k_pair_keys<12>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
+                           keys.p + off, bits, src.p + off, o32);
       off += static_cast<size_t>(cnt);
     }
-    sort_keys(keys, sorted, m, 2 * bits, s);
+    sort_pairs(keys, sorted, src, src_sorted, m, 2 * bits, s);
     keys.release();
-    nnz_topo = unique_pairs(sorted, m, bits, t_row, t_col, s);
+    src.release();
+    nnz_topo = unique_pairs(sorted, m, bits, t_row, t_col, s, &seg);
   }
-  DBuf<int> t_rp;
-  t_rp.resize(static_cast<size_t>(n_) + 1);
-  hipLaunchKernelGGL(k_row_ptr, dim3((nnz_topo + 1 + 255) / 256), dim3(256), 0, s, nnz_topo, n_,
-                     t_row.p, t_rp.p);
   VBH_HIP(hipEventRecord(e1, s));
 
-  // 3. element matrices + scatter
-  DBuf<double> t_val;
-  DBuf<int> touched;
-  t_val.resize(static_cast<size_t>(nnz_topo));
-  touched.resize(static_cast<size_t>(nnz_topo));
-  dzero(t_val, s);
-  dzero(touched, s);
+  // 3. element matrices (entries at their pair positions), then slot sums
+  DBuf<double> ke;
+  ke.resize(m);
   for (int k = 0; k < 5; ++k) {
     const int ns = static_cast<int>(groups[k].size());
     if (!ns) continue;
+    double* ke_k = ke.p + off_k[k];
     if (k == kQuad4) {
       launch_quad4_stiffness(ns, d_sel[k].p, d_ep.p, d_conn.p, d_xyz.p, d_E.p, d_nu.p, d_t.p,
-                             shell_form, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p, s);
+                             shell_form, d_gd.p, ke_k, d_err.p, s);
       continue;
     }
     const dim3 g(static_cast<unsigned>(std::min((ns + kElemWaves - 1) / kElemWaves, 65535)));
     const dim3 b(64 * kElemWaves);
     if (k == kHexEas)
       hipLaunchKernelGGL(k_element_stiffness<kHexEas>, g, b, 0, s, ns, d_sel[k].p, d_ep.p, d_conn.p,
-                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p);
+                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, nullptr, nullptr, ke_k, nullptr, d_err.p);
     else if (k == kHexSri)
       hipLaunchKernelGGL(k_element_stiffness<kHexSri>, g, b, 0, s, ns, d_sel[k].p, d_ep.p, d_conn.p,
-                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p);
+                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, nullptr, nullptr, ke_k, nullptr, d_err.p);
     else if (k == kPentaEas)
       hipLaunchKernelGGL(k_element_stiffness<kPentaEas>, g, b, 0, s, ns, d_sel[k].p, d_ep.p,
-                         d_conn.p, d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p,
-                         touched.p, d_err.p);
+                         d_conn.p, d_xyz.p, d_E.p, d_nu.p, d_gd.p, nullptr, nullptr, ke_k, nullptr,
+                         d_err.p);
     else
       hipLaunchKernelGGL(k_element_stiffness<kTet4>, g, b, 0, s, ns, d_sel[k].p, d_ep.p, d_conn.p,
-                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, t_rp.p, t_col.p, t_val.p, touched.p, d_err.p);
+                         d_xyz.p, d_E.p, d_nu.p, d_gd.p, nullptr, nullptr, ke_k, nullptr, d_err.p);
   }
+  DBuf<double> t_val;
+  DBuf<int> touched;
+  t_val.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
+  touched.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
+  if (nnz_topo > 0)
+    hipLaunchKernelGGL(k_slot_sums, dim3((nnz_topo + 255) / 256), dim3(256), 0, s, nnz_topo, seg.p,
+                       src_sorted.p, ke.p, t_val.p, touched.p);
   VBH_HIP(hipEventRecord(e2, s));
   int herr[4];
   VBH_HIP(hipMemcpyAsync(herr, d_err.p, sizeof herr, hipMemcpyDeviceToHost, s));

@@ -45,12 +45,12 @@ __device__ __forceinline__ int find_slot(const int* __restrict__ rp, const int* 
   return -1;
 }
 
-// Batched CQUAD4 stiffness + scatter into the lower CSR (rp, ci, val), marking
-// touched slots; `form` 0 = MITC4+, 1 = Mindlin.
+// Batched CQUAD4 stiffness: element s's 300 lower entries (pair order of
+// PairTable<24>) at ke[s * 300 ..]; `form` 0 = MITC4+, 1 = Mindlin.
 void launch_quad4_stiffness(int nsel, const int* sel, const int* eptr, const int* conn,
                             const double* xyz, const double* E, const double* nu,
-                            const double* thick, int form, const int* gd, const int* rp,
-                            const int* ci, double* val, int* touched, int* err, hipStream_t s);
+                            const double* thick, int form, const int* gd, double* ke, int* err,
+                            hipStream_t s);
 
 // out[e] = -u_e^T (dK_e/dp_e) v_e for the selected CQUAD4 elements; param 0:
 // p = E, 1: p = thickness.

@@ -510,7 +510,10 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
         atomicOr(&err[0], bad);
         atomicMin(&err[1], e);
       }
+      // entry `it` of element s at val[s * 300 + it] (its pair position; the
+      // slot sums read only pairs whose DOFs are both free)
       const int* g = gd + static_cast<size_t>(e) * kGdStride;
+      double* dst = val + static_cast<size_t>(s) * 300;
       for (int it = lane; it < 300; it += 64) {
         const unsigned short pr = c_spairs.v[it];
         int I = pr >> 8, J = pr & 0xff;
@@ -532,16 +535,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
           for (int q = 0; q < 3; ++q) inner += klocal(L, ni, la[p], nj, lb[q]) * tb[q];
           v += ta[p] * inner;
         }
-        if (v != 0.0) {
-          const int row = gi > gj ? gi : gj, col = gi > gj ? gj : gi;
-          const int slot = find_slot(rp, ci, row, col);
-          if (slot < 0) {
-            atomicOr(&err[0], kErrSlot);
-          } else {
-            atomicAdd(&val[slot], v);
-            touched[slot] = 1;
-          }
-        }
+        dst[it] = v;
       }
     }
     __syncthreads();
@@ -706,13 +700,13 @@ void launch_quad4_forces(int nsel, const int* sel, const int* eptr, const int* c
 
 void launch_quad4_stiffness(int nsel, const int* sel, const int* eptr, const int* conn,
                             const double* xyz, const double* E, const double* nu,
-                            const double* thick, int form, const int* gd, const int* rp,
-                            const int* ci, double* val, int* touched, int* err, hipStream_t s) {
+                            const double* thick, int form, const int* gd, double* ke, int* err,
+                            hipStream_t s) {
   if (nsel <= 0) return;
   const int blocks = std::min((nsel + kElemWaves - 1) / kElemWaves, 65535);
   hipLaunchKernelGGL(k_quad4<0>, dim3(blocks), dim3(64 * kElemWaves), 0, s, nsel, sel, eptr, conn,
-                     xyz, E, nu, thick, form, gd, rp, ci, val, touched, err, nullptr, nullptr,
-                     nullptr);
+                     xyz, E, nu, thick, form, gd, nullptr, nullptr, ke, nullptr, err, nullptr,
+                     nullptr, nullptr);
 }
 
 void launch_quad4_sensitivity(int nsel, const int* sel, const int* eptr, const int* conn,
