@@ -243,11 +243,33 @@ struct ShellLds {
   double w[4];          // Gauss weights (detJ)
   double c2[8];         // Mindlin centre in-plane shear row
   double w0;
-  double Km[8][8];
-  double Kp[12][12];
+  double Km[8][8];      // local membrane block (both triangles)
+  double Kp[12][12];    // local plate block, columns (rx, ry, w) per node
   double drill;
   double dg[20];  // Kl diagonal before differentiation (thickness mode)
 };
+
+// One entry of the B^T D B product of a B with M strain rows, in the
+// reference's operation order (Eigen-free restatement of the element's
+// K += B^T (D B) w, cquad4.cpp:622-655): (sum_k B[k][i] (sum_l D[k][l] B[l][j])) w,
+// every sum from 0.0 in increasing index order. With floating-point
+// contraction off (this file is compiled with -ffp-contract=off) the entry is
+// bit-identical to the CPU oracle's, so entries that cancel to exactly 0.0
+// there do so here, and the CSR pattern's exact-zero drop matches.
+template <int M, typename BCOL>
+__device__ __forceinline__ double btdb_entry(const double (&D)[M][M], BCOL col, int i, int j,
+                                             double w) {
+  double bi[M], bj[M];
+  col(i, bi);
+  col(j, bj);
+  double s = 0.0;
+  for (int k = 0; k < M; ++k) {
+    double db = 0.0;
+    for (int l = 0; l < M; ++l) db += D[k][l] * bj[l];
+    s += bi[k] * db;
+  }
+  return s * w;
+}
 
 // Bending curvature column of plate column k: (kx, ky, kxy).
 __device__ __forceinline__ void bend_col(const ShellLds& L, int g, int k, double b[3]) {
@@ -333,6 +355,19 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
     const double ts = 0.833333 * t * G;         // PSHELL TS/T = 0.833333
     const double A00 = t * c, A01 = t * c * nu, A22 = t * c * (1.0 - nu) / 2.0;
     const double D00 = ib * c, D01 = ib * c * nu, D22 = ib * c * (1.0 - nu) / 2.0;
+    // section matrices as the reference forms them (cquad4.cpp:189-218):
+    // Dp plane stress, membrane A = t Dp, bending ib Dp, shear tst t G I2
+    const double Dp[3][3] = {{c, c * nu, 0.0}, {c * nu, c, 0.0}, {0.0, 0.0, c * (1.0 - nu) / 2.0}};
+    double Am[3][3], Db[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        Am[a][b] = t * Dp[a][b];
+        Db[a][b] = ib * Dp[a][b];
+      }
+    const double A22x = Am[2][2];  // Mindlin: in-plane shear at the centre only
+    if (form == 1) Am[2][2] = 0.0;
+    const double Ts[2][2] = {{ts, 0.0}, {0.0, ts}};
+    const double Ash[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, A22x}};
 
     // b. strain-displacement columns: 32 membrane, 48 shear, 4 points, centre
     for (int it = lane; it < 85; it += 64) {
@@ -371,8 +406,47 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
     }
     __syncthreads();
 
-    // c. local membrane (36 lower entries) and plate (78) blocks
-    for (int it = lane; it < 36 + 78; it += 64) {
+    // c. local membrane (8 x 8) and plate (12 x 12) blocks, every entry in
+    //    the reference's order: per Gauss point the membrane B^T A B w, then
+    //    the bending B^T Db B w and the shear B^T Ts B w (cquad4.cpp:622-655),
+    //    the Mindlin centre in-plane shear last
+    if (MODE != 2) {
+      for (int it = lane; it < 64 + 144; it += 64) {
+        if (it < 64) {
+          const int i = it >> 3, j = it & 7;
+          double acc = 0.0;
+          for (int g = 0; g < 4; ++g) {
+            auto col = [&](int k, double (&b)[3]) {
+              for (int m = 0; m < 3; ++m) b[m] = L.Bm[g][k][m];
+            };
+            acc += btdb_entry<3>(Am, col, i, j, L.w[g]);
+          }
+          if (form == 1) {
+            auto col0 = [&](int k, double (&b)[3]) {  // standard membrane B at the centre
+              b[0] = (k & 1) ? 0.0 : L.c2[k + 1];
+              b[1] = (k & 1) ? L.c2[k - 1] : 0.0;
+              b[2] = L.c2[k];
+            };
+            acc += btdb_entry<3>(Ash, col0, i, j, L.w0);
+          }
+          L.Km[i][j] = acc;
+        } else {
+          const int i = (it - 64) / 12, j = (it - 64) % 12;
+          double acc = 0.0;
+          for (int g = 0; g < 4; ++g) {
+            auto bcol = [&](int k, double (&b)[3]) { bend_col(L, g, k, b); };
+            auto scol = [&](int k, double (&b)[2]) {
+              b[0] = L.Bs[g][k][0];
+              b[1] = L.Bs[g][k][1];
+            };
+            acc += btdb_entry<3>(Db, bcol, i, j, L.w[g]);
+            acc += btdb_entry<2>(Ts, scol, i, j, L.w[g]);
+          }
+          L.Kp[i][j] = acc;
+        }
+      }
+    }
+    for (int it = lane; MODE == 2 && it < 36 + 78; it += 64) {
       if (it < 36) {
         const unsigned short pr = c_spairs.v[it];
         const int i = pr >> 8, j = pr & 0xff;
