@@ -30,9 +30,9 @@
 //   4. compaction                  empty slots vanish, so the pattern equals
 //                                  the reference's.
 // K is therefore bit-identical from run to run: every slot is summed in
-// element order (element kind groups in turn: CHEXA8, CPENTA6, CTETRA4,
-// CQUAD4), one fixed order of the serial assembly (the reference's TBB merge
-// order is thread-dependent, assembly_parallel.hpp:84-100).
+// element order, one fixed order of the serial assembly (the reference's TBB
+// merge order is thread-dependent, assembly_parallel.hpp:84-100), the order
+// of the CPU oracle's stable build_csr.
 #include <cstring>
 #include <numeric>
 
@@ -103,14 +103,18 @@ __global__ void k_gdofs(int nelem, const int* __restrict__ etype, const int* __r
 // ---- 2. topological pattern keys ----------------------------------------------------------
 // keys[idx] = (row << bits) | col of the element's lower pair, src[idx] =
 // off + idx: the pair's position in the concatenated list of all kinds.
+// With eid, also the element of each pair (meshes of several element kinds,
+// whose slots are summed in element order across the kind groups).
 template <int ND>
 __global__ void k_pair_keys(int nsel, const int* __restrict__ sel, const int* __restrict__ gd,
                             unsigned long long* __restrict__ keys, int bits,
-                            unsigned* __restrict__ src = nullptr, unsigned off = 0) {
+                            unsigned* __restrict__ src = nullptr, unsigned off = 0,
+                            int* __restrict__ eid = nullptr) {
   constexpr int NL = ND * (ND + 1) / 2;
   const long long idx = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (idx >= static_cast<long long>(nsel) * NL) return;
   if (src) src[idx] = off + static_cast<unsigned>(idx);
+  if (eid) eid[idx] = sel[idx / NL];
   const int s = static_cast<int>(idx / NL), t = static_cast<int>(idx % NL);
   const unsigned short pr = pair_at<ND>(t);
   const int e = sel[s];
@@ -158,6 +162,42 @@ __global__ void k_seg_end(long long m, const unsigned long long* __restrict__ ke
   seg[cnt] = lo;
 }
 
+// Slot sums of a mesh with several element kinds: the run of slot p holds
+// the kind groups one after the other, so its entries are visited in
+// increasing element index (eid of each pair position; an element
+// contributes at most once to a slot). Same skip and mark rules.
+__global__ void k_slot_sums_by_element(int nslots, const long long* __restrict__ seg,
+                                       const unsigned* __restrict__ src,
+                                       const int* __restrict__ eid,
+                                       const double* __restrict__ ke, double* __restrict__ val,
+                                       int* __restrict__ touched) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nslots) return;
+  const long long j0 = seg[p], j1 = seg[p + 1];
+  double acc = 0.0;
+  int any = 0, last = -1;
+  for (long long c = j0; c < j1; ++c) {
+    int best = 0x7fffffff;
+    long long bj = j0;
+    for (long long j = j0; j < j1; ++j) {
+      const int ej = eid[src[j]];
+      if (ej > last && ej < best) {
+        best = ej;
+        bj = j;
+      }
+    }
+    if (best == 0x7fffffff) break;  // every element visited
+    last = best;
+    const double v = ke[src[bj]];
+    if (v != 0.0) {
+      acc += v;
+      any = 1;
+    }
+  }
+  val[p] = acc;
+  touched[p] = any;
+}
+
 // Slot sums in element order: slot p's contributions are the sorted pair
 // positions seg[p] .. seg[p+1]-1, whose entries the element kernels stored at
 // ke[src[j]]. Exact zeros are skipped; touched[p] = 1 if any entry was nonzero.
@@ -199,12 +239,19 @@ __device__ __forceinline__ double det3(const M3d& m) {
          m.a[2][0] * (m.a[0][1] * m.a[1][2] - m.a[0][2] * m.a[1][1]);
 }
 
-__device__ __forceinline__ M3d inv3(const M3d& m, double det) {
-  const double id = 1.0 / det;
+// Inverse by cofactors with its own determinant from the first-row
+// expansion, in the operation order of the CPU oracle's inv3 (bit-identical
+// with contraction off). `bad` replaces a non-positive determinant by 1.
+__device__ __forceinline__ M3d inv3(const M3d& m, bool bad) {
+  const double c00 = m.a[1][1] * m.a[2][2] - m.a[1][2] * m.a[2][1];
+  const double c10 = m.a[1][2] * m.a[2][0] - m.a[1][0] * m.a[2][2];
+  const double c20 = m.a[1][0] * m.a[2][1] - m.a[1][1] * m.a[2][0];
+  const double det = m.a[0][0] * c00 + m.a[0][1] * c10 + m.a[0][2] * c20;
+  const double id = 1.0 / (bad ? 1.0 : det);
   M3d r;
-  r.a[0][0] = (m.a[1][1] * m.a[2][2] - m.a[1][2] * m.a[2][1]) * id;
-  r.a[1][0] = (m.a[1][2] * m.a[2][0] - m.a[1][0] * m.a[2][2]) * id;
-  r.a[2][0] = (m.a[1][0] * m.a[2][1] - m.a[1][1] * m.a[2][0]) * id;
+  r.a[0][0] = c00 * id;
+  r.a[1][0] = c10 * id;
+  r.a[2][0] = c20 * id;
   r.a[0][1] = (m.a[0][2] * m.a[2][1] - m.a[0][1] * m.a[2][2]) * id;
   r.a[1][1] = (m.a[0][0] * m.a[2][2] - m.a[0][2] * m.a[2][0]) * id;
   r.a[2][1] = (m.a[0][1] * m.a[2][0] - m.a[0][0] * m.a[2][1]) * id;
@@ -258,12 +305,38 @@ __device__ __forceinline__ M3d jac(const double (&X)[NN][3], const double (&d)[N
   return J;
 }
 
-// Isotropic bilinear form of two "gradient vectors" a (mode direction p) and
-// b (mode direction q): lam*a_p*b_q + mu*a_q*b_p + mu*delta_pq*(a.b).
-__device__ __forceinline__ double iso(double lam, double mu, const double* a, int p,
-                                      const double* b, int q) {
-  double v = lam * a[p] * b[q] + mu * a[q] * b[p];
-  if (p == q) v += mu * (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+// An isotropic material matrix in strain order (xx, yy, zz, xy, yz, zx): its
+// normal 3x3 block and the shear diagonal.
+struct Dmat {
+  double n[3][3];
+  double s;
+};
+
+// One entry of B^T D B, B(:, i) the strain column of displacement component p
+// with gradient a, B(:, j) component q with gradient b (build_B layout,
+// solid_elements.cpp:111-125; an enhanced mode column has the same shape,
+// fill_G_col 1140-1151), in the order of the reference's product
+// (sum_k B[k][i] (sum_l D[k][l] B[l][j])), every sum from 0.0 in increasing
+// index: each (D B)[k][j] has a single nonzero product, and the row sum
+// visits the normal row p, then the two shear rows of p in increasing order.
+// Bit-identical to the CPU oracle's add_BtDB with contraction off (terms that
+// are exact zeros there leave the sums unchanged).
+__device__ __forceinline__ double btdb6(const Dmat& D, const double* a, int p, const double* b,
+                                        int q) {
+  const double s3 = (q == 0) ? D.s * b[1] : (q == 1 ? D.s * b[0] : 0.0);
+  const double s4 = (q == 1) ? D.s * b[2] : (q == 2 ? D.s * b[1] : 0.0);
+  const double s5 = (q == 0) ? D.s * b[2] : (q == 2 ? D.s * b[0] : 0.0);
+  double v = a[p] * (D.n[p][q] * b[q]);
+  if (p == 0) {
+    v += a[1] * s3;
+    v += a[2] * s5;
+  } else if (p == 1) {
+    v += a[0] * s3;
+    v += a[2] * s4;
+  } else {
+    v += a[1] * s4;
+    v += a[0] * s5;
+  }
   return v;
 }
 
@@ -391,6 +464,37 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
     const double E = Ev[e], nu = nuv[e];
     const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
     const double mu = E / (2 * (1 + nu));
+    // isotropic_D (solid_elements.cpp:17-27); SRI splits it into the
+    // deviatoric D - Dvol at the Gauss points and Dvol (lam in the normal
+    // block) at the centroid (95-180)
+    Dmat Dfull, Ddev, Dvol;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        Dfull.n[a][b] = (a == b) ? lam + 2 * mu : lam;
+        Ddev.n[a][b] = Dfull.n[a][b] - lam;
+        Dvol.n[a][b] = lam;
+      }
+    Dfull.s = mu;
+    Ddev.s = mu - 0.0;
+    Dvol.s = 0.0;
+    // lane-owned lower pairs, oriented so the row has the larger global index
+    // (add_element_stiffness takes ke(i, j) with gdof[i] >= gdof[j],
+    // sparse_matrix.cpp:16-35)
+    int ei[KD::SU], ej[KD::SU];
+#pragma unroll
+    for (int q = 0; q < KD::SU; ++q) {
+      const int t = lane + 64 * q;
+      const unsigned short pr = pair_at<ND>(t < NL ? t : 0);
+      int i = pr >> 8, j = pr & 0xff;
+      const int* gg = g_of(gd, e);
+      if (gg[i] < gg[j]) {
+        const int tmp = i;
+        i = j;
+        j = tmp;
+      }
+      ei[q] = i;
+      ej[q] = j;
+    }
     double kuu[KD::SU];
 #pragma unroll
     for (int q = 0; q < KD::SU; ++q) kuu[q] = 0.0;
@@ -418,7 +522,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
       const M3d J0 = jac<NN>(X, d0);
       const double detJ0 = det3(J0);
       if (!(detJ0 > 0)) bad = true;
-      const M3d J0i = inv3(J0, bad ? 1.0 : detJ0);
+      const M3d J0i = inv3(J0, bad);
 #pragma unroll 1
       for (int g = 0; g < NG; ++g) {
         double r0, r1, r2, wq;
@@ -439,7 +543,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
         const M3d J = jac<NN>(X, d);
         const double detJ = det3(J);
         if (!(detJ > 0)) bad = true;
-        const M3d Ji = inv3(J, bad ? 1.0 : detJ);
+        const M3d Ji = inv3(J, bad);
         double dN[NN][3];
 #pragma unroll
         for (int n = 0; n < NN; ++n)
@@ -447,14 +551,13 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
           for (int c = 0; c < 3; ++c)
             dN[n][c] = Ji.a[c][0] * d[n][0] + Ji.a[c][1] * d[n][1] + Ji.a[c][2] * d[n][2];
         const double w = detJ * wq;
-        const double lamg = (K == kHexSri) ? 0.0 : lam;  // SRI: deviatoric part here
+        const Dmat& Dg = (K == kHexSri) ? Ddev : Dfull;  // SRI: deviatoric part here
 #pragma unroll
         for (int q = 0; q < KD::SU; ++q) {
           const int t = lane + 64 * q;
           if (t < NL) {
-            const unsigned short pr = pair_at<ND>(t);
-            const int i = pr >> 8, j = pr & 0xff;
-            kuu[q] += w * iso(lamg, mu, dN[i / 3], i % 3, dN[j / 3], j % 3);
+            const int i = ei[q], j = ej[q];
+            kuu[q] += btdb6(Dg, dN[i / 3], i % 3, dN[j / 3], j % 3) * w;
           }
         }
         if constexpr (KD::EAS) {
@@ -481,13 +584,13 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
             const int t = lane + 64 * q;
             if (t < ND * 9) {
               const int i = t / 9, m = t % 9;
-              kua[q] += w * iso(lam, mu, dN[i / 3], i % 3, gk[m / 3], m % 3);
+              kua[q] += btdb6(Dfull, dN[i / 3], i % 3, gk[m / 3], m % 3) * w;
             }
           }
           if (lane < 45) {
             const unsigned short pr = pair_at<9>(lane);
             const int m1 = pr >> 8, m2 = pr & 0xff;
-            kaa += w * iso(lam, mu, gk[m1 / 3], m1 % 3, gk[m2 / 3], m2 % 3);
+            kaa += btdb6(Dfull, gk[m1 / 3], m1 % 3, gk[m2 / 3], m2 % 3) * w;
           }
         }
       }
@@ -504,9 +607,8 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
         for (int q = 0; q < KD::SU; ++q) {
           const int t = lane + 64 * q;
           if (t < NL) {
-            const unsigned short pr = pair_at<ND>(t);
-            const int i = pr >> 8, j = pr & 0xff;
-            kuu[q] += w * lam * dN[i / 3][i % 3] * dN[j / 3][j % 3];
+            const int i = ei[q], j = ej[q];
+            kuu[q] += btdb6(Dvol, dN[i / 3], i % 3, dN[j / 3], j % 3) * w;
           }
         }
       }
@@ -569,8 +671,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
       for (int q = 0; q < KD::SU; ++q) {
         const int t = lane + 64 * q;
         if (t < NL) {
-          const unsigned short pr = pair_at<ND>(t);
-          const int i = pr >> 8, j = pr & 0xff;
+          const int i = ei[q], j = ej[q];
           double sv = 0.0;
 #pragma unroll
           for (int k = 0; k < 9; ++k) sv += sKua[i * 9 + k] * sX[k * ND + j];
@@ -1014,6 +1115,9 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   DBuf<int> t_row, t_col;
   DBuf<long long> seg;       // slot p's run in the sorted pair list
   DBuf<unsigned> src_sorted;  // pair positions in sorted key order
+  DBuf<int> eid;             // element of each pair position (several kinds only)
+  int ngroups = 0;
+  for (int k = 0; k < 5; ++k) ngroups += !groups[k].empty();
   size_t off_k[5] = {0, 0, 0, 0, 0};
   int nnz_topo = 0;
   {
@@ -1023,6 +1127,7 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
     sorted.resize(m);
     src.resize(m);
     src_sorted.resize(m);
+    if (ngroups > 1) eid.resize(m);
     size_t off = 0;
     for (int k = 0; k < 5; ++k) {
       off_k[k] = off;
@@ -1034,13 +1139,13 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
       const unsigned o32 = static_cast<unsigned>(off);
       if (ndk[k] == 24)
         hipLaunchKernelGGL(k_pair_keys<24>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
-                           keys.p + off, bits, src.p + off, o32);
+                           keys.p + off, bits, src.p + off, o32, eid.p ? eid.p + off : nullptr);
       else if (ndk[k] == 18)
         hipLaunchKernelGGL(k_pair_keys<18>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
-                           keys.p + off, bits, src.p + off, o32);
+                           keys.p + off, bits, src.p + off, o32, eid.p ? eid.p + off : nullptr);
       else
         hipLaunchKernelGGL(k_pair_keys<12>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
-                           keys.p + off, bits, src.p + off, o32);
+                           keys.p + off, bits, src.p + off, o32, eid.p ? eid.p + off : nullptr);
       off += static_cast<size_t>(cnt);
     }
     sort_pairs(keys, sorted, src, src_sorted, m, 2 * bits, s);
@@ -1082,7 +1187,10 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   DBuf<int> touched;
   t_val.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
   touched.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
-  if (nnz_topo > 0)
+  if (nnz_topo > 0 && ngroups > 1)
+    hipLaunchKernelGGL(k_slot_sums_by_element, dim3((nnz_topo + 255) / 256), dim3(256), 0, s,
+                       nnz_topo, seg.p, src_sorted.p, eid.p, ke.p, t_val.p, touched.p);
+  else if (nnz_topo > 0)
     hipLaunchKernelGGL(k_slot_sums, dim3((nnz_topo + 255) / 256), dim3(256), 0, s, nnz_topo, seg.p,
                        src_sorted.p, ke.p, t_val.p, touched.p);
   VBH_HIP(hipEventRecord(e2, s));
