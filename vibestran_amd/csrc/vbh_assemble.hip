@@ -306,10 +306,9 @@ __device__ __forceinline__ M3d jac(const double (&X)[NN][3], const double (&d)[N
 }
 
 // An isotropic material matrix in strain order (xx, yy, zz, xy, yz, zx): its
-// normal 3x3 block and the shear diagonal.
+// normal 3x3 block (diagonal nd, off-diagonal no) and the shear diagonal s.
 struct Dmat {
-  double n[3][3];
-  double s;
+  double nd, no, s;
 };
 
 // One entry of B^T D B, B(:, i) the strain column of displacement component p
@@ -326,7 +325,7 @@ __device__ __forceinline__ double btdb6(const Dmat& D, const double* a, int p, c
   const double s3 = (q == 0) ? D.s * b[1] : (q == 1 ? D.s * b[0] : 0.0);
   const double s4 = (q == 1) ? D.s * b[2] : (q == 2 ? D.s * b[1] : 0.0);
   const double s5 = (q == 0) ? D.s * b[2] : (q == 2 ? D.s * b[0] : 0.0);
-  double v = a[p] * (D.n[p][q] * b[q]);
+  double v = a[p] * ((p == q ? D.nd : D.no) * b[q]);
   if (p == 0) {
     v += a[1] * s3;
     v += a[2] * s5;
@@ -425,6 +424,40 @@ __device__ void tet_entries(const double (&X)[4][3], double E, double nu, int la
   }
 }
 
+// Per-wave geometry of the current point (solid kernels).
+template <int NN>
+struct Geo {
+  double X[NN][3];    // node coordinates
+  double d[NN][3];    // parametric shape derivatives at the point
+  double J[9];        // Jacobian, row-major
+  double dN[NN][3];   // physical gradients at the point
+  double dN0[NN][3];  // physical gradients at the centroid (SRI volumetric part)
+  double gk[3][3];    // enhanced-mode gradients at the point (EAS)
+  double Ji[9];       // inverse Jacobian at the point, row-major
+  double J0i[9];      // inverse centroidal Jacobian
+};
+
+// shape_d for one node (the same expressions, so the same values).
+template <int K>
+__device__ __forceinline__ void shape_d_node(int i, double r0, double r1, double r2, double (&d)[3]) {
+  if constexpr (Kind<K>::NN == 8) {
+    const double xn = (i == 1 || i == 2 || i == 5 || i == 6) ? 1.0 : -1.0;
+    const double en = (i == 2 || i == 3 || i == 6 || i == 7) ? 1.0 : -1.0;
+    const double zn = (i >= 4) ? 1.0 : -1.0;
+    d[0] = 0.125 * xn * (1 + en * r1) * (1 + zn * r2);
+    d[1] = 0.125 * en * (1 + xn * r0) * (1 + zn * r2);
+    d[2] = 0.125 * zn * (1 + xn * r0) * (1 + en * r1);
+  } else {
+    const double L3 = 1.0 - r0 - r1, zm = (1.0 - r2) * 0.5, zp = (1.0 + r2) * 0.5;
+    const double zs = (i < 3) ? zm : zp;  // node i's bottom/top factor
+    const int k = i % 3;
+    d[0] = (k == 0) ? ((i < 3) ? zm : zp) : (k == 1 ? 0.0 : -zs);
+    d[1] = (k == 1) ? zs : (k == 0 ? 0.0 : -zs);
+    const double Lk = (k == 0) ? r0 : (k == 1 ? r1 : L3);
+    d[2] = (i < 3) ? -Lk * 0.5 : Lk * 0.5;
+  }
+}
+
 template <int K>
 struct ElemLds {
   static constexpr int ND = Kind<K>::ND;
@@ -448,6 +481,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
   constexpr int NN = KD::NN, ND = KD::ND, NL = KD::NL;
   __shared__ double lds_all[kElemWaves][ElemLds<K>::WORDS];
   __shared__ double facc_all[kElemWaves][MODE == 2 ? KD::ND : 1];
+  __shared__ Geo<NN> geo_all[kElemWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double* lds = lds_all[wave];
   double* facc = facc_all[wave];
@@ -455,12 +489,7 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
     const int s = base + wave;
     const bool active = s < nsel;
     const int e = active ? sel[s] : sel[base];
-    double X[NN][3];
     const int o = eptr[e];
-#pragma unroll
-    for (int a = 0; a < NN; ++a)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) X[a][d] = xyz[3 * conn[o + a] + d];
     const double E = Ev[e], nu = nuv[e];
     const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
     const double mu = E / (2 * (1 + nu));
@@ -468,14 +497,14 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
     // deviatoric D - Dvol at the Gauss points and Dvol (lam in the normal
     // block) at the centroid (95-180)
     Dmat Dfull, Ddev, Dvol;
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) {
-        Dfull.n[a][b] = (a == b) ? lam + 2 * mu : lam;
-        Ddev.n[a][b] = Dfull.n[a][b] - lam;
-        Dvol.n[a][b] = lam;
-      }
+    Dfull.nd = lam + 2 * mu;
+    Dfull.no = lam;
     Dfull.s = mu;
+    Ddev.nd = Dfull.nd - lam;
+    Ddev.no = lam - lam;
     Ddev.s = mu - 0.0;
+    Dvol.nd = lam;
+    Dvol.no = lam;
     Dvol.s = 0.0;
     // lane-owned lower pairs, oriented so the row has the larger global index
     // (add_element_stiffness takes ke(i, j) with gdof[i] >= gdof[j],
@@ -510,19 +539,77 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
       // operation order as the CPU oracle without FMA contraction, so that
       // entries which cancel to exactly 0.0 there do so here too (the CSR
       // pattern drops exact zeros).
+      double X[NN][3];
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) X[a][d] = xyz[3 * conn[o + a] + d];
       tet_entries<KD::SU>(X, E, nu, lane, g_of(gd, e), kuu, bad);
     } else {
       // Gauss rule: hexa 2x2x2 (weight 1), penta 3 triangle x 2 axial (1/6).
+      // The geometry of each point is formed once per wave in LDS (shape
+      // derivatives by node lanes, J by entry lanes, physical gradients by
+      // DOF lanes); every lane then reads the gradients of its entries, so
+      // no lane holds whole per-node arrays (which spilled to scratch).
       constexpr int NG = (NN == 8) ? 8 : 6;
       const double gp = 0.57735026918962584;  // 1/sqrt(3)
+      Geo<NN>& G = geo_all[wave];
+      // the shape derivatives of node `lane` at (r0, r1, r2) into G.d, then J
+      // (sum over nodes in node order, the oracle's jacobian) into G.J;
+      // returns the point's inverse Jacobian and determinant in all lanes
+      auto point_geometry = [&](double r0, double r1, double r2, M3d& Ji, double& detJ) {
+        if (lane < NN) {
+          double dl[3];
+          shape_d_node<K>(lane, r0, r1, r2, dl);
+          G.d[lane][0] = dl[0];
+          G.d[lane][1] = dl[1];
+          G.d[lane][2] = dl[2];
+        }
+        __syncthreads();
+        if (lane < 9) {
+          const int ia = lane / 3, ib = lane % 3;
+          double sj = 0.0;
+          for (int n = 0; n < NN; ++n) sj += G.d[n][ia] * G.X[n][ib];
+          G.J[lane] = sj;
+        }
+        __syncthreads();
+        M3d J;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) J.a[q / 3][q % 3] = G.J[q];
+        detJ = det3(J);
+        if (!(detJ > 0)) bad = true;
+        Ji = inv3(J, bad);
+        // lanes index the inverse by their own row: through LDS, not a
+        // private array (which would live in scratch memory)
+        if (lane == 0)
+#pragma unroll
+          for (int q = 0; q < 9; ++q) G.Ji[q] = Ji.a[q / 3][q % 3];
+        __syncthreads();
+      };
+      // physical gradients dN[n][c] (the oracle's phys_grad) by DOF lanes
+      auto grads = [&]() {
+        if (lane < ND) {
+          const int n = lane / 3, c = lane % 3;
+          G.dN[n][c] = G.Ji[3 * c] * G.d[n][0] + G.Ji[3 * c + 1] * G.d[n][1] +
+                       G.Ji[3 * c + 2] * G.d[n][2];
+        }
+      };
+      if (lane < ND) G.X[lane / 3][lane % 3] = xyz[3 * conn[o + lane / 3] + lane % 3];
+      __syncthreads();
       // centroidal Jacobian (EAS mapping / SRI volumetric point)
-      double d0[NN][3];
-      if constexpr (NN == 8) shape_d<K>(0.0, 0.0, 0.0, d0);
-      else shape_d<K>(1.0 / 3.0, 1.0 / 3.0, 0.0, d0);
-      const M3d J0 = jac<NN>(X, d0);
-      const double detJ0 = det3(J0);
-      if (!(detJ0 > 0)) bad = true;
-      const M3d J0i = inv3(J0, bad);
+      M3d J0i;
+      double detJ0;
+      if constexpr (NN == 8) point_geometry(0.0, 0.0, 0.0, J0i, detJ0);
+      else point_geometry(1.0 / 3.0, 1.0 / 3.0, 0.0, J0i, detJ0);
+      if (lane < 9) G.J0i[lane] = G.Ji[lane];
+      if constexpr (K == kHexSri) {
+        // volumetric part at the centroid, weight 8 (solid_elements.cpp:153-177),
+        // added after the deviatoric Gauss points below
+        grads();
+        __syncthreads();
+        if (lane < ND) G.dN0[lane / 3][lane % 3] = G.dN[lane / 3][lane % 3];
+      }
+      __syncthreads();
 #pragma unroll 1
       for (int g = 0; g < NG; ++g) {
         double r0, r1, r2, wq;
@@ -538,18 +625,29 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
           r2 = (g & 1) ? gp : -gp;
           wq = 1.0 / 6.0;
         }
-        double d[NN][3];
-        shape_d<K>(r0, r1, r2, d);
-        const M3d J = jac<NN>(X, d);
-        const double detJ = det3(J);
-        if (!(detJ > 0)) bad = true;
-        const M3d Ji = inv3(J, bad);
-        double dN[NN][3];
-#pragma unroll
-        for (int n = 0; n < NN; ++n)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            dN[n][c] = Ji.a[c][0] * d[n][0] + Ji.a[c][1] * d[n][1] + Ji.a[c][2] * d[n][2];
+        M3d Ji;
+        double detJ;
+        point_geometry(r0, r1, r2, Ji, detJ);
+        grads();
+        if constexpr (KD::EAS) {
+          // enhanced-mode gradients g_k (scaled by detJ0/detJ):
+          //   hexa: rows of J0^{-1} times (xi, eta, zeta)   (solid_elements.cpp:1131-1133)
+          //   penta: columns of J0^{-1} times (L1-1/3, L2-1/3, zeta) (1399-1410)
+          if (lane >= 32 && lane < 41) {
+            const double sc = detJ0 / detJ;
+            const int k = (lane - 32) / 3, c = (lane - 32) % 3;
+            double gv;
+            if constexpr (NN == 8) {
+              const double prm = k == 0 ? r0 : (k == 1 ? r1 : r2);
+              gv = G.J0i[3 * k + c] * prm * sc;
+            } else {
+              const double f = k == 0 ? r0 - 1.0 / 3.0 : (k == 1 ? r1 - 1.0 / 3.0 : r2);
+              gv = G.J0i[3 * c + k] * f * sc;
+            }
+            G.gk[k][c] = gv;
+          }
+        }
+        __syncthreads();
         const double w = detJ * wq;
         const Dmat& Dg = (K == kHexSri) ? Ddev : Dfull;  // SRI: deviatoric part here
 #pragma unroll
@@ -557,58 +655,34 @@ __global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
           const int t = lane + 64 * q;
           if (t < NL) {
             const int i = ei[q], j = ej[q];
-            kuu[q] += btdb6(Dg, dN[i / 3], i % 3, dN[j / 3], j % 3) * w;
+            kuu[q] += btdb6(Dg, G.dN[i / 3], i % 3, G.dN[j / 3], j % 3) * w;
           }
         }
         if constexpr (KD::EAS) {
-          // enhanced-mode gradients g_k (scaled by detJ0/detJ):
-          //   hexa: rows of J0^{-1} times (xi, eta, zeta)   (solid_elements.cpp:1131-1133)
-          //   penta: columns of J0^{-1} times (L1-1/3, L2-1/3, zeta) (1399-1410)
-          const double sc = detJ0 / detJ;
-          double gk[3][3];
-          if constexpr (NN == 8) {
-            const double prm[3] = {r0, r1, r2};
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) gk[k][c] = J0i.a[k][c] * prm[k] * sc;
-          } else {
-            const double f[3] = {r0 - 1.0 / 3.0, r1 - 1.0 / 3.0, r2};
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) gk[k][c] = J0i.a[c][k] * f[k] * sc;
-          }
 #pragma unroll
           for (int q = 0; q < KD::SA; ++q) {
             const int t = lane + 64 * q;
             if (t < ND * 9) {
               const int i = t / 9, m = t % 9;
-              kua[q] += btdb6(Dfull, dN[i / 3], i % 3, gk[m / 3], m % 3) * w;
+              kua[q] += btdb6(Dfull, G.dN[i / 3], i % 3, G.gk[m / 3], m % 3) * w;
             }
           }
           if (lane < 45) {
             const unsigned short pr = pair_at<9>(lane);
             const int m1 = pr >> 8, m2 = pr & 0xff;
-            kaa += btdb6(Dfull, gk[m1 / 3], m1 % 3, gk[m2 / 3], m2 % 3) * w;
+            kaa += btdb6(Dfull, G.gk[m1 / 3], m1 % 3, G.gk[m2 / 3], m2 % 3) * w;
           }
         }
+        __syncthreads();
       }
       if constexpr (K == kHexSri) {
-        // volumetric part at the centroid, weight 8 (solid_elements.cpp:153-177)
-        double dN[NN][3];
-#pragma unroll
-        for (int n = 0; n < NN; ++n)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            dN[n][c] = J0i.a[c][0] * d0[n][0] + J0i.a[c][1] * d0[n][1] + J0i.a[c][2] * d0[n][2];
         const double w = detJ0 * 8.0;
 #pragma unroll
         for (int q = 0; q < KD::SU; ++q) {
           const int t = lane + 64 * q;
           if (t < NL) {
             const int i = ei[q], j = ej[q];
-            kuu[q] += btdb6(Dvol, dN[i / 3], i % 3, dN[j / 3], j % 3) * w;
+            kuu[q] += btdb6(Dvol, G.dN0[i / 3], i % 3, G.dN0[j / 3], j % 3) * w;
           }
         }
       }
