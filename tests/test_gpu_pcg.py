@@ -193,16 +193,22 @@ def _sell_cases():
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-def test_sell_16bit_block_columns_match_32bit(gpu_backend_factory, monkeypatch, mixed):
-    """The sliced ELL's 16-bit block-column offsets (node0-relative) and its
-    32-bit block columns (VBH_SELL_IDX16=0) give bit-identical solves and
-    iteration counts on the same operator."""
+def test_sell_layouts_give_identical_solves(gpu_backend_factory, monkeypatch, mixed):
+    """The three sliced-ELL layouts -- relative (one block-column offset per
+    slot and slice; VBH_SELL_REL), absolute with 16-bit node0-relative block
+    columns, absolute with 32-bit block columns (VBH_SELL_IDX16=0) -- sum every
+    row's products in the same column order (the relative layout's extra zero
+    blocks add exact zeros), so solves and iteration counts are bit-identical
+    on the same operator. The shuffled-numbering hex keeps the absolute 32-bit
+    layout (its slices see too many offsets for the relative one)."""
     for csr, F in _sell_cases():
         res = []
-        for flag in ("1", "0"):
-            monkeypatch.setenv("VBH_SELL_IDX16", flag)
+        for rel, idx16 in (("1", "1"), ("0", "1"), ("0", "0")):
+            monkeypatch.setenv("VBH_SELL_REL", rel)
+            monkeypatch.setenv("VBH_SELL_IDX16", idx16)
             be = gpu_backend_factory(mixed=mixed, tolerance=1e-8, max_iters=50000)
             u = be.solve(csr, F)
             res.append((u, be.last_iteration_count()))
-        (u16, it16), (u32, it32) = res
-        assert it16 == it32 and np.array_equal(u16, u32)
+        (u0, it0) = res[0]
+        for u, it in res[1:]:
+            assert it == it0 and np.array_equal(u, u0)

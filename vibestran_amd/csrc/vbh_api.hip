@@ -837,6 +837,13 @@ int vbh_operator_stream_bytes(const vbh_ctx* h, int precision, long long* bytes)
   }
   const vbh::BsrDev& A = c->bsrd;
   const long long B = A.B;
+  if (A.rk64) {  // relative sliced ELL: values, one 32-bit offset per slot, slice offsets
+    const long long slots = precision == VBH_PREC_FP64 ? c->rel_slots64 : c->rel_slots32;
+    long long b = slots * vbh::kSliceRows * B * sv + slots * 4 + 4ll * (A.nslices + 1);
+    if (precision == VBH_PREC_MIXED && A.dblk) b += static_cast<long long>(A.nbr) * B * B * 8;
+    *bytes = b;
+    return VBH_OK;
+  }
   const long long slots = c->sell_slots;
   long long b = slots * vbh::kSliceRows * B * sv;                        // values
   b += slots * vbh::sell_nps(A.B) * (A.sidx16 ? 2 : 4);                 // block columns

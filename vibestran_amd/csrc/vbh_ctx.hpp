@@ -117,6 +117,11 @@ struct Ctx {
   int rowsum_fp32 = 1;      // split the FP32 stream as above (VBH_ROWSUM=0 disables)
   long long sell_padded_blocks = 0;
   long long sell_slots = 0;  // block slots of all slices (sum of the slice widths)
+  // relative sliced ELL (BsrDev::rk64 ...)
+  DBuf<int> rel_k64, rel_o64, rel_k32, rel_o32;
+  DBuf<double> rel_v64;
+  DBuf<float> rel_v32;
+  long long rel_slots64 = 0, rel_slots32 = 0;
   DBuf<double> diag64;
   DBuf<float> diag32;
   CsrDev csr;

@@ -158,6 +158,18 @@ struct BsrDev {
   // (blocks kSellCap.. of the row in the BSR arrays above) after the SpMV.
   int nlong = 0;
   const int* lrows = nullptr;  // block rows with more than kSellCap blocks
+  // Relative sliced ELL (set when it streams fewer bytes; replaces koff /
+  // sidx / s64 / s32 for the SpMVs): slice s holds the union of its nodes'
+  // block-column offsets J - I, rel_off[rel_koff[s] .. rel_koff[s+1]), one
+  // uniform offset per slot for the whole slice, so no per-row column index is
+  // read; a row's value block is zero where its node has no such neighbour.
+  // The FP32 list leaves out offset 0 when dblk carries the diagonal blocks.
+  const int* rk64 = nullptr;  // slot offsets per slice (nslices + 1), FP64 list
+  const int* ro64 = nullptr;  // block-column offsets, FP64 list
+  const double* rv64 = nullptr;
+  const int* rk32 = nullptr;
+  const int* ro32 = nullptr;
+  const float* rv32 = nullptr;
 };
 
 constexpr int kSliceRows = 64;

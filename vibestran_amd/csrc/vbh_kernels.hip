@@ -156,11 +156,47 @@ __device__ __forceinline__ Acc sell_row_ix(const T* __restrict__ v, const T* __r
   return acc;
 }
 
+template <typename T>
+__device__ __forceinline__ const T* rel_vals(const BsrDev& A);
+template <>
+__device__ __forceinline__ const double* rel_vals<double>(const BsrDev& A) { return A.rv64; }
+template <>
+__device__ __forceinline__ const float* rel_vals<float>(const BsrDev& A) { return A.rv32; }
+
+// Row g = 64 s + l of the relative sliced ELL (BsrDev::rk64 ...) times x: the
+// slot offsets are uniform over the wave's slice (scalar loads); block
+// columns outside the operator (a boundary node without that neighbour,
+// whose value block is zero) are clamped into range.
+template <typename T, int B, typename Acc>
+__device__ __forceinline__ Acc sell_row_rel(const BsrDev& A, const T* __restrict__ x, int s,
+                                            int l) {
+  const bool f32 = sizeof(T) == 4;
+  const int* __restrict__ rk = f32 ? A.rk32 : A.rk64;
+  const int* __restrict__ ro = f32 ? A.ro32 : A.ro64;
+  s = __builtin_amdgcn_readfirstlane(s);
+  const int k0 = rk[s], km = rk[s + 1] - k0;
+  const int node = (kSliceRows * s + l) / B;
+  const int last = A.nbr - 1;
+  const T* __restrict__ v = rel_vals<T>(A) + (static_cast<size_t>(k0) * kSliceRows + l) * B;
+  const int* __restrict__ off = ro + k0;
+  Acc acc = Acc(0);
+#pragma unroll 4
+  for (int k = 0; k < km; ++k) {
+    int J = node + off[k];
+    J = J < 0 ? 0 : (J > last ? last : J);
+    const T* vk = v + static_cast<size_t>(k) * kSliceRows * B;
+#pragma unroll
+    for (int c = 0; c < B; ++c) acc += static_cast<Acc>(vk[c]) * static_cast<Acc>(x[B * J + c]);
+  }
+  return acc;
+}
+
 // Row g = 64 s + l of the sliced-ELL operator times x (one lane, no reduction),
 // accumulated in Acc.
 template <typename T, int B, typename Acc = T>
 __device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ val,
                                         const T* __restrict__ x, int s, int l) {
+  if (A.rk64) return sell_row_rel<T, B, Acc>(A, x, s, l);
   constexpr int NPS = sell_nps(B);
   const int k0 = A.koff[s], km = A.koff[s + 1] - k0;
   const int node = (kSliceRows * s + l) / B;

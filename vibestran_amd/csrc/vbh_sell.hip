@@ -4,6 +4,8 @@
 // blocks; with 64-row slices the B values a lane needs per block are
 // contiguous and the lanes of a wave read one contiguous span per block slot,
 // with no cross-lane reduction. Built on the device from the BSR operator.
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -132,6 +134,46 @@ __global__ void k_sell_zero_diag(int nslices, int nbr, const int* __restrict__ k
   for (int c = 0; c < B; ++c) s32[(static_cast<size_t>(koff[s] + kd) * kSliceRows + l) * B + c] = 0.0f;
 }
 
+// Values of the relative sliced ELL: slice s, lane l = row 64 s + l of node
+// I; slot k holds the row's block at column I + off[k] (found by binary
+// search in the sorted BSR row) or zeros.
+template <int B, typename T>
+__global__ void k_rel_fill(int nslices, int nbr, const int* __restrict__ brp,
+                           const int* __restrict__ bci, const double* __restrict__ bv,
+                           const int* __restrict__ rk, const int* __restrict__ ro,
+                           T* __restrict__ rv) {
+  const int s = blockIdx.x;
+  const int l = threadIdx.x;  // 0..63
+  if (s >= nslices) return;
+  const int k0 = rk[s], km = rk[s + 1] - k0;
+  const int g = kSliceRows * s + l;
+  const int I = g / B, r = g - I * B;
+  const bool row = g < nbr * B;
+  const int o = row ? brp[I] : 0, nb = row ? brp[I + 1] - o : 0;
+  for (int k = 0; k < km; ++k) {
+    const int J = I + ro[k0 + k];
+    int kk = -1;
+    if (row) {
+      int lo = 0, hi = nb - 1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = bci[o + mid];
+        if (c == J) {
+          kk = mid;
+          break;
+        }
+        if (c < J) lo = mid + 1;
+        else hi = mid - 1;
+      }
+    }
+    T* dst = rv + (static_cast<size_t>(k0 + k) * kSliceRows + l) * B;
+    for (int c = 0; c < B; ++c)
+      dst[c] = kk >= 0 ? static_cast<T>(bv[static_cast<size_t>(B * B) * o +
+                                          static_cast<size_t>(r * B + c) * nb + kk])
+                       : T(0);
+  }
+}
+
 __global__ void k_to_f32(long long m, const double* __restrict__ a, float* __restrict__ b) {
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -139,6 +181,8 @@ __global__ void k_to_f32(long long m, const double* __restrict__ a, float* __res
 }
 
 }  // namespace
+
+void build_sell_relative(Ctx& c);
 
 void build_sell(Ctx& c) {
   hipStream_t s = c.stream;
@@ -238,6 +282,87 @@ void build_sell(Ctx& c) {
   A.s32 = c.sell_v32.p;
   c.sell_padded_blocks = static_cast<long long>(ktot) * kSliceRows / B;
   c.sell_slots = ktot;
+  build_sell_relative(c);
+}
+
+// The relative sliced ELL (BsrDev::rk64 ...), used when its slot count stays
+// within 10 % of the absolute one's and no row overflows: on meshes numbered
+// by a structured sweep every slice sees the same few neighbour offsets, so
+// the per-row block columns (and, for the FP32 stream, the zeroed diagonal
+// blocks) drop out of the operator stream. VBH_SELL_REL=0 keeps the absolute
+// layout.
+void build_sell_relative(Ctx& c) {
+  BsrDev& A = c.bsrd;
+  A.rk64 = A.ro64 = A.rk32 = A.ro32 = nullptr;
+  A.rv64 = nullptr;
+  A.rv32 = nullptr;
+  c.rel_k64.release();
+  c.rel_o64.release();
+  c.rel_k32.release();
+  c.rel_o32.release();
+  c.rel_v64.release();
+  c.rel_v32.release();
+  c.rel_slots64 = c.rel_slots32 = 0;
+  const char* env = std::getenv("VBH_SELL_REL");
+  if ((env && env[0] == '0') || A.nlong > 0) return;
+  hipStream_t s = c.stream;
+  const int B = A.B, nbr = A.nbr, nslices = A.nslices;
+  std::vector<int> brp(static_cast<size_t>(nbr) + 1), bci(static_cast<size_t>(A.nnzb));
+  VBH_HIP(hipMemcpyAsync(brp.data(), A.brp, sizeof(int) * brp.size(), hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipMemcpyAsync(bci.data(), A.bci, sizeof(int) * bci.size(), hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  const bool split = A.dblk != nullptr;  // FP32 list without the diagonal offset
+  std::vector<int> k64(static_cast<size_t>(nslices) + 1, 0), k32(static_cast<size_t>(nslices) + 1, 0);
+  std::vector<int> o64, o32, u;
+  for (int sl = 0; sl < nslices; ++sl) {
+    const int n0 = (kSliceRows * sl) / B;
+    const int n1 = std::min((kSliceRows * sl + kSliceRows - 1) / B, nbr - 1);
+    u.clear();
+    for (int I = n0; I <= n1; ++I)
+      for (int q = brp[static_cast<size_t>(I)]; q < brp[static_cast<size_t>(I) + 1]; ++q)
+        u.push_back(bci[static_cast<size_t>(q)] - I);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    if (static_cast<int>(u.size()) > kSellCap) return;
+    for (int off : u) {
+      o64.push_back(off);
+      if (!(split && off == 0)) o32.push_back(off);
+    }
+    k64[static_cast<size_t>(sl) + 1] = static_cast<int>(o64.size());
+    k32[static_cast<size_t>(sl) + 1] = static_cast<int>(o32.size());
+  }
+  if (static_cast<double>(o64.size()) > 1.1 * static_cast<double>(c.sell_slots)) return;
+  c.rel_k64.upload(k64.data(), k64.size(), s);
+  c.rel_o64.upload(o64.data(), o64.size(), s);
+  c.rel_k32.upload(k32.data(), k32.size(), s);
+  c.rel_o32.upload(o32.data(), std::max<size_t>(o32.size(), 1), s);
+  c.rel_v64.resize(o64.size() * kSliceRows * B);
+  c.rel_v32.resize(std::max<size_t>(o32.size(), 1) * kSliceRows * B);
+  if (B == 3) {
+    hipLaunchKernelGGL((k_rel_fill<3, double>), dim3(nslices), dim3(kSliceRows), 0, s, nslices, nbr,
+                       A.brp, A.bci, A.v64, c.rel_k64.p, c.rel_o64.p, c.rel_v64.p);
+    hipLaunchKernelGGL((k_rel_fill<3, float>), dim3(nslices), dim3(kSliceRows), 0, s, nslices, nbr,
+                       A.brp, A.bci, A.v64, c.rel_k32.p, c.rel_o32.p, c.rel_v32.p);
+  } else {
+    hipLaunchKernelGGL((k_rel_fill<6, double>), dim3(nslices), dim3(kSliceRows), 0, s, nslices, nbr,
+                       A.brp, A.bci, A.v64, c.rel_k64.p, c.rel_o64.p, c.rel_v64.p);
+    hipLaunchKernelGGL((k_rel_fill<6, float>), dim3(nslices), dim3(kSliceRows), 0, s, nslices, nbr,
+                       A.brp, A.bci, A.v64, c.rel_k32.p, c.rel_o32.p, c.rel_v32.p);
+  }
+  VBH_HIP(hipStreamSynchronize(s));
+  A.rk64 = c.rel_k64.p;
+  A.ro64 = c.rel_o64.p;
+  A.rv64 = c.rel_v64.p;
+  A.rk32 = c.rel_k32.p;
+  A.ro32 = c.rel_o32.p;
+  A.rv32 = c.rel_v32.p;
+  c.rel_slots64 = static_cast<long long>(o64.size());
+  c.rel_slots32 = static_cast<long long>(o32.size());
+  // the absolute layout's value arrays are no longer read by the solver
+  c.sell_v64.release();
+  c.sell_v32.release();
+  A.s64 = nullptr;
+  A.s32 = nullptr;
 }
 
 }  // namespace vbh
