@@ -239,11 +239,12 @@ def main():
                 "rel_res": rel64}
         log(f"[bench] fp64 solve: {it64} iters, rel {rel64:.3e}, {t64*1e3:.1f} ms")
 
-    # Dominant kernel: the fused FP32 PCG SpMV (K p + p.Ap partials). Its
-    # duration is measured live with dispatch start/stop events
-    # (hipExtLaunchKernelGGL on the context stream, inside directly launched
-    # PCG iterations, the kernel's own begin and end as a profiler reports
-    # them); back-to-back launches between one event pair are reported beside.
+    # Dominant kernel: the fused FP32 PCG SpMV (K p + the p.Ap, z.Ap and
+    # Ap.diag^-1.Ap partials). Its duration is measured live with dispatch
+    # start/stop events (hipExtLaunchKernelGGL on the context stream, inside
+    # directly launched PCG iterations, the kernel's own begin and end as a
+    # profiler reports them); back-to-back launches between one event pair are
+    # reported beside.
     def kt(which, reps=args.spmv_reps):
         ms = C.c_double(0.0)
         if lib.vbh_bench_spmv(h, which, reps, C.byref(ms)) != 0:
@@ -258,8 +259,7 @@ def main():
     lib.vbh_memcpy_h2d(h, dF, F.ctypes.data_as(C.c_void_p), nbytes)
     ms_b2b32, ms_b2b64 = kt(14), kt(13)  # back to back, one event pair around all launches
     ms32, ms64 = kt(8), kt(7)  # inside iterations, the dispatch's own start/stop
-    ms_upd32, ms_upd64 = kt(10), kt(9)
-    ms_dir32, ms_dir64 = kt(12), kt(11)
+    ms_vec32, ms_vec64 = kt(10), kt(9)
     ms_plain32 = kt(1)
     ms_copy = kt(6, 5)
     copy_gbs = 2 * (256 << 20) / (ms_copy * 1e-3) / 1e9
@@ -267,26 +267,27 @@ def main():
     n_pad = ((n + fmt - 1) // fmt) * fmt if fmt else n
     # Algorithmic bytes of one fused SpMV launch in the layout it streams: the
     # operator (padded block values, block columns, slice offsets and, for the
-    # FP32 stream, the FP64 diagonal node blocks) plus p gathered once and Ap
-    # written. The SURVEY yardstick (full-CSR equivalent: value + int32 column
-    # per nonzero) is reported beside as an effective figure.
-    spmv32_bytes = op_bytes(_lib.VBH_PREC_MIXED) + 2 * 4 * n_pad
-    spmv64_bytes = op_bytes(_lib.VBH_PREC_FP64) + 2 * 8 * n_pad
-    csr32_bytes = nnz_full * (4 + 4) + 4 * (n + 1) + 2 * 4 * n
-    upd32_bytes = (8 + 4 * 4 + 8 + 4) * n
+    # FP32 stream, the FP64 diagonal node blocks) plus p gathered once, Ap
+    # written, and z and diag read for the z.Ap and Ap.diag^-1.Ap partials.
+    # The full-CSR equivalent (value + int32 column per nonzero, the same four
+    # vectors) is reported beside as an effective figure.
+    spmv32_bytes = op_bytes(_lib.VBH_PREC_MIXED) + 4 * 4 * n_pad
+    spmv64_bytes = op_bytes(_lib.VBH_PREC_FP64) + 4 * 8 * n_pad
+    csr32_bytes = nnz_full * (4 + 4) + 4 * (n + 1) + 4 * 4 * n
+    # vector kernel: u (FP64) read and written, p, Ap, diag, r read, r, z, p written
+    vec32_bytes = (8 + 4 * 4 + 8 + 3 * 4) * n
     # the live duration: `reps` launches back to back between one event pair on
     # the context stream (the kernel trace of the same loop gives the same
     # per-dispatch duration); the dispatch-event figure inside iterations adds
     # the events' own cost and is reported beside
     ach32 = spmv32_bytes / (ms_b2b32 * 1e-3) / 1e9
     ach64 = spmv64_bytes / (ms_b2b64 * 1e-3) / 1e9
-    # whole iteration: the SpMV's bytes plus the 11 other vector streams of the
-    # minimal fused iteration (u f64 r/w counted as 4 FP32-sized streams)
-    b_iter32 = spmv32_bytes + 11 * n * 4
+    # whole iteration: the SpMV's bytes plus the vector kernel's
+    b_iter32 = spmv32_bytes + vec32_bytes
     iter_gbs = b_iter32 * iters_mixed / (devs[-1]) / 1e9
     log(f"[bench] cg_spmv dispatch events (us): fp32 {ms32*1e3:.2f} fp64 {ms64*1e3:.2f}; back to "
-        f"back fp32 {ms_b2b32*1e3:.2f} fp64 {ms_b2b64*1e3:.2f}; cg_update fp32 {ms_upd32*1e3:.2f} "
-        f"fp64 {ms_upd64*1e3:.2f}, cg_direction fp32 {ms_dir32*1e3:.2f} fp64 {ms_dir64*1e3:.2f}; "
+        f"back fp32 {ms_b2b32*1e3:.2f} fp64 {ms_b2b64*1e3:.2f}; cg_vector fp32 {ms_vec32*1e3:.2f} "
+        f"fp64 {ms_vec64*1e3:.2f}; "
         f"iteration {devs[-1]/iters_mixed*1e6:.2f} us; fp32 SpMV stream {spmv32_bytes/1e6:.1f} MB")
 
     # result recovery of the converged u: centroid + corner stresses on the
@@ -376,10 +377,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"k_cg_spmv_bsr<float, {fmt}, double> (FP32 PCG SpMV + p.Ap)",
+                     "kernel": f"k_cg_spmv_bsr<float, {fmt}, double> (FP32 PCG SpMV + p.Ap, z.Ap, Ap.diag^-1.Ap partials)",
                      "bytes_per_launch": int(spmv32_bytes),
                      "bytes_definition": "sliced-ELL FP32 off-diagonal blocks + FP64 diagonal "
-                                         "blocks + block columns + slice offsets + p + Ap",
+                                         "blocks + block columns + slice offsets + p + Ap "
+                                         "+ z + diag",
                      "us_per_launch": round(ms_b2b32 * 1e3, 3),
                      "us_per_launch_dispatch_events": round(ms32 * 1e3, 3),
                      "rocprof_us_per_launch_back_to_back": prof_b2b_us,
@@ -390,14 +392,12 @@ def main():
                          "bytes_per_launch": int(csr32_bytes),
                          "GBs": round(csr32_bytes / (ms_b2b32 * 1e-3) / 1e9, 1)}},
         "kernels_us": {"cg_spmv_f32": round(ms32 * 1e3, 2), "cg_spmv_f64": round(ms64 * 1e3, 2),
-                       "cg_update_f32": round(ms_upd32 * 1e3, 2),
-                       "cg_update_f64": round(ms_upd64 * 1e3, 2),
-                       "cg_direction_f32": round(ms_dir32 * 1e3, 2),
-                       "cg_direction_f64": round(ms_dir64 * 1e3, 2),
+                       "cg_vector_f32": round(ms_vec32 * 1e3, 2),
+                       "cg_vector_f64": round(ms_vec64 * 1e3, 2),
                        "cg_spmv_f32_back_to_back": round(ms_b2b32 * 1e3, 2),
                        "cg_spmv_f64_back_to_back": round(ms_b2b64 * 1e3, 2),
                        "spmv_plain_f32": round(ms_plain32 * 1e3, 2),
-                       "cg_update_f32_GBs": round(upd32_bytes / (ms_upd32 * 1e-3) / 1e9, 1),
+                       "cg_vector_f32_GBs": round(vec32_bytes / (ms_vec32 * 1e-3) / 1e9, 1),
                        "cg_spmv_f64_bytes": int(spmv64_bytes),
                        "cg_spmv_f64_GBs": round(ach64, 1),
                        "stream_copy_256MiB_us": round(ms_copy * 1e3, 2),

@@ -305,8 +305,8 @@ int vbh_solve_device(vbh_ctx* ctx, const double* dF, double* du, int* iters, dou
  * vbh_assemble, which may allocate); the solver's work buffers live in one
  * static device array of 96 MiB per GPU, taken at vbh_ctx_create, and the
  * iterations are launched directly (no graph). Operators of more than
- * about 2,093,000 rows in FP64 (2,511,000 in mixed precision: 48 / 40 bytes
- * per row after 197 KiB of state and partial sums) are rejected with
+ * about 2,510,000 rows in FP64 (2,788,000 in mixed precision: 40 / 36 bytes
+ * per row after 257 KiB of state and partial sums) are rejected with
  * VBH_ERR_ARG. Every call on one GPU shares that
  * GPU's array and keeps the last call's contents: calls that overlap on one
  * GPU (other threads or contexts) race. */
@@ -346,16 +346,17 @@ int vbh_last_stats(const vbh_ctx* ctx, vbh_solve_stats* out);
 int vbh_spmv(vbh_ctx* ctx, const double* x, double* y);
 /* Time `reps` back-to-back launches of one solver kernel on device-resident
  * vectors; returns the mean launch duration in ms measured with hipEvents on
- * the context stream. which: 0/1 plain SpMV FP64/FP32, 2/3 the fused PCG
- * SpMV kernel (search direction + K p + p.Ap) FP64/FP32, 4/5 the fused PCG
- * update kernel FP64/FP32, 6 a 256 MiB device-to-device streaming copy
- * (calibration of byte counters); 7..12 time one kernel inside `reps`
- * directly launched PCG iterations (direction, SpMV, update) from the start
- * and stop timestamps of that kernel's own dispatch (hipExtLaunchKernelGGL
- * events): 7/8 the fused SpMV FP64/FP32, 9/10 the update, 11/12 the
- * direction; 13/14 the fused SpMV FP64/FP32 launched `reps` times back to
+ * the context stream. which: 0/1 plain SpMV FP64/FP32, 2/3 the PCG SpMV
+ * kernel (K p + the p.Ap, z.Ap, Ap.diag^-1.Ap partial sums) FP64/FP32, 4/5
+ * the PCG vector kernel (update + next search direction) FP64/FP32, 6 a
+ * 256 MiB device-to-device streaming copy (calibration of byte counters);
+ * 7..10 time one kernel inside `reps` directly launched PCG iterations
+ * (SpMV, vector kernel) from the start and stop timestamps of that kernel's
+ * own dispatch (hipExtLaunchKernelGGL events): 7/8 the SpMV FP64/FP32, 9/10
+ * the vector kernel; 13/14 the SpMV FP64/FP32 launched `reps` times back to
  * back between one event pair (kernel plus dispatch gap, no per-launch event
- * cost). 2..5 and 7..14 use the F of the last solve. */
+ * cost). 11 and 12 are not used. 2..5 and 7..14 use the F of the last
+ * solve. */
 int vbh_bench_spmv(vbh_ctx* ctx, int which, int reps, double* ms_per_launch);
 /* Bytes one launch of the single-load-case solver SpMV reads from the device
  * operator in the layout it actually streams (no reference counterpart;

@@ -272,16 +272,14 @@ void Ctx::ensure_work(int nvec_len) {
   w_u.resize(N);
   w_r64.resize(N);
   w_r.resize(N);
-  w_z.resize(N);
   w_p.resize(N);
   w_Ap.resize(N);
   w_r32.resize(N);
-  w_z32.resize(N);
   w_p32.resize(N);
   w_Ap32.resize(N);
   if (!part_a.p) {
     part_a.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
-    part_b.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
+    part_b.resize(2 * static_cast<size_t>(kPartB));
     part_r.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
   }
   if (!st_dev.p) st_dev.resize(1);
@@ -290,7 +288,7 @@ void Ctx::ensure_work(int nvec_len) {
   launch_fill_f64(static_cast<long long>(N), w_F.p, 0.0, stream);
   const int rows = (format == 0) ? n : bsrd.nslices;
   red_n_a = grid_for(rows, rows_per_block(format));
-  red_n_b = grid_for(static_cast<long long>(N), kBlock * 4);
+  red_n_b = vector_grid(static_cast<long long>(N), n_cu);
   drop_graphs();
   work_len = N;
   work_version = operator_version;
@@ -323,28 +321,27 @@ CgRed Ctx::red() const {
 }
 
 CgBufs<double> Ctx::bufs64() {
-  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_z.p, w_p.p, w_Ap.p, w_u.p};
+  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_p.p, w_Ap.p, w_u.p};
 }
 
 CgBufs<float> Ctx::bufs32() {
-  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_z32.p, w_p32.p, w_Ap32.p, w_u.p};
+  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_p32.p, w_Ap32.p, w_u.p};
 }
 
 void Ctx::enqueue_iterations(int precision_mode, int count) {
+  if (count % 2 != 0) throw_error(VBH_ERR_ARG, "HIP PCG: iteration launch sequences have an even length");
   const int N = static_cast<int>(work_len);
   const PcgParams prm = params();
   const CgRed rd = red();
   for (int i = 0; i < count; ++i) {
     if (precision_mode == VBH_PREC_FP64) {
       const CgBufs<double> b = bufs64();
-      launch_cg_direction<double>(N, b, st_dev.p, rd, prm, 0, stream);
       launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, stream, 0);
-      launch_cg_update<double>(N, b, st_dev.p, rd, stream);
+      launch_cg_vector<double>(N, b, st_dev.p, rd, prm, 0, i, stream);
     } else {
       const CgBufs<float> b = bufs32();
-      launch_cg_direction<float>(N, b, st_dev.p, rd, prm, 1, stream);
       launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, stream, spmv_acc64);
-      launch_cg_update<float>(N, b, st_dev.p, rd, stream);
+      launch_cg_vector<float>(N, b, st_dev.p, rd, prm, 1, i, stream);
     }
   }
 }
@@ -398,7 +395,7 @@ PcgScalars Ctx::run_solve(int precision_mode) {
     if (!st_host->done) continue;
     if (precision_mode == VBH_PREC_MIXED && (st_host->flags & kFlagRefresh)) {
       launch_refresh_residual(format, csr, bsrd, w_F.p, w_u.p, w_r64.p, st_dev.p, rd, stream);
-      launch_refresh_reload(N, w_r64.p, diag32.p, w_r32.p, w_z32.p, st_dev.p, rd, prm, stream);
+      launch_refresh_reload(N, w_r64.p, diag32.p, w_r32.p, st_dev.p, rd, prm, stream);
       read_state();
       if (!st_host->done) continue;
     }
@@ -487,6 +484,7 @@ int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
                 std::to_string(prop.multiProcessorCount) + " CUs)";
     }
     c->arch = prop.gcnArchName;
+    c->n_cu = prop.multiProcessorCount;
     c->tol = tol > 0.0 ? tol : 1e-8;
     c->max_iters = max_iters > 0 ? max_iters : 10000;
     if (const char* e = std::getenv("VBH_REFRESH_DELTA")) c->refresh_delta = std::atof(e);
@@ -669,7 +667,8 @@ int vbh_spmv(vbh_ctx* h, const double* x, double* y) {
 }
 
 int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
-  if (!h || reps <= 0 || reps > 1000 || !ms_per_launch || precision < 0 || precision > 14) {
+  if (!h || reps <= 0 || reps > 1000 || !ms_per_launch || precision < 0 || precision > 14 ||
+      precision == 11 || precision == 12) {
     vbh::set_error("vbh_bench_spmv: bad argument");
     return VBH_ERR_ARG;
   }
@@ -685,16 +684,20 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
     if (precision >= 2) {  // fused solver kernels need an initialised state (F in w_F)
       if ((precision < 13 && precision % 2 == 0) || precision == 13) {
         vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
-        vbh::launch_cg_direction<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, prm, 0, c->stream);
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
                                     c->stream, 0);
       } else {
         vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
-        vbh::launch_cg_direction<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, prm, 1, c->stream);
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
                                    c->stream, c->spmv_acc64);
       }
     }
+    // tol 0, no iteration limit and no refresh pauses: timed solver kernels
+    // never find the state stopped
+    vbh::PcgParams run_all = prm;
+    run_all.tol = 0.0;
+    run_all.max_iters = 1 << 30;
+    int pos = 0;  // position of the next vector kernel in its launch sequence
     DBuf<char> cp_a, cp_b;
     const long long cp_bytes = 256ll << 20;
     if (precision == 6) {
@@ -712,9 +715,11 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
                                    c->stream, c->spmv_acc64);
       } else if (precision == 4) {
-        vbh::launch_cg_update<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
+        vbh::launch_cg_vector<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, run_all, 0, pos++,
+                                      c->stream);
       } else if (precision == 5) {
-        vbh::launch_cg_update<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
+        vbh::launch_cg_vector<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, run_all, 0, pos++,
+                                     c->stream);
       } else if (precision == 0) {
         if (c->format == 0) vbh::launch_spmv_csr<double>(c->csr, c->w_r64.p, c->w_Ap.p, c->stream);
         else vbh::launch_spmv_bsr<double>(c->bsrd, c->w_r64.p, c->w_Ap.p, c->stream);
@@ -723,18 +728,14 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         else vbh::launch_spmv_bsr<float>(c->bsrd, c->w_p32.p, c->w_Ap32.p, c->stream);
       }
     };
-    if (precision >= 7 && precision <= 12) {
-      // In context: `reps` PCG iterations (direction, SpMV, update) launched
+    if (precision >= 7 && precision <= 10) {
+      // In context: `reps` PCG iterations (SpMV, vector kernel) launched
       // directly; the timed kernel goes out through hipExtLaunchKernelGGL
       // with a start/stop event pair, which records that dispatch's own begin
       // and end, so the mean is the kernel duration a profiler reports, with
-      // p just written by the direction kernel as in a solve. tol 0 and no
-      // refresh pauses keep every iteration running.
+      // p just written by the vector kernel as in a solve.
       const bool f32 = (precision % 2) == 0;
-      const int kind = (precision - 7) / 2;  // 0 SpMV, 1 update, 2 direction
-      vbh::PcgParams run = prm;
-      run.tol = 0.0;
-      run.max_iters = 1 << 30;
+      const int kind = (precision - 7) / 2;  // 0 SpMV, 1 vector kernel
       std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps), nullptr);
       for (auto& e : ev) VBH_HIP(hipEventCreate(&e));
       if (f32) vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
@@ -742,19 +743,18 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
       for (int i = 0; i < reps + 1; ++i) {  // iteration 0 warms up
         hipEvent_t e0 = i ? ev[2 * static_cast<size_t>(i - 1)] : nullptr;
         hipEvent_t e1 = i ? ev[2 * static_cast<size_t>(i - 1) + 1] : nullptr;
-        hipEvent_t d0 = kind == 2 ? e0 : nullptr, d1 = kind == 2 ? e1 : nullptr;
         hipEvent_t s0 = kind == 0 ? e0 : nullptr, s1 = kind == 0 ? e1 : nullptr;
-        hipEvent_t u0 = kind == 1 ? e0 : nullptr, u1 = kind == 1 ? e1 : nullptr;
+        hipEvent_t v0 = kind == 1 ? e0 : nullptr, v1 = kind == 1 ? e1 : nullptr;
         if (f32) {
-          vbh::launch_cg_direction<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, run, 0, c->stream, d0, d1);
           vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
                                      c->stream, c->spmv_acc64, s0, s1);
-          vbh::launch_cg_update<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream, u0, u1);
+          vbh::launch_cg_vector<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, run_all, 0, i,
+                                       c->stream, v0, v1);
         } else {
-          vbh::launch_cg_direction<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, run, 0, c->stream, d0, d1);
           vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
                                       c->stream, 0, s0, s1);
-          vbh::launch_cg_update<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream, u0, u1);
+          vbh::launch_cg_vector<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, run_all, 0, i,
+                                        c->stream, v0, v1);
         }
       }
       VBH_HIP(hipStreamSynchronize(c->stream));

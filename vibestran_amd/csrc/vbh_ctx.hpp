@@ -61,6 +61,7 @@ struct MeshDev {
 struct Ctx {
   // device / stream
   int device = 0;
+  int n_cu = 256;  // compute units of the device
   std::string name, arch;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -130,8 +131,8 @@ struct Ctx {
   // solve work space
   size_t work_len = 0;
   long long work_version = -1;
-  DBuf<double> w_F, w_u, w_r64, w_r, w_z, w_p, w_Ap;
-  DBuf<float> w_r32, w_z32, w_p32, w_Ap32;
+  DBuf<double> w_F, w_u, w_r64, w_r, w_p, w_Ap;
+  DBuf<float> w_r32, w_p32, w_Ap32;
   DBuf<double> part_a, part_b, part_r;
   int red_n_a = 1, red_n_b = 1;
   DBuf<PcgScalars> st_dev;

@@ -60,6 +60,7 @@ struct DBuf {
 constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kMaxGrid = 2048;         // grid-stride cap: bounds the partial sums
 constexpr int kMaxRed = 4;             // values per grid reduction
+constexpr int kPartB = kMaxRed * kMaxGrid;  // stride of the two partB buffers
 
 // Device-side PCG scalars (one instance per right-hand side). Written only by
 // block 0 of a kernel, in fields no other block of that kernel reads.
@@ -70,8 +71,8 @@ struct PcgScalars {
   double rel_true;       // last FP64 true relative residual (mixed)
   double hist_rel[2];    // refresh history, slot = refresh count & 1
   int hist_stalls[2];
-  int iter_done;         // completed iterations (written by cg_update)
-  int iter_cur;          // iteration in progress (written by cg_direction)
+  int iter_done;         // completed iterations (written by cg_vector)
+  int iter_cur;          // iteration in progress (written by cg_spmv)
   int resume_iter;       // iteration resumed after a refresh (written by refresh_reload)
   int done, flags, refreshes;
 };
@@ -95,8 +96,10 @@ struct PcgParams {
 // Per-block partial sums passed from one kernel to the next. nA/nB/nR are the
 // grid sizes of the producing kernels (<= kMaxGrid).
 struct CgRed {
-  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap
-  double* partB;  // from cg_init / cg_update / refresh_reload: r.z, r.r, (F.F)
+  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap, z.Ap, Ap.diag^-1.Ap
+  double* partB;  // two buffers of kPartB: r.z, r.r, (F.F) from cg_init (buffer 0),
+                  // cg_vector of iteration i (buffer (i+1)&1), refresh_reload
+                  // (the paused iteration's buffer)
   double* partR;  // from refresh_residual: |F - K u|^2
   int nA, nB, nR;
 };
@@ -106,7 +109,6 @@ struct CgBufs {
   const double* F;
   const T* diag;
   T* r;
-  T* z;
   T* p;
   T* Ap;
   double* u64;
@@ -178,6 +180,15 @@ __host__ __device__ constexpr int sell_nps(int B) { return kSliceRows / B + 2; }
 
 // ---- kernel launchers (vbh_kernels.hip) ------------------------------------------
 int grid_for(long long work_items, int items_per_block);
+// Grid of the PCG vector kernels: four rows per thread, at most two
+// workgroups per compute unit. Every workgroup reduces all partial sums of
+// the previous kernels, so fewer, fuller workgroups cost less than one quad
+// per thread over more of them (MI355X, 605k rows: 12.7 us at 512
+// workgroups, 14.7 us at 591).
+inline int vector_grid(long long n, int n_cu) {
+  const int g = grid_for(n, kBlock * 4);
+  return g < 2 * n_cu ? g : 2 * n_cu;
+}
 
 template <typename T>
 void launch_spmv_csr(const CsrDev& A, const T* x, T* y, hipStream_t s);
@@ -194,19 +205,19 @@ template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
                     PcgScalars* st, const CgRed& red, hipStream_t s, int acc64,
                     hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+// position: the launch's place in its sequence of iterations (a captured
+// graph or a run of direct launches); sequences have an even length and
+// start after cg_init or refresh_reload (which write partB buffer 0).
 template <typename T>
-void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
-                         const PcgParams& prm, int mixed, hipStream_t s, hipEvent_t t0 = nullptr,
-                         hipEvent_t t1 = nullptr);
-template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s,
+void launch_cg_vector(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
+                      const PcgParams& prm, int mixed, int position, hipStream_t s,
                       hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // Mixed refresh: r64 = F - K64 u64 (FP64), then decide and reload r32.
 void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
                              const double* u64, double* r64, PcgScalars* st, const CgRed& red,
                              hipStream_t s);
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
-                           PcgScalars* st, const CgRed& red, const PcgParams& prm, hipStream_t s);
+void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
+                           const CgRed& red, const PcgParams& prm, hipStream_t s);
 
 // Device address of the allocation-free solver's scratch array on `device`
 // (current), taken once per device (vbh_noalloc.hip).

@@ -252,32 +252,52 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
   }
 }
 
-// ---- Jacobi-PCG: three kernels per iteration -------------------------------------------
+// ---- Jacobi-PCG: two kernels per iteration ---------------------------------------------
 // The reference's recurrence (cuda_pcg_backend_impl.hpp:1867-1954) with every
 // global sum produced as per-block partials and reduced by the NEXT kernel
 // (every block redundantly, fixed order): deterministic, no atomics, no
-// in-kernel fences, scalars bit-identical across blocks.
+// in-kernel fences, scalars bit-identical across blocks. The search direction
+// of the next iteration is formed by the vector kernel of this one:
 //
-//   init          u = 0, r = F, z = r/diag, p = 0; partials (r.z, r.r, F.F)   -> partB
-//   cg_direction  reduce partB -> rz_i, rr_i; convergence / pause / limit
-//                 decision (the reference's finalize step, impl:704-738);
-//                 beta = rz_i / rz_{i-1};  p = z + beta p
-//   cg_spmv       Ap = K p (plain SpMV); partial p.Ap                          -> partA
-//   cg_update     reduce partA -> alpha = rz_i / p.Ap (breakdown if p.Ap <= 0);
-//                 u += alpha p (FP64); r -= alpha Ap; z = r/diag;
-//                 partials (r.z, r.r)                                          -> partB
+//   init       u = 0, r = F, p = r/diag; partials (r.z, r.r, F.F)            -> partB
+//   cg_spmv    Ap = K p; partials (p.Ap, z.Ap, Ap.diag^-1.Ap), z = r/diag
+//              formed in registers                                           -> partA
+//   cg_vector  reduce partB -> rz_i = r.z, rr_i = r.r of the current r; the
+//              convergence / pause / limit decision (the reference's
+//              finalize step, impl:704-738); reduce partA ->
+//              alpha = rz_i / p.Ap (breakdown if p.Ap <= 0),
+//              rz_{i+1} = rz_i - 2 alpha z.Ap + alpha^2 Ap.diag^-1.Ap (the
+//              value of (r - alpha Ap).diag^-1.(r - alpha Ap)),
+//              beta = rz_{i+1} / rz_i;
+//              u += alpha p (FP64); r -= alpha Ap; p = r/diag + beta p;
+//              partials (r.z, r.r) of the new r                              -> partB
 //
-// Mixed precision pauses in cg_direction when the recursive residual fell by
-// refresh_delta since the last refresh (or below tol); the host then runs
+// z = r/diag is never stored: both kernels form it from r (the same FP
+// operation, so the same value).
+//
+// alpha always takes r.z summed over the vectors themselves; only beta takes
+// the expanded rz_{i+1}. (A CPU model of the 605k-DOF hex block: 1,639 mixed
+// iterations so, 2,465 with the expansion in alpha as well, 1,657 for the
+// three-kernel form that reduced r.z before forming p.)
+//
+// Mixed precision pauses in cg_vector, before its update, when the
+// recursive residual fell by refresh_delta since the last refresh (or below
+// tol); the host then runs
 //   refresh_residual  r64 = F - K64 u (FP64)                                   -> partR
 //   refresh_reload    reduce partR: converged / stalled / continue with
-//                     r = (float) r64, z = r/diag; partials (r.z, r.r)        -> partB
-// and cg_direction resumes the paused iteration with the refreshed residual
-// (beta from the refreshed r.z: reliable-update CG).
+//                     r = (float) r64; partials (r.z, r.r)                    -> partB
+// and the next cg_spmv recomputes K p with p kept (reliable update, van der
+// Vorst and Ye: only r is replaced), so its z.Ap uses the refreshed z, and
+// cg_vector resumes the paused iteration.
 //
 // Only block 0 writes the scalar state, and never a field that other blocks
-// of the same kernel read: iteration counters are split (iter_cur written by
-// cg_direction, iter_done by cg_update) and rz is double-buffered.
+// of the same kernel read: cg_vector reads the iteration as iter_cur, which
+// cg_spmv copies from iter_done (written by cg_vector); norm_b and
+// rr_refresh are written only in iterations whose decision does not read
+// them. partB has two buffers: the vector kernel launched at position j of
+// a launch sequence (a captured graph, the directly launched iterations)
+// reads buffer j & 1 and writes the other; cg_init and refresh_reload write
+// buffer 0, which position 0 reads, and every sequence has an even length.
 
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nprod,
@@ -307,58 +327,66 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restri
     for (int i = 0; i < NV; ++i) part[i * kMaxGrid + blockIdx.x] = v[i];
 }
 
-constexpr int kLds = 5 * kMaxRed;
+constexpr int kLds = 8 * kMaxRed;
 
-// Broadcast of state words: thread 0 reads, the block agrees.
+// Broadcast of the SpMV's state words (done, iter_done): thread 0 reads, the
+// block agrees.
 __device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
   if (threadIdx.x == 0) {
     s_i[0] = st->done;
     s_i[1] = st->iter_done;
-    s_i[2] = st->iter_cur;
-    s_i[3] = st->resume_iter;
   }
   __syncthreads();
 }
 
-// The head of a vector kernel, split in two so the caller can put its own
+// The head of the vector kernel, split in two so the caller can put its own
 // stream loads between them:
 //   head_issue   every load the head needs, all in flight at once: this
-//                thread's kMaxGrid/kBlock partial sums (unrolled, so one
+//                thread's partial sums of both producers (unrolled, so one
 //                memory round trip instead of one per grid-stride step) and
 //                the state words (thread 0);
-//   head_finish  the block sum of the partials and the broadcast of the words.
+//   head_finish  one block sum of all partials and the broadcast of the words.
 // Partials are issued before any stream load, so they return first (vector
 // loads complete in order) and are read while they are still in L2.
-// Only the four state words go through the head: the double scalars
-// (norm_b, rr_refresh, rz) are read after the decision that needs them --
-// loading them up front with the words made both vector kernels ~12 us slower
-// per launch on MI355X.
+// Only the state words go through the head: the double scalars (norm_b,
+// rr_refresh) are read after the decision that needs them.
 struct Head {
   int done, iter_done, iter_cur, resume_iter;
 };
 
-// NJ partial sums per thread: the launchers pick the smallest NJ with
-// NJ * kBlock >= the producer's grid (1, 2, 4 or kMaxGrid / kBlock).
-template <int NV, int NJ>
+// NJA / NJB partial sums per thread for the SpMV's / the previous vector
+// kernel's grid: the smallest NJ with NJ * kBlock >= the grid (1, 2, 4, 8).
+template <int NJA, int NJB>
 struct HeadLoads {
-  double part[NV][NJ];
+  double a[3][NJA];  // p.Ap, z.Ap, Ap.diag^-1.Ap
+  double b[3][NJB];  // r.z, r.r, F.F (F.F: buffer 0 from cg_init)
   Head st;
 };
 
-template <int NV, int NJ>
-__device__ __forceinline__ void head_issue(const PcgScalars* st, const double* __restrict__ part,
-                                           int nprod, HeadLoads<NV, NJ>& hl) {
+template <int NJA, int NJB>
+__device__ __forceinline__ void head_issue(const PcgScalars* st, const CgRed& red,
+                                           const double* __restrict__ partB,
+                                           HeadLoads<NJA, NJB>& hl) {
   // every slot of the kMaxGrid-wide partial arrays is allocated, so the loads
   // are unconditional (no branch between them) and the unused ones masked
 #pragma unroll
-  for (int j = 0; j < NJ; ++j)
+  for (int j = 0; j < NJA; ++j)
 #pragma unroll
-    for (int i = 0; i < NV; ++i) hl.part[i][j] = part[i * kMaxGrid + threadIdx.x + j * kBlock];
+    for (int i = 0; i < 3; ++i) hl.a[i][j] = red.partA[i * kMaxGrid + threadIdx.x + j * kBlock];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j)
-    if (threadIdx.x + j * kBlock >= nprod)
+  for (int j = 0; j < NJB; ++j)
 #pragma unroll
-      for (int i = 0; i < NV; ++i) hl.part[i][j] = 0.0;
+    for (int i = 0; i < 3; ++i) hl.b[i][j] = partB[i * kMaxGrid + threadIdx.x + j * kBlock];
+#pragma unroll
+  for (int j = 0; j < NJA; ++j)
+    if (threadIdx.x + j * kBlock >= red.nA)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) hl.a[i][j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < NJB; ++j)
+    if (threadIdx.x + j * kBlock >= red.nB)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) hl.b[i][j] = 0.0;
   if (threadIdx.x == 0) {
     hl.st.done = st->done;
     hl.st.iter_done = st->iter_done;
@@ -367,24 +395,28 @@ __device__ __forceinline__ void head_issue(const PcgScalars* st, const double* _
   }
 }
 
-template <int NV, int NJ>
-__device__ __forceinline__ void head_finish(HeadLoads<NV, NJ>& hl, double (&out)[NV], Head& h,
+// out[0..2] = partA sums, out[3..5] = partB sums
+template <int NJA, int NJB>
+__device__ __forceinline__ void head_finish(HeadLoads<NJA, NJB>& hl, double (&out)[6], Head& h,
                                             double* lds, Head* sh) {
-  double t[NV];
+  double t[6];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
+  for (int i = 0; i < 3; ++i) {
     t[i] = 0.0;
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) t[i] += hl.part[i][j];
+    for (int j = 0; j < NJA; ++j) t[i] += hl.a[i][j];
+    t[3 + i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJB; ++j) t[3 + i] += hl.b[i][j];
   }
   if (threadIdx.x == 0) *sh = hl.st;
-  block_sum<NV>(t, lds);  // its barriers also publish *sh
+  block_sum<6>(t, lds);  // its barriers also publish *sh
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int i = 0; i < NV; ++i) lds[4 * kMaxRed + i] = t[i];
+    for (int i = 0; i < 6; ++i) lds[6 * kMaxRed + i] = t[i];
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < NV; ++i) out[i] = lds[4 * kMaxRed + i];
+  for (int i = 0; i < 6; ++i) out[i] = lds[6 * kMaxRed + i];
   h = *sh;
   __syncthreads();
 }
@@ -400,8 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScala
     const T zi = ri / b.diag[i];
     b.u64[i] = 0.0;
     b.r[i] = ri;
-    b.z[i] = zi;
-    b.p[i] = T(0);
+    b.p[i] = zi;
     v[0] += static_cast<double>(ri) * static_cast<double>(zi);
     v[1] += static_cast<double>(ri) * static_cast<double>(ri);
     v[2] += f * f;
@@ -423,33 +454,108 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScala
   }
 }
 
-template <typename T, int NJ>
-__global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, PcgScalars* st,
-                                                         CgRed red, PcgParams prm, int mixed) {
+// The three partial sums of the PCG SpMV for row g with Ap_g = v.
+template <typename T>
+__device__ __forceinline__ void spmv_sums(const CgBufs<T>& b, int g, T v, double (&d)[3]) {
+  const double av = static_cast<double>(v);
+  const T dg = b.diag[g];
+  const T zg = b.r[g] / dg;
+  d[0] += static_cast<double>(b.p[g]) * av;
+  d[1] += static_cast<double>(zg) * av;
+  d[2] += av * av / static_cast<double>(dg);
+}
+
+// Block 0 copies the iteration count for the vector kernel (see above).
+__device__ __forceinline__ void spmv_iteration(PcgScalars* st, const int* s_i) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = s_i[1];
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
+                                                        CgRed red) {
+  __shared__ double lds[kLds];
+  __shared__ int s_i[2];
+  read_words(st, s_i);
+  if (s_i[0]) return;
+  spmv_iteration(st, s_i);
+  const T* val = vals<T>(A);
+  const int lg = threadIdx.x % G;
+  const int stride = gridDim.x * (blockDim.x / G);
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
+    const T s = csr_row<T, G>(A, val, b.p, r, lg);
+    if (lg == 0) {
+      b.Ap[r] = s;
+      spmv_sums(b, r, s, d);
+    }
+  }
+  write_partials<3>(d, red.partA, lds);
+}
+
+template <typename T, int B, typename Acc = T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
+                                                        CgRed red) {
+  __shared__ double lds[kLds];
+  __shared__ int s_i[2];
+  read_words(st, s_i);
+  if (s_i[0]) return;
+  spmv_iteration(st, s_i);
+  const T* val = svals<T>(A);
+  const int l = threadIdx.x & (kSliceRows - 1);
+  const int nrows = A.nbr * B;
+  const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
+       s += gridDim.x * (blockDim.x / kSliceRows)) {
+    const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
+    const int g = kSliceRows * s + l;
+    if (g < nrows) {
+      // split FP32 stream: off-diagonal blocks summed in FP32, the corrected
+      // FP64 diagonal block added in FP64
+      const T v = split ? static_cast<T>(static_cast<double>(acc) +
+                                         diag_block_row<T, B>(A.dblk, b.p, g))
+                        : static_cast<T>(acc);
+      b.Ap[g] = v;
+      spmv_sums(b, g, v, d);
+    }
+  }
+  write_partials<3>(d, red.partA, lds);
+}
+
+template <typename T, int NJA, int NJB>
+__global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgScalars* st,
+                                                      CgRed red, PcgParams prm, int mixed,
+                                                      int par) {
   __shared__ double lds[kLds];
   __shared__ Head sh;
   const int n4 = n >> 2;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  HeadLoads<3, NJ> hl;
-  head_issue<3, NJ>(st, red.partB, red.nB, hl);
+  HeadLoads<NJA, NJB> hl;
+  head_issue<NJA, NJB>(st, red, red.partB + par * kPartB, hl);
   // this thread's first rows go out behind the head loads
-  Quad<T> z0{}, p0{};
+  Quad<double> u0{};
+  Quad<T> p0{}, ap0{}, d0{}, r0{};
   if (t0 < n4) {
-    z0 = ld4(b.z + 4 * t0);
-    p0 = ld4(b.p + 4 * t0);
+    const int k = 4 * t0;
+    u0 = ld4(b.u64 + k);
+    p0 = ld4(b.p + k);
+    ap0 = ld4(b.Ap + k);
+    d0 = ld4(b.diag + k);
+    r0 = ld4(b.r + k);
   }
-  double v[3];
+  double v6[6];
   Head h;
-  head_finish<3, NJ>(hl, v, h, lds, &sh);
+  head_finish<NJA, NJB>(hl, v6, h, lds, &sh);
   if (h.done) return;
-  const int i = h.iter_done;
+  const int i = h.iter_cur;
   const bool resumed = (h.resume_iter == i);
   const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
-  const double rz = v[0], rr = v[1];
-  double norm_b, beta = 0.0, rel;
+  const double pAp = v6[0], zAp = v6[1], ApDAp = v6[2];
+  const double rz = v6[3], rr = v6[4];
+  double norm_b, rel;
   int stop = 0;
   if (i == 0 && !resumed) {
-    norm_b = sqrt(fabs(v[2]));
+    norm_b = sqrt(fabs(v6[5]));
     rel = 1.0;
     if (norm_b < 1e-300) {  // zero load: u = 0 after 0 iterations (impl:1822-1830)
       stop = kFlagZeroRhs;
@@ -469,157 +575,46 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, CgBufs<T> b, Pcg
       stop = kFlagRefresh;  // pause for an FP64 true-residual refresh
     }
     if (!stop && i >= prm.max_iters) stop = kFlagMaxIter;
-    if (!stop) beta = rz / st->rz[(i - 1) & 1];
   }
+  if (!stop && !(pAp > 0.0)) stop = kFlagBreakdown;  // impl:691-696
   if (stop) {
     if (lead) {
       if (i == 0 && !resumed) st->norm_b = norm_b;
-      st->rz[i & 1] = rz;
+      st->rz[0] = rz;
       st->rr = rr;
-      st->rel = rel;
+      st->pAp = pAp;
+      st->rel = (stop == kFlagBreakdown) ? 1e300 : rel;
       st->flags |= stop;
       st->done = 1;
     }
     return;
   }
-  const T bt = static_cast<T>(beta);
-  if (t0 < n4) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) p0.v[j] = z0.v[j] + bt * p0.v[j];
-    st4(b.p + 4 * t0, p0);
-  }
-  for (int q = t0 + nt; q < n4; q += nt) {
-    const Quad<T> z = ld4(b.z + 4 * q);
-    Quad<T> p = ld4(b.p + 4 * q);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) p.v[j] = z.v[j] + bt * p.v[j];
-    st4(b.p + 4 * q, p);
-  }
-  for (int k = 4 * n4 + t0; k < n; k += nt) b.p[k] = b.z[k] + bt * b.p[k];
-  if (lead) {
-    if (i == 0 && !resumed) {
-      st->norm_b = norm_b;
-      st->rr_refresh = rr;
-    }
-    if (resumed) st->rr_refresh = rr;
-    st->rz[i & 1] = rz;
-    st->rr = rr;
-    st->rel = rel;
-    st->beta = beta;
-    st->iter_cur = i;
-  }
-}
-
-template <typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red) {
-  __shared__ double lds[kLds];
-  __shared__ int s_i[4];
-  read_words(st, s_i);
-  if (s_i[0]) return;
-  const T* val = vals<T>(A);
-  const int lg = threadIdx.x % G;
-  const int stride = gridDim.x * (blockDim.x / G);
-  double d[1] = {0.0};
-  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
-    const T s = csr_row<T, G>(A, val, b.p, r, lg);
-    if (lg == 0) {
-      b.Ap[r] = s;
-      d[0] += static_cast<double>(b.p[r]) * static_cast<double>(s);
-    }
-  }
-  write_partials<1>(d, red.partA, lds);
-}
-
-template <typename T, int B, typename Acc = T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red) {
-  __shared__ double lds[kLds];
-  __shared__ int s_i[4];
-  read_words(st, s_i);
-  if (s_i[0]) return;
-  const T* val = svals<T>(A);
-  const int l = threadIdx.x & (kSliceRows - 1);
-  const int nrows = A.nbr * B;
-  const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
-  double d[1] = {0.0};
-  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
-       s += gridDim.x * (blockDim.x / kSliceRows)) {
-    const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
-    const int g = kSliceRows * s + l;
-    if (g < nrows) {
-      // split FP32 stream: off-diagonal blocks summed in FP32, the corrected
-      // FP64 diagonal block added in FP64
-      const T v = split ? static_cast<T>(static_cast<double>(acc) +
-                                         diag_block_row<T, B>(A.dblk, b.p, g))
-                        : static_cast<T>(acc);
-      b.Ap[g] = v;
-      d[0] += static_cast<double>(b.p[g]) * static_cast<double>(v);
-    }
-  }
-  write_partials<1>(d, red.partA, lds);
-}
-
-template <typename T, int NJ>
-__global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgScalars* st,
-                                                      CgRed red) {
-  __shared__ double lds[kLds];
-  __shared__ Head sh;
-  const int n4 = n >> 2;
-  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  HeadLoads<1, NJ> hl;
-  head_issue<1, NJ>(st, red.partA, red.nA, hl);
-  // this thread's first rows go out behind the head loads
-  Quad<double> u0{};
-  Quad<T> p0{}, ap0{}, d0{}, r0{};
-  if (t0 < n4) {
-    const int k = 4 * t0;
-    u0 = ld4(b.u64 + k);
-    p0 = ld4(b.p + k);
-    ap0 = ld4(b.Ap + k);
-    d0 = ld4(b.diag + k);
-    r0 = ld4(b.r + k);
-  }
-  double v1[1];
-  Head h;
-  head_finish<1, NJ>(hl, v1, h, lds, &sh);
-  if (h.done) return;
-  const int i = h.iter_cur;
-  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
-  const double pAp = v1[0];
-  if (!(pAp > 0.0)) {  // breakdown (impl:691-696)
-    if (lead) {
-      st->pAp = pAp;
-      st->alpha = 0.0;
-      st->rel = 1e300;
-      st->flags |= kFlagBreakdown;
-      st->done = 1;
-    }
-    return;
-  }
-  const double alpha = st->rz[i & 1] / pAp;
-  const T a = static_cast<T>(alpha);
-  double v[3] = {0.0, 0.0, 0.0};
-  auto rows4 = [&](int k, Quad<double>& u, const Quad<T>& p, const Quad<T>& ap,
-                   const Quad<T>& d, Quad<T>& r) {
-    Quad<T> z;
+  const double alpha = rz / pAp;
+  const double rz_next = rz - 2.0 * alpha * zAp + alpha * alpha * ApDAp;
+  const double beta = rz_next > 0.0 ? rz_next / rz : 0.0;
+  const T a = static_cast<T>(alpha), bt = static_cast<T>(beta);
+  double v[2] = {0.0, 0.0};
+  auto rows4 = [&](int k, Quad<double>& u, Quad<T>& p, const Quad<T>& ap, const Quad<T>& d,
+                   Quad<T>& r) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       u.v[j] += alpha * static_cast<double>(p.v[j]);
       r.v[j] = r.v[j] - a * ap.v[j];
-      z.v[j] = r.v[j] / d.v[j];
-      v[0] += static_cast<double>(r.v[j]) * static_cast<double>(z.v[j]);
+      const T z = r.v[j] / d.v[j];
+      p.v[j] = z + bt * p.v[j];
+      v[0] += static_cast<double>(r.v[j]) * static_cast<double>(z);
       v[1] += static_cast<double>(r.v[j]) * static_cast<double>(r.v[j]);
     }
     st4(b.u64 + k, u);
     st4(b.r + k, r);
-    st4(b.z + k, z);
+    st4(b.p + k, p);
   };
   if (t0 < n4) rows4(4 * t0, u0, p0, ap0, d0, r0);
   for (int q = t0 + nt; q < n4; q += nt) {
     const int k = 4 * q;
     Quad<double> u = ld4(b.u64 + k);
-    const Quad<T> p = ld4(b.p + k), ap = ld4(b.Ap + k), d = ld4(b.diag + k);
+    Quad<T> p = ld4(b.p + k);
+    const Quad<T> ap = ld4(b.Ap + k), d = ld4(b.diag + k);
     Quad<T> r = ld4(b.r + k);
     rows4(k, u, p, ap, d, r);
   }
@@ -628,14 +623,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, CgBufs<T> b, PcgSca
     const T r = b.r[k] - a * b.Ap[k];
     const T z = r / b.diag[k];
     b.r[k] = r;
-    b.z[k] = z;
+    b.p[k] = z + bt * b.p[k];
     v[0] += static_cast<double>(r) * static_cast<double>(z);
     v[1] += static_cast<double>(r) * static_cast<double>(r);
   }
-  write_partials<3>(v, red.partB, lds);
+  write_partials<2>(v, red.partB + (par ^ 1) * kPartB, lds);
   if (lead) {
+    if (i == 0 && !resumed) {
+      st->norm_b = norm_b;
+      st->rr_refresh = rr;
+    }
+    if (resumed) st->rr_refresh = rr;
+    st->rz[0] = rz;
+    st->rr = rr;
+    st->rel = rel;
     st->pAp = pAp;
     st->alpha = alpha;
+    st->beta = beta;
     st->iter_done = i + 1;
   }
 }
@@ -685,12 +689,12 @@ __global__ __launch_bounds__(kBlock) void k_refresh_residual_bsr(BsrDev A, const
   if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
-// Decide from the FP64 true residual; on "continue" reload r = (float) r64,
-// z = r/diag (partials for cg_direction) and mark the paused iteration resumed.
+// Decide from the FP64 true residual; on "continue" reload r = (float) r64
+// (partials r.z, r.r into partB buffer 0, which the next launch sequence's
+// first vector kernel reads) and mark the paused iteration resumed.
 __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* __restrict__ r64,
                                                            const float* __restrict__ diag,
-                                                           float* __restrict__ r,
-                                                           float* __restrict__ z, PcgScalars* st,
+                                                           float* __restrict__ r, PcgScalars* st,
                                                            CgRed red, PcgParams prm) {
   __shared__ double lds[kLds];
   double v1[1];
@@ -721,16 +725,15 @@ __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* 
     }
     return;  // done stays set
   }
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[2] = {0.0, 0.0};
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const float rk = static_cast<float>(r64[k]);
     const float zk = rk / diag[k];
     r[k] = rk;
-    z[k] = zk;
     v[0] += static_cast<double>(rk) * static_cast<double>(zk);
     v[1] += static_cast<double>(rk) * static_cast<double>(rk);
   }
-  write_partials<3>(v, red.partB, lds);
+  write_partials<2>(v, red.partB, lds);
   if (lead) {
     st->resume_iter = st->iter_done;
     st->flags &= ~kFlagRefresh;
@@ -742,15 +745,18 @@ __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* 
 // One workgroup per long block row (grid-stride over the rows): the row's
 // remaining blocks (read from the BSR arrays, FP64 sums) are reduced over the
 // workgroup and added to what the sliced-ELL pass wrote for the row's B
-// scalar rows. MODE 0: y += rest. MODE 1 (PCG SpMV): y += rest, and the
-// change of p.Ap, p_g (y_new - y_old), is added to partial slot blockIdx.x,
-// which the SpMV pass wrote (grid <= its grid; one writer per slot, so the
-// sums stay deterministic). MODE 2 (refresh, FP64): r -= rest and the change
-// of r.r into the refresh's slot. With the split FP32 stream the diagonal
-// block is skipped (the SpMV added its FP64 form).
+// scalar rows. MODE 0: y += rest. MODE 1 (PCG SpMV, x = p, y = Ap, z = r):
+// y += rest, and the changes of the three SpMV sums (p.Ap, z.Ap, Ap.diag^-1.Ap)
+// are added to partial slot blockIdx.x, which the SpMV pass wrote (grid <=
+// its grid; one writer per slot, so the sums stay deterministic). MODE 2
+// (refresh, FP64): r -= rest and the change of r.r into the refresh's slot.
+// With the split FP32 stream the diagonal block is skipped (the SpMV added
+// its FP64 form).
 template <typename T, int B, int MODE>
 __global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __restrict__ x,
                                                           T* __restrict__ y,
+                                                          const T* __restrict__ z,  // r (MODE 1)
+                                                          const T* __restrict__ diag,
                                                           double* __restrict__ part,
                                                           const PcgScalars* st) {
   __shared__ double lds[4 * 8];
@@ -763,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __r
   const bool split = sizeof(T) == 4 && A.dblk != nullptr;
   const T* __restrict__ val = sizeof(T) == 4 ? reinterpret_cast<const T*>(A.v32)
                                              : reinterpret_cast<const T*>(A.v64);
-  double corr = 0.0;
+  double corr[3] = {0.0, 0.0, 0.0};
   for (int li = blockIdx.x; li < A.nlong; li += gridDim.x) {
     const int I = A.lrows[li];
     const int o = A.brp[I], nb = A.brp[I + 1] - o;
@@ -790,16 +796,27 @@ __global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __r
         if (MODE == 2) {
           const double rn = y0 - acc[r];
           y[g] = static_cast<T>(rn);
-          corr += rn * rn - y0 * y0;
+          corr[0] += rn * rn - y0 * y0;
         } else {
           const T yn = static_cast<T>(y0 + acc[r]);
           y[g] = yn;
-          if (MODE == 1) corr += static_cast<double>(x[g]) * (static_cast<double>(yn) - y0);
+          if (MODE == 1) {
+            const double y1 = static_cast<double>(yn), dy = y1 - y0;
+            corr[0] += static_cast<double>(x[g]) * dy;
+            corr[1] += static_cast<double>(z[g] / diag[g]) * dy;
+            corr[2] += (y1 * y1 - y0 * y0) / static_cast<double>(diag[g]);
+          }
         }
       }
     }
   }
-  if (MODE != 0 && threadIdx.x == 0) part[blockIdx.x] += corr;
+  if (threadIdx.x == 0) {
+    if (MODE == 1)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) part[i * kMaxGrid + blockIdx.x] += corr[i];
+    else if (MODE == 2)
+      part[blockIdx.x] += corr[0];
+  }
 }
 
 __global__ void k_cast_f64_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
@@ -828,14 +845,16 @@ constexpr int kCsrG = 8;  // lanes per scalar CSR row
 // ---- launchers ------------------------------------------------------------------------
 // The overflow pass of long rows after a sliced-ELL pass (see k_sell_overflow).
 template <typename T, int MODE>
-static void launch_overflow(const BsrDev& A, const T* x, T* y, double* part, int nprod,
-                            const PcgScalars* st, hipStream_t s) {
+static void launch_overflow(const BsrDev& A, const T* x, T* y, const T* z, const T* diag,
+                            double* part, int nprod, const PcgScalars* st, hipStream_t s) {
   if (A.nlong == 0) return;
   const int g = MODE == 0 ? std::min(A.nlong, kMaxGrid) : std::min(A.nlong, nprod);
   if (A.B == 3)
-    hipLaunchKernelGGL((k_sell_overflow<T, 3, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, part, st);
+    hipLaunchKernelGGL((k_sell_overflow<T, 3, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, z, diag,
+                       part, st);
   else
-    hipLaunchKernelGGL((k_sell_overflow<T, 6, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, part, st);
+    hipLaunchKernelGGL((k_sell_overflow<T, 6, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, z, diag,
+                       part, st);
 }
 
 template <typename T>
@@ -858,7 +877,7 @@ void launch_spmv_bsr(const BsrDev& A, const T* x, T* y, hipStream_t s) {
     hipLaunchKernelGGL((k_spmv_bsr<T, 6, double>), g, dim3(kBlock), 0, s, A, x, y);
   else
     hipLaunchKernelGGL((k_spmv_bsr<T, 6>), g, dim3(kBlock), 0, s, A, x, y);
-  launch_overflow<T, 0>(A, x, y, nullptr, 0, nullptr, s);
+  launch_overflow<T, 0>(A, x, y, nullptr, nullptr, nullptr, 0, nullptr, s);
 }
 
 template <typename T>
@@ -891,7 +910,7 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red);
   else
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red);
-  if (format != 0) launch_overflow<T, 1>(Ab, b.p, b.Ap, red.partA, red.nA, st, s);
+  if (format != 0) launch_overflow<T, 1>(Ab, b.p, b.Ap, b.r, b.diag, red.partA, red.nA, st, s);
 }
 
 // partial sums per thread for a producer grid of nprod blocks
@@ -901,28 +920,21 @@ static int head_nj(int nprod) {
 static_assert(kMaxGrid / kBlock == 8, "head_nj covers up to 8 partials per thread");
 
 template <typename T>
-void launch_cg_direction(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
-                         const PcgParams& prm, int mixed, hipStream_t s, hipEvent_t t0,
-                         hipEvent_t t1) {
-  const dim3 g(red.nB);
-  switch (head_nj(red.nB)) {
-    case 1: VBH_LAUNCH((k_cg_direction<T, 1>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
-    case 2: VBH_LAUNCH((k_cg_direction<T, 2>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
-    case 4: VBH_LAUNCH((k_cg_direction<T, 4>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
-    default: VBH_LAUNCH((k_cg_direction<T, 8>), g, s, t0, t1, n, b, st, red, prm, mixed); break;
-  }
-}
-
-template <typename T>
-void launch_cg_update(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s,
+void launch_cg_vector(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
+                      const PcgParams& prm, int mixed, int position, hipStream_t s,
                       hipEvent_t t0, hipEvent_t t1) {
   const dim3 g(red.nB);
-  switch (head_nj(red.nA)) {
-    case 1: VBH_LAUNCH((k_cg_update<T, 1>), g, s, t0, t1, n, b, st, red); break;
-    case 2: VBH_LAUNCH((k_cg_update<T, 2>), g, s, t0, t1, n, b, st, red); break;
-    case 4: VBH_LAUNCH((k_cg_update<T, 4>), g, s, t0, t1, n, b, st, red); break;
-    default: VBH_LAUNCH((k_cg_update<T, 8>), g, s, t0, t1, n, b, st, red); break;
+  const int ja = head_nj(red.nA), jb = head_nj(red.nB), par = position & 1;
+#define VBH_VEC(JA, JB)                                                                  \
+  if (ja == JA && jb == JB) {                                                            \
+    VBH_LAUNCH((k_cg_vector<T, JA, JB>), g, s, t0, t1, n, b, st, red, prm, mixed, par);  \
+    return;                                                                              \
   }
+  VBH_VEC(1, 1) VBH_VEC(1, 2) VBH_VEC(1, 4) VBH_VEC(1, 8)
+  VBH_VEC(2, 1) VBH_VEC(2, 2) VBH_VEC(2, 4) VBH_VEC(2, 8)
+  VBH_VEC(4, 1) VBH_VEC(4, 2) VBH_VEC(4, 4) VBH_VEC(4, 8)
+  VBH_VEC(8, 1) VBH_VEC(8, 2) VBH_VEC(8, 4) VBH_VEC(8, 8)
+#undef VBH_VEC
 }
 
 #undef VBH_LAUNCH
@@ -939,12 +951,13 @@ void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, cons
   else
     hipLaunchKernelGGL((k_refresh_residual_bsr<6>), dim3(red.nR), dim3(kBlock), 0, s, Ab, F, u64,
                        r64, st, red);
-  if (format != 0) launch_overflow<double, 2>(Ab, u64, r64, red.partR, red.nR, st, s);
+  if (format != 0)
+    launch_overflow<double, 2>(Ab, u64, r64, nullptr, nullptr, red.partR, red.nR, st, s);
 }
 
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, float* z,
-                           PcgScalars* st, const CgRed& red, const PcgParams& prm, hipStream_t s) {
-  hipLaunchKernelGGL(k_refresh_reload, dim3(red.nB), dim3(kBlock), 0, s, n, r64, diag, r, z, st, red,
+void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
+                           const CgRed& red, const PcgParams& prm, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_reload, dim3(red.nB), dim3(kBlock), 0, s, n, r64, diag, r, st, red,
                      prm);
 }
 
@@ -978,15 +991,11 @@ template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const Cg
 template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
                                     PcgScalars*, const CgRed&, hipStream_t, int, hipEvent_t,
                                     hipEvent_t);
-template void launch_cg_direction<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
-                                          const PcgParams&, int, hipStream_t, hipEvent_t,
-                                          hipEvent_t);
-template void launch_cg_direction<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
-                                         const PcgParams&, int, hipStream_t, hipEvent_t,
-                                         hipEvent_t);
-template void launch_cg_update<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
-                                       hipStream_t, hipEvent_t, hipEvent_t);
-template void launch_cg_update<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
-                                      hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_cg_vector<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
+                                       const PcgParams&, int, int, hipStream_t, hipEvent_t,
+                                       hipEvent_t);
+template void launch_cg_vector<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&,
+                                      const PcgParams&, int, int, hipStream_t, hipEvent_t,
+                                      hipEvent_t);
 
 }  // namespace vbh

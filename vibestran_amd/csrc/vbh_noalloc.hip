@@ -41,7 +41,7 @@ size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // Carves the scratch array into the buffers of one solve of n_pad rows.
 struct ScratchLayout {
-  size_t st, partA, partB, partR, F, u, r64, r, z, p, Ap, end;
+  size_t st, partA, partB, partR, F, u, r64, r, p, Ap, end;
   ScratchLayout(size_t n_pad, bool mixed) {
     const size_t part = sizeof(double) * kMaxRed * kMaxGrid;
     const size_t vs = mixed ? sizeof(float) : sizeof(double);
@@ -51,7 +51,7 @@ struct ScratchLayout {
     partA = o;
     o = align_up(o + part);
     partB = o;
-    o = align_up(o + part);
+    o = align_up(o + 2 * part);
     partR = o;
     o = align_up(o + part);
     F = o;
@@ -61,8 +61,6 @@ struct ScratchLayout {
     r64 = o;
     if (mixed) o = align_up(o + sizeof(double) * n_pad);
     r = o;
-    o = align_up(o + vs * n_pad);
-    z = o;
     o = align_up(o + vs * n_pad);
     p = o;
     o = align_up(o + vs * n_pad);
@@ -108,7 +106,7 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
                 "vbh_solve_device_noalloc: " + std::to_string(N) + " operator rows need " +
                     std::to_string(L.end) + " bytes of scratch, more than its " +
                     std::to_string(kScratchBytes) + " (at most " +
-                    std::to_string((kScratchBytes - L.F) / (mixed ? 40 : 48)) +
+                    std::to_string((kScratchBytes - L.F) / (mixed ? 36 : 40)) +
                     " rows in this precision)");
   unsigned char* base = scratch_of(c.device);
   PcgScalars* st = reinterpret_cast<PcgScalars*>(base + L.st);
@@ -118,17 +116,15 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
   rd.partR = reinterpret_cast<double*>(base + L.partR);
   const int rows = (c.format == 0) ? c.n : c.bsrd.nslices;
   rd.nA = grid_for(rows, rows_per_block(c.format));
-  rd.nB = grid_for(static_cast<long long>(N), kBlock * 4);
+  rd.nB = vector_grid(static_cast<long long>(N), c.n_cu);
   rd.nR = rd.nA;
   double* F = reinterpret_cast<double*>(base + L.F);
   double* u = reinterpret_cast<double*>(base + L.u);
   double* r64 = reinterpret_cast<double*>(base + L.r64);
-  const CgBufs<double> b64{F, c.diag64.p,
-                           reinterpret_cast<double*>(base + L.r), reinterpret_cast<double*>(base + L.z),
-                           reinterpret_cast<double*>(base + L.p), reinterpret_cast<double*>(base + L.Ap),
-                           u};
-  const CgBufs<float> b32{F, c.diag32.p,
-                          reinterpret_cast<float*>(base + L.r), reinterpret_cast<float*>(base + L.z),
+  const CgBufs<double> b64{F, c.diag64.p, reinterpret_cast<double*>(base + L.r),
+                           reinterpret_cast<double*>(base + L.p),
+                           reinterpret_cast<double*>(base + L.Ap), u};
+  const CgBufs<float> b32{F, c.diag32.p, reinterpret_cast<float*>(base + L.r),
                           reinterpret_cast<float*>(base + L.p), reinterpret_cast<float*>(base + L.Ap),
                           u};
   hipStream_t s = c.stream;
@@ -145,7 +141,7 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
     VBH_HIP(hipMemcpyAsync(c.st_host, st, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     VBH_HIP(hipStreamSynchronize(s));
   };
-  const int per = c.iters_per_graph;
+  const int per = c.iters_per_graph;  // even (see launch_cg_vector)
   const long long cap = 2ll * (c.max_iters / per + 2) + 64;
   long long launches = 0;
   for (;;) {
@@ -153,20 +149,18 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
       throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
     for (int i = 0; i < per; ++i) {
       if (mixed) {
-        launch_cg_direction<float>(static_cast<int>(N), b32, st, rd, prm, 1, s);
         launch_cg_spmv<float>(c.format, c.csr, c.bsrd, b32, st, rd, s, c.spmv_acc64);
-        launch_cg_update<float>(static_cast<int>(N), b32, st, rd, s);
+        launch_cg_vector<float>(static_cast<int>(N), b32, st, rd, prm, 1, i, s);
       } else {
-        launch_cg_direction<double>(static_cast<int>(N), b64, st, rd, prm, 0, s);
         launch_cg_spmv<double>(c.format, c.csr, c.bsrd, b64, st, rd, s, 0);
-        launch_cg_update<double>(static_cast<int>(N), b64, st, rd, s);
+        launch_cg_vector<double>(static_cast<int>(N), b64, st, rd, prm, 0, i, s);
       }
     }
     read_state();
     if (!c.st_host->done) continue;
     if (mixed && (c.st_host->flags & kFlagRefresh)) {
       launch_refresh_residual(c.format, c.csr, c.bsrd, F, u, r64, st, rd, s);
-      launch_refresh_reload(static_cast<int>(N), r64, c.diag32.p, b32.r, b32.z, st, rd, prm, s);
+      launch_refresh_reload(static_cast<int>(N), r64, c.diag32.p, b32.r, st, rd, prm, s);
       read_state();
       if (!c.st_host->done) continue;
     }
