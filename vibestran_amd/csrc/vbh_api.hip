@@ -280,10 +280,10 @@ void Ctx::ensure_work(int nvec_len) {
   if (!part_a.p) {
     part_a.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
     part_b.resize(2 * static_cast<size_t>(kPartB));
-    part_r.resize(static_cast<size_t>(kMaxRed) * kMaxGrid);
   }
   if (!st_dev.p) st_dev.resize(1);
-  if (!st_host) VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PcgScalars), 0));
+  if (!st_host)
+    VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&st_host), 2 * sizeof(PcgScalars), 0));
   // zero the padding tails once (F beyond n stays zero)
   launch_fill_f64(static_cast<long long>(N), w_F.p, 0.0, stream);
   const int rows = (format == 0) ? n : bsrd.nslices;
@@ -313,19 +313,17 @@ CgRed Ctx::red() const {
   CgRed r;
   r.partA = part_a.p;
   r.partB = part_b.p;
-  r.partR = part_r.p;
   r.nA = red_n_a;
   r.nB = red_n_b;
-  r.nR = red_n_a;
   return r;
 }
 
 CgBufs<double> Ctx::bufs64() {
-  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_p.p, w_Ap.p, w_u.p};
+  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_p.p, w_Ap.p, w_u.p, nullptr};
 }
 
 CgBufs<float> Ctx::bufs32() {
-  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_p32.p, w_Ap32.p, w_u.p};
+  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_p32.p, w_Ap32.p, w_u.p, w_r64.p};
 }
 
 void Ctx::enqueue_iterations(int precision_mode, int count) {
@@ -364,15 +362,38 @@ hipGraphExec_t Ctx::iteration_graph(int precision_mode) {
   return g.exec;
 }
 
-void Ctx::read_state() {
-  VBH_HIP(hipMemcpyAsync(st_host, st_dev.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
-  VBH_HIP(hipStreamSynchronize(stream));
+// Launches batches of iterations until the device state reports done. The
+// state of batch k is copied out behind it and read while batch k + 1 is
+// already queued, so the device never waits for the host; the batch queued
+// behind the last one finds the state stopped and every kernel of it exits at
+// entry. A batch that ends neither stopped nor short of its iterations always
+// advances the state, so the cap is never reached by a healthy solve
+// (iterations plus refresh steps are at most twice the iterations).
+PcgScalars Ctx::drive(const PcgScalars* st, const std::function<void()>& batch) {
+  const long long cap = 2ll * (max_iters / iters_per_graph + 2) + 64;
+  long long launches = 0;
+  auto issue = [&](int slot) {
+    if (++launches > cap)
+      throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
+    batch();
+    VBH_HIP(hipMemcpyAsync(st_host + slot, st, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+    VBH_HIP(hipEventRecord(ev_poll[slot], stream));
+  };
+  issue(0);
+  int cur = 0;
+  for (;;) {
+    issue(cur ^ 1);
+    VBH_HIP(hipEventSynchronize(ev_poll[cur]));
+    if (st_host[cur].done) break;
+    cur ^= 1;
+  }
+  VBH_HIP(hipEventSynchronize(ev_poll[cur ^ 1]));
+  return st_host[cur ^ 1];
 }
 
 // Runs one solve on device-resident F (w_F) into w_u. Returns the scalars.
 PcgScalars Ctx::run_solve(int precision_mode) {
   const int N = static_cast<int>(work_len);
-  const PcgParams prm = params();
   const CgRed rd = red();
   VBH_HIP(hipEventRecord(ev_start, stream));
   if (precision_mode == VBH_PREC_FP64)
@@ -382,30 +403,14 @@ PcgScalars Ctx::run_solve(int precision_mode) {
   const char* ge_env = std::getenv("VBH_USE_GRAPH");
   const bool use_graph = !(ge_env && ge_env[0] == '0');
   hipGraphExec_t ge = use_graph ? iteration_graph(precision_mode) : nullptr;
-  // Each graph launch that does not end paused or finished completes
-  // iters_per_graph iterations, so this cap is never reached by a healthy solve.
-  const long long cap = 2ll * (max_iters / iters_per_graph + 2) + 64;
-  long long launches = 0;
-  for (;;) {
-    if (++launches > cap)
-      throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
+  const PcgScalars s = drive(st_dev.p, [&] {
     if (use_graph) VBH_HIP(hipGraphLaunch(ge, stream));
     else enqueue_iterations(precision_mode, iters_per_graph);
-    read_state();
-    if (!st_host->done) continue;
-    if (precision_mode == VBH_PREC_MIXED && (st_host->flags & kFlagRefresh)) {
-      launch_refresh_residual(format, csr, bsrd, w_F.p, w_u.p, w_r64.p, st_dev.p, rd, stream);
-      launch_refresh_reload(N, w_r64.p, diag32.p, w_r32.p, st_dev.p, rd, prm, stream);
-      read_state();
-      if (!st_host->done) continue;
-    }
-    break;
-  }
+  });
   VBH_HIP(hipEventRecord(ev_stop, stream));
   VBH_HIP(hipEventSynchronize(ev_stop));
   float ms = 0.0f;
   VBH_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
-  PcgScalars s = *st_host;
   stats.iterations = s.iter_done;
   stats.refinements = s.refreshes;
   stats.rel_res = (precision_mode == VBH_PREC_MIXED && (s.flags & (kFlagConverged | kFlagStall)))
@@ -493,9 +498,10 @@ int vbh_ctx_create(int device, double tol, int max_iters, vbh_ctx** out) {
     VBH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     VBH_HIP(hipEventCreate(&c->ev_start));
     VBH_HIP(hipEventCreate(&c->ev_stop));
-    // set-up for the allocation-free solve: the pinned state word buffer and
+    for (hipEvent_t& e : c->ev_poll) VBH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // set-up for the allocation-free solve: the pinned state word slots and
     // the device address of this device's solver scratch array
-    VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->st_host), sizeof(vbh::PcgScalars), 0));
+    VBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->st_host), 2 * sizeof(vbh::PcgScalars), 0));
     vbh::scratch_init(device);
     *out = reinterpret_cast<vbh_ctx*>(c.release());
   });

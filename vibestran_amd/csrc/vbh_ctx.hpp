@@ -1,5 +1,6 @@
 // vibestran_amd/csrc/vbh_ctx.hpp -- the context behind vbh_ctx*.
 #pragma once
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -65,6 +66,7 @@ struct Ctx {
   std::string name, arch;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  hipEvent_t ev_poll[2] = {nullptr, nullptr};  // state copies of the last two iteration batches
 
   // solver configuration (CudaPCGContext equivalent, impl:2030-2040)
   double tol = 1e-8;
@@ -133,10 +135,10 @@ struct Ctx {
   long long work_version = -1;
   DBuf<double> w_F, w_u, w_r64, w_r, w_p, w_Ap;
   DBuf<float> w_r32, w_p32, w_Ap32;
-  DBuf<double> part_a, part_b, part_r;
+  DBuf<double> part_a, part_b;
   int red_n_a = 1, red_n_b = 1;
   DBuf<PcgScalars> st_dev;
-  PcgScalars* st_host = nullptr;
+  PcgScalars* st_host = nullptr;  // pinned, two slots (see drive)
   GraphSlot graphs[2];
 
   // mesh of the last device assembly
@@ -158,6 +160,8 @@ struct Ctx {
     if (st_host) (void)hipHostFree(st_host);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
+    for (hipEvent_t e : ev_poll)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -193,7 +197,7 @@ struct Ctx {
   CgBufs<float> bufs32();
   void enqueue_iterations(int precision_mode, int count);
   hipGraphExec_t iteration_graph(int precision_mode);
-  void read_state();
+  PcgScalars drive(const PcgScalars* st, const std::function<void()>& batch);
   PcgScalars run_solve(int precision_mode);
   void check_outcome(const PcgScalars& s, int precision_mode);
 };

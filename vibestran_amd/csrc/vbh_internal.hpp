@@ -73,7 +73,9 @@ struct PcgScalars {
   int hist_stalls[2];
   int iter_done;         // completed iterations (written by cg_vector)
   int iter_cur;          // iteration in progress (written by cg_spmv)
-  int resume_iter;       // iteration resumed after a refresh (written by refresh_reload)
+  int resume_iter;       // iteration resumed after a refresh (written by cg_vector)
+  int refresh_req;       // mixed: cg_vector paused for an FP64 refresh (read by cg_spmv)
+  int spmv_mode;         // the last cg_spmv computed the refresh residual (read by cg_vector)
   int done, flags, refreshes;
 };
 
@@ -83,7 +85,7 @@ enum : int {
   kFlagMaxIter = 4,
   kFlagRz0 = 8,          // r.z <= 0 at start (impl:1859-1862)
   kFlagZeroRhs = 16,     // ||F|| < 1e-300 (impl:1822-1830)
-  kFlagRefresh = 32,     // mixed: paused, FP64 true residual requested
+  kFlagRefresh = 32,     // block PCG, mixed: column paused for an FP64 true-residual refresh
   kFlagStall = 64        // mixed: true residual stopped improving
 };
 
@@ -93,15 +95,14 @@ struct PcgParams {
   double refresh_delta;  // mixed: refresh when rr < delta^2 * rr_refresh
 };
 
-// Per-block partial sums passed from one kernel to the next. nA/nB/nR are the
+// Per-block partial sums passed from one kernel to the next. nA/nB are the
 // grid sizes of the producing kernels (<= kMaxGrid).
 struct CgRed {
-  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap, z.Ap, Ap.diag^-1.Ap
-  double* partB;  // two buffers of kPartB: r.z, r.r, (F.F) from cg_init (buffer 0),
-                  // cg_vector of iteration i (buffer (i+1)&1), refresh_reload
-                  // (the paused iteration's buffer)
-  double* partR;  // from refresh_residual: |F - K u|^2
-  int nA, nB, nR;
+  double* partA;  // [kMaxRed][kMaxGrid] from cg_spmv: p.Ap, z.Ap, Ap.diag^-1.Ap, or
+                  // |F - K u|^2 (slot 0) when it computed a refresh residual
+  double* partB;  // two buffers of kPartB: r.z, r.r, (F.F) from cg_init (buffer 0)
+                  // and cg_vector (buffer (position + 1) & 1)
+  int nA, nB;
 };
 
 template <typename T>
@@ -112,6 +113,7 @@ struct CgBufs {
   T* p;
   T* Ap;
   double* u64;
+  double* r64;  // mixed: FP64 true residual of a refresh (nullptr in FP64)
 };
 
 // ---- operator formats ---------------------------------------------------------
@@ -207,17 +209,11 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
                     hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // position: the launch's place in its sequence of iterations (a captured
 // graph or a run of direct launches); sequences have an even length and
-// start after cg_init or refresh_reload (which write partB buffer 0).
+// start after cg_init (which writes partB buffer 0).
 template <typename T>
 void launch_cg_vector(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
                       const PcgParams& prm, int mixed, int position, hipStream_t s,
                       hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
-// Mixed refresh: r64 = F - K64 u64 (FP64), then decide and reload r32.
-void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
-                             const double* u64, double* r64, PcgScalars* st, const CgRed& red,
-                             hipStream_t s);
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
-                           const CgRed& red, const PcgParams& prm, hipStream_t s);
 
 // Device address of the allocation-free solver's scratch array on `device`
 // (current), taken once per device (vbh_noalloc.hip).

@@ -67,25 +67,59 @@ __device__ __forceinline__ void st4(double* p, const Quad<double>& q) {
 }
 
 // ---- deterministic reductions ---------------------------------------------------
+// A double moved between lanes by DPP (both 32-bit halves), CTRL one of the
+// in-row patterns below: every source lane lies in the destination's row of
+// 16, so masks and bound control never apply.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// Sum over the 64 lanes of a wave, the same bits in every lane: a butterfly
+// inside each row of 16 lanes (DPP quad_perm lane^1 and lane^2, then
+// row_half_mirror and row_mirror, which swap quads and half rows), where the
+// two lanes of every exchange add the same two values (x + y == y + x), then
+// the four row sums (r0 + r1) + (r2 + r3). No LDS traffic, no barrier.
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+
+// Sum of v over the workgroup, returned in every thread with the same bits
+// and in the same order in every workgroup (results are deterministic for a
+// given grid): wave sums, then the waves' sums in wave order through LDS.
+// Called by every thread of the workgroup with all lanes active; `lds` holds
+// NV x (waves per block) doubles and may be reused after the call.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] += __shfl_down(v[i], off, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0)
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
 #pragma unroll
     for (int i = 0; i < NV; ++i) lds[w * NV + i] = v[i];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
+  const int nw = blockDim.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double s = 0.0;
-      for (int k = 0; k < nw; ++k) s += lds[k * NV + i];
-      v[i] = s;
-    }
+  for (int i = 0; i < NV; ++i) {
+    double t = lds[i];
+    for (int k = 1; k < nw; ++k) t += lds[k * NV + i];
+    v[i] = t;
   }
   __syncthreads();
 }
@@ -280,43 +314,29 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
 // iterations so, 2,465 with the expansion in alpha as well, 1,657 for the
 // three-kernel form that reduced r.z before forming p.)
 //
-// Mixed precision pauses in cg_vector, before its update, when the
+// Mixed precision refreshes the residual inside the same two-kernel sequence,
+// with no host round trip. cg_vector pauses, before its update, when the
 // recursive residual fell by refresh_delta since the last refresh (or below
-// tol); the host then runs
-//   refresh_residual  r64 = F - K64 u (FP64)                                   -> partR
-//   refresh_reload    reduce partR: converged / stalled / continue with
-//                     r = (float) r64; partials (r.z, r.r)                    -> partB
+// tol), and sets refresh_req; then
+//   cg_spmv    (refresh_req set) r64 = F - K64 u over the FP64 operator;
+//              partial |r64|^2 in partA slot 0; spmv_mode = 1
+//   cg_vector  (spmv_mode set) decide from the FP64 true residual: converged /
+//              stalled, or continue with r = (float) r64, partials (r.z, r.r);
+//              refresh_req = 0, the paused iteration marked resumed
 // and the next cg_spmv recomputes K p with p kept (reliable update, van der
 // Vorst and Ye: only r is replaced), so its z.Ap uses the refreshed z, and
 // cg_vector resumes the paused iteration.
 //
 // Only block 0 writes the scalar state, and never a field that other blocks
-// of the same kernel read: cg_vector reads the iteration as iter_cur, which
-// cg_spmv copies from iter_done (written by cg_vector); norm_b and
-// rr_refresh are written only in iterations whose decision does not read
-// them. partB has two buffers: the vector kernel launched at position j of
-// a launch sequence (a captured graph, the directly launched iterations)
-// reads buffer j & 1 and writes the other; cg_init and refresh_reload write
-// buffer 0, which position 0 reads, and every sequence has an even length.
-
-template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nprod,
-                                                double (&out)[NV], double* lds) {
-  double t[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) t[i] = 0.0;
-  for (int b = threadIdx.x; b < nprod; b += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) t[i] += part[i * kMaxGrid + b];
-  block_sum<NV>(t, lds);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) lds[4 * kMaxRed + i] = t[i];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; ++i) out[i] = lds[4 * kMaxRed + i];
-  __syncthreads();
-}
+// of the same kernel read: cg_vector reads the iteration as iter_cur and the
+// refresh step as spmv_mode, which cg_spmv copies from iter_done and
+// refresh_req (written by cg_vector); norm_b and rr_refresh are written only
+// in iterations whose decision does not read them; done is set only together
+// with a stop every block of the kernel reaches on its own. partB has two
+// buffers: the vector kernel launched at position j of a launch sequence (a
+// captured graph, the directly launched iterations) reads buffer j & 1 and
+// writes the other; cg_init writes buffer 0, which position 0 reads, and every
+// sequence has an even length.
 
 template <int NV>
 __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restrict__ part,
@@ -329,12 +349,13 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restri
 
 constexpr int kLds = 8 * kMaxRed;
 
-// Broadcast of the SpMV's state words (done, iter_done): thread 0 reads, the
-// block agrees.
+// Broadcast of the SpMV's state words (done, iter_done, refresh_req): thread
+// 0 reads, the block agrees.
 __device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
   if (threadIdx.x == 0) {
     s_i[0] = st->done;
     s_i[1] = st->iter_done;
+    s_i[2] = st->refresh_req;
   }
   __syncthreads();
 }
@@ -351,14 +372,14 @@ __device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
 // Only the state words go through the head: the double scalars (norm_b,
 // rr_refresh) are read after the decision that needs them.
 struct Head {
-  int done, iter_done, iter_cur, resume_iter;
+  int done, iter_cur, resume_iter, spmv_mode;
 };
 
 // NJA / NJB partial sums per thread for the SpMV's / the previous vector
 // kernel's grid: the smallest NJ with NJ * kBlock >= the grid (1, 2, 4, 8).
 template <int NJA, int NJB>
 struct HeadLoads {
-  double a[3][NJA];  // p.Ap, z.Ap, Ap.diag^-1.Ap
+  double a[3][NJA];  // p.Ap, z.Ap, Ap.diag^-1.Ap (or |F - K u|^2 in slot 0)
   double b[3][NJB];  // r.z, r.r, F.F (F.F: buffer 0 from cg_init)
   Head st;
 };
@@ -389,36 +410,28 @@ __device__ __forceinline__ void head_issue(const PcgScalars* st, const CgRed& re
       for (int i = 0; i < 3; ++i) hl.b[i][j] = 0.0;
   if (threadIdx.x == 0) {
     hl.st.done = st->done;
-    hl.st.iter_done = st->iter_done;
     hl.st.iter_cur = st->iter_cur;
     hl.st.resume_iter = st->resume_iter;
+    hl.st.spmv_mode = st->spmv_mode;
   }
 }
 
-// out[0..2] = partA sums, out[3..5] = partB sums
+// out[0..2] = partA sums, out[3..5] = partB sums, the same in every thread
 template <int NJA, int NJB>
 __device__ __forceinline__ void head_finish(HeadLoads<NJA, NJB>& hl, double (&out)[6], Head& h,
                                             double* lds, Head* sh) {
-  double t[6];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    t[i] = 0.0;
+    out[i] = 0.0;
 #pragma unroll
-    for (int j = 0; j < NJA; ++j) t[i] += hl.a[i][j];
-    t[3 + i] = 0.0;
+    for (int j = 0; j < NJA; ++j) out[i] += hl.a[i][j];
+    out[3 + i] = 0.0;
 #pragma unroll
-    for (int j = 0; j < NJB; ++j) t[3 + i] += hl.b[i][j];
+    for (int j = 0; j < NJB; ++j) out[3 + i] += hl.b[i][j];
   }
   if (threadIdx.x == 0) *sh = hl.st;
-  block_sum<6>(t, lds);  // its barriers also publish *sh
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int i = 0; i < 6; ++i) lds[6 * kMaxRed + i] = t[i];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 6; ++i) out[i] = lds[6 * kMaxRed + i];
+  block_sum<6>(out, lds);  // its barriers also publish *sh
   h = *sh;
-  __syncthreads();
 }
 
 template <typename T>
@@ -442,6 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, CgBufs<T> b, PcgScala
     st->iter_done = 0;
     st->iter_cur = 0;
     st->resume_iter = -1;
+    st->refresh_req = 0;
+    st->spmv_mode = 0;
     st->flags = 0;
     st->done = 0;
     st->refreshes = 0;
@@ -465,22 +480,51 @@ __device__ __forceinline__ void spmv_sums(const CgBufs<T>& b, int g, T v, double
   d[2] += av * av / static_cast<double>(dg);
 }
 
-// Block 0 copies the iteration count for the vector kernel (see above).
+// Block 0 passes the iteration count and the refresh step on to the vector
+// kernel (see above).
 __device__ __forceinline__ void spmv_iteration(PcgScalars* st, const int* s_i) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = s_i[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter_cur = s_i[1];
+    st->spmv_mode = s_i[2];
+  }
+}
+
+// Refresh step of the SpMV kernels (mixed precision): r64 = F - K64 u over
+// the FP64 operator, |r64|^2 as partial slot 0 of partA, and the refresh
+// count (block 0).
+template <typename T>
+__device__ __forceinline__ void refresh_done(const CgBufs<T>& b, PcgScalars* st, double (&v)[1],
+                                             const CgRed& red, double* lds) {
+  write_partials<1>(v, red.partA, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[2];
+  __shared__ int s_i[3];
   read_words(st, s_i);
   if (s_i[0]) return;
   spmv_iteration(st, s_i);
-  const T* val = vals<T>(A);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
+  if constexpr (sizeof(T) == 4) {
+    if (s_i[2]) {
+      double v[1] = {0.0};
+      for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
+        const double s = csr_row<double, G>(A, A.v64, b.u64, r, lg);
+        if (lg == 0) {
+          const double ri = b.F[r] - s;
+          b.r64[r] = ri;
+          v[0] += ri * ri;
+        }
+      }
+      refresh_done(b, st, v, red, lds);
+      return;
+    }
+  }
+  const T* val = vals<T>(A);
   double d[3] = {0.0, 0.0, 0.0};
   for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
     const T s = csr_row<T, G>(A, val, b.p, r, lg);
@@ -496,17 +540,34 @@ template <typename T, int B, typename Acc = T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[2];
+  __shared__ int s_i[3];
   read_words(st, s_i);
   if (s_i[0]) return;
   spmv_iteration(st, s_i);
-  const T* val = svals<T>(A);
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
+  const int s0 = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows;
+  const int sstride = gridDim.x * (blockDim.x / kSliceRows);
+  if constexpr (sizeof(T) == 4) {
+    if (s_i[2]) {
+      double v[1] = {0.0};
+      for (int s = s0; s < A.nslices; s += sstride) {
+        const double ku = sell_row<double, B>(A, A.s64, b.u64, s, l);
+        const int g = kSliceRows * s + l;
+        if (g < nrows) {
+          const double ri = b.F[g] - ku;
+          b.r64[g] = ri;
+          v[0] += ri * ri;
+        }
+      }
+      refresh_done(b, st, v, red, lds);
+      return;
+    }
+  }
+  const T* val = svals<T>(A);
   const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
   double d[3] = {0.0, 0.0, 0.0};
-  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
-       s += gridDim.x * (blockDim.x / kSliceRows)) {
+  for (int s = s0; s < A.nslices; s += sstride) {
     const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
     const int g = kSliceRows * s + l;
     if (g < nrows) {
@@ -520,6 +581,56 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
     }
   }
   write_partials<3>(d, red.partA, lds);
+}
+
+// The refresh step of the vector kernel: the decision on the FP64 true
+// residual |r64|^2 = rr_true that the SpMV kernel just computed; on
+// "continue", r = (T) r64 with the partials (r.z, r.r) of the new r, and the
+// paused iteration marked resumed.
+template <typename T>
+__device__ __forceinline__ void cg_reload(int n, const CgBufs<T>& b, PcgScalars* st,
+                                          const CgRed& red, const PcgParams& prm, double rr_true,
+                                          const Head& h, int par, double* lds) {
+  const double rel = sqrt(fabs(rr_true)) / st->norm_b;
+  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+  // stall history is double-buffered by refresh count (the SpMV kernel
+  // advanced the count), so no block reads what block 0 writes here
+  const int rc = st->refreshes;
+  const double rel_prev = st->hist_rel[(rc - 1) & 1];
+  int stalls = st->hist_stalls[(rc - 1) & 1];
+  int stop = 0;
+  if (rel < prm.tol) {
+    stop = kFlagConverged;
+  } else {
+    stalls = (rel > 0.9 * rel_prev) ? stalls + 1 : 0;  // true residual stopped following
+    if (stalls >= 4) stop = kFlagStall;
+  }
+  if (lead) {
+    st->rel_true = rel;
+    st->hist_rel[rc & 1] = rel;
+    st->hist_stalls[rc & 1] = stalls;
+  }
+  if (stop) {
+    if (lead) {
+      st->rel = rel;
+      st->flags |= stop;
+      st->done = 1;
+    }
+    return;
+  }
+  double v[2] = {0.0, 0.0};
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const T rk = static_cast<T>(b.r64[k]);
+    const T zk = rk / b.diag[k];
+    b.r[k] = rk;
+    v[0] += static_cast<double>(rk) * static_cast<double>(zk);
+    v[1] += static_cast<double>(rk) * static_cast<double>(rk);
+  }
+  write_partials<2>(v, red.partB + (par ^ 1) * kPartB, lds);
+  if (lead) {
+    st->resume_iter = h.iter_cur;
+    st->refresh_req = 0;
+  }
 }
 
 template <typename T, int NJA, int NJB>
@@ -547,6 +658,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
   Head h;
   head_finish<NJA, NJB>(hl, v6, h, lds, &sh);
   if (h.done) return;
+  if constexpr (sizeof(T) == 4) {
+    if (mixed && h.spmv_mode) {
+      cg_reload<T>(n, b, st, red, prm, v6[0], h, par, lds);
+      return;
+    }
+  }
   const int i = h.iter_cur;
   const bool resumed = (h.resume_iter == i);
   const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
@@ -554,6 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
   const double rz = v6[3], rr = v6[4];
   double norm_b, rel;
   int stop = 0;
+  bool pause = false;
   if (i == 0 && !resumed) {
     norm_b = sqrt(fabs(v6[5]));
     rel = 1.0;
@@ -572,11 +690,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
       if (rel < prm.tol) stop = kFlagConverged;
     } else if (!resumed &&
                (rel < prm.tol || rr < prm.refresh_delta * prm.refresh_delta * st->rr_refresh)) {
-      stop = kFlagRefresh;  // pause for an FP64 true-residual refresh
+      pause = true;  // for an FP64 true-residual refresh
     }
-    if (!stop && i >= prm.max_iters) stop = kFlagMaxIter;
+    if (!stop && !pause && i >= prm.max_iters) stop = kFlagMaxIter;
   }
-  if (!stop && !(pAp > 0.0)) stop = kFlagBreakdown;  // impl:691-696
+  if (!stop && !pause && !(pAp > 0.0)) stop = kFlagBreakdown;  // impl:691-696
+  if (pause) {
+    if (lead) {
+      st->rz[0] = rz;
+      st->rr = rr;
+      st->pAp = pAp;
+      st->rel = rel;
+      st->refresh_req = 1;
+    }
+    return;
+  }
   if (stop) {
     if (lead) {
       if (i == 0 && !resumed) st->norm_b = norm_b;
@@ -644,103 +772,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
   }
 }
 
-// ---- mixed precision refresh ----------------------------------------------------------
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_refresh_residual_csr(CsrDev A, const double* __restrict__ F,
-                                                                 const double* __restrict__ u,
-                                                                 double* __restrict__ r64,
-                                                                 PcgScalars* st, CgRed red) {
-  __shared__ double lds[kLds];
-  const int lg = threadIdx.x % G;
-  const int stride = gridDim.x * (blockDim.x / G);
-  double v[1] = {0.0};
-  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
-    const double s = csr_row<double, G>(A, A.v64, u, r, lg);
-    if (lg == 0) {
-      const double ri = F[r] - s;
-      r64[r] = ri;
-      v[0] += ri * ri;
-    }
-  }
-  write_partials<1>(v, red.partR, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
-}
-
-template <int B>
-__global__ __launch_bounds__(kBlock) void k_refresh_residual_bsr(BsrDev A, const double* __restrict__ F,
-                                                                 const double* __restrict__ u,
-                                                                 double* __restrict__ r64,
-                                                                 PcgScalars* st, CgRed red) {
-  __shared__ double lds[kLds];
-  const int l = threadIdx.x & (kSliceRows - 1);
-  const int nrows = A.nbr * B;
-  double v[1] = {0.0};
-  for (int s = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows; s < A.nslices;
-       s += gridDim.x * (blockDim.x / kSliceRows)) {
-    const double ku = sell_row<double, B>(A, A.s64, u, s, l);
-    const int g = kSliceRows * s + l;
-    if (g < nrows) {
-      const double ri = F[g] - ku;
-      r64[g] = ri;
-      v[0] += ri * ri;
-    }
-  }
-  write_partials<1>(v, red.partR, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
-}
-
-// Decide from the FP64 true residual; on "continue" reload r = (float) r64
-// (partials r.z, r.r into partB buffer 0, which the next launch sequence's
-// first vector kernel reads) and mark the paused iteration resumed.
-__global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* __restrict__ r64,
-                                                           const float* __restrict__ diag,
-                                                           float* __restrict__ r, PcgScalars* st,
-                                                           CgRed red, PcgParams prm) {
-  __shared__ double lds[kLds];
-  double v1[1];
-  reduce_partials<1>(red.partR, red.nR, v1, lds);
-  const double rel = sqrt(fabs(v1[0])) / st->norm_b;
-  const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
-  // stall history is double-buffered by refresh count (refresh_residual
-  // advanced the count), so no block reads what block 0 writes here
-  const int rc = st->refreshes;
-  const double rel_prev = st->hist_rel[(rc - 1) & 1];
-  int stalls = st->hist_stalls[(rc - 1) & 1];
-  int stop = 0;
-  if (rel < prm.tol) {
-    stop = kFlagConverged;
-  } else {
-    stalls = (rel > 0.9 * rel_prev) ? stalls + 1 : 0;  // true residual stopped following
-    if (stalls >= 4) stop = kFlagStall;
-  }
-  if (lead) {
-    st->rel_true = rel;
-    st->hist_rel[rc & 1] = rel;
-    st->hist_stalls[rc & 1] = stalls;
-  }
-  if (stop) {
-    if (lead) {
-      st->rel = rel;
-      st->flags = (st->flags & ~kFlagRefresh) | stop;
-    }
-    return;  // done stays set
-  }
-  double v[2] = {0.0, 0.0};
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const float rk = static_cast<float>(r64[k]);
-    const float zk = rk / diag[k];
-    r[k] = rk;
-    v[0] += static_cast<double>(rk) * static_cast<double>(zk);
-    v[1] += static_cast<double>(rk) * static_cast<double>(rk);
-  }
-  write_partials<2>(v, red.partB, lds);
-  if (lead) {
-    st->resume_iter = st->iter_done;
-    st->flags &= ~kFlagRefresh;
-    st->done = 0;
-  }
-}
-
 // ---- long rows: the blocks past kSellCap -------------------------------------------------
 // One workgroup per long block row (grid-stride over the rows): the row's
 // remaining blocks (read from the BSR arrays, FP64 sums) are reduced over the
@@ -749,23 +780,15 @@ __global__ __launch_bounds__(kBlock) void k_refresh_reload(int n, const double* 
 // y += rest, and the changes of the three SpMV sums (p.Ap, z.Ap, Ap.diag^-1.Ap)
 // are added to partial slot blockIdx.x, which the SpMV pass wrote (grid <=
 // its grid; one writer per slot, so the sums stay deterministic). MODE 2
-// (refresh, FP64): r -= rest and the change of r.r into the refresh's slot.
+// (refresh step of the PCG SpMV, FP64, x = u, y = r64): r64 -= rest and the
+// change of r.r into slot 0 of the same partials.
 // With the split FP32 stream the diagonal block is skipped (the SpMV added
 // its FP64 form).
 template <typename T, int B, int MODE>
-__global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __restrict__ x,
-                                                          T* __restrict__ y,
-                                                          const T* __restrict__ z,  // r (MODE 1)
-                                                          const T* __restrict__ diag,
-                                                          double* __restrict__ part,
-                                                          const PcgScalars* st) {
-  __shared__ double lds[4 * 8];
-  if (MODE == 1) {
-    __shared__ int dn;
-    if (threadIdx.x == 0) dn = st->done;
-    __syncthreads();
-    if (dn) return;
-  }
+__device__ __forceinline__ void overflow_rows(const BsrDev& A, const T* __restrict__ x,
+                                              T* __restrict__ y, const T* __restrict__ z,
+                                              const T* __restrict__ diag,
+                                              double* __restrict__ part, double* lds) {
   const bool split = sizeof(T) == 4 && A.dblk != nullptr;
   const T* __restrict__ val = sizeof(T) == 4 ? reinterpret_cast<const T*>(A.v32)
                                              : reinterpret_cast<const T*>(A.v64);
@@ -819,6 +842,35 @@ __global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __r
   }
 }
 
+// MODE 0: after a plain SpMV. MODE 1: after a PCG SpMV, which either formed
+// Ap = K p (MODE 1 rows) or, in a mixed refresh step, r64 = F - K64 u (MODE 2
+// rows on the FP64 blocks); it reads the same state words the SpMV read.
+template <typename T, int B, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __restrict__ x,
+                                                          T* __restrict__ y,
+                                                          const T* __restrict__ z,  // r (MODE 1)
+                                                          const T* __restrict__ diag,
+                                                          double* __restrict__ part,
+                                                          const PcgScalars* st,
+                                                          const double* __restrict__ u64,
+                                                          double* __restrict__ r64) {
+  __shared__ double lds[4 * 8];
+  if (MODE == 1) {
+    __shared__ int w[2];
+    if (threadIdx.x == 0) {
+      w[0] = st->done;
+      w[1] = st->refresh_req;
+    }
+    __syncthreads();
+    if (w[0]) return;
+    if (sizeof(T) == 4 && w[1]) {
+      overflow_rows<double, B, 2>(A, u64, r64, nullptr, nullptr, part, lds);
+      return;
+    }
+  }
+  overflow_rows<T, B, MODE>(A, x, y, z, diag, part, lds);
+}
+
 __global__ void k_cast_f64_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     b[i] = static_cast<float>(a[i]);
@@ -846,15 +898,16 @@ constexpr int kCsrG = 8;  // lanes per scalar CSR row
 // The overflow pass of long rows after a sliced-ELL pass (see k_sell_overflow).
 template <typename T, int MODE>
 static void launch_overflow(const BsrDev& A, const T* x, T* y, const T* z, const T* diag,
-                            double* part, int nprod, const PcgScalars* st, hipStream_t s) {
+                            double* part, int nprod, const PcgScalars* st, hipStream_t s,
+                            const double* u64 = nullptr, double* r64 = nullptr) {
   if (A.nlong == 0) return;
   const int g = MODE == 0 ? std::min(A.nlong, kMaxGrid) : std::min(A.nlong, nprod);
   if (A.B == 3)
     hipLaunchKernelGGL((k_sell_overflow<T, 3, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, z, diag,
-                       part, st);
+                       part, st, u64, r64);
   else
     hipLaunchKernelGGL((k_sell_overflow<T, 6, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, z, diag,
-                       part, st);
+                       part, st, u64, r64);
 }
 
 template <typename T>
@@ -910,7 +963,8 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red);
   else
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red);
-  if (format != 0) launch_overflow<T, 1>(Ab, b.p, b.Ap, b.r, b.diag, red.partA, red.nA, st, s);
+  if (format != 0)
+    launch_overflow<T, 1>(Ab, b.p, b.Ap, b.r, b.diag, red.partA, red.nA, st, s, b.u64, b.r64);
 }
 
 // partial sums per thread for a producer grid of nprod blocks
@@ -938,28 +992,6 @@ void launch_cg_vector(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& re
 }
 
 #undef VBH_LAUNCH
-
-void launch_refresh_residual(int format, const CsrDev& A, const BsrDev& Ab, const double* F,
-                             const double* u64, double* r64, PcgScalars* st, const CgRed& red,
-                             hipStream_t s) {
-  if (format == 0)
-    hipLaunchKernelGGL((k_refresh_residual_csr<kCsrG>), dim3(red.nR), dim3(kBlock), 0, s, A, F, u64,
-                       r64, st, red);
-  else if (format == 3)
-    hipLaunchKernelGGL((k_refresh_residual_bsr<3>), dim3(red.nR), dim3(kBlock), 0, s, Ab, F, u64,
-                       r64, st, red);
-  else
-    hipLaunchKernelGGL((k_refresh_residual_bsr<6>), dim3(red.nR), dim3(kBlock), 0, s, Ab, F, u64,
-                       r64, st, red);
-  if (format != 0)
-    launch_overflow<double, 2>(Ab, u64, r64, nullptr, nullptr, red.partR, red.nR, st, s);
-}
-
-void launch_refresh_reload(int n, const double* r64, const float* diag, float* r, PcgScalars* st,
-                           const CgRed& red, const PcgParams& prm, hipStream_t s) {
-  hipLaunchKernelGGL(k_refresh_reload, dim3(red.nB), dim3(kBlock), 0, s, n, r64, diag, r, st, red,
-                     prm);
-}
 
 int rows_per_block(int format) {
   return format == 0 ? kBlock / kCsrG : kBlock / kSliceRows;  // block formats: slices per block

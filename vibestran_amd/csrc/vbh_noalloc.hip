@@ -41,7 +41,7 @@ size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // Carves the scratch array into the buffers of one solve of n_pad rows.
 struct ScratchLayout {
-  size_t st, partA, partB, partR, F, u, r64, r, p, Ap, end;
+  size_t st, partA, partB, F, u, r64, r, p, Ap, end;
   ScratchLayout(size_t n_pad, bool mixed) {
     const size_t part = sizeof(double) * kMaxRed * kMaxGrid;
     const size_t vs = mixed ? sizeof(float) : sizeof(double);
@@ -52,8 +52,6 @@ struct ScratchLayout {
     o = align_up(o + part);
     partB = o;
     o = align_up(o + 2 * part);
-    partR = o;
-    o = align_up(o + part);
     F = o;
     o = align_up(o + sizeof(double) * n_pad);
     u = o;
@@ -113,20 +111,18 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
   CgRed rd;
   rd.partA = reinterpret_cast<double*>(base + L.partA);
   rd.partB = reinterpret_cast<double*>(base + L.partB);
-  rd.partR = reinterpret_cast<double*>(base + L.partR);
   const int rows = (c.format == 0) ? c.n : c.bsrd.nslices;
   rd.nA = grid_for(rows, rows_per_block(c.format));
   rd.nB = vector_grid(static_cast<long long>(N), c.n_cu);
-  rd.nR = rd.nA;
   double* F = reinterpret_cast<double*>(base + L.F);
   double* u = reinterpret_cast<double*>(base + L.u);
   double* r64 = reinterpret_cast<double*>(base + L.r64);
   const CgBufs<double> b64{F, c.diag64.p, reinterpret_cast<double*>(base + L.r),
                            reinterpret_cast<double*>(base + L.p),
-                           reinterpret_cast<double*>(base + L.Ap), u};
+                           reinterpret_cast<double*>(base + L.Ap), u, nullptr};
   const CgBufs<float> b32{F, c.diag32.p, reinterpret_cast<float*>(base + L.r),
                           reinterpret_cast<float*>(base + L.p), reinterpret_cast<float*>(base + L.Ap),
-                          u};
+                          u, r64};
   hipStream_t s = c.stream;
   const PcgParams prm = c.params();
   // F in the operator's layout; the padding rows past n stay zero (a fill
@@ -137,16 +133,8 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
   VBH_HIP(hipEventRecord(c.ev_start, s));
   if (mixed) launch_cg_init<float>(static_cast<int>(N), b32, st, rd, s);
   else launch_cg_init<double>(static_cast<int>(N), b64, st, rd, s);
-  auto read_state = [&] {
-    VBH_HIP(hipMemcpyAsync(c.st_host, st, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    VBH_HIP(hipStreamSynchronize(s));
-  };
   const int per = c.iters_per_graph;  // even (see launch_cg_vector)
-  const long long cap = 2ll * (c.max_iters / per + 2) + 64;
-  long long launches = 0;
-  for (;;) {
-    if (++launches > cap)
-      throw_error(VBH_ERR_HIP, "HIP PCG: device iteration state stopped advancing");
+  const PcgScalars out = c.drive(st, [&] {
     for (int i = 0; i < per; ++i) {
       if (mixed) {
         launch_cg_spmv<float>(c.format, c.csr, c.bsrd, b32, st, rd, s, c.spmv_acc64);
@@ -156,23 +144,13 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
         launch_cg_vector<double>(static_cast<int>(N), b64, st, rd, prm, 0, i, s);
       }
     }
-    read_state();
-    if (!c.st_host->done) continue;
-    if (mixed && (c.st_host->flags & kFlagRefresh)) {
-      launch_refresh_residual(c.format, c.csr, c.bsrd, F, u, r64, st, rd, s);
-      launch_refresh_reload(static_cast<int>(N), r64, c.diag32.p, b32.r, st, rd, prm, s);
-      read_state();
-      if (!c.st_host->done) continue;
-    }
-    break;
-  }
+  });
   VBH_HIP(hipEventRecord(c.ev_stop, s));
   VBH_HIP(hipEventSynchronize(c.ev_stop));
   float ms = 0.0f;
   VBH_HIP(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
   c.from_operator(u, du, true);
   VBH_HIP(hipStreamSynchronize(s));
-  const PcgScalars out = *c.st_host;
   c.stats.iterations = out.iter_done;
   c.stats.refinements = out.refreshes;
   c.stats.rel_res = (mixed && (out.flags & (kFlagConverged | kFlagStall))) ? out.rel_true : out.rel;
