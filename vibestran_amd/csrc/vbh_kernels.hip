@@ -197,18 +197,29 @@ __device__ __forceinline__ const double* rel_vals<double>(const BsrDev& A) { ret
 template <>
 __device__ __forceinline__ const float* rel_vals<float>(const BsrDev& A) { return A.rv32; }
 
+// The block slots of slice s in the stream of value type T: [k0, k0 + km)
+// (scalar loads of the slice offsets).
+struct Slots {
+  int k0, km;
+};
+template <typename T>
+__device__ __forceinline__ Slots sell_slots(const BsrDev& A, int s) {
+  s = __builtin_amdgcn_readfirstlane(s);
+  const int* __restrict__ k = A.rk64 ? (sizeof(T) == 4 ? A.rk32 : A.rk64) : A.koff;
+  const int k0 = k[s];
+  return {k0, k[s + 1] - k0};
+}
+
 // Row g = 64 s + l of the relative sliced ELL (BsrDev::rk64 ...) times x: the
 // slot offsets are uniform over the wave's slice (scalar loads); block
 // columns outside the operator (a boundary node without that neighbour,
 // whose value block is zero) are clamped into range.
 template <typename T, int B, typename Acc>
 __device__ __forceinline__ Acc sell_row_rel(const BsrDev& A, const T* __restrict__ x, int s,
-                                            int l) {
+                                            int l, Slots sl) {
   const bool f32 = sizeof(T) == 4;
-  const int* __restrict__ rk = f32 ? A.rk32 : A.rk64;
   const int* __restrict__ ro = f32 ? A.ro32 : A.ro64;
-  s = __builtin_amdgcn_readfirstlane(s);
-  const int k0 = rk[s], km = rk[s + 1] - k0;
+  const int k0 = sl.k0, km = sl.km;
   const int node = (kSliceRows * s + l) / B;
   const int last = A.nbr - 1;
   const T* __restrict__ v = rel_vals<T>(A) + (static_cast<size_t>(k0) * kSliceRows + l) * B;
@@ -226,13 +237,13 @@ __device__ __forceinline__ Acc sell_row_rel(const BsrDev& A, const T* __restrict
 }
 
 // Row g = 64 s + l of the sliced-ELL operator times x (one lane, no reduction),
-// accumulated in Acc.
+// accumulated in Acc; sl = sell_slots<T>(A, s).
 template <typename T, int B, typename Acc = T>
 __device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ val,
-                                        const T* __restrict__ x, int s, int l) {
-  if (A.rk64) return sell_row_rel<T, B, Acc>(A, x, s, l);
+                                        const T* __restrict__ x, int s, int l, Slots sl) {
+  if (A.rk64) return sell_row_rel<T, B, Acc>(A, x, s, l, sl);
   constexpr int NPS = sell_nps(B);
-  const int k0 = A.koff[s], km = A.koff[s + 1] - k0;
+  const int k0 = sl.k0, km = sl.km;
   const int node = (kSliceRows * s + l) / B;
   const int node0 = (kSliceRows * s) / B;
   const T* v = val + (static_cast<size_t>(k0) * kSliceRows + l) * B;
@@ -241,6 +252,12 @@ __device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ v
                                   node0, km);
   return sell_row_ix<T, B, Acc>(v, x, A.sidx + static_cast<size_t>(k0) * NPS + (node - node0), 0,
                                 km);
+}
+
+template <typename T, int B, typename Acc = T>
+__device__ __forceinline__ Acc sell_row(const BsrDev& A, const T* __restrict__ val,
+                                        const T* __restrict__ x, int s, int l) {
+  return sell_row<T, B, Acc>(A, val, x, s, l, sell_slots<T>(A, s));
 }
 
 // FP64 diagonal node block of row g (row-sum preserving FP32 stream, see
@@ -349,15 +366,15 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restri
 
 constexpr int kLds = 8 * kMaxRed;
 
-// Broadcast of the SpMV's state words (done, iter_done, refresh_req): thread
-// 0 reads, the block agrees.
-__device__ __forceinline__ void read_words(const PcgScalars* st, int* s_i) {
-  if (threadIdx.x == 0) {
-    s_i[0] = st->done;
-    s_i[1] = st->iter_done;
-    s_i[2] = st->refresh_req;
-  }
-  __syncthreads();
+// The state words the SpMV kernels branch on (done, iter_done, refresh_req),
+// read by every wave itself: uniform addresses, so scalar loads, and no
+// barrier. They are written only by the vector kernels, never while an SpMV
+// runs.
+struct SpmvWords {
+  int done, iter_done, refresh_req;
+};
+__device__ __forceinline__ SpmvWords spmv_words(const PcgScalars* st) {
+  return {st->done, st->iter_done, st->refresh_req};
 }
 
 // The head of the vector kernel, split in two so the caller can put its own
@@ -482,10 +499,10 @@ __device__ __forceinline__ void spmv_sums(const CgBufs<T>& b, int g, T v, double
 
 // Block 0 passes the iteration count and the refresh step on to the vector
 // kernel (see above).
-__device__ __forceinline__ void spmv_iteration(PcgScalars* st, const int* s_i) {
+__device__ __forceinline__ void spmv_iteration(PcgScalars* st, const SpmvWords& w) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->iter_cur = s_i[1];
-    st->spmv_mode = s_i[2];
+    st->iter_cur = w.iter_done;
+    st->spmv_mode = w.refresh_req;
   }
 }
 
@@ -503,14 +520,13 @@ template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[3];
-  read_words(st, s_i);
-  if (s_i[0]) return;
-  spmv_iteration(st, s_i);
+  const SpmvWords w = spmv_words(st);
+  if (w.done) return;
+  spmv_iteration(st, w);
   const int lg = threadIdx.x % G;
   const int stride = gridDim.x * (blockDim.x / G);
   if constexpr (sizeof(T) == 4) {
-    if (s_i[2]) {
+    if (w.refresh_req) {
       double v[1] = {0.0};
       for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < A.n; r += stride) {
         const double s = csr_row<double, G>(A, A.v64, b.u64, r, lg);
@@ -540,16 +556,22 @@ template <typename T, int B, typename Acc = T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
                                                         CgRed red) {
   __shared__ double lds[kLds];
-  __shared__ int s_i[3];
-  read_words(st, s_i);
-  if (s_i[0]) return;
-  spmv_iteration(st, s_i);
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
   const int s0 = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows;
   const int sstride = gridDim.x * (blockDim.x / kSliceRows);
+  // the state words and the first slice's slot range load together (one
+  // memory round trip before the stream starts, not two); the early exit
+  // reads the slot range too, so its loads stay ahead of the branch
+  const SpmvWords w = spmv_words(st);
+  const Slots sl0 = s0 < A.nslices ? sell_slots<T>(A, s0) : Slots{0, 0};
+  if (w.done) {
+    if (sl0.km < 0) b.Ap[0] = T(0);  // never true
+    return;
+  }
+  spmv_iteration(st, w);
   if constexpr (sizeof(T) == 4) {
-    if (s_i[2]) {
+    if (w.refresh_req) {
       double v[1] = {0.0};
       for (int s = s0; s < A.nslices; s += sstride) {
         const double ku = sell_row<double, B>(A, A.s64, b.u64, s, l);
@@ -568,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
   double d[3] = {0.0, 0.0, 0.0};
   for (int s = s0; s < A.nslices; s += sstride) {
-    const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l);
+    const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l, s == s0 ? sl0 : sell_slots<T>(A, s));
     const int g = kSliceRows * s + l;
     if (g < nrows) {
       // split FP32 stream: off-diagonal blocks summed in FP32, the corrected
