@@ -506,6 +506,24 @@ __device__ __forceinline__ void spmv_iteration(PcgScalars* st, const SpmvWords& 
   }
 }
 
+// The slices a wave of the PCG SpMV handles: s0, s0 + stride, ... < hi, in
+// the order BsrDev::xcd selects for value type T (XCD-contiguous ranges
+// when the grid is a multiple of 8).
+struct SliceRange {
+  int s0, stride, hi;
+};
+template <typename T>
+__device__ __forceinline__ SliceRange slice_range(const BsrDev& A) {
+  const int wpb = blockDim.x / kSliceRows, wave = threadIdx.x / kSliceRows;
+  if (A.xcd[sizeof(T) == 4] && (gridDim.x & 7) == 0) {
+    const int x = blockIdx.x & 7, per = (A.nslices + 7) >> 3;
+    const int lo = min(A.nslices, x * per);
+    return {lo + (static_cast<int>(blockIdx.x >> 3) * wpb + wave),
+            static_cast<int>(gridDim.x >> 3) * wpb, min(A.nslices, lo + per)};
+  }
+  return {static_cast<int>(blockIdx.x) * wpb + wave, static_cast<int>(gridDim.x) * wpb, A.nslices};
+}
+
 // Refresh step of the SpMV kernels (mixed precision): r64 = F - K64 u over
 // the FP64 operator, |r64|^2 as partial slot 0 of partA, and the refresh
 // count (block 0).
@@ -558,13 +576,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   __shared__ double lds[kLds];
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
-  const int s0 = (blockIdx.x * blockDim.x + threadIdx.x) / kSliceRows;
-  const int sstride = gridDim.x * (blockDim.x / kSliceRows);
+  const SliceRange sr = slice_range<T>(A);
+  const int s0 = sr.s0;
   // the state words and the first slice's slot range load together (one
   // memory round trip before the stream starts, not two); the early exit
   // reads the slot range too, so its loads stay ahead of the branch
   const SpmvWords w = spmv_words(st);
-  const Slots sl0 = s0 < A.nslices ? sell_slots<T>(A, s0) : Slots{0, 0};
+  const Slots sl0 = s0 < sr.hi ? sell_slots<T>(A, s0) : Slots{0, 0};
   if (w.done) {
     if (sl0.km < 0) b.Ap[0] = T(0);  // never true
     return;
@@ -573,7 +591,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   if constexpr (sizeof(T) == 4) {
     if (w.refresh_req) {
       double v[1] = {0.0};
-      for (int s = s0; s < A.nslices; s += sstride) {
+      const SliceRange r64 = slice_range<double>(A);
+      for (int s = r64.s0; s < r64.hi; s += r64.stride) {
         const double ku = sell_row<double, B>(A, A.s64, b.u64, s, l);
         const int g = kSliceRows * s + l;
         if (g < nrows) {
@@ -589,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
   const T* val = svals<T>(A);
   const bool split = (sizeof(T) == 4) && A.dblk != nullptr;
   double d[3] = {0.0, 0.0, 0.0};
-  for (int s = s0; s < A.nslices; s += sstride) {
+  for (int s = s0; s < sr.hi; s += sr.stride) {
     const Acc acc = sell_row<T, B, Acc>(A, val, b.p, s, l, s == s0 ? sl0 : sell_slots<T>(A, s));
     const int g = kSliceRows * s + l;
     if (g < nrows) {
