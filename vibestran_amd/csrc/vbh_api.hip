@@ -319,11 +319,13 @@ CgRed Ctx::red() const {
 }
 
 CgBufs<double> Ctx::bufs64() {
-  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_p.p, w_Ap.p, w_u.p, nullptr};
+  return CgBufs<double>{w_F.p, diag64.p, w_r.p, w_p.p, w_Ap.p, w_u.p, nullptr,
+                        format != 0 ? bsrd.xcd[0] : 0};
 }
 
 CgBufs<float> Ctx::bufs32() {
-  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_p32.p, w_Ap32.p, w_u.p, w_r64.p};
+  return CgBufs<float>{w_F.p, diag32.p, w_r32.p, w_p32.p, w_Ap32.p, w_u.p, w_r64.p,
+                       format != 0 ? bsrd.xcd[1] : 0};
 }
 
 void Ctx::enqueue_iterations(int precision_mode, int count) {

@@ -114,6 +114,10 @@ struct CgBufs {
   T* Ap;
   double* u64;
   double* r64;  // mixed: FP64 true residual of a refresh (nullptr in FP64)
+  // the vector kernel's rows follow the SpMV's XCD-contiguous slice ranges
+  // (BsrDev::xcd of this value type), so each XCD updates the rows its SpMV
+  // wrote and will gather
+  int xcd = 0;
 };
 
 // ---- operator formats ---------------------------------------------------------

@@ -681,14 +681,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
   __shared__ double lds[kLds];
   __shared__ Head sh;
   const int n4 = n >> 2;
-  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  const int tg = blockIdx.x * blockDim.x + threadIdx.x, ng = gridDim.x * blockDim.x;
+  // row quads [q_lo, q_hi) of this workgroup's XCD range (CgBufs::xcd) or of
+  // the whole vector, from q_lo + t0 in steps of nt
+  int q_lo = 0, q_hi = n4, t0 = tg, nt = ng;
+  if (b.xcd && (gridDim.x & 7) == 0) {
+    const int per = (((n + kSliceRows - 1) / kSliceRows) + 7) >> 3;  // SpMV slices per XCD
+    const int x = blockIdx.x & 7;
+    q_lo = min(n4, x * per * (kSliceRows / 4));
+    q_hi = min(n4, (x + 1) * per * (kSliceRows / 4));
+    t0 = static_cast<int>(blockIdx.x >> 3) * blockDim.x + threadIdx.x;
+    nt = static_cast<int>(gridDim.x >> 3) * blockDim.x;
+  }
   HeadLoads<NJA, NJB> hl;
   head_issue<NJA, NJB>(st, red, red.partB + par * kPartB, hl);
   // this thread's first rows go out behind the head loads
   Quad<double> u0{};
   Quad<T> p0{}, ap0{}, d0{}, r0{};
-  if (t0 < n4) {
-    const int k = 4 * t0;
+  if (q_lo + t0 < q_hi) {
+    const int k = 4 * (q_lo + t0);
     u0 = ld4(b.u64 + k);
     p0 = ld4(b.p + k);
     ap0 = ld4(b.Ap + k);
@@ -778,8 +789,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
     st4(b.r + k, r);
     st4(b.p + k, p);
   };
-  if (t0 < n4) rows4(4 * t0, u0, p0, ap0, d0, r0);
-  for (int q = t0 + nt; q < n4; q += nt) {
+  if (q_lo + t0 < q_hi) rows4(4 * (q_lo + t0), u0, p0, ap0, d0, r0);
+  for (int q = q_lo + t0 + nt; q < q_hi; q += nt) {
     const int k = 4 * q;
     Quad<double> u = ld4(b.u64 + k);
     Quad<T> p = ld4(b.p + k);
@@ -787,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
     Quad<T> r = ld4(b.r + k);
     rows4(k, u, p, ap, d, r);
   }
-  for (int k = 4 * n4 + t0; k < n; k += nt) {
+  for (int k = 4 * n4 + tg; k < n; k += ng) {
     b.u64[k] += alpha * static_cast<double>(b.p[k]);
     const T r = b.r[k] - a * b.Ap[k];
     const T z = r / b.diag[k];

@@ -119,10 +119,11 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
   double* r64 = reinterpret_cast<double*>(base + L.r64);
   const CgBufs<double> b64{F, c.diag64.p, reinterpret_cast<double*>(base + L.r),
                            reinterpret_cast<double*>(base + L.p),
-                           reinterpret_cast<double*>(base + L.Ap), u, nullptr};
+                           reinterpret_cast<double*>(base + L.Ap), u, nullptr,
+                           c.format != 0 ? c.bsrd.xcd[0] : 0};
   const CgBufs<float> b32{F, c.diag32.p, reinterpret_cast<float*>(base + L.r),
                           reinterpret_cast<float*>(base + L.p), reinterpret_cast<float*>(base + L.Ap),
-                          u, r64};
+                          u, r64, c.format != 0 ? c.bsrd.xcd[1] : 0};
   hipStream_t s = c.stream;
   const PcgParams prm = c.params();
   // F in the operator's layout; the padding rows past n stay zero (a fill
