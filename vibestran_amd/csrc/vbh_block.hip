@@ -1251,7 +1251,9 @@ int log2_of(int p) {
 }
 
 // SpMM candidates for ldk columns and block size B: every (CW, LG) with a
-// kernel whose tile is the full width (or half of it with CW = 4), two grids.
+// kernel whose tile is the full width (or half of it with CW = 4), three grids (512
+// workgroups keep fewer block rows in flight per XCD, so the X rows their
+// neighbours gather stay in that XCD's L2 for wide FP64 blocks).
 std::vector<BlkTune> candidates(int ldk, int B) {
   const int kt = std::min(ldk, 64);
   std::vector<BlkTune> c;
@@ -1259,7 +1261,7 @@ std::vector<BlkTune> candidates(int ldk, int B) {
     for (int stage : {2, 1, 0}) {
       if (stage == 2 && (!has_group_bcast(lg) || B == 1)) continue;
       for (int xcd : {1, 0})
-        for (int nrb : {2048, 1024}) {
+        for (int nrb : {2048, 1024, 512}) {
           BlkTune t;
           t.cw = cw;
           t.lg = lg;
