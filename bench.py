@@ -122,7 +122,9 @@ def profiled_traffic(profile, kernel):
     try:
         table = None
         for line in open(path):
-            if line.startswith("| kernel | calls | avg us"):
+            if line.startswith("#"):
+                table = None  # a new section: its tables are not read
+            elif line.startswith("| kernel | calls | avg us"):
                 table = "dur"
             elif line.startswith("| kernel | FETCH_SIZE"):
                 table = "traffic"

@@ -1,5 +1,6 @@
 """GPU load vectors through the C ABI: FORCE/MOMENT + PLOAD4 against the host
-mirror, GRAV (consistent mass) against the oracle, on solids and shells."""
+mirror and against the oracle's element face loads, GRAV (consistent mass)
+against the oracle, on solids and shells."""
 import ctypes as C
 
 import numpy as np
@@ -23,6 +24,29 @@ def test_force_and_pload4_match_host(gpu_backend_factory):
     F = assemble_rhs_device(be, m, eq, n, m.loads[0])
     Fh = M.assemble_rhs(m, eq, n, m.loads[0])
     assert np.abs(F - Fh).max() <= 1e-13 * np.abs(Fh).max()
+
+
+def test_force_and_pload4_match_oracle(gpu_backend_factory):
+    """FORCE + PLOAD4 on a jittered hex block, built independently of the host
+    mirror: the oracle's CHEXA8 face load (linear_static_pressure.cpp:154-196,
+    430-517) per loaded element, scattered through the equation map."""
+    be = gpu_backend_factory()
+    m = decks.config2_hex_jitter(nx=8, ny=5, nz=4, seed=11)
+    pl = m.loads[0].pload4
+    pl.p = np.random.default_rng(4).uniform(0.5, 2.0, (len(pl.elem), 4))
+    m.loads[0].nodal[::5, 0] = np.linspace(-3, 3, len(m.loads[0].nodal[::5]))
+    eq, n = dof_map_device(be, m)
+    assemble_model(be, m, eq, n)
+    F = assemble_rhs_device(be, m, eq, n, m.loads[0])
+    nodal = np.array(m.loads[0].nodal, dtype=np.float64, copy=True)
+    conn = m.conn2d()
+    for i, e in enumerate(pl.elem):
+        f8 = O.pload4_hexa8(m.xyz[conn[e]], int(pl.g1[i]), int(pl.g34[i]), pl.p[i])
+        np.add.at(nodal[:, 0:3], conn[e], f8)
+    Fo = np.zeros(n)
+    mask = eq >= 0
+    np.add.at(Fo, eq[mask], nodal[mask])
+    assert np.abs(F - Fo).max() <= 1e-13 * np.abs(Fo).max()
 
 
 @pytest.mark.parametrize("mk", [mixed_mesh, lambda: decks.cquad4_tube(n_circ=16, n_axial=8, n_cases=1)])
