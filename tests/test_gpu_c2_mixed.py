@@ -78,3 +78,26 @@ def test_c2_workload_mixed_solve_vs_oracle(gpu_backend_factory, c2small):
     print(f"[c2 40x16x16 fp64 ] iters={be64.last_iteration_count()} oracle true rel={rel64:.3e} "
           f"max-norm error vs FP64 oracle={err64:.3e}; mixed vs fp64 GPU {_max_err(u, u64):.3e}")
     assert rel64 < 1e-8 and err64 < 1e-7
+
+
+def test_c2_workload_slice_orders_vs_oracle(gpu_backend_factory, c2small, monkeypatch):
+    """The FP32 SpMV of a 3x3-block operator sweeps XCD-contiguous slice
+    ranges (and the vector kernel the matching row ranges) by default, the
+    grid-stride order with VBH_SPMV_XCD=0. Both orders, mixed precision, meet
+    the same oracle bars; they differ only in the summation order of the
+    per-block partial sums, so their iteration counts stay within a few per
+    cent of each other."""
+    m, eq, n, oc, F, u_ref, it_ref = c2small
+    rp, ci, v = oc.arrays()
+    res = {}
+    for xcd in ("1", "0"):
+        monkeypatch.setenv("VBH_SPMV_XCD", xcd)
+        be = gpu_backend_factory(mixed=True, tolerance=1e-8, max_iters=100000)
+        u = be.solve(CsrData.from_arrays(rp.copy(), ci.copy(), v.copy(), lower_only=True), F)
+        rel = O.true_relative_residual(oc, u, F)
+        err = _max_err(u, u_ref)
+        res[xcd] = be.last_iteration_count()
+        print(f"\n[c2 40x16x16 mixed, VBH_SPMV_XCD={xcd}] iters={res[xcd]} oracle true rel={rel:.3e} "
+              f"max-norm error vs FP64 oracle={err:.3e}")
+        assert rel < 1e-8 and err < 1e-7
+    assert abs(res["1"] - res["0"]) <= 0.1 * res["0"]

@@ -118,6 +118,28 @@ def test_error_paths(gpu_backend_factory):
                   np.array([0.0, 1.0]))
 
 
+def test_mixed_error_paths(gpu_backend_factory):
+    """The reference's outcomes in mixed precision, where pauses for an FP64
+    refresh run inside the iteration kernels: the iteration limit, a zero
+    load, p.Ap <= 0 and a non-positive initial r.z."""
+    be = gpu_backend_factory(mixed=True, max_iters=3)
+    with pytest.raises(SolverError, match="did not converge after 3"):
+        be.solve(csr_of(tridiag(50, 2.0, -1.0)), np.ones(50))
+    be2 = gpu_backend_factory(mixed=True)
+    u = be2.solve(csr_of(tridiag(20, 2.0, -1.0)), np.zeros(20))
+    assert not u.any() and be2.last_iteration_count() == 0
+    with pytest.raises(SolverError, match="did not converge"):  # p.Ap <= 0 breakdown
+        be2.solve(CsrData.from_arrays([0, 1, 3], [0, 0, 1], [1.0, 2.0, 1.0], lower_only=True),
+                  np.array([1.0, -1.0]))
+    with pytest.raises(SolverError, match="non-positive initial r"):
+        be2.solve(CsrData.from_arrays([0, 1, 2], [0, 1], [1.0, -1.0], lower_only=True),
+                  np.array([0.0, 1.0]))
+    # the context stays usable after each failure
+    oc = tridiag(200, 2.0, -1.0)
+    u = be2.solve(csr_of(oc), np.ones(200))
+    assert np.allclose(u, direct(oc, np.ones(200)), atol=5e-3)
+
+
 @pytest.fixture(scope="module")
 def c1():
     m = decks.config1_hex_cantilever()

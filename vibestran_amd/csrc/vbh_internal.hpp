@@ -169,7 +169,8 @@ struct BsrDev {
   // Slice order of the PCG SpMV per value type (0 FP64 stream, 1 FP32):
   // 1 = XCD-contiguous (workgroup w runs on XCD w mod 8, so the slices are
   // cut into 8 contiguous ranges, one per XCD, and the gathered vector of
-  // each range is fetched into that XCD's L2 only), 0 = grid-stride over all.
+  // each range is fetched into that XCD's L2 only; grids of at least 8
+  // workgroups), 0 = grid-stride over all.
   int xcd[2] = {0, 0};
   // Relative sliced ELL (set when it streams fewer bytes; replaces koff /
   // sidx / s64 / s32 for the SpMVs): slice s holds the union of its nodes'

@@ -506,20 +506,28 @@ __device__ __forceinline__ void spmv_iteration(PcgScalars* st, const SpmvWords& 
   }
 }
 
+// XCD x of workgroup w (round-robin dispatch) and the number of workgroups
+// of a grid of g (>= 8) on XCD x.
+__device__ __forceinline__ int xcd_of_block() { return static_cast<int>(blockIdx.x) & 7; }
+__device__ __forceinline__ int blocks_on_xcd(int x) {
+  return (static_cast<int>(gridDim.x) - x + 7) >> 3;
+}
+
 // The slices a wave of the PCG SpMV handles: s0, s0 + stride, ... < hi, in
-// the order BsrDev::xcd selects for value type T (XCD-contiguous ranges
-// when the grid is a multiple of 8).
+// the order BsrDev::xcd selects for value type T: XCD-contiguous ranges of
+// ceil(nslices / 8) slices (grids of at least 8 workgroups, so every range
+// has workgroups), or grid-stride over all slices.
 struct SliceRange {
   int s0, stride, hi;
 };
 template <typename T>
 __device__ __forceinline__ SliceRange slice_range(const BsrDev& A) {
   const int wpb = blockDim.x / kSliceRows, wave = threadIdx.x / kSliceRows;
-  if (A.xcd[sizeof(T) == 4] && (gridDim.x & 7) == 0) {
-    const int x = blockIdx.x & 7, per = (A.nslices + 7) >> 3;
+  if (A.xcd[sizeof(T) == 4] && gridDim.x >= 8) {
+    const int x = xcd_of_block(), per = (A.nslices + 7) >> 3;
     const int lo = min(A.nslices, x * per);
-    return {lo + (static_cast<int>(blockIdx.x >> 3) * wpb + wave),
-            static_cast<int>(gridDim.x >> 3) * wpb, min(A.nslices, lo + per)};
+    return {lo + (static_cast<int>(blockIdx.x >> 3) * wpb + wave), blocks_on_xcd(x) * wpb,
+            min(A.nslices, lo + per)};
   }
   return {static_cast<int>(blockIdx.x) * wpb + wave, static_cast<int>(gridDim.x) * wpb, A.nslices};
 }
@@ -685,13 +693,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
   // row quads [q_lo, q_hi) of this workgroup's XCD range (CgBufs::xcd) or of
   // the whole vector, from q_lo + t0 in steps of nt
   int q_lo = 0, q_hi = n4, t0 = tg, nt = ng;
-  if (b.xcd && (gridDim.x & 7) == 0) {
+  if (b.xcd && gridDim.x >= 8) {
     const int per = (((n + kSliceRows - 1) / kSliceRows) + 7) >> 3;  // SpMV slices per XCD
-    const int x = blockIdx.x & 7;
+    const int x = xcd_of_block();
     q_lo = min(n4, x * per * (kSliceRows / 4));
     q_hi = min(n4, (x + 1) * per * (kSliceRows / 4));
     t0 = static_cast<int>(blockIdx.x >> 3) * blockDim.x + threadIdx.x;
-    nt = static_cast<int>(gridDim.x >> 3) * blockDim.x;
+    nt = blocks_on_xcd(x) * blockDim.x;
   }
   HeadLoads<NJA, NJB> hl;
   head_issue<NJA, NJB>(st, red, red.partB + par * kPartB, hl);

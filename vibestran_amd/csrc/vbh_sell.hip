@@ -286,8 +286,10 @@ void build_sell(Ctx& c) {
   // PCG SpMV slice order (BsrDev::xcd): XCD-contiguous ranges for the FP32
   // stream of 3x3 blocks (C2 hex: 35.4 -> 33.7 us per launch); the FP64
   // stream (81.6 -> 82.6 us) and 6x6 shell blocks are faster grid-stride
+  // (VBH_SPMV_XCD=0: grid-stride for both, a diagnosis knob)
+  const char* xe = std::getenv("VBH_SPMV_XCD");
   A.xcd[0] = 0;
-  A.xcd[1] = B == 3 ? 1 : 0;
+  A.xcd[1] = (B == 3 && !(xe && xe[0] == '0')) ? 1 : 0;
 }
 
 // The relative sliced ELL (BsrDev::rk64 ...), used when its slot count stays
