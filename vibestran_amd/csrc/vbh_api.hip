@@ -336,11 +336,11 @@ void Ctx::enqueue_iterations(int precision_mode, int count) {
   for (int i = 0; i < count; ++i) {
     if (precision_mode == VBH_PREC_FP64) {
       const CgBufs<double> b = bufs64();
-      launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, stream, 0);
+      launch_cg_spmv<double>(format, csr, bsrd, b, st_dev.p, rd, stream, 0, i);
       launch_cg_vector<double>(N, b, st_dev.p, rd, prm, 0, i, stream);
     } else {
       const CgBufs<float> b = bufs32();
-      launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, stream, spmv_acc64);
+      launch_cg_spmv<float>(format, csr, bsrd, b, st_dev.p, rd, stream, spmv_acc64, i);
       launch_cg_vector<float>(N, b, st_dev.p, rd, prm, 1, i, stream);
     }
   }
@@ -693,11 +693,11 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
       if ((precision < 13 && precision % 2 == 0) || precision == 13) {
         vbh::launch_cg_init<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, c->stream);
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
-                                    c->stream, 0);
+                                    c->stream, 0, 0);
       } else {
         vbh::launch_cg_init<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, c->stream);
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
-                                   c->stream, c->spmv_acc64);
+                                   c->stream, c->spmv_acc64, 0);
       }
     }
     // tol 0, no iteration limit and no refresh pauses: timed solver kernels
@@ -718,10 +718,10 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         vbh::launch_stream_copy(cp_bytes, cp_a.p, cp_b.p, c->stream);
       } else if (precision == 2) {
         vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
-                                    c->stream, 0);
+                                    c->stream, 0, 0);
       } else if (precision == 3) {
         vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
-                                   c->stream, c->spmv_acc64);
+                                   c->stream, c->spmv_acc64, 0);
       } else if (precision == 4) {
         vbh::launch_cg_vector<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, run_all, 0, pos++,
                                       c->stream);
@@ -755,12 +755,12 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
         hipEvent_t v0 = kind == 1 ? e0 : nullptr, v1 = kind == 1 ? e1 : nullptr;
         if (f32) {
           vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd,
-                                     c->stream, c->spmv_acc64, s0, s1);
+                                     c->stream, c->spmv_acc64, i, s0, s1);
           vbh::launch_cg_vector<float>(c->n_pad, c->bufs32(), c->st_dev.p, rd, run_all, 0, i,
                                        c->stream, v0, v1);
         } else {
           vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd,
-                                      c->stream, 0, s0, s1);
+                                      c->stream, 0, i, s0, s1);
           vbh::launch_cg_vector<double>(c->n_pad, c->bufs64(), c->st_dev.p, rd, run_all, 0, i,
                                         c->stream, v0, v1);
         }
@@ -787,16 +787,16 @@ int vbh_bench_spmv(vbh_ctx* h, int precision, int reps, double* ms_per_launch) {
       // and none of the per-launch event overhead
       const int w = precision == 13 ? 2 : 3;
       for (int i = 0; i < 2; ++i) {  // warm up
-        if (w == 2) vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd, c->stream, 0);
-        else vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd, c->stream, c->spmv_acc64);
+        if (w == 2) vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd, c->stream, 0, 0);
+        else vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd, c->stream, c->spmv_acc64, 0);
       }
       hipEvent_t e0 = nullptr, e1 = nullptr;
       VBH_HIP(hipEventCreate(&e0));
       VBH_HIP(hipEventCreate(&e1));
       VBH_HIP(hipEventRecord(e0, c->stream));
       for (int i = 0; i < reps; ++i) {
-        if (w == 2) vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd, c->stream, 0);
-        else vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd, c->stream, c->spmv_acc64);
+        if (w == 2) vbh::launch_cg_spmv<double>(c->format, c->csr, c->bsrd, c->bufs64(), c->st_dev.p, rd, c->stream, 0, 0);
+        else vbh::launch_cg_spmv<float>(c->format, c->csr, c->bsrd, c->bufs32(), c->st_dev.p, rd, c->stream, c->spmv_acc64, 0);
       }
       VBH_HIP(hipEventRecord(e1, c->stream));
       VBH_HIP(hipEventSynchronize(e1));

@@ -77,6 +77,8 @@ struct PcgScalars {
   int refresh_req;       // mixed: cg_vector paused for an FP64 refresh (read by cg_spmv)
   int spmv_mode;         // the last cg_spmv computed the refresh residual (read by cg_vector)
   int done, flags, refreshes;
+  double sumB[3];        // r.z, r.r, F.F of the current r: the previous vector kernel's (or
+                         // cg_init's) partials, reduced by cg_spmv's last workgroup
 };
 
 enum : int {
@@ -213,9 +215,11 @@ template <typename T>
 void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red, hipStream_t s);
 // acc64: accumulate the SpMV rows in FP64 (products of the stored values).
 // t0/t1 (optional): events that receive the kernel's own start and end times.
+// position: as for launch_cg_vector (the SpMV reduces the partB buffer that
+// the vector kernel at the same position would read).
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64,
+                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64, int position,
                     hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // position: the launch's place in its sequence of iterations (a captured
 // graph or a run of direct launches); sequences have an even length and

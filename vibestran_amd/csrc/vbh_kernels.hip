@@ -313,7 +313,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
 //   init       u = 0, r = F, p = r/diag; partials (r.z, r.r, F.F)            -> partB
 //   cg_spmv    Ap = K p; partials (p.Ap, z.Ap, Ap.diag^-1.Ap), z = r/diag
 //              formed in registers                                           -> partA
-//   cg_vector  reduce partB -> rz_i = r.z, rr_i = r.r of the current r; the
+//              its last workgroup (one slice per wave: it has time to spare)
+//              reduces partB                                                 -> sumB
+//   cg_vector  rz_i = r.z, rr_i = r.r of the current r from sumB; the
 //              convergence / pause / limit decision (the reference's
 //              finalize step, impl:704-738); reduce partA ->
 //              alpha = rz_i / p.Ap (breakdown if p.Ap <= 0),
@@ -349,10 +351,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(BsrDev A, const T* __restri
 // refresh step as spmv_mode, which cg_spmv copies from iter_done and
 // refresh_req (written by cg_vector); norm_b and rr_refresh are written only
 // in iterations whose decision does not read them; done is set only together
-// with a stop every block of the kernel reaches on its own. partB has two
-// buffers: the vector kernel launched at position j of a launch sequence (a
-// captured graph, the directly launched iterations) reads buffer j & 1 and
-// writes the other; cg_init writes buffer 0, which position 0 reads, and every
+// with a stop every block of the kernel reaches on its own; sumB is written
+// by the SpMV's last workgroup only. partB has two buffers: the SpMV launched
+// at position j of a launch sequence (a captured graph, the directly launched
+// iterations) reduces buffer j & 1 and the vector kernel at position j writes
+// the other; cg_init writes buffer 0, which position 0 reduces, and every
 // sequence has an even length.
 
 template <int NV>
@@ -392,19 +395,18 @@ struct Head {
   int done, iter_cur, resume_iter, spmv_mode;
 };
 
-// NJA / NJB partial sums per thread for the SpMV's / the previous vector
-// kernel's grid: the smallest NJ with NJ * kBlock >= the grid (1, 2, 4, 8).
-template <int NJA, int NJB>
+// NJA partial sums per thread for the SpMV's grid: the smallest NJ with
+// NJ * kBlock >= the grid (1, 2, 4, 8).
+template <int NJA>
 struct HeadLoads {
   double a[3][NJA];  // p.Ap, z.Ap, Ap.diag^-1.Ap (or |F - K u|^2 in slot 0)
-  double b[3][NJB];  // r.z, r.r, F.F (F.F: buffer 0 from cg_init)
+  double b[3];       // sumB: r.z, r.r, F.F
   Head st;
 };
 
-template <int NJA, int NJB>
+template <int NJA>
 __device__ __forceinline__ void head_issue(const PcgScalars* st, const CgRed& red,
-                                           const double* __restrict__ partB,
-                                           HeadLoads<NJA, NJB>& hl) {
+                                           HeadLoads<NJA>& hl) {
   // every slot of the kMaxGrid-wide partial arrays is allocated, so the loads
   // are unconditional (no branch between them) and the unused ones masked
 #pragma unroll
@@ -412,19 +414,12 @@ __device__ __forceinline__ void head_issue(const PcgScalars* st, const CgRed& re
 #pragma unroll
     for (int i = 0; i < 3; ++i) hl.a[i][j] = red.partA[i * kMaxGrid + threadIdx.x + j * kBlock];
 #pragma unroll
-  for (int j = 0; j < NJB; ++j)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) hl.b[i][j] = partB[i * kMaxGrid + threadIdx.x + j * kBlock];
-#pragma unroll
   for (int j = 0; j < NJA; ++j)
     if (threadIdx.x + j * kBlock >= red.nA)
 #pragma unroll
       for (int i = 0; i < 3; ++i) hl.a[i][j] = 0.0;
 #pragma unroll
-  for (int j = 0; j < NJB; ++j)
-    if (threadIdx.x + j * kBlock >= red.nB)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) hl.b[i][j] = 0.0;
+  for (int i = 0; i < 3; ++i) hl.b[i] = st->sumB[i];  // uniform: scalar loads
   if (threadIdx.x == 0) {
     hl.st.done = st->done;
     hl.st.iter_cur = st->iter_cur;
@@ -433,21 +428,24 @@ __device__ __forceinline__ void head_issue(const PcgScalars* st, const CgRed& re
   }
 }
 
-// out[0..2] = partA sums, out[3..5] = partB sums, the same in every thread
-template <int NJA, int NJB>
-__device__ __forceinline__ void head_finish(HeadLoads<NJA, NJB>& hl, double (&out)[6], Head& h,
+// out[0..2] = partA sums, out[3..5] = sumB, the same in every thread
+template <int NJA>
+__device__ __forceinline__ void head_finish(HeadLoads<NJA>& hl, double (&out)[6], Head& h,
                                             double* lds, Head* sh) {
+  double a[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    out[i] = 0.0;
+    a[i] = 0.0;
 #pragma unroll
-    for (int j = 0; j < NJA; ++j) out[i] += hl.a[i][j];
-    out[3 + i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NJB; ++j) out[3 + i] += hl.b[i][j];
+    for (int j = 0; j < NJA; ++j) a[i] += hl.a[i][j];
   }
   if (threadIdx.x == 0) *sh = hl.st;
-  block_sum<6>(out, lds);  // its barriers also publish *sh
+  block_sum<3>(a, lds);  // its barriers also publish *sh
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    out[i] = a[i];
+    out[3 + i] = hl.b[i];
+  }
   h = *sh;
 }
 
@@ -542,9 +540,27 @@ __device__ __forceinline__ void refresh_done(const CgBufs<T>& b, PcgScalars* st,
   if (blockIdx.x == 0 && threadIdx.x == 0) st->refreshes += 1;
 }
 
+// The SpMV's last workgroup reduces the partB buffer par (r.z, r.r, F.F of
+// the current r) into sumB for the vector kernel that follows: the last
+// workgroup has one slice per wave in either slice order, so it finishes its
+// rows early and the reduction costs the SpMV nothing.
+__device__ __forceinline__ void reduce_partB(PcgScalars* st, const CgRed& red, int par,
+                                             double* lds) {
+  if (blockIdx.x != gridDim.x - 1) return;
+  const double* __restrict__ pb = red.partB + par * kPartB;
+  double t[3] = {0.0, 0.0, 0.0};
+  for (int q = threadIdx.x; q < red.nB; q += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t[i] += pb[i * kMaxGrid + q];
+  block_sum<3>(t, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) st->sumB[i] = t[i];
+}
+
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red) {
+                                                        CgRed red, int par) {
   __shared__ double lds[kLds];
   const SpmvWords w = spmv_words(st);
   if (w.done) return;
@@ -576,11 +592,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
     }
   }
   write_partials<3>(d, red.partA, lds);
+  reduce_partB(st, red, par, lds);
 }
 
 template <typename T, int B, typename Acc = T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red) {
+                                                        CgRed red, int par) {
   __shared__ double lds[kLds];
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
@@ -630,6 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, P
     }
   }
   write_partials<3>(d, red.partA, lds);
+  reduce_partB(st, red, par, lds);
 }
 
 // The refresh step of the vector kernel: the decision on the FP64 true
@@ -682,7 +700,7 @@ __device__ __forceinline__ void cg_reload(int n, const CgBufs<T>& b, PcgScalars*
   }
 }
 
-template <typename T, int NJA, int NJB>
+template <typename T, int NJA>
 __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgScalars* st,
                                                       CgRed red, PcgParams prm, int mixed,
                                                       int par) {
@@ -701,8 +719,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
     t0 = static_cast<int>(blockIdx.x >> 3) * blockDim.x + threadIdx.x;
     nt = blocks_on_xcd(x) * blockDim.x;
   }
-  HeadLoads<NJA, NJB> hl;
-  head_issue<NJA, NJB>(st, red, red.partB + par * kPartB, hl);
+  HeadLoads<NJA> hl;
+  head_issue<NJA>(st, red, hl);
   // this thread's first rows go out behind the head loads
   Quad<double> u0{};
   Quad<T> p0{}, ap0{}, d0{}, r0{};
@@ -716,7 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_vector(int n, CgBufs<T> b, PcgSca
   }
   double v6[6];
   Head h;
-  head_finish<NJA, NJB>(hl, v6, h, lds, &sh);
+  head_finish<NJA>(hl, v6, h, lds, &sh);
   if (h.done) return;
   if constexpr (sizeof(T) == 4) {
     if (mixed && h.spmv_mode) {
@@ -1009,20 +1027,21 @@ void launch_cg_init(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& red,
 
 template <typename T>
 void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<T>& b,
-                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64, hipEvent_t t0,
-                    hipEvent_t t1) {
+                    PcgScalars* st, const CgRed& red, hipStream_t s, int acc64, int position,
+                    hipEvent_t t0, hipEvent_t t1) {
   if (sizeof(T) == 4 && Ab.dblk != nullptr) acc64 = 1;  // split FP32 stream: FP64 sums
   const dim3 g(red.nA);
+  const int par = position & 1;
   if (format == 0)
-    VBH_LAUNCH((k_cg_spmv_csr<T, kCsrG>), g, s, t0, t1, A, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_csr<T, kCsrG>), g, s, t0, t1, A, b, st, red, par);
   else if (format == 3 && acc64)
-    VBH_LAUNCH((k_cg_spmv_bsr<T, 3, double>), g, s, t0, t1, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 3, double>), g, s, t0, t1, Ab, b, st, red, par);
   else if (format == 3)
-    VBH_LAUNCH((k_cg_spmv_bsr<T, 3>), g, s, t0, t1, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 3>), g, s, t0, t1, Ab, b, st, red, par);
   else if (acc64)
-    VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red, par);
   else
-    VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red);
+    VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red, par);
   if (format != 0)
     launch_overflow<T, 1>(Ab, b.p, b.Ap, b.r, b.diag, red.partA, red.nA, st, s, b.u64, b.r64);
 }
@@ -1038,16 +1057,13 @@ void launch_cg_vector(int n, const CgBufs<T>& b, PcgScalars* st, const CgRed& re
                       const PcgParams& prm, int mixed, int position, hipStream_t s,
                       hipEvent_t t0, hipEvent_t t1) {
   const dim3 g(red.nB);
-  const int ja = head_nj(red.nA), jb = head_nj(red.nB), par = position & 1;
-#define VBH_VEC(JA, JB)                                                                  \
-  if (ja == JA && jb == JB) {                                                            \
-    VBH_LAUNCH((k_cg_vector<T, JA, JB>), g, s, t0, t1, n, b, st, red, prm, mixed, par);  \
-    return;                                                                              \
+  const int ja = head_nj(red.nA), par = position & 1;
+#define VBH_VEC(JA)                                                                 \
+  if (ja == JA) {                                                                   \
+    VBH_LAUNCH((k_cg_vector<T, JA>), g, s, t0, t1, n, b, st, red, prm, mixed, par); \
+    return;                                                                         \
   }
-  VBH_VEC(1, 1) VBH_VEC(1, 2) VBH_VEC(1, 4) VBH_VEC(1, 8)
-  VBH_VEC(2, 1) VBH_VEC(2, 2) VBH_VEC(2, 4) VBH_VEC(2, 8)
-  VBH_VEC(4, 1) VBH_VEC(4, 2) VBH_VEC(4, 4) VBH_VEC(4, 8)
-  VBH_VEC(8, 1) VBH_VEC(8, 2) VBH_VEC(8, 4) VBH_VEC(8, 8)
+  VBH_VEC(1) VBH_VEC(2) VBH_VEC(4) VBH_VEC(8)
 #undef VBH_VEC
 }
 
@@ -1078,10 +1094,10 @@ template void launch_spmv_bsr<float>(const BsrDev&, const float*, float*, hipStr
 template void launch_cg_init<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_init<float>(int, const CgBufs<float>&, PcgScalars*, const CgRed&, hipStream_t);
 template void launch_cg_spmv<double>(int, const CsrDev&, const BsrDev&, const CgBufs<double>&,
-                                     PcgScalars*, const CgRed&, hipStream_t, int, hipEvent_t,
+                                     PcgScalars*, const CgRed&, hipStream_t, int, int, hipEvent_t,
                                      hipEvent_t);
 template void launch_cg_spmv<float>(int, const CsrDev&, const BsrDev&, const CgBufs<float>&,
-                                    PcgScalars*, const CgRed&, hipStream_t, int, hipEvent_t,
+                                    PcgScalars*, const CgRed&, hipStream_t, int, int, hipEvent_t,
                                     hipEvent_t);
 template void launch_cg_vector<double>(int, const CgBufs<double>&, PcgScalars*, const CgRed&,
                                        const PcgParams&, int, int, hipStream_t, hipEvent_t,

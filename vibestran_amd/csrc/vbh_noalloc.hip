@@ -138,10 +138,10 @@ static PcgScalars run_solve_noalloc(Ctx& c, int precision_mode, const double* dF
   const PcgScalars out = c.drive(st, [&] {
     for (int i = 0; i < per; ++i) {
       if (mixed) {
-        launch_cg_spmv<float>(c.format, c.csr, c.bsrd, b32, st, rd, s, c.spmv_acc64);
+        launch_cg_spmv<float>(c.format, c.csr, c.bsrd, b32, st, rd, s, c.spmv_acc64, i);
         launch_cg_vector<float>(static_cast<int>(N), b32, st, rd, prm, 1, i, s);
       } else {
-        launch_cg_spmv<double>(c.format, c.csr, c.bsrd, b64, st, rd, s, 0);
+        launch_cg_spmv<double>(c.format, c.csr, c.bsrd, b64, st, rd, s, 0, i);
         launch_cg_vector<double>(static_cast<int>(N), b64, st, rd, prm, 0, i, s);
       }
     }
