@@ -967,6 +967,9 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_lds(BlkOp A, int ldk, int xcd,
 //   LG = 16  DPP row_newbcast:n (gfx90a and later: lane n of each 16-lane row
 //            to the whole row), folded by the compiler into the instruction
 //            that consumes it (the FP32 -> FP64 conversion of the shell sums);
+//            a plain DPP move for FP64 values (mov_dpp: every source lane is
+//            in the row, so no old value; update_dpp with one costs a move
+//            that initialises each destination);
 //   LG = 4   DPP quad_perm [n,n,n,n], folded the same way;
 //   LG = 8   ds_swizzle in bit-mask mode (lane (l & 0x18) | n of each 32),
 //            which runs in the LDS unit without touching LDS memory.
@@ -976,9 +979,9 @@ template <int LG, int N>
 __device__ __forceinline__ int group_bcast_i32(int v) {
   static_assert(N >= 0 && N < LG, "source lane outside the group");
   if constexpr (LG == 16) {
-    return __builtin_amdgcn_update_dpp(0, v, kDppRowNewBcast + N, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, kDppRowNewBcast + N, 0xF, 0xF, true);
   } else if constexpr (LG == 4) {
-    return __builtin_amdgcn_update_dpp(0, v, N | (N << 2) | (N << 4) | (N << 6), 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, N | (N << 2) | (N << 4) | (N << 6), 0xF, 0xF, true);
   } else {
     static_assert(LG == 8, "group broadcast for 4, 8 or 16 lanes");
     return __builtin_amdgcn_ds_swizzle(v, 0x18 | (N << 5));
@@ -1018,10 +1021,10 @@ constexpr int dblk_lanes() {
   return (MODE == 0 && sizeof(T) == 4) ? (B * B + LG - 1) / LG : 0;
 }
 
-// FP32 values with at most 2 load cases per lane: at least 4 waves per SIMD
-// (<= 128 VGPRs, which those shapes fit without spilling)
+// No minimum of resident waves: capping the 6x6 FP32 shapes with two load
+// cases per lane at 128 VGPRs (4 waves per SIMD) spills their accumulators.
 template <typename T, int B, int CW, int LG, int MODE>
-__global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_blk_spmm_bcast(
+__global__ __launch_bounds__(kBT) void k_blk_spmm_bcast(
     BlkOp A, int ldk, int xcd, const T* __restrict__ X, T* __restrict__ Y,
     const double* __restrict__ F, const int* __restrict__ act, double* __restrict__ part) {
   constexpr int KT = CW * LG, GR = 64 / LG, NB2 = B * B, NL = (NB2 + LG - 1) / LG;
@@ -1052,17 +1055,44 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
     row_hi = min(A.nbr, row_lo + per);
   }
   // every lane of a group follows the same block row and block sequence, so
-  // the broadcast source lane is always active when its row is
-  for (int I0 = row_lo + (rb * (kBT / 64) + wave) * GR; I0 < row_hi;
-       I0 += nrbx * (kBT / 64) * GR) {
+  // the broadcast source lane is always active when its row is.
+  // Rows are software-pipelined: the row pointers two rows ahead and the first
+  // block columns of the next row load while this row is summed, so a row
+  // starts with its block loads instead of two dependent index round trips;
+  // where registers allow (kPreX), the row's own X rows load at its start too.
+  constexpr bool kPreX = CW <= 2;
+  const int step = nrbx * (kBT / 64) * GR;
+  auto row_ptrs = [&](int I, int& ro, int& re) {
+    ro = 0;
+    re = 0;
+    if (I < row_hi) {
+      ro = A.brp[I];
+      re = A.brp[I + 1];
+    }
+  };
+  auto first_cols = [&](int ro, int re) { return (lg < min(LG, re - ro)) ? A.bci[ro + lg] : 0; };
+  int I0 = row_lo + (rb * (kBT / 64) + wave) * GR;
+  int o, e, o1, e1, o2, e2;
+  row_ptrs(I0 + g, o, e);
+  row_ptrs(I0 + step + g, o1, e1);
+  int jl0 = first_cols(o, e), jl1 = 0;
+  for (; I0 < row_hi; I0 += step, o = o1, e = e1, o1 = o2, e1 = e2, jl0 = jl1) {
     const int I = I0 + g;
+    row_ptrs(I + 2 * step, o2, e2);
+    jl1 = first_cols(o1, e1);
     if (I >= row_hi) continue;
     Acc acc[B][CW];
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
       for (int w = 0; w < CW; ++w) acc[r][w] = Acc(0);
-    const int o = A.brp[I], e = A.brp[I + 1];
+    // the row's own X rows (for p.Ap, and the diagonal block)
+    T xI[B][CW];
+    if constexpr (MODE == 0 && kPreX) {
+#pragma unroll
+      for (int s = 0; s < B; ++s)
+        vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xI[s]);
+    }
     // the row's block columns LG at a time (lane l holds the l-th); blocks in
     // pairs with two register sets, so the next block's values and X rows
     // load while the current one is summed (the diagonal block of an FP32
@@ -1091,7 +1121,7 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
     };
     for (int q0 = o; q0 < e; q0 += LG) {
       const int qn = min(LG, e - q0);
-      const int jl = (lg < qn) ? A.bci[q0 + lg] : 0;
+      const int jl = (q0 == o) ? jl0 : ((lg < qn) ? A.bci[q0 + lg] : 0);
       int J = __shfl(jl, 0, LG);
       bool use0 = !(dblk && J == I), use1 = false;
       if (use0) fetch(m0, x0, q0, J);
@@ -1112,11 +1142,11 @@ __global__ __launch_bounds__(kBT, (sizeof(T) == 4 && CW <= 2) ? 4 : 1) void k_bl
       }
     }
     if constexpr (MODE == 0) {
-      // the row's own X rows (for p.Ap, and the diagonal block)
-      T xI[B][CW];
+      if constexpr (!kPreX) {
 #pragma unroll
-      for (int s = 0; s < B; ++s)
-        vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xI[s]);
+        for (int s = 0; s < B; ++s)
+          vload<T, CW>(X + static_cast<size_t>(B * I + s) * ldk + c0, xI[s]);
+      }
       // FP64 corrected diagonal block (row-major B*B per block row), held
       // like the block values: lane l of the group has entries l, l + 16, ...
       constexpr int ND = dblk_lanes<T, B, LG, MODE>();

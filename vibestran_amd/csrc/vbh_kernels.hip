@@ -69,12 +69,13 @@ __device__ __forceinline__ void st4(double* p, const Quad<double>& q) {
 // ---- deterministic reductions ---------------------------------------------------
 // A double moved between lanes by DPP (both 32-bit halves), CTRL one of the
 // in-row patterns below: every source lane lies in the destination's row of
-// 16, so masks and bound control never apply.
+// 16, so masks and bound control never apply, and the move needs no old value
+// (update_dpp with an old value makes the compiler initialise the destination).
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double x) {
   const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double((static_cast<long long>(hi) << 32) |
                               static_cast<unsigned int>(lo));
 }
