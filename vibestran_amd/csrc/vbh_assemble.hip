@@ -9,7 +9,13 @@
 // with four device passes:
 //   1. element DOF lists           (solid_global_dof_indices, solid_elements.cpp:29-42)
 //   2. topological lower pattern   every (max(gi,gj), min(gi,gj)) pair of every
-//                                  element, radix-sorted and de-duplicated
+//                                  element, radix-sorted and de-duplicated;
+//                                  when every node carries the same DOFs (3
+//                                  for solids, 6 for CQUAD4) the node pairs are
+//                                  sorted instead (36 keys per CHEXA8, not 300)
+//                                  and each is expanded into its DOF slots in
+//                                  CSR order (pass 2': same slots, same sums in
+//                                  the same order)
 //   3. element matrices           one wave64 per element; each lane owns a few
 //                                  lower entries, accumulated with the closed
 //                                  isotropic form lam*a_p*b_q + mu*a_q*b_p +
@@ -33,6 +39,7 @@
 // element order, one fixed order of the serial assembly (the reference's TBB
 // merge order is thread-dependent, assembly_parallel.hpp:84-100), the order
 // of the CPU oracle's stable build_csr.
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -216,6 +223,207 @@ __global__ void k_slot_sums(int nslots, const long long* __restrict__ seg,
     }
   }
   val[p] = acc;
+  touched[p] = any;
+}
+
+// ---- 2'. node-level pattern ------------------------------------------------------------
+// The element pairs of pass 2 repeat each node pair's D x D DOF block, so a
+// mesh whose nodes all carry D DOFs (solids 3, shells 6) is sorted by node
+// pair instead (36 keys per CHEXA8 instead of 300) and each unique node pair
+// is expanded into its DOF slots in CSR order. Requires every node's active
+// DOFs d < D to have consecutive equation numbers in d order (the DofMap
+// numbers node by node), checked by k_node_dofs.
+
+// base[a] = first active equation of node a among DOFs d < D, nact[a] their
+// count, dmask[a] bit d set for an active DOF d, node_of_eq[eq] = a; *bad = 1
+// if the active equations of a node are not consecutive.
+__global__ void k_node_dofs(int nnodes, int D, const int* __restrict__ eq, int* __restrict__ base,
+                            int* __restrict__ nact, int* __restrict__ dmask,
+                            int* __restrict__ node_of_eq, int* __restrict__ bad) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nnodes) return;
+  int b = -1, cnt = 0, prev = -1, mask = 0;
+  for (int d = 0; d < D; ++d) {
+    const int q = eq[6 * a + d];
+    if (q < 0) continue;
+    if (b < 0) b = q;
+    else if (q != prev + 1) *bad = 1;
+    prev = q;
+    ++cnt;
+    mask |= 1 << d;
+    node_of_eq[q] = a;
+  }
+  base[a] = b;
+  nact[a] = cnt;
+  dmask[a] = mask;
+}
+
+// Node-pair keys of element e at npo[e] .. (its local node pairs a >= b):
+// key (larger base << bits) | smaller base, or the sentinel if a node has no
+// active DOF; val = ebase[e] << 8 | lr << 4 | lc, lr the local index of the
+// row node (larger base), lc of the column node, ebase[e] the position of the
+// element's lower entries in ke. Elements are laid out in element order, so
+// a stable sort keeps each node pair's contributions in element order.
+// *bad = 1 for a repeated node.
+__global__ void k_node_pair_keys(int nelem, const int* __restrict__ eptr,
+                                 const int* __restrict__ conn, const int* __restrict__ npo,
+                                 const int* __restrict__ base, const int* __restrict__ nact,
+                                 const unsigned* __restrict__ ebase, int bits,
+                                 unsigned long long* __restrict__ keys,
+                                 unsigned long long* __restrict__ vals, int* __restrict__ bad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int o = eptr[e], nn = eptr[e + 1] - o;
+  const unsigned long long sentinel = (1ull << (2 * bits)) - 1ull;
+  const unsigned long long kb = static_cast<unsigned long long>(ebase[e]) << 8;
+  int p = npo[e];
+  for (int a = 0; a < nn; ++a) {
+    const int A = conn[o + a];
+    for (int b = 0; b <= a; ++b, ++p) {
+      const int B = conn[o + b];
+      unsigned long long key = sentinel;
+      int lr = a, lc = b;
+      if (nact[A] > 0 && nact[B] > 0) {
+        if (a != b && A == B) *bad = 1;
+        const unsigned long long ba = static_cast<unsigned long long>(base[A]);
+        const unsigned long long bb = static_cast<unsigned long long>(base[B]);
+        if (bb > ba) {
+          lr = b;
+          lc = a;
+        }
+        key = ((ba > bb ? ba : bb) << bits) | (ba > bb ? bb : ba);
+      }
+      keys[p] = key;
+      vals[p] = kb | static_cast<unsigned long long>((lr << 4) | lc);
+    }
+  }
+}
+
+// Per unique node pair u (sorted by row node, the diagonal pair last):
+// cnt[u] = DOFs of its column node unless diagonal; first_of[A] = A's first
+// pair, diag_of[A] = its diagonal pair; info[u] = DOF mask of the row node |
+// mask of the column node << 8 | (diagonal) << 16.
+__global__ void k_np_info(int U, const int* __restrict__ rb, const int* __restrict__ cb,
+                          const int* __restrict__ node_of_eq, const int* __restrict__ nact,
+                          const int* __restrict__ dmask, int* __restrict__ cnt,
+                          int* __restrict__ first_of, int* __restrict__ diag_of,
+                          int* __restrict__ info) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const int A = node_of_eq[rb[u]], B = node_of_eq[cb[u]];
+  cnt[u] = (A != B) ? nact[B] : 0;
+  if (u == 0 || rb[u - 1] != rb[u]) first_of[A] = u;
+  if (A == B) diag_of[A] = u;
+  info[u] = dmask[A] | (dmask[B] << 8) | ((A == B) ? (1 << 16) : 0);
+}
+
+// off[u]: the column of node pair u's first slot within its rows (the DOFs
+// of the row node's earlier column nodes).
+__global__ void k_np_off(int U, const int* __restrict__ rb, const int* __restrict__ node_of_eq,
+                         const int* __restrict__ pre, const int* __restrict__ first_of,
+                         int* __restrict__ off) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  off[u] = pre[u] - pre[first_of[node_of_eq[rb[u]]]];
+}
+
+// Lower CSR row lengths: row r of node A (active index i) holds the DOFs of
+// A's off-diagonal column nodes and then the diagonal block's first i + 1.
+__global__ void k_np_row_len(int n, const int* __restrict__ node_of_eq,
+                             const int* __restrict__ base, const int* __restrict__ pre,
+                             const int* __restrict__ first_of, const int* __restrict__ diag_of,
+                             int* __restrict__ len) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int A = node_of_eq[r];
+  if (A < 0 || diag_of[A] < 0) {
+    len[r] = 0;
+    return;
+  }
+  len[r] = pre[diag_of[A]] - pre[first_of[A]] + (r - base[A]) + 1;
+}
+
+// Slot (i, j) of node pair u: row r = rb + i, column c = cb + j (j <= i on
+// the diagonal pair) at rowptr[r] + off[u] + j. Thread t: j = t mod D, then
+// u, then i, so a wave's lanes read adjacent entries of the same element
+// matrices and write adjacent slots of the same rows. Each slot sums its
+// contributions in element order (the sorted run seg[u] .. seg[u + 1]): the
+// element's entry of local DOFs (D lr + d_i, D lc + d_j) at ke[kb + t], with
+// the skip and mark rules of k_slot_sums. Runs of up to kRunCache elements
+// (every node pair of hexa and shell meshes) are loaded at once; longer runs
+// one entry at a time.
+constexpr int kRunCache = 8;
+
+template <int D>
+__device__ __forceinline__ double np_entry(const double* __restrict__ ke, unsigned long long v,
+                                           int di, int dj) {
+  const int lr = static_cast<int>((v >> 4) & 15u), lc = static_cast<int>(v & 15u);
+  const int li = D * lr + di, lj = D * lc + dj;
+  const int hi = li > lj ? li : lj, lo = li > lj ? lj : li;
+  return ke[(v >> 8) + static_cast<unsigned long long>(hi * (hi + 1) / 2 + lo)];
+}
+
+// the k-th set bit of m (k < popcount(m)), or -1
+__device__ __forceinline__ int nth_bit(int m, int k) {
+  for (int d = 0; d < 8; ++d)
+    if ((m >> d) & 1) {
+      if (k == 0) return d;
+      --k;
+    }
+  return -1;
+}
+
+template <int D>
+__global__ void k_np_slot_sums(int U, const int* __restrict__ rb, const int* __restrict__ cb,
+                               const long long* __restrict__ seg,
+                               const unsigned long long* __restrict__ vals,
+                               const int* __restrict__ info, const int* __restrict__ off,
+                               const int* __restrict__ rowptr, const double* __restrict__ ke,
+                               int* __restrict__ t_row, int* __restrict__ t_col,
+                               double* __restrict__ t_val, int* __restrict__ touched) {
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long long>(U) * D * D) return;
+  const int j = static_cast<int>(t % D);
+  const long long rest = t / D;
+  const int u = static_cast<int>(rest % U), i = static_cast<int>(rest / U);
+  const int inf = info[u];
+  const int mA = inf & 0xff, mB = (inf >> 8) & 0xff;
+  const int di = nth_bit(mA, i);
+  if (di < 0) return;
+  const int jn = ((inf >> 16) & 1) ? i + 1 : __popc(mB);
+  if (j >= jn) return;
+  const int dj = nth_bit(mB, j);
+  const int r = rb[u] + i;
+  const long long j0 = seg[u], j1 = seg[u + 1];
+  const int L = static_cast<int>(j1 - j0);
+  double acc = 0.0;
+  int any = 0;
+  if (L <= kRunCache) {
+    // entries past L repeat the last one (a valid address); only the first L
+    // are summed
+    double x[kRunCache];
+#pragma unroll
+    for (int q = 0; q < kRunCache; ++q) x[q] = np_entry<D>(ke, vals[j0 + (q < L ? q : L - 1)], di, dj);
+#pragma unroll
+    for (int q = 0; q < kRunCache; ++q) {
+      if (q < L && x[q] != 0.0) {
+        acc += x[q];
+        any = 1;
+      }
+    }
+  } else {
+    for (long long q = j0; q < j1; ++q) {
+      const double x = np_entry<D>(ke, vals[q], di, dj);
+      if (x != 0.0) {
+        acc += x;
+        any = 1;
+      }
+    }
+  }
+  const int p = rowptr[r] + off[u] + j;
+  t_row[p] = r;
+  t_col[p] = cb[u] + j;
+  t_val[p] = acc;
   touched[p] = any;
 }
 
@@ -921,6 +1129,12 @@ __global__ void k_zero_i32(long long m, int* __restrict__ a) {
     a[i] = 0;
 }
 
+__global__ void k_set_i32(long long m, int* __restrict__ a, int v) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    a[i] = v;
+}
+
 __global__ void k_zero_f64(long long m, double* __restrict__ a) {
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < m;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -1001,8 +1215,9 @@ void sort_keys(DBuf<unsigned long long>& in, DBuf<unsigned long long>& out, size
 
 // Stable radix sort of (key, position) pairs: equal keys keep their positions
 // in increasing order.
-void sort_pairs(DBuf<unsigned long long>& kin, DBuf<unsigned long long>& kout, DBuf<unsigned>& vin,
-                DBuf<unsigned>& vout, size_t m, int end_bit, hipStream_t s) {
+template <typename V>
+void sort_pairs(DBuf<unsigned long long>& kin, DBuf<unsigned long long>& kout, DBuf<V>& vin,
+                DBuf<V>& vout, size_t m, int end_bit, hipStream_t s) {
   size_t tmp = 0;
   VBH_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin.p, kout.p, vin.p, vout.p, m, 0u,
                                     static_cast<unsigned>(end_bit), s));
@@ -1183,18 +1398,112 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   const int bits = bits_for(n_);
   size_t m = 0;
   const int ndk[5] = {24, 24, 18, 12, 24};
-  for (int k = 0; k < 5; ++k) m += groups[k].size() * static_cast<size_t>(ndk[k] * (ndk[k] + 1) / 2);
+  size_t off_k[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < 5; ++k) {
+    off_k[k] = m;
+    m += groups[k].size() * static_cast<size_t>(ndk[k] * (ndk[k] + 1) / 2);
+  }
   if (m >= (1ull << 32))
     throw_error(VBH_ERR_ARG, "vbh_assemble: more than 2^32 element matrix entries");
+  int ngroups = 0;
+  for (int k = 0; k < 5; ++k) ngroups += !groups[k].empty();
+  int nnz_topo = 0;
   DBuf<int> t_row, t_col;
+  // node-level pattern (pass 2') when every element carries the same DOFs per
+  // node: D = 3 (solids only) or 6 (CQUAD4 only); VBH_PATTERN=dof takes the
+  // pair-level pass instead
+  int D = 0;
+  {
+    const bool shells = !groups[kQuad4].empty();
+    const bool solids = ngroups > (shells ? 1 : 0);
+    const char* pe = std::getenv("VBH_PATTERN");
+    const bool force_pairs = pe && std::strcmp(pe, "dof") == 0;
+    if (!force_pairs && shells != solids && nelem < (1 << 26)) D = shells ? 6 : 3;
+  }
+  DBuf<int> np_rb, np_cb, np_pre, np_first, np_rowptr, np_node_of_eq, np_info, np_off;
+  DBuf<long long> np_seg;
+  DBuf<unsigned long long> np_vals;
+  DBuf<unsigned> np_ebase;
+  int np_u = 0;
+  bool node_level = false;
+  if (D) {
+    std::vector<int> npo(static_cast<size_t>(nelem) + 1, 0);
+    long long tot = 0;
+    for (int e = 0; e < nelem; ++e) {
+      const int nn = eptr_h[e + 1] - eptr_h[e];
+      npo[e] = static_cast<int>(tot);
+      tot += nn * (nn + 1) / 2;
+    }
+    if (tot < (1ll << 31)) {
+      npo[nelem] = static_cast<int>(tot);
+      std::vector<unsigned> eb(static_cast<size_t>(nelem), 0u);
+      for (int k = 0; k < 5; ++k) {
+        const unsigned nl = static_cast<unsigned>(ndk[k] * (ndk[k] + 1) / 2);
+        for (size_t q = 0; q < groups[k].size(); ++q)
+          eb[groups[k][q]] = static_cast<unsigned>(off_k[k]) + static_cast<unsigned>(q) * nl;
+      }
+      DBuf<int> d_npo, nbase, nact, dmask, bad;
+      d_npo.upload(npo.data(), npo.size(), s);
+      np_ebase.upload(eb.data(), eb.size(), s);
+      nbase.resize(static_cast<size_t>(nnodes));
+      nact.resize(static_cast<size_t>(nnodes));
+      dmask.resize(static_cast<size_t>(nnodes));
+      np_node_of_eq.resize(static_cast<size_t>(n_));
+      hipLaunchKernelGGL(k_set_i32, dim3(grid_for(n_, 1024)), dim3(256), 0, s,
+                         static_cast<long long>(n_), np_node_of_eq.p, -1);
+      bad.resize(1);
+      dzero(bad, s);
+      hipLaunchKernelGGL(k_node_dofs, dim3((nnodes + 255) / 256), dim3(256), 0, s, nnodes, D,
+                         d_eq.p, nbase.p, nact.p, dmask.p, np_node_of_eq.p, bad.p);
+      const size_t mn = static_cast<size_t>(tot);
+      DBuf<unsigned long long> keys, sorted, vals;
+      keys.resize(std::max<size_t>(mn, 1));
+      sorted.resize(std::max<size_t>(mn, 1));
+      vals.resize(std::max<size_t>(mn, 1));
+      np_vals.resize(std::max<size_t>(mn, 1));
+      hipLaunchKernelGGL(k_node_pair_keys, dim3((nelem + 255) / 256), dim3(256), 0, s, nelem,
+                         d_ep.p, d_conn.p, d_npo.p, nbase.p, nact.p, np_ebase.p, bits, keys.p,
+                         vals.p, bad.p);
+      int hbad = 0;
+      VBH_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      VBH_HIP(hipStreamSynchronize(s));
+      if (!hbad && mn > 0) {
+        sort_pairs(keys, sorted, vals, np_vals, mn, 2 * bits, s);
+        keys.release();
+        vals.release();
+        np_u = unique_pairs(sorted, mn, bits, np_rb, np_cb, s, &np_seg);
+        sorted.release();
+        DBuf<int> cnt, diag, len;
+        cnt.resize(static_cast<size_t>(std::max(np_u, 1)));
+        np_info.resize(static_cast<size_t>(std::max(np_u, 1)));
+        np_off.resize(static_cast<size_t>(std::max(np_u, 1)));
+        np_first.resize(static_cast<size_t>(nnodes));
+        diag.resize(static_cast<size_t>(nnodes));
+        hipLaunchKernelGGL(k_set_i32, dim3(grid_for(nnodes, 1024)), dim3(256), 0, s,
+                           static_cast<long long>(nnodes), diag.p, -1);
+        if (np_u > 0)
+          hipLaunchKernelGGL(k_np_info, dim3((np_u + 255) / 256), dim3(256), 0, s, np_u, np_rb.p,
+                             np_cb.p, np_node_of_eq.p, nact.p, dmask.p, cnt.p, np_first.p, diag.p,
+                             np_info.p);
+        np_pre.resize(static_cast<size_t>(std::max(np_u, 1)));
+        excl_scan(cnt.p, np_pre.p, static_cast<size_t>(np_u), s);
+        if (np_u > 0)
+          hipLaunchKernelGGL(k_np_off, dim3((np_u + 255) / 256), dim3(256), 0, s, np_u, np_rb.p,
+                             np_node_of_eq.p, np_pre.p, np_first.p, np_off.p);
+        len.resize(static_cast<size_t>(n_));
+        hipLaunchKernelGGL(k_np_row_len, dim3((n_ + 255) / 256), dim3(256), 0, s, n_,
+                           np_node_of_eq.p, nbase.p, np_pre.p, np_first.p, diag.p, len.p);
+        np_rowptr.resize(static_cast<size_t>(n_));
+        excl_scan(len.p, np_rowptr.p, static_cast<size_t>(n_), s);
+        nnz_topo = read_last_plus(np_rowptr.p, len.p, static_cast<size_t>(n_), s);
+        node_level = true;
+      }
+    }
+  }
   DBuf<long long> seg;       // slot p's run in the sorted pair list
   DBuf<unsigned> src_sorted;  // pair positions in sorted key order
   DBuf<int> eid;             // element of each pair position (several kinds only)
-  int ngroups = 0;
-  for (int k = 0; k < 5; ++k) ngroups += !groups[k].empty();
-  size_t off_k[5] = {0, 0, 0, 0, 0};
-  int nnz_topo = 0;
-  {
+  if (!node_level) {
     DBuf<unsigned long long> keys, sorted;
     DBuf<unsigned> src;
     keys.resize(m);
@@ -1202,14 +1511,13 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
     src.resize(m);
     src_sorted.resize(m);
     if (ngroups > 1) eid.resize(m);
-    size_t off = 0;
     for (int k = 0; k < 5; ++k) {
-      off_k[k] = off;
       const int ns = static_cast<int>(groups[k].size());
       if (!ns) continue;
       const int NL = ndk[k] * (ndk[k] + 1) / 2;
       const long long cnt = static_cast<long long>(ns) * NL;
       const dim3 g(static_cast<unsigned>((cnt + 255) / 256));
+      const size_t off = off_k[k];
       const unsigned o32 = static_cast<unsigned>(off);
       if (ndk[k] == 24)
         hipLaunchKernelGGL(k_pair_keys<24>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
@@ -1220,7 +1528,6 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
       else
         hipLaunchKernelGGL(k_pair_keys<12>, g, dim3(256), 0, s, ns, d_sel[k].p, d_gd.p,
                            keys.p + off, bits, src.p + off, o32, eid.p ? eid.p + off : nullptr);
-      off += static_cast<size_t>(cnt);
     }
     sort_pairs(keys, sorted, src, src_sorted, m, 2 * bits, s);
     keys.release();
@@ -1261,7 +1568,21 @@ void Ctx::assemble(int n_, int nnodes, const double* xyz_h, const int* eq_h, int
   DBuf<int> touched;
   t_val.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
   touched.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
-  if (nnz_topo > 0 && ngroups > 1)
+  if (node_level) {
+    t_row.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
+    t_col.resize(static_cast<size_t>(std::max(nnz_topo, 1)));
+    if (np_u > 0) {
+      const dim3 g(static_cast<unsigned>((static_cast<long long>(np_u) * D * D + 255) / 256));
+      if (D == 3)
+        hipLaunchKernelGGL(k_np_slot_sums<3>, g, dim3(256), 0, s, np_u, np_rb.p, np_cb.p, np_seg.p,
+                           np_vals.p, np_info.p, np_off.p, np_rowptr.p, ke.p, t_row.p, t_col.p,
+                           t_val.p, touched.p);
+      else
+        hipLaunchKernelGGL(k_np_slot_sums<6>, g, dim3(256), 0, s, np_u, np_rb.p, np_cb.p, np_seg.p,
+                           np_vals.p, np_info.p, np_off.p, np_rowptr.p, ke.p, t_row.p, t_col.p,
+                           t_val.p, touched.p);
+    }
+  } else if (nnz_topo > 0 && ngroups > 1)
     hipLaunchKernelGGL(k_slot_sums_by_element, dim3((nnz_topo + 255) / 256), dim3(256), 0, s,
                        nnz_topo, seg.p, src_sorted.p, eid.p, ke.p, t_val.p, touched.p);
   else if (nnz_topo > 0)
