@@ -677,8 +677,13 @@ struct ElemLds {
 // MODE 2 adds the element's internal force K_e u_e to out[6 * node + dof]
 // (every DOF of the element's nodes, constrained ones included; u_e is 0 at
 // DOFs without an equation).
+// Resident waves per SIMD asked of the compiler: 4 for assembly (MODE 0; it
+// then holds 128 VGPRs and spills the rest to scratch, and the extra waves
+// hide more latency than the spills cost: C2 6.5 -> 5.4 ms, 2 waves at 239
+// VGPRs without the bound), 2 for the contractions (C2 adjoint 12.7 -> 7.9 ms,
+// 1 wave at 257 registers without it).
 template <int K, int MODE = 0>
-__global__ __launch_bounds__(64 * kElemWaves) void k_element_stiffness(
+__global__ __launch_bounds__(64 * kElemWaves, MODE == 0 ? 4 : 2) void k_element_stiffness(
     int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
     const int* __restrict__ conn, const double* __restrict__ xyz, const double* __restrict__ Ev,
     const double* __restrict__ nuv, const int* __restrict__ gd, const int* __restrict__ rp,

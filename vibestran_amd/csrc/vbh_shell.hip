@@ -322,8 +322,12 @@ __device__ __forceinline__ void tcol(const Frame& f, int a, double t[3], int l[3
 // MODE 2: out[e] = -u_e^T (dK_e/dt_e) v_e: membrane and transverse shear scale
 //         with t, bending with t^3, the drilling penalty with the rotational
 //         diagonal entry it was taken from.
+// 2 resident waves per SIMD asked of the compiler (256 VGPRs, a few spilled):
+// 1 wave without the bound (256 VGPRs + AGPRs). Tube assembly 8.8 -> 6.9 ms,
+// adjoint contractions 12.5 -> 10.4 ms, element forces 12.1 -> 10.1 ms; a
+// bound of 4 waves spills too much (assembly 10.9 ms).
 template <int MODE>
-__global__ __launch_bounds__(64 * kElemWaves) void k_quad4(
+__global__ __launch_bounds__(64 * kElemWaves, 2) void k_quad4(
     int nsel, const int* __restrict__ sel, const int* __restrict__ eptr,
     const int* __restrict__ conn, const double* __restrict__ xyz, const double* __restrict__ Ev,
     const double* __restrict__ nuv, const double* __restrict__ thick, int form,
