@@ -596,9 +596,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
   reduce_partB(st, red, par, lds);
 }
 
+// FP64 streams ask for 8 resident waves per SIMD with 3x3 blocks (58 VGPRs
+// instead of 92: C2 83.3 -> 81.5 us per launch) and 7 with 6x6 blocks (66
+// VGPRs; 8 would spill). The FP32 stream keeps its 88 VGPRs: at 8 waves its
+// fewer loads in flight per wave cost more (34.3 -> 40.4 us).
 template <typename T, int B, typename Acc = T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_bsr(BsrDev A, CgBufs<T> b, PcgScalars* st,
-                                                        CgRed red, int par) {
+__global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? (B == 3 ? 8 : 7) : 1) void k_cg_spmv_bsr(
+    BsrDev A, CgBufs<T> b, PcgScalars* st, CgRed red, int par) {
   __shared__ double lds[kLds];
   const int l = threadIdx.x & (kSliceRows - 1);
   const int nrows = A.nbr * B;
