@@ -37,7 +37,7 @@ struct Frame {
   double xl[4], yl[4];
 };
 
-__device__ void make_frame(const double (&X)[4][3], Frame& f) {
+__device__ __forceinline__ void make_frame(const double (&X)[4][3], Frame& f) {
   const double v12[3] = {X[1][0] - X[0][0], X[1][1] - X[0][1], X[1][2] - X[0][2]};
   const double v14[3] = {X[3][0] - X[0][0], X[3][1] - X[0][1], X[3][2] - X[0][2]};
   double e3[3] = {v12[1] * v14[2] - v12[2] * v14[1], v12[2] * v14[0] - v12[0] * v14[2],
@@ -61,6 +61,13 @@ __device__ void make_frame(const double (&X)[4][3], Frame& f) {
     f.yl[n] = X[n][0] * e2[0] + X[n][1] * e2[1] + X[n][2] * e2[2];
   }
 }
+
+// a[n] for a run-time n in 0..3 / 0..1 as selects: private arrays indexed at
+// run time live in scratch memory, selected ones stay in registers.
+__device__ __forceinline__ double sel4(const double (&a)[4], int n) {
+  return n == 0 ? a[0] : (n == 1 ? a[1] : (n == 2 ? a[2] : a[3]));
+}
+__device__ __forceinline__ double sel2(const double (&a)[2], int c) { return c == 0 ? a[0] : a[1]; }
 
 // Bilinear shape functions and their parametric derivatives.
 struct Shape {
@@ -92,6 +99,7 @@ struct Point {
 __device__ __forceinline__ void point(const Frame& f, double xi, double eta, Point& q) {
   q.s = shape(xi, eta);
   q.J[0][0] = q.J[0][1] = q.J[1][0] = q.J[1][1] = 0.0;
+#pragma unroll
   for (int n = 0; n < 4; ++n) {
     q.J[0][0] += q.s.dxi[n] * f.xl[n];
     q.J[0][1] += q.s.dxi[n] * f.yl[n];
@@ -104,6 +112,7 @@ __device__ __forceinline__ void point(const Frame& f, double xi, double eta, Poi
   q.Ji[0][1] = -q.J[0][1] * id;
   q.Ji[1][0] = -q.J[1][0] * id;
   q.Ji[1][1] = q.J[0][0] * id;
+#pragma unroll
   for (int n = 0; n < 4; ++n) {
     q.dN[0][n] = q.Ji[0][0] * q.s.dxi[n] + q.Ji[0][1] * q.s.deta[n];
     q.dN[1][n] = q.Ji[1][0] * q.s.dxi[n] + q.Ji[1][1] * q.s.deta[n];
@@ -114,9 +123,10 @@ __device__ __forceinline__ void point(const Frame& f, double xi, double eta, Poi
 struct MitcCoef {
   double top, bottom, right, left, center;
 };
-__device__ int mitc_coefficients(const Frame& f, MitcCoef& a) {
+__device__ __forceinline__ int mitc_coefficients(const Frame& f, MitcCoef& a) {
   int bad = 0;
   const double tie[5][2] = {{0, 1}, {0, -1}, {1, 0}, {-1, 0}, {0, 0}};
+#pragma unroll
   for (int p = 0; p < 5; ++p) {  // tying points must map with positive Jacobian
     Point q;
     point(f, tie[p][0], tie[p][1], q);
@@ -124,8 +134,10 @@ __device__ int mitc_coefficients(const Frame& f, MitcCoef& a) {
   }
   const double RS[4] = {-1, 1, 1, -1}, SS[4] = {-1, -1, 1, 1};
   double xr[2] = {0, 0}, xs[2] = {0, 0}, xd[2] = {0, 0};
+#pragma unroll
   for (int n = 0; n < 4; ++n) {
     const double p[2] = {f.xl[n], f.yl[n]};
+#pragma unroll
     for (int k = 0; k < 2; ++k) {
       xr[k] += 0.25 * RS[n] * p[k];
       xs[k] += 0.25 * SS[n] * p[k];
@@ -156,22 +168,24 @@ __device__ __forceinline__ void cov_membrane(const Frame& f, double xi, double e
   Point q;
   point(f, xi, eta, q);
   const int n = k >> 1, c = k & 1;
-  const double gr = q.J[0][c], gs = q.J[1][c];
-  rr = gr * q.s.dxi[n];
-  ss = gs * q.s.deta[n];
-  rs = 0.5 * (gr * q.s.deta[n] + gs * q.s.dxi[n]);
+  const double gr = sel2(q.J[0], c), gs = sel2(q.J[1], c);
+  const double dxi = sel4(q.s.dxi, n), deta = sel4(q.s.deta, n);
+  rr = gr * dxi;
+  ss = gs * deta;
+  rs = 0.5 * (gr * deta + gs * dxi);
 }
 
 // Membrane B column k (u or v of node k/2) at Gauss point (r, s).
-__device__ void membrane_col(const Frame& f, const MitcCoef& a, int form, double r, double s,
+__device__ __forceinline__ void membrane_col(const Frame& f, const MitcCoef& a, int form, double r, double s,
                              int k, double out[3]) {
   Point q;
   point(f, r, s, q);
   const int n = k >> 1, c = k & 1;
   if (form == 1) {
-    out[0] = c == 0 ? q.dN[0][n] : 0.0;
-    out[1] = c == 1 ? q.dN[1][n] : 0.0;
-    out[2] = c == 0 ? q.dN[1][n] : q.dN[0][n];
+    const double d0 = sel4(q.dN[0], n), d1 = sel4(q.dN[1], n);
+    out[0] = c == 0 ? d0 : 0.0;
+    out[1] = c == 1 ? d1 : 0.0;
+    out[2] = c == 0 ? d1 : d0;
     return;
   }
   double t_rr, t_ss, t_rs, b_rr, b_ss, b_rs, r_rr, r_ss, r_rs, l_rr, l_ss, l_rs, c_rr, c_ss, c_rs;
@@ -206,22 +220,24 @@ __device__ __forceinline__ void cov_shear(const Frame& f, double xi, double eta,
   point(f, xi, eta, q);
   const int n = k / 3, c = k % 3;
   if (c == 2) {
-    gxi = q.s.dxi[n];
-    geta = q.s.deta[n];
+    gxi = sel4(q.s.dxi, n);
+    geta = sel4(q.s.deta, n);
   } else {
-    gxi = -q.s.N[n] * q.J[0][c];
-    geta = -q.s.N[n] * q.J[1][c];
+    const double Nn = sel4(q.s.N, n);
+    gxi = -Nn * sel2(q.J[0], c);
+    geta = -Nn * sel2(q.J[1], c);
   }
 }
 
 // Transverse shear B column k at Gauss point (r, s): MITC4 tying or Mindlin.
-__device__ void shear_col(const Frame& f, int form, double r, double s, int k, double out[2]) {
+__device__ __forceinline__ void shear_col(const Frame& f, int form, double r, double s, int k, double out[2]) {
   Point q;
   point(f, r, s, q);
   const int n = k / 3, c = k % 3;
   if (form == 1) {
-    out[0] = c == 2 ? q.dN[0][n] : (c == 0 ? -q.s.N[n] : 0.0);
-    out[1] = c == 2 ? q.dN[1][n] : (c == 1 ? -q.s.N[n] : 0.0);
+    const double d0 = sel4(q.dN[0], n), d1 = sel4(q.dN[1], n), Nn = sel4(q.s.N, n);
+    out[0] = c == 2 ? d0 : (c == 0 ? -Nn : 0.0);
+    out[1] = c == 2 ? d1 : (c == 1 ? -Nn : 0.0);
     return;
   }
   double xb, eb, xt, et, xr, er, xl, el, xc, ec;
@@ -263,8 +279,10 @@ __device__ __forceinline__ double btdb_entry(const double (&D)[M][M], BCOL col, 
   col(i, bi);
   col(j, bj);
   double s = 0.0;
+#pragma unroll
   for (int k = 0; k < M; ++k) {
     double db = 0.0;
+#pragma unroll
     for (int l = 0; l < M; ++l) db += D[k][l] * bj[l];
     s += bi[k] * db;
   }
@@ -299,17 +317,22 @@ __device__ __forceinline__ double klocal(const ShellLds& L, int i, int p, int j,
 
 // Column of T for global DOF a (0..5) of a node: local DOF index of each term
 // and its coefficient; T = blockdiag(R, [-R1; R0; R2]).
+__device__ __forceinline__ double rsel(const Frame& f, int c, int b) {
+  return b == 0 ? f.R[c][0] : (b == 1 ? f.R[c][1] : f.R[c][2]);
+}
+
 __device__ __forceinline__ void tcol(const Frame& f, int a, double t[3], int l[3]) {
   if (a < 3) {
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      t[c] = f.R[c][a];
+      t[c] = rsel(f, c, a);
       l[c] = c;
     }
   } else {
     const int b = a - 3;
-    t[0] = -f.R[1][b];
-    t[1] = f.R[0][b];
-    t[2] = f.R[2][b];
+    t[0] = -rsel(f, 1, b);
+    t[1] = rsel(f, 0, b);
+    t[2] = rsel(f, 2, b);
     l[0] = 3;
     l[1] = 4;
     l[2] = 5;
