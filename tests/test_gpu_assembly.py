@@ -217,3 +217,78 @@ def test_node_and_pair_patterns_identical(gpu_backend_factory, monkeypatch, mesh
     if mesh != "tube-spc":
         rp, ci, v = oracle_K(m, eq, n).arrays()
         assert np.array_equal(rp0, rp) and np.array_equal(ci0, ci)
+
+
+def _random_solid_mesh(seed):
+    """Jittered hex block whose cells become a random mix of CHEXA8, two
+    CPENTA6 and five CTETRA4 (conforming alternate splits), with some cells
+    left out (their nodes may then belong to no element), shuffled node IDs
+    and random partial SPCs."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = (int(v) for v in rng.integers(2, 5, size=3))
+    hm = decks.hex_block(nx, ny, nz, jitter=0.1, jitter_seed=seed)
+    c = hm.conn2d()
+    ijk = np.array(np.unravel_index(np.arange(len(c)), (nx, ny, nz))).T
+    kinds = rng.choice(4, size=len(c), p=[0.4, 0.2, 0.3, 0.1])  # hex, wedges, tets, none
+    et, conn = [], []
+    for e in range(len(c)):
+        if kinds[e] == 0:
+            et.append(M.CHEXA8); conn.extend(c[e])
+        elif kinds[e] == 1:
+            for w in ([0, 1, 2, 4, 5, 6], [0, 2, 3, 4, 6, 7]):
+                et.append(M.CPENTA6); conn.extend(c[e, w])
+        elif kinds[e] == 2:
+            split = ([[0, 1, 3, 4], [1, 2, 3, 6], [1, 4, 5, 6], [3, 4, 6, 7], [1, 3, 4, 6]]
+                     if ijk[e].sum() % 2 == 0 else
+                     [[0, 1, 2, 5], [0, 2, 3, 7], [0, 5, 7, 4], [2, 5, 6, 7], [0, 2, 5, 7]])
+            for t in split:
+                q = c[e, t]
+                X = hm.xyz[q]
+                if np.linalg.det(np.array([X[1] - X[0], X[2] - X[0], X[3] - X[0]])) < 0:
+                    q = q[[0, 2, 1, 3]]
+                et.append(M.CTETRA4); conn.extend(q)
+    if not et:
+        et.append(M.CHEXA8); conn.extend(c[0])
+    ep = np.cumsum([0] + [M.NODES_PER[t] for t in et]).astype(np.int32)
+    ne = len(et)
+    m = M.Model(node_ids=rng.permutation(np.arange(1, hm.nn + 1) * 3).astype(np.int32), xyz=hm.xyz,
+                elem_type=np.array(et, np.int32), elem_ptr=ep, conn=np.array(conn, np.int32),
+                E=rng.uniform(1e5, 3e5, ne), nu=np.full(ne, 0.3), rho=np.zeros(ne),
+                spc_nodes=hm.spc_nodes, spc_masks=hm.spc_masks)
+    return _partial_spc(m, seed + 100, [1, 2, 4, 3, 5, 6, 7])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_node_pair_pattern_random_meshes(gpu_backend_factory, monkeypatch, seed):
+    """Randomised meshes for the node-pair pattern pass: solids of random
+    kinds with holes, shuffled IDs and partial SPCs (even seeds), shell tubes
+    of random size with partial SPCs (odd seeds); the pass and the
+    element-pair pass agree bit for bit, and solids match the oracle's CSR."""
+    if seed % 2 == 0:
+        m = _random_solid_mesh(seed)
+    else:
+        rng = np.random.default_rng(seed)
+        m = _partial_spc(decks.cquad4_tube(n_circ=int(rng.integers(6, 14)),
+                                           n_axial=int(rng.integers(3, 9)), n_cases=1),
+                         seed + 200, [1, 2, 8, 16, 32, 9, 34, 56, 63])
+    eq, n = M.analysis_dof_map(m)
+    if n == 0:
+        pytest.skip("every DOF constrained")
+    out = []
+    for mode in ("node", "dof"):
+        if mode == "dof":
+            monkeypatch.setenv("VBH_PATTERN", "dof")
+        else:
+            monkeypatch.delenv("VBH_PATTERN", raising=False)
+        be = gpu_backend_factory()
+        K = assemble_model(be, m, eq, n).fetch()
+        out.append((K.row_ptr.copy(), K.col_ind.copy(), K.values.copy(),
+                    assembly_stats(be).nnz_topological))
+        be.close()
+    (rp0, ci0, v0, t0), (rp1, ci1, v1, t1) = out
+    assert np.array_equal(rp0, rp1) and np.array_equal(ci0, ci1) and t0 == t1
+    assert np.array_equal(v0.view(np.int64), v1.view(np.int64))
+    if seed % 2 == 0:
+        rp, ci, v = oracle_K(m, eq, n).arrays()
+        assert np.array_equal(rp0, rp) and np.array_equal(ci0, ci)
+        assert np.array_equal(v0.view(np.int64), v.view(np.int64))
