@@ -1784,23 +1784,15 @@ static void run_block_solve(Ctx& c, BlockState& b, int precision) {
     }
     if (paused) {
       ++n_refresh_passes;
-      // r = F - K u for the paused columns: tiles of 8 columns, tiles without
-      // a paused column skipped, when that streams the matrix fewer times
-      // than the iteration's own tile width would
+      // r = F - K u with the tuned shape of the current width, column tiles
+      // without a paused column skipped (pmask). Every pass streams the whole
+      // FP64 operator, so narrower shapes for a few paused columns are not
+      // cheaper (C3, one 8-column tile: 1.11 ms with direct loads in groups of
+      // 8, 0.84 ms with the 4-lane broadcast; all 32 columns with the tuned
+      // shape: 0.64 ms). Only paused columns are reloaded.
       std::vector<int> pm(static_cast<size_t>(b.ldk), 0);
-      int tiles = 0;
       for (int j = 0; j < b.ldk; ++j) pm[static_cast<size_t>(j)] = b.cols_host[j].state == kColPaused;
-      for (int t0 = 0; t0 < b.ldk; t0 += 8) {
-        int any = 0;
-        for (int j = t0; j < t0 + 8; ++j) any |= pm[static_cast<size_t>(j)];
-        tiles += any;
-      }
-      BlkTune rt = b.cur;
-      if (b.op.B > 1 && tiles * 8 < b.cur.cw * b.cur.lg) {
-        rt.cw = 1;
-        rt.lg = 8;
-        rt.stage = 0;
-      }
+      const BlkTune rt = b.cur;
       pmask.upload(pm.data(), pm.size(), s);
       spmm<double, 1>(rt, b.op, b.ldk, b.u.p, b.r64.p, b.F.p, pmask.p, b.partR.p, s);
       hipLaunchKernelGGL(k_blk_scal_refresh, dim3(b.ldk), dim3(kBT), 0, s, b.ldk, b.partR.p,
