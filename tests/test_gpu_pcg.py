@@ -234,3 +234,174 @@ def test_sell_layouts_give_identical_solves(gpu_backend_factory, monkeypatch, mi
         (u0, it0) = res[0]
         for u, it in res[1:]:
             assert it == it0 and np.array_equal(u, u0)
+
+
+def _reversed_rows(rp, ci, v):
+    """The same lower CSR with each row's entries in reverse order: rows that
+    are not increasing go through vbh_set_csr's host conversion, sorted rows
+    through its device conversion."""
+    ci2, v2 = ci.copy(), v.copy()
+    for r in range(len(rp) - 1):
+        a, b = rp[r], rp[r + 1]
+        ci2[a:b] = ci[a:b][::-1]
+        v2[a:b] = v[a:b][::-1]
+    return ci2, v2
+
+
+def _small_tube():
+    m = decks.cquad4_tube(n_circ=24, n_axial=30, n_cases=1)
+    eq, n = M.analysis_dof_map(m)
+    oc = O.assemble_model(n, m, eq)
+    return oc, M.assemble_rhs(m, eq, n, m.loads[0])
+
+
+@pytest.mark.parametrize("case,fmt", [("c1", -1), ("c1", 3), ("c1", 6), ("tube", -1),
+                                      ("tube", 3)])
+def test_set_csr_device_conversion_matches_host(gpu_backend_factory, c1, case, fmt):
+    """vbh_set_csr converts sorted lower storage (the reference's CsrData) on
+    the device; its operator must equal the host conversion's bit for bit:
+    same block format, same SpMV, same PCG iterates."""
+    if case == "c1":
+        oc, F = c1[3], c1[4]
+    else:
+        oc, F = _small_tube()
+    rp, ci, v = oc.arrays()
+    assert oc.stores_lower_triangle_only()
+    ci_r, v_r = _reversed_rows(rp, ci, v)
+    x = np.random.default_rng(3).standard_normal(len(rp) - 1)
+    res = []
+    for cols, vals in ((ci, v), (ci_r, v_r)):
+        for mixed in (False, True):
+            be = gpu_backend_factory(mixed=mixed, tolerance=1e-9, max_iters=20000)
+            be.set_format(fmt)
+            K = CsrData.from_arrays(rp, cols, vals, lower_only=True)
+            be.set_matrix(K)
+            y = be.spmv(x)
+            u = be.solve(K, F)
+            res.append((be.stats().format, y, u, be.last_iteration_count()))
+    dev, host = res[:2], res[2:]
+    for a, b in zip(dev, host):
+        assert a[0] == b[0] and a[0] in (3, 6)
+        assert np.array_equal(a[1], b[1])
+        assert np.array_equal(a[2], b[2]) and a[3] == b[3]
+    yo = oc.multiply(x)
+    assert np.abs(dev[0][1] - yo).max() <= 1e-13 * np.abs(yo).max()
+
+
+def test_set_csr_device_conversion_keeps_lower_csr(gpu_backend_factory, c1):
+    """After a device conversion the context hands back the caller's CSR."""
+    from vibestran_amd.assembly import DeviceCsr
+    oc = c1[3]
+    rp, ci, v = oc.arrays()
+    be = gpu_backend_factory()
+    be.set_matrix(CsrData.from_arrays(rp, ci, v, lower_only=True))
+    K = DeviceCsr(be, len(rp) - 1, len(ci)).fetch()
+    assert np.array_equal(K.row_ptr, rp) and np.array_equal(K.col_ind, ci)
+    assert np.array_equal(K.values, v)
+
+
+def test_set_csr_error_order(gpu_backend_factory):
+    """The first failing row decides the error, as in the reference's row loop
+    (cuda_pcg_backend_impl.hpp:2076-2101); the context keeps its operator."""
+    oc = tridiag(40, 4.0, -1.0, lower_only=True)
+    rp, ci, v = (a.copy() for a in oc.arrays())
+    be = gpu_backend_factory()
+    good = CsrData.from_arrays(rp, ci, v, lower_only=True)
+    u0 = be.solve(good, np.ones(40))
+
+    def bad(edit):
+        c2, v2 = ci.copy(), v.copy()
+        edit(c2, v2)
+        return CsrData.from_arrays(rp, c2, v2, lower_only=True)
+
+    def diag_of(r):
+        return rp[r + 1] - 1  # the diagonal ends each lower row
+
+    def zero_then_range(c2, v2):  # zero diagonal at row 7 before a bad column at row 9
+        v2[diag_of(7)] = 0.0
+        c2[rp[9]] = 99
+
+    def range_then_zero(c2, v2):  # bad column at row 5 before a zero diagonal at row 8
+        c2[rp[5]] = -1
+        v2[diag_of(8)] = 0.0
+
+    def missing_then_upper(c2, v2):  # row 6 loses its diagonal, row 12 gets an upper entry
+        c2[diag_of(6)] = 4
+        c2[rp[12]] = 13
+
+    with pytest.raises(SolverError, match="zero diagonal at row 7 "):
+        be.solve(bad(zero_then_range), np.ones(40))
+    with pytest.raises(ValueError, match="column index out of range"):
+        be.solve(bad(range_then_zero), np.ones(40))
+    with pytest.raises(SolverError, match="missing diagonal at row 6 "):
+        be.solve(bad(missing_then_upper), np.ones(40))
+    with pytest.raises(ValueError, match="row_ptr decreases"):
+        rp2 = rp.copy()
+        rp2[3], rp2[4] = rp2[4], rp2[3]
+        be.solve(CsrData.from_arrays(rp2, ci, v, lower_only=True), np.ones(40))
+    u1 = be.solve(good, np.ones(40))  # the operator installed before the failures
+    assert np.array_equal(u0, u1)
+
+
+def _expected_stream_bytes(rp, ci, n, B):
+    """Operator stream bytes of the sliced-ELL layouts, computed from the lower
+    CSR pattern alone: the full symmetric block pattern, each 64-row slice's
+    widest block row (absolute layout) and its distinct block-column offsets
+    (relative layout, chosen when within 10 % of the absolute slot count).
+    Returns (fp64, mixed) for the relative layout, or None if absolute."""
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    nbr = -(-n // B)
+    bi = np.concatenate([rows // B, ci // B, np.arange(nbr)])
+    bj = np.concatenate([ci // B, rows // B, np.arange(nbr)])
+    pair = np.unique(bi.astype(np.int64) * nbr + bj)
+    I, J = pair // nbr, pair % nbr
+    rowlen = np.bincount(I, minlength=nbr)
+    ns = -(-nbr * B // 64)
+    if rowlen.max() > 128:
+        return None
+    s0, s1 = (B * I) // 64, (B * I + B - 1) // 64
+    sl = np.concatenate([s0, s1[s1 != s0]])
+    off = np.concatenate([J - I, (J - I)[s1 != s0]])
+    key = np.unique(sl * (2 * nbr) + (off + nbr))
+    ksl, koff = key // (2 * nbr), key % (2 * nbr) - nbr
+    rel64 = len(key)
+    rel32 = rel64 - int(np.count_nonzero(koff == 0))
+    rs0, rs1 = (B * np.arange(nbr)) // 64, (B * np.arange(nbr) + B - 1) // 64
+    kmax = np.zeros(ns, np.int64)
+    np.maximum.at(kmax, rs0, rowlen)
+    np.maximum.at(kmax, rs1, rowlen)
+    if np.bincount(ksl, minlength=ns).max() > 128 or rel64 > 1.1 * kmax.sum():
+        return None
+    fp64 = rel64 * 64 * B * 8 + rel64 * 4 + 4 * (ns + 1)
+    mixed = rel32 * 64 * B * 4 + rel32 * 4 + 4 * (ns + 1) + nbr * B * B * 8
+    return fp64, mixed
+
+
+def test_relative_sell_slots_match_pattern(gpu_backend_factory):
+    """The relative sliced ELL's slot counts, built from the device BSR, equal
+    those computed from the CSR pattern on the host (C1 BSR3, a shell tube
+    BSR6, a small tube through forced 3x3 blocks)."""
+    import ctypes as C
+    from vibestran_amd import _lib as L
+    m1 = decks.config1_hex_cantilever()
+    eq, n = M.analysis_dof_map(m1)
+    c1 = O.assemble_solids(n, m1.elem_type, m1.elem_ptr, m1.conn, m1.xyz, eq, m1.E, m1.nu)
+    tube, _ = _small_tube()
+    seen_rel = 0
+    for oc, fmt, B in ((c1, -1, 3), (tube, -1, 6), (tube, 3, 3)):
+        rp, ci, v = oc.arrays()
+        n = len(rp) - 1
+        be = gpu_backend_factory(max_iters=100000)
+        be.set_format(fmt)
+        be.solve(CsrData.from_arrays(rp, ci, v, lower_only=True), np.ones(n))
+        assert be.stats().format == B
+        exp = _expected_stream_bytes(rp, ci, n, B)
+        got = []
+        for prec in (L.VBH_PREC_FP64, L.VBH_PREC_MIXED):
+            b = C.c_longlong(0)
+            assert L.load().vbh_operator_stream_bytes(be._h, prec, C.byref(b)) == L.VBH_OK
+            got.append(b.value)
+        if exp is not None:
+            seen_rel += 1
+            assert tuple(got) == exp, (B, got, exp)
+    assert seen_rel >= 2

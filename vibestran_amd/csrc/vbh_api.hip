@@ -140,6 +140,10 @@ void Ctx::upload_operator(int n_, int nnz, const int* rp, const int* ci, const d
   if (n_ <= 0) throw_error(VBH_ERR_SOLVER, "HIP PCG: stiffness matrix is empty -- no free DOFs");
   if (!rp || !ci || (!v && nnz > 0)) throw_error(VBH_ERR_ARG, "vbh_set_csr: null array");
   if (rp[0] != 0 || rp[n_] != nnz) throw_error(VBH_ERR_ARG, "vbh_set_csr: row_ptr inconsistent with nnz");
+  for (int i = 0; i < n_; ++i)
+    if (rp[i + 1] < rp[i]) throw_error(VBH_ERR_ARG, "vbh_set_csr: row_ptr decreases");
+  // Sorted lower storage (the reference's CsrData) is converted on the device.
+  if (lower && format_req != 0 && set_lower_device(n_, nnz, rp, ci, v)) return;
   // Jacobi diagonal with the reference's checks (build_jacobi_diagonal).
   std::vector<double> diag(static_cast<size_t>(n_), 0.0);
   for (int i = 0; i < n_; ++i) {

@@ -1074,6 +1074,68 @@ __global__ void k_diag_from_lower(int n, int n_pad, const int* __restrict__ rp,
   d32[r] = static_cast<float>(d);
 }
 
+// Row checks of a caller's lower CSR, made in the order of the host loop of
+// build_jacobi_diagonal (cuda_pcg_backend_impl.hpp:2076-2101) with the
+// argument checks of vbh_set_csr added: the first failing row decides,
+// and in a row its first failing entry, the missing-diagonal check coming
+// after the row's last entry. first_err = min over rows of row << 3 | code,
+// code 1 column out of range, 2 entry above the diagonal, 3 zero diagonal,
+// 4 missing diagonal. Also writes each entry's row and sets *unsorted when a
+// row's columns are not strictly increasing (a duplicate included).
+__global__ void k_check_lower_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                   const double* __restrict__ v, int* __restrict__ row_of,
+                                   unsigned long long* __restrict__ first_err,
+                                   int* __restrict__ unsorted) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int k0 = rp[r], k1 = rp[r + 1];
+  int code = 0, prev = -1;
+  bool found = false, sorted = true;
+  for (int k = k0; k < k1; ++k) {
+    row_of[k] = r;
+    const int c = ci[k];
+    if (code) continue;
+    if (c < 0 || c >= n) {
+      code = 1;
+    } else if (c > r) {
+      code = 2;
+    } else {
+      sorted = sorted && c > prev;
+      prev = c;
+      if (c == r && !found) {
+        if (v[k] == 0.0) code = 3;
+        found = true;
+      }
+    }
+  }
+  if (!code && !found) code = 4;
+  if (code) atomicMin(first_err, (static_cast<unsigned long long>(r) << 3) | static_cast<unsigned>(code));
+  if (!sorted) atomicOr(unsorted, 1);
+}
+
+__global__ void k_init_check(unsigned long long* first_err, int* unsorted) {
+  *first_err = ~0ull;
+  *unsorted = 0;
+}
+
+// (slice, block-column offset) keys of the relative sliced ELL: block row I
+// covers rows B*I .. B*I+B-1 and so lies in one slice or two; each of its
+// blocks q gives the key slice << bits | (bci[q] - I + nbr) for each of them
+// (the sentinel where the row lies in one slice only).
+__global__ void k_rel_keys(int nbr, int B, const int* __restrict__ brp, const int* __restrict__ bci,
+                           int bits, unsigned long long sentinel,
+                           unsigned long long* __restrict__ keys) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const unsigned long long s0 = static_cast<unsigned>((B * I) / kSliceRows);
+  const unsigned long long s1 = static_cast<unsigned>((B * I + B - 1) / kSliceRows);
+  for (int q = brp[I]; q < brp[I + 1]; ++q) {
+    const unsigned long long off = static_cast<unsigned>(bci[q] - I + nbr);
+    keys[2 * static_cast<size_t>(q)] = (s0 << bits) | off;
+    keys[2 * static_cast<size_t>(q) + 1] = s1 != s0 ? ((s1 << bits) | off) : sentinel;
+  }
+}
+
 // ---- BSR (full symmetric) from the lower CSR ----------------------------------------------------
 __global__ void k_block_keys(int nnz, int B, const int* __restrict__ row_of,
                              const int* __restrict__ ci, unsigned long long* __restrict__ keys,
@@ -1279,29 +1341,70 @@ static int unique_pairs(DBuf<unsigned long long>& keys, size_t m, int bits, DBuf
   return cnt;
 }
 
-// Build the BSR(B) operator on the device from the device lower CSR held in
-// ctx (l_rp/l_ci/l_v/l_row). Returns false if the block fill is too high.
-bool build_device_bsr(Ctx& c, int B) {
-  hipStream_t s = c.stream;
-  const int n = c.n, nnz = c.l_nnz;
-  const int nbr = (n + B - 1) / B;
-  const int bits = bits_for(nbr);
+// Block pattern of the full symmetric operator of a lower CSR (entry rows
+// row_of, columns ci): the sorted unique block pairs (brow, bcol).
+struct BlockPattern {
+  int B = 0, nbr = 0, nnzb = 0;
+  DBuf<int> brow, bcol;
+};
+
+static void block_pattern(int n, int nnz, const int* row_of, const int* ci, int B, BlockPattern& p,
+                          hipStream_t s) {
+  p.B = B;
+  p.nbr = (n + B - 1) / B;
+  const int bits = bits_for(p.nbr);
   DBuf<unsigned long long> keys, sorted;
   const size_t m = 2 * static_cast<size_t>(nnz);
   keys.resize(m);
   sorted.resize(m);
-  hipLaunchKernelGGL(k_block_keys, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, B, c.l_row.p,
-                     c.l_ci.p, keys.p, bits);
+  hipLaunchKernelGGL(k_block_keys, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, B, row_of, ci,
+                     keys.p, bits);
   sort_keys(keys, sorted, m, 2 * bits, s);
   keys.release();
-  DBuf<int> brow, bcol;
-  const int nnzb = unique_pairs(sorted, m, bits, brow, bcol, s);
+  p.nnzb = unique_pairs(sorted, m, bits, p.brow, p.bcol, s);
+}
+
+// Sorted distinct block-column offsets (bci - I) of the block rows each
+// slice of kSliceRows rows overlaps: offsets of slice sl at
+// offs[koff[sl] .. koff[sl + 1]).
+void sell_slice_offsets(const BsrDev& A, int nslices, std::vector<int>& koff,
+                        std::vector<int>& offs, hipStream_t s) {
+  const int nbr = A.nbr;
+  const int bits = bits_for(2ll * nbr);
+  const unsigned long long sentinel = (1ull << (2 * bits)) - 1ull;
+  const size_t m = 2 * static_cast<size_t>(A.nnzb);
+  DBuf<unsigned long long> keys, sorted;
+  keys.resize(std::max<size_t>(m, 1));
+  sorted.resize(std::max<size_t>(m, 1));
+  hipLaunchKernelGGL(k_rel_keys, dim3((nbr + 255) / 256), dim3(256), 0, s, nbr, A.B, A.brp, A.bci,
+                     bits, sentinel, keys.p);
+  sort_keys(keys, sorted, m, 2 * bits, s);
+  keys.release();
+  DBuf<int> slice_of, col, d_koff;
+  const int cnt = unique_pairs(sorted, m, bits, slice_of, col, s);
   sorted.release();
-  const long long full_nnz = 2ll * nnz - n;
-  const double fill = static_cast<double>(nnzb) * B * B / static_cast<double>(full_nnz);
-  // 6x6 blocks are requested for shell meshes, whose node blocks hold
-  // structural zeros (fill about 1.2, the same as with 3x3 blocks)
-  if (fill > (B == 6 ? 1.3 : 1.25)) return false;
+  d_koff.resize(static_cast<size_t>(nslices) + 1);
+  hipLaunchKernelGGL(k_row_ptr, dim3((cnt + 1 + 255) / 256), dim3(256), 0, s, cnt, nslices,
+                     slice_of.p, d_koff.p);
+  koff.resize(static_cast<size_t>(nslices) + 1);
+  offs.resize(static_cast<size_t>(cnt));
+  VBH_HIP(hipMemcpyAsync(koff.data(), d_koff.p, sizeof(int) * koff.size(), hipMemcpyDeviceToHost, s));
+  if (cnt)
+    VBH_HIP(hipMemcpyAsync(offs.data(), col.p, sizeof(int) * offs.size(), hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  for (int& o : offs) o -= nbr;
+}
+
+static double block_fill(const BlockPattern& p, long long full_nnz) {
+  return static_cast<double>(p.nnzb) * p.B * p.B / static_cast<double>(full_nnz);
+}
+
+// BSR values of the device lower CSR held in ctx (l_row/l_ci/l_v) on pattern p.
+static void fill_bsr(Ctx& c, BlockPattern& p) {
+  hipStream_t s = c.stream;
+  const int n = c.n, nnz = c.l_nnz, B = p.B, nbr = p.nbr, nnzb = p.nnzb;
+  DBuf<int>& brow = p.brow;
+  DBuf<int>& bcol = p.bcol;
   c.bsr_rp.resize(static_cast<size_t>(nbr) + 1);
   hipLaunchKernelGGL(k_row_ptr, dim3((nnzb + 1 + 255) / 256), dim3(256), 0, s, nnzb, nbr, brow.p,
                      c.bsr_rp.p);
@@ -1334,6 +1437,17 @@ bool build_device_bsr(Ctx& c, int B) {
   c.bsrd.v64 = c.bsr_v64.p;
   c.bsrd.v32 = c.bsr_v32.p;
   build_sell(c);
+}
+
+// Build the BSR(B) operator on the device from the device lower CSR held in
+// ctx (l_rp/l_ci/l_v/l_row). Returns false if the block fill is too high.
+bool build_device_bsr(Ctx& c, int B) {
+  BlockPattern p;
+  block_pattern(c.n, c.l_nnz, c.l_row.p, c.l_ci.p, B, p, c.stream);
+  // 6x6 blocks are requested for shell meshes, whose node blocks hold
+  // structural zeros (fill about 1.2, the same as with 3x3 blocks)
+  if (block_fill(p, 2ll * c.l_nnz - c.n) > (B == 6 ? 1.3 : 1.25)) return false;
+  fill_bsr(c, p);
   return true;
 }
 
@@ -1800,6 +1914,13 @@ void Ctx::finish_device_operator() {
     upload_operator(n, static_cast<int>(ci.size()), rp.data(), ci.data(), v.data(), true);
     return;
   }
+  finish_block_operator(B);
+}
+
+// Jacobi diagonal, checks and bookkeeping of a BSR(B) operator built on the
+// device from the device lower CSR.
+void Ctx::finish_block_operator(int B) {
+  hipStream_t s = stream;
   format = B;
   n_pad = bsrd.nbr * B;
   DBuf<int> d_err;
@@ -1841,6 +1962,84 @@ void Ctx::fetch_lower_to_host() {
   }
   VBH_HIP(hipStreamSynchronize(stream));
   lower_on_host = true;
+}
+
+// Operator from a caller's lower CSR (vbh_set_csr) built on the device: the
+// arrays are uploaded as they are, checked row by row, and blocked by the
+// same rule as the host conversion. Returns false, with the context
+// untouched, when the rows are not strictly increasing or no block format
+// fits (the host conversion then handles the matrix); throws the host
+// conversion's errors.
+bool Ctx::set_lower_device(int n_, int nnz, const int* rp, const int* ci, const double* v) {
+  hipStream_t s = stream;
+  DBuf<int> t_rp, t_ci, t_row;
+  DBuf<double> t_v;
+  t_rp.upload(rp, static_cast<size_t>(n_) + 1, s);
+  t_ci.upload(ci, static_cast<size_t>(nnz), s);
+  t_v.upload(v, static_cast<size_t>(nnz), s);
+  t_row.resize(static_cast<size_t>(std::max(nnz, 1)));
+  struct Check {
+    unsigned long long first_err;
+    int unsorted;
+  };
+  DBuf<Check> d_chk;
+  d_chk.resize(1);
+  hipLaunchKernelGGL(k_init_check, dim3(1), dim3(1), 0, s, &d_chk.p->first_err, &d_chk.p->unsorted);
+  hipLaunchKernelGGL(k_check_lower_rows, dim3((n_ + 255) / 256), dim3(256), 0, s, n_, t_rp.p,
+                     t_ci.p, t_v.p, t_row.p, &d_chk.p->first_err, &d_chk.p->unsorted);
+  Check h{};
+  VBH_HIP(hipMemcpyAsync(&h, d_chk.p, sizeof h, hipMemcpyDeviceToHost, s));
+  VBH_HIP(hipStreamSynchronize(s));
+  if (h.first_err != ~0ull) {
+    const std::string row = std::to_string(h.first_err >> 3);
+    switch (static_cast<int>(h.first_err & 7)) {
+      case 1: throw_error(VBH_ERR_ARG, "vbh_set_csr: column index out of range");
+      case 2: throw_error(VBH_ERR_ARG, "vbh_set_csr: upper entry in lower storage");
+      case 3:
+        throw_error(VBH_ERR_SOLVER, "HIP PCG: zero diagonal at row " + row +
+                                        " -- matrix is singular. Check boundary conditions.");
+      default:
+        throw_error(VBH_ERR_SOLVER, "HIP PCG: missing diagonal at row " + row +
+                                        " -- matrix is singular. Check boundary conditions.");
+    }
+  }
+  if (h.unsorted) return false;
+  // Block size: the host conversion's rule (6x6 node blocks when they carry
+  // no more padding than 3x3 blocks would, else 3x3 when the fill is low).
+  const long long full_nnz = 2ll * nnz - n_;
+  BlockPattern p6, p3;
+  BlockPattern* pick = nullptr;
+  if (format_req == 3 || format_req == 6) {
+    BlockPattern& p = format_req == 6 ? p6 : p3;
+    block_pattern(n_, nnz, t_row.p, t_ci.p, format_req, p, s);
+    pick = &p;
+  } else {
+    double f6 = 1e300, f3 = 1e300;
+    if (n_ >= 24) {
+      block_pattern(n_, nnz, t_row.p, t_ci.p, 6, p6, s);
+      f6 = block_fill(p6, full_nnz);
+    }
+    if (f6 > 1.12 && n_ >= 12) {
+      block_pattern(n_, nnz, t_row.p, t_ci.p, 3, p3, s);
+      f3 = block_fill(p3, full_nnz);
+    }
+    if (f6 <= 1.12 || (f6 <= 1.3 && f6 <= 1.05 * f3)) pick = &p6;
+    else if (f3 <= 1.25) pick = &p3;
+    else return false;
+  }
+  n = n_;
+  l_nnz = nnz;
+  l_rp.swap(t_rp);
+  l_ci.swap(t_ci);
+  l_v.swap(t_v);
+  l_row.swap(t_row);
+  lower_on_host = false;
+  lower_rp.clear();
+  lower_ci.clear();
+  lower_v.clear();
+  fill_bsr(*this, *pick);
+  finish_block_operator(pick->B);
+  return true;
 }
 
 // GPU DOF numbering (closed form of build_analysis_dof_map).

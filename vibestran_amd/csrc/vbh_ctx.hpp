@@ -172,6 +172,8 @@ struct Ctx {
                 const int* eptr, const int* conn, const double* E, const double* nu,
                 const double* thick, int solid_form, int shell_form);
   void finish_device_operator();
+  void finish_block_operator(int B);
+  bool set_lower_device(int n, int nnz, const int* rp, const int* ci, const double* v);
   void apply_mpc_to_lower();
   // API vector (n_api, host or device) <-> operator vector (n, device)
   void to_operator(const double* x, bool dev, double* dst, int k = 1);

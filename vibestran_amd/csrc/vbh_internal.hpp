@@ -236,6 +236,10 @@ void scratch_init(int device);
 // Builds the sliced-ELL arrays of a device BSR (vbh_sell.hip).
 struct Ctx;
 void build_sell(Ctx& c);
+// Sorted distinct block-column offsets (bci - I) of the block rows each slice
+// overlaps, as a CSR over slices (vbh_assemble.hip).
+void sell_slice_offsets(const BsrDev& A, int nslices, std::vector<int>& koff,
+                        std::vector<int>& offs, hipStream_t s);
 
 // Plain elementwise helpers.
 void launch_cast_f64_to_f32(int n, const double* a, float* b, hipStream_t s);

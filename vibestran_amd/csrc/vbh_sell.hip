@@ -314,24 +314,18 @@ void build_sell_relative(Ctx& c) {
   if ((env && env[0] == '0') || A.nlong > 0) return;
   hipStream_t s = c.stream;
   const int B = A.B, nbr = A.nbr, nslices = A.nslices;
-  std::vector<int> brp(static_cast<size_t>(nbr) + 1), bci(static_cast<size_t>(A.nnzb));
-  VBH_HIP(hipMemcpyAsync(brp.data(), A.brp, sizeof(int) * brp.size(), hipMemcpyDeviceToHost, s));
-  VBH_HIP(hipMemcpyAsync(bci.data(), A.bci, sizeof(int) * bci.size(), hipMemcpyDeviceToHost, s));
-  VBH_HIP(hipStreamSynchronize(s));
+  std::vector<int> koff, offs;
+  sell_slice_offsets(A, nslices, koff, offs, s);
   const bool split = A.dblk != nullptr;  // FP32 list without the diagonal offset
   std::vector<int> k64(static_cast<size_t>(nslices) + 1, 0), k32(static_cast<size_t>(nslices) + 1, 0);
-  std::vector<int> o64, o32, u;
+  std::vector<int> o64, o32;
+  o64.reserve(offs.size());
+  o32.reserve(offs.size());
   for (int sl = 0; sl < nslices; ++sl) {
-    const int n0 = (kSliceRows * sl) / B;
-    const int n1 = std::min((kSliceRows * sl + kSliceRows - 1) / B, nbr - 1);
-    u.clear();
-    for (int I = n0; I <= n1; ++I)
-      for (int q = brp[static_cast<size_t>(I)]; q < brp[static_cast<size_t>(I) + 1]; ++q)
-        u.push_back(bci[static_cast<size_t>(q)] - I);
-    std::sort(u.begin(), u.end());
-    u.erase(std::unique(u.begin(), u.end()), u.end());
-    if (static_cast<int>(u.size()) > kSellCap) return;
-    for (int off : u) {
+    const int a = koff[static_cast<size_t>(sl)], b = koff[static_cast<size_t>(sl) + 1];
+    if (b - a > kSellCap) return;
+    for (int q = a; q < b; ++q) {
+      const int off = offs[static_cast<size_t>(q)];
       o64.push_back(off);
       if (!(split && off == 0)) o32.push_back(off);
     }
