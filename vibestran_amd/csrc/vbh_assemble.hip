@@ -1136,6 +1136,16 @@ __global__ void k_rel_keys(int nbr, int B, const int* __restrict__ brp, const in
   }
 }
 
+// 6x6 block keys from a 3x3 block pattern (both triangles): 6x6 block
+// (I/2, J/2) holds 3x3 block (I, J).
+__global__ void k_coarse_keys(int nnzb, const int* __restrict__ brow, const int* __restrict__ bcol,
+                              int bits, unsigned long long* __restrict__ keys) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnzb) return;
+  keys[q] = (static_cast<unsigned long long>(brow[q] >> 1) << bits) |
+            static_cast<unsigned long long>(bcol[q] >> 1);
+}
+
 // ---- BSR (full symmetric) from the lower CSR ----------------------------------------------------
 __global__ void k_block_keys(int nnz, int B, const int* __restrict__ row_of,
                              const int* __restrict__ ci, unsigned long long* __restrict__ keys,
@@ -1393,6 +1403,23 @@ void sell_slice_offsets(const BsrDev& A, int nslices, std::vector<int>& koff,
     VBH_HIP(hipMemcpyAsync(offs.data(), col.p, sizeof(int) * offs.size(), hipMemcpyDeviceToHost, s));
   VBH_HIP(hipStreamSynchronize(s));
   for (int& o : offs) o -= nbr;
+}
+
+// The 6x6 block pattern of the same matrix from its 3x3 one (p3.nnzb keys
+// instead of two per lower entry).
+static void coarsen_pattern(int n, const BlockPattern& p3, BlockPattern& p6, hipStream_t s) {
+  p6.B = 6;
+  p6.nbr = (n + 5) / 6;
+  const int bits = bits_for(p6.nbr);
+  const size_t m = static_cast<size_t>(p3.nnzb);
+  DBuf<unsigned long long> keys, sorted;
+  keys.resize(std::max<size_t>(m, 1));
+  sorted.resize(std::max<size_t>(m, 1));
+  hipLaunchKernelGGL(k_coarse_keys, dim3((p3.nnzb + 255) / 256), dim3(256), 0, s, p3.nnzb,
+                     p3.brow.p, p3.bcol.p, bits, keys.p);
+  sort_keys(keys, sorted, m, 2 * bits, s);
+  keys.release();
+  p6.nnzb = unique_pairs(sorted, m, bits, p6.brow, p6.bcol, s);
 }
 
 static double block_fill(const BlockPattern& p, long long full_nnz) {
@@ -2014,14 +2041,15 @@ bool Ctx::set_lower_device(int n_, int nnz, const int* rp, const int* ci, const 
     block_pattern(n_, nnz, t_row.p, t_ci.p, format_req, p, s);
     pick = &p;
   } else {
+    // (the 6x6 pattern is derived from the 3x3 one, which is built first)
     double f6 = 1e300, f3 = 1e300;
-    if (n_ >= 24) {
-      block_pattern(n_, nnz, t_row.p, t_ci.p, 6, p6, s);
-      f6 = block_fill(p6, full_nnz);
-    }
-    if (f6 > 1.12 && n_ >= 12) {
+    if (n_ >= 12) {
       block_pattern(n_, nnz, t_row.p, t_ci.p, 3, p3, s);
-      f3 = block_fill(p3, full_nnz);
+      if (n_ >= 24) {
+        coarsen_pattern(n_, p3, p6, s);
+        f6 = block_fill(p6, full_nnz);
+      }
+      if (f6 > 1.12) f3 = block_fill(p3, full_nnz);
     }
     if (f6 <= 1.12 || (f6 <= 1.3 && f6 <= 1.05 * f3)) pick = &p6;
     else if (f3 <= 1.25) pick = &p3;
