@@ -405,3 +405,20 @@ def test_relative_sell_slots_match_pattern(gpu_backend_factory):
             seen_rel += 1
             assert tuple(got) == exp, (B, got, exp)
     assert seen_rel >= 2
+
+
+def test_set_csr_rejects_operators_beyond_32_bit_indices(gpu_backend_factory):
+    """A lower CSR whose full symmetric operator would hold 2^31 or more
+    entries is refused before any entry is read (only row_ptr's two ends are
+    looked at here), and the context stays usable."""
+    import ctypes as C
+    from vibestran_amd import _lib as L
+    be = gpu_backend_factory()
+    oc = tridiag(20, 4.0, -1.0, lower_only=True)
+    u0 = be.solve(csr_of(oc), np.ones(20))
+    rp = np.array([0, 2 ** 30], dtype=np.int32)
+    ci = np.zeros(1, dtype=np.int32)
+    v = np.ones(1)
+    rc = L.load().vbh_set_csr(be._h, 1, 2 ** 30, L.ip(rp), L.ip(ci), L.dp(v), 1)
+    assert rc == L.VBH_ERR_ARG and "2^31" in L.last_error()
+    assert np.array_equal(be.solve(csr_of(oc), np.ones(20)), u0)

@@ -142,6 +142,9 @@ void Ctx::upload_operator(int n_, int nnz, const int* rp, const int* ci, const d
   if (rp[0] != 0 || rp[n_] != nnz) throw_error(VBH_ERR_ARG, "vbh_set_csr: row_ptr inconsistent with nnz");
   for (int i = 0; i < n_; ++i)
     if (rp[i + 1] < rp[i]) throw_error(VBH_ERR_ARG, "vbh_set_csr: row_ptr decreases");
+  if ((lower ? 2ll * nnz - n_ : static_cast<long long>(nnz)) >= 0x7fffffffll)
+    throw_error(VBH_ERR_ARG, "vbh_set_csr: the full symmetric operator has 2^31 or more entries "
+                             "(32-bit indices)");
   // Sorted lower storage (the reference's CsrData) is converted on the device.
   if (lower && format_req != 0 && set_lower_device(n_, nnz, rp, ci, v)) return;
   // Jacobi diagonal with the reference's checks (build_jacobi_diagonal).

@@ -1927,7 +1927,9 @@ void Ctx::element_forces(const double* u, double* fout, bool dev) {
 // Device operator from the device lower CSR: diagonal checks, BSR build (or a
 // host round trip for matrices without block structure).
 void Ctx::finish_device_operator() {
-  hipStream_t s = stream;
+  if (2ll * l_nnz >= 0x7fffffffll)
+    throw_error(VBH_ERR_ARG, "assembly: 2^30 or more lower entries (the block build counts its "
+                             "keys with 32-bit indices)");
   // Block size: 6 when all rows come in node groups of six, else 3.
   int B = format_req;
   if (B < 0) B = block_hint;
@@ -1995,10 +1997,12 @@ void Ctx::fetch_lower_to_host() {
 // arrays are uploaded as they are, checked row by row, and blocked by the
 // same rule as the host conversion. Returns false, with the context
 // untouched, when the rows are not strictly increasing or no block format
-// fits (the host conversion then handles the matrix); throws the host
-// conversion's errors.
+// fits, or when the matrix has 2^30 entries or more (the host conversion then
+// handles the matrix); throws the host conversion's errors.
 bool Ctx::set_lower_device(int n_, int nnz, const int* rp, const int* ci, const double* v) {
   hipStream_t s = stream;
+  // the block keys (two per entry) are counted with 32-bit scans
+  if (2ll * nnz >= 0x7fffffffll) return false;
   DBuf<int> t_rp, t_ci, t_row;
   DBuf<double> t_v;
   t_rp.upload(rp, static_cast<size_t>(n_) + 1, s);
