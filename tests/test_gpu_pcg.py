@@ -422,3 +422,41 @@ def test_set_csr_rejects_operators_beyond_32_bit_indices(gpu_backend_factory):
     rc = L.load().vbh_set_csr(be._h, 1, 2 ** 30, L.ip(rp), L.ip(ci), L.dp(v), 1)
     assert rc == L.VBH_ERR_ARG and "2^31" in L.last_error()
     assert np.array_equal(be.solve(csr_of(oc), np.ones(20)), u0)
+
+
+def test_set_csr_full_and_duplicate_storage(gpu_backend_factory, c1):
+    """The same operator handed over as the oracle's lower CSR (device
+    conversion), as its expanded full symmetric CSR, and as a lower CSR with
+    every off-diagonal entry split into two halves (both host conversions,
+    duplicates summed, exactly): same block format and bit-identical SpMV and
+    PCG iterates."""
+    oc, F = c1[3], c1[4]
+    rp, ci, v = oc.arrays()
+    full = oc.expanded_symmetric()
+    frp, fci, fv = full.arrays()
+    assert not full.stores_lower_triangle_only()
+    # duplicates: each off-diagonal lower entry v -> v/2 twice (exact halves)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = ci != rows
+    reps = np.where(off, 2, 1)
+    drp = np.concatenate([[0], np.cumsum(np.add.reduceat(reps, rp[:-1]))]).astype(np.int32)
+    dci = np.repeat(ci, reps).astype(np.int32)
+    dv = np.repeat(np.where(off, v / 2.0, v), reps)
+    x = np.random.default_rng(5).standard_normal(n)
+    yo = oc.multiply(x)
+    res = []
+    for K in (CsrData.from_arrays(rp, ci, v, lower_only=True),
+              CsrData.from_arrays(frp, fci, fv, lower_only=False),
+              CsrData.from_arrays(drp, dci, dv, lower_only=True)):
+        be = gpu_backend_factory(tolerance=1e-10, max_iters=20000)
+        be.set_matrix(K)
+        y = be.spmv(x)
+        assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+        u = be.solve(K, F)
+        res.append((be.stats().format, be.last_iteration_count(), u, y))
+    assert np.array_equal(res[1][3], res[0][3]) and np.array_equal(res[2][3], res[0][3])
+    assert res[0][0] == res[1][0] == res[2][0] == 3
+    assert res[0][1] == res[1][1] == res[2][1]
+    for r in res[1:]:
+        assert np.array_equal(r[2], res[0][2])

@@ -14,6 +14,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "vbh_ctx.hpp"
@@ -80,6 +81,46 @@ HostFull expand_full(int n, int nnz, const int* rp, const int* ci, const double*
       }
     }
   return f;
+}
+
+// Rows of f sorted by column with repeated entries summed in storage order
+// (callers may pass unsorted rows or repeated entries; the reference's
+// CsrData, built by build_csr, has neither). Rows already strictly
+// increasing are copied as they are.
+void canonical_rows(HostFull& f) {
+  bool clean = true;
+  for (int r = 0; r < f.n && clean; ++r)
+    for (int k = f.rp[static_cast<size_t>(r)] + 1; k < f.rp[static_cast<size_t>(r) + 1]; ++k)
+      if (f.ci[static_cast<size_t>(k)] <= f.ci[static_cast<size_t>(k) - 1]) {
+        clean = false;
+        break;
+      }
+  if (clean) return;
+  std::vector<int> rp(static_cast<size_t>(f.n) + 1, 0), ci;
+  std::vector<double> v;
+  ci.reserve(f.ci.size());
+  v.reserve(f.v.size());
+  std::vector<std::pair<int, int>> row;  // (column, position)
+  for (int r = 0; r < f.n; ++r) {
+    row.clear();
+    for (int k = f.rp[static_cast<size_t>(r)]; k < f.rp[static_cast<size_t>(r) + 1]; ++k)
+      row.emplace_back(f.ci[static_cast<size_t>(k)], k);
+    std::stable_sort(row.begin(), row.end(),
+                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first < b.first; });
+    for (size_t q = 0; q < row.size(); ++q) {
+      const double x = f.v[static_cast<size_t>(row[q].second)];
+      if (q > 0 && row[q].first == row[q - 1].first) {
+        v.back() += x;
+      } else {
+        ci.push_back(row[q].first);
+        v.push_back(x);
+      }
+    }
+    rp[static_cast<size_t>(r) + 1] = static_cast<int>(ci.size());
+  }
+  f.rp.swap(rp);
+  f.ci.swap(ci);
+  f.v.swap(v);
 }
 
 struct HostBsr {
@@ -187,6 +228,7 @@ void Ctx::upload_operator(int n_, int nnz, const int* rp, const int* ci, const d
     }
   }
   HostFull full = expand_full(n_, nnz, rp, ci, v, lower);
+  canonical_rows(full);
   // Format choice: BSR when the block fill stays low.
   int fmt = format_req;
   HostBsr bsr;
