@@ -182,7 +182,62 @@ __global__ void k_to_f32(long long m, const double* __restrict__ a, float* __res
 
 }  // namespace
 
-void build_sell_relative(Ctx& c);
+bool build_sell_relative(Ctx& c);
+
+// Fills the absolute sliced ELL of ktot slots: FP64 and FP32 values, block
+// columns (16-bit where every offset fits), and the FP32 diagonal zeroed
+// where the corrected FP64 diagonal blocks carry it.
+static void build_sell_absolute(Ctx& c, int ktot) {
+  hipStream_t s = c.stream;
+  BsrDev& A = c.bsrd;
+  const int B = A.B, nslices = A.nslices;
+  const size_t nval = static_cast<size_t>(ktot) * kSliceRows * B;
+  c.sell_v64.resize(nval > 0 ? nval : 1);
+  c.sell_v32.resize(nval > 0 ? nval : 1);
+  c.sell_idx.resize(static_cast<size_t>(ktot) * sell_nps(B) + 1);
+  if (B == 3)
+    hipLaunchKernelGGL(k_sell_fill<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr, A.brp,
+                       A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
+  else
+    hipLaunchKernelGGL(k_sell_fill<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr, A.brp,
+                       A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(static_cast<long long>(nval), 1024)), dim3(256), 0, s,
+                     static_cast<long long>(nval), c.sell_v64.p, c.sell_v32.p);
+  if (A.dblk) {
+    if (B == 3)
+      hipLaunchKernelGGL(k_sell_zero_diag<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_kdiag.p, c.sell_v32.p);
+    else
+      hipLaunchKernelGGL(k_sell_zero_diag<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_kdiag.p, c.sell_v32.p);
+  }
+  // 16-bit block columns when every offset fits (VBH_SELL_IDX16=0 keeps the
+  // 32-bit ones)
+  A.sidx16 = nullptr;
+  c.sell_idx16.release();
+  const char* i16_env = std::getenv("VBH_SELL_IDX16");
+  if (!(i16_env && i16_env[0] == '0')) {
+    c.sell_idx16.resize(static_cast<size_t>(ktot) * sell_nps(B) + 1);
+    DBuf<int> bad;
+    bad.resize(1);
+    VBH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    if (B == 3)
+      hipLaunchKernelGGL(k_sell_idx16<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_idx.p, c.sell_idx16.p, bad.p);
+    else
+      hipLaunchKernelGGL(k_sell_idx16<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
+                         c.sell_koff.p, c.sell_idx.p, c.sell_idx16.p, bad.p);
+    int h_bad = 0;
+    VBH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    VBH_HIP(hipStreamSynchronize(s));
+    if (h_bad) c.sell_idx16.release();
+    else A.sidx16 = c.sell_idx16.p;
+  }
+  VBH_HIP(hipStreamSynchronize(s));
+  A.sidx = c.sell_idx.p;
+  A.s64 = c.sell_v64.p;
+  A.s32 = c.sell_v32.p;
+}
 
 void build_sell(Ctx& c) {
   hipStream_t s = c.stream;
@@ -209,33 +264,16 @@ void build_sell(Ctx& c) {
   int ktot = 0;
   VBH_HIP(hipMemcpyAsync(&ktot, c.sell_koff.p + nslices, sizeof(int), hipMemcpyDeviceToHost, s));
   VBH_HIP(hipStreamSynchronize(s));
-  const size_t nval = static_cast<size_t>(ktot) * kSliceRows * B;
-  c.sell_v64.resize(nval > 0 ? nval : 1);
-  c.sell_v32.resize(nval > 0 ? nval : 1);
-  c.sell_idx.resize(static_cast<size_t>(ktot) * sell_nps(B) + 1);
-  if (B == 3)
-    hipLaunchKernelGGL(k_sell_fill<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr, A.brp,
-                       A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
-  else
-    hipLaunchKernelGGL(k_sell_fill<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr, A.brp,
-                       A.bci, A.v64, c.sell_koff.p, c.sell_v64.p, c.sell_idx.p);
-  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(static_cast<long long>(nval), 1024)), dim3(256), 0, s,
-                     static_cast<long long>(nval), c.sell_v64.p, c.sell_v32.p);
   A.dblk = nullptr;
   if (c.rowsum_fp32) {
     c.sell_dblk.resize(static_cast<size_t>(A.nbr) * B * B);
     c.sell_kdiag.resize(static_cast<size_t>(A.nbr));
-    if (B == 3) {
+    if (B == 3)
       hipLaunchKernelGGL(k_diag_rowsum<3>, dim3((A.nbr + 255) / 256), dim3(256), 0, s, A.nbr, A.brp,
                          A.bci, A.v64, c.sell_dblk.p, c.sell_kdiag.p);
-      hipLaunchKernelGGL(k_sell_zero_diag<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
-                         c.sell_koff.p, c.sell_kdiag.p, c.sell_v32.p);
-    } else {
+    else
       hipLaunchKernelGGL(k_diag_rowsum<6>, dim3((A.nbr + 255) / 256), dim3(256), 0, s, A.nbr, A.brp,
                          A.bci, A.v64, c.sell_dblk.p, c.sell_kdiag.p);
-      hipLaunchKernelGGL(k_sell_zero_diag<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
-                         c.sell_koff.p, c.sell_kdiag.p, c.sell_v32.p);
-    }
     A.dblk = c.sell_dblk.p;
   } else {
     c.sell_dblk.release();
@@ -252,37 +290,24 @@ void build_sell(Ctx& c) {
     A.nlong = static_cast<int>(lr.size());
     A.lrows = lr.empty() ? nullptr : c.sell_lrows.p;
   }
-  // 16-bit block columns when every offset fits (VBH_SELL_IDX16=0 keeps the
-  // 32-bit ones)
-  A.sidx16 = nullptr;
-  c.sell_idx16.release();
-  const char* i16_env = std::getenv("VBH_SELL_IDX16");
-  if (!(i16_env && i16_env[0] == '0')) {
-    c.sell_idx16.resize(static_cast<size_t>(ktot) * sell_nps(B) + 1);
-    DBuf<int> bad;
-    bad.resize(1);
-    VBH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-    if (B == 3)
-      hipLaunchKernelGGL(k_sell_idx16<3>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
-                         c.sell_koff.p, c.sell_idx.p, c.sell_idx16.p, bad.p);
-    else
-      hipLaunchKernelGGL(k_sell_idx16<6>, dim3(nslices), dim3(kSliceRows), 0, s, nslices, A.nbr,
-                         c.sell_koff.p, c.sell_idx.p, c.sell_idx16.p, bad.p);
-    int h_bad = 0;
-    VBH_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    VBH_HIP(hipStreamSynchronize(s));
-    if (h_bad) c.sell_idx16.release();
-    else A.sidx16 = c.sell_idx16.p;
-  }
-  VBH_HIP(hipStreamSynchronize(s));
   A.nslices = nslices;
   A.koff = c.sell_koff.p;
-  A.sidx = c.sell_idx.p;
-  A.s64 = c.sell_v64.p;
-  A.s32 = c.sell_v32.p;
   c.sell_padded_blocks = static_cast<long long>(ktot) * kSliceRows / B;
   c.sell_slots = ktot;
-  build_sell_relative(c);
+  // the absolute layout is filled only where the relative one is not taken
+  // (no reader of the absolute arrays runs while BsrDev::rk64 is set)
+  A.sidx = nullptr;
+  A.sidx16 = nullptr;
+  A.s64 = nullptr;
+  A.s32 = nullptr;
+  if (build_sell_relative(c)) {
+    c.sell_v64.release();
+    c.sell_v32.release();
+    c.sell_idx.release();
+    c.sell_idx16.release();
+  } else {
+    build_sell_absolute(c, ktot);
+  }
   // PCG SpMV slice order (BsrDev::xcd): XCD-contiguous ranges for the FP32
   // stream of 3x3 blocks (C2 hex: 35.4 -> 33.7 us per launch); the FP64
   // stream (81.6 -> 82.6 us) and 6x6 shell blocks are faster grid-stride
@@ -298,7 +323,7 @@ void build_sell(Ctx& c) {
 // the per-row block columns (and, for the FP32 stream, the zeroed diagonal
 // blocks) drop out of the operator stream. VBH_SELL_REL=0 keeps the absolute
 // layout.
-void build_sell_relative(Ctx& c) {
+bool build_sell_relative(Ctx& c) {
   BsrDev& A = c.bsrd;
   A.rk64 = A.ro64 = A.rk32 = A.ro32 = nullptr;
   A.rv64 = nullptr;
@@ -311,7 +336,7 @@ void build_sell_relative(Ctx& c) {
   c.rel_v32.release();
   c.rel_slots64 = c.rel_slots32 = 0;
   const char* env = std::getenv("VBH_SELL_REL");
-  if ((env && env[0] == '0') || A.nlong > 0) return;
+  if ((env && env[0] == '0') || A.nlong > 0) return false;
   hipStream_t s = c.stream;
   const int B = A.B, nbr = A.nbr, nslices = A.nslices;
   std::vector<int> koff, offs;
@@ -323,7 +348,7 @@ void build_sell_relative(Ctx& c) {
   o32.reserve(offs.size());
   for (int sl = 0; sl < nslices; ++sl) {
     const int a = koff[static_cast<size_t>(sl)], b = koff[static_cast<size_t>(sl) + 1];
-    if (b - a > kSellCap) return;
+    if (b - a > kSellCap) return false;
     for (int q = a; q < b; ++q) {
       const int off = offs[static_cast<size_t>(q)];
       o64.push_back(off);
@@ -332,7 +357,7 @@ void build_sell_relative(Ctx& c) {
     k64[static_cast<size_t>(sl) + 1] = static_cast<int>(o64.size());
     k32[static_cast<size_t>(sl) + 1] = static_cast<int>(o32.size());
   }
-  if (static_cast<double>(o64.size()) > 1.1 * static_cast<double>(c.sell_slots)) return;
+  if (static_cast<double>(o64.size()) > 1.1 * static_cast<double>(c.sell_slots)) return false;
   c.rel_k64.upload(k64.data(), k64.size(), s);
   c.rel_o64.upload(o64.data(), o64.size(), s);
   c.rel_k32.upload(k32.data(), k32.size(), s);
@@ -359,11 +384,7 @@ void build_sell_relative(Ctx& c) {
   A.rv32 = c.rel_v32.p;
   c.rel_slots64 = static_cast<long long>(o64.size());
   c.rel_slots32 = static_cast<long long>(o32.size());
-  // the absolute layout's value arrays are no longer read by the solver
-  c.sell_v64.release();
-  c.sell_v32.release();
-  A.s64 = nullptr;
-  A.s32 = nullptr;
+  return true;
 }
 
 }  // namespace vbh
