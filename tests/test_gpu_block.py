@@ -137,6 +137,10 @@ def _variants():
             # the group's lanes (groups of 4, 8 or 16 lanes)
             for stage in ((0, 1, 2) if lg in (4, 8, 16) else (0, 1)):
                 out.append(f"{cw},{lg},1024,{xcd},{stage}")
+    # 3: the sliced ELL, one lane per row, 8 or 4 columns per lane
+    for kc in (8, 4):
+        for xcd in (0, 1):
+            out.append(f"{kc},1,1024,{xcd},3")
     return out
 
 
@@ -173,10 +177,13 @@ def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
 
 @pytest.mark.parametrize("k,cfg", [(8, "1,8,1024,0,2"), (8, "2,4,1024,1,2"), (16, "2,8,1024,0,2"),
                                    (16, "1,16,1024,1,2"), (64, "4,16,1024,0,2"),
-                                   (64, "8,8,1024,1,2"), (5, "2,4,1024,0,2")])
+                                   (64, "8,8,1024,1,2"), (5, "2,4,1024,0,2"),
+                                   (8, "8,1,2048,1,3"), (5, "4,1,1024,0,3"),
+                                   (16, "8,1,1024,0,3"), (64, "4,1,2048,1,3")])
 def test_block_broadcast_widths(gpu_backend_factory, monkeypatch, k, cfg):
     """The register-broadcast SpMM at the other column widths (groups of 4,
-    8 and 16 lanes; k = 5 pads to 8 columns): the shell tube (BSR6) in FP64
+    8 and 16 lanes; k = 5 pads to 8 columns), and the sliced-ELL SpMM (value
+    path 3, 8 or 4 columns per lane): the shell tube (BSR6) in FP64
     and mixed precision and a jittered hex (BSR3) in mixed precision, each
     column to the tolerance against the FP64 oracle operator."""
     monkeypatch.setenv("VBH_BLOCK_TUNE", cfg)
@@ -236,3 +243,30 @@ def test_block_column_compaction(gpu_backend_factory, c1, monkeypatch, mixed):
         if not mixed:
             uo, it_o, _ = O.pcg_jacobi(oc, G[:, j], tol=tol, max_iters=20000)
             assert abs(int(its[j]) - it_o) <= 2, (j, its[j], it_o)
+
+
+@pytest.mark.parametrize("rel", ["1", "0"])
+def test_block_sliced_ell_layouts(gpu_backend_factory, monkeypatch, rel):
+    """The sliced-ELL SpMM reads whichever single-RHS layout the operator
+    carries: the relative one (slot offsets per slice) or, with
+    VBH_SELL_REL=0, the absolute one with 16-bit block columns, and with
+    32-bit columns on a hex whose shuffled node IDs put neighbours more than
+    32,767 nodes apart. FP64 and mixed block solves of 8 load cases, every
+    checked column to the tolerance against the oracle operator."""
+    monkeypatch.setenv("VBH_SELL_REL", rel)
+    monkeypatch.setenv("VBH_BLOCK_TUNE", "8,1,1024,1,3")
+    hx = decks.config2_hex_jitter(nx=64, ny=24, nz=24, seed=3)
+    hx.node_ids = np.random.default_rng(9).permutation(hx.node_ids).astype(np.int32)
+    cases = [decks.config2_hex_jitter(nx=10, ny=4, nz=4, seed=7),
+             decks.cquad4_tube(n_circ=16, n_axial=24, n_cases=1), hx]
+    for m in cases:
+        eq, n = M.analysis_dof_map(m)
+        oc = O.assemble_model(n, m, eq)
+        G = np.random.default_rng(11).standard_normal((n, 8))
+        for mixed, tol, bound in ((False, 1e-10, 2e-10), (True, 1e-8, 1e-8)):
+            be = gpu_backend_factory(mixed=mixed, tolerance=tol, max_iters=50000)
+            U = be.solve_block(csr_of(oc), G)
+            used = be.block_config()
+            assert (used.cols_per_lane, used.lanes_per_group, used.value_path) == (8, 1, 3)
+            for j in (0, 3, 7):
+                assert O.true_relative_residual(oc, U[:, j], G[:, j]) < bound

@@ -71,13 +71,17 @@ struct BlkOp {
   // (row-major B*B per block row); the FP32 SpMM skips the FP32 diagonal
   // block and adds this block's product in FP64 after the row.
   const double* dblk = nullptr;
+  // the single-RHS sliced ELL of the same operator (relative or absolute
+  // layout; nslices = 0 when there is none), read by the sliced-ELL SpMM
+  BsrDev sl;
 };
 
 struct BlkTune {
   int cw = 0, lg = 0, nrb = 0;  // columns per lane, lanes per group, grid rows
   int xcd = 0;                  // XCD-contiguous row ranges
   int stage = 0;                // block values: 0 direct loads, 1 staged through LDS,
-                                // 2 broadcast across the group's lanes (lg = 4, 8, 16)
+                                // 2 broadcast across the group's lanes (lg = 4, 8, 16),
+                                // 3 the sliced ELL, one lane per row (lg = 1, cw = 8 or 4)
   double ms = 0.0;
 };
 
@@ -1215,11 +1219,159 @@ __global__ __launch_bounds__(kBT) void k_blk_spmm_bcast(
   }
 }
 
+// Sliced-ELL variant (value path 3): one lane per scalar row of a 64-row
+// slice, streaming the single-RHS operator layout (BsrDev: relative slot
+// offsets or absolute block columns), KC columns per lane gathered as whole
+// X rows. Every block value is read once per column tile, by the one lane
+// whose row it is on, with no cross-lane broadcast; the X gathers grow with
+// KC, so it suits narrow column tiles. Same inputs, outputs and partial sums
+// as the other shapes (column tile blockIdx.y of KC columns, part slot
+// blockIdx.x).
+template <typename T, int B, int KC, int MODE>
+__global__ __launch_bounds__(kBT) void k_blk_spmm_sell(
+    BlkOp A, int ldk, int xcd, const T* __restrict__ X, T* __restrict__ Y,
+    const double* __restrict__ F, const int* __restrict__ act, double* __restrict__ part) {
+  __shared__ double lds[kBT * KC];
+  const int l = threadIdx.x & (kSliceRows - 1), wave = threadIdx.x >> 6;
+  const int c0 = blockIdx.y * KC;
+  {  // a column tile with no column to compute leaves at once
+    int on = 0;
+#pragma unroll
+    for (int w = 0; w < KC; ++w) on |= act[c0 + w];
+    if (!__syncthreads_or(on)) return;
+  }
+  using Acc = typename std::conditional<(MODE == 0 && sizeof(T) == 4 && B == 6), double, T>::type;
+  constexpr int NPS = sell_nps(B);
+  const BsrDev& S = A.sl;
+  const bool rel = S.rk64 != nullptr;
+  const int* __restrict__ kk = rel ? (sizeof(T) == 4 ? S.rk32 : S.rk64) : S.koff;
+  const int* __restrict__ ro = sizeof(T) == 4 ? S.ro32 : S.ro64;
+  const T* __restrict__ vals;
+  if constexpr (sizeof(T) == 4) vals = rel ? S.rv32 : S.s32;
+  else vals = rel ? S.rv64 : S.s64;
+  const double* __restrict__ dblk = (MODE == 0 && sizeof(T) == 4) ? A.dblk : nullptr;
+  const int nrows = S.nbr * B, last = S.nbr - 1;
+  double dot[KC];
+#pragma unroll
+  for (int w = 0; w < KC; ++w) dot[w] = 0.0;
+  // slices: XCD-contiguous ranges (xcd) or grid-stride, one per wave
+  const int wpb = kBT / kSliceRows;
+  int s0 = static_cast<int>(blockIdx.x) * wpb + wave, stride = static_cast<int>(gridDim.x) * wpb;
+  int s_hi = S.nslices;
+  if (xcd && gridDim.x >= 8) {
+    const int x = blockIdx.x & 7, per = (S.nslices + 7) >> 3;
+    const int lo = min(S.nslices, x * per);
+    s0 = lo + static_cast<int>(blockIdx.x >> 3) * wpb + wave;
+    stride = static_cast<int>(gridDim.x >> 3) * wpb;
+    s_hi = min(S.nslices, lo + per);
+  }
+  for (int sc = s0; sc < s_hi; sc += stride) {
+    const int sw = __builtin_amdgcn_readfirstlane(sc);
+    const int k0 = kk[sw], km = kk[sw + 1] - k0;
+    const int g = kSliceRows * sw + l;
+    const int node = g / B, node0 = (kSliceRows * sw) / B;
+    const T* __restrict__ v = vals + (static_cast<size_t>(k0) * kSliceRows + l) * B;
+    Acc acc[KC];
+#pragma unroll
+    for (int w = 0; w < KC; ++w) acc[w] = Acc(0);
+    // two slots in flight (C5, 8 columns: 153.4 -> 145.9 us; four: 155.5)
+#pragma unroll 2
+    for (int k = 0; k < km; ++k) {
+      int J;
+      if (rel) {
+        J = node + ro[k0 + k];
+        J = J < 0 ? 0 : (J > last ? last : J);  // outside the operator: zero values
+      } else if (S.sidx16) {
+        J = node0 + static_cast<int>(S.sidx16[static_cast<size_t>(k0 + k) * NPS + (node - node0)]);
+      } else {
+        J = S.sidx[static_cast<size_t>(k0 + k) * NPS + (node - node0)];
+      }
+      const T* __restrict__ vk = v + static_cast<size_t>(k) * kSliceRows * B;
+      const T* __restrict__ xj = X + static_cast<size_t>(B * J) * ldk + c0;
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        T x[KC];
+        vload<T, KC>(xj + static_cast<size_t>(c) * ldk, x);
+        const Acc a = static_cast<Acc>(vk[c]);
+#pragma unroll
+        for (int w = 0; w < KC; ++w) acc[w] += a * static_cast<Acc>(x[w]);
+      }
+    }
+    if (g < nrows) {
+      const size_t idx = static_cast<size_t>(g) * ldk + c0;
+      if constexpr (MODE == 0) {
+        T xo[KC], yo[KC];
+        vload<T, KC>(X + idx, xo);
+#pragma unroll
+        for (int w = 0; w < KC; ++w) yo[w] = static_cast<T>(acc[w]);
+        if (dblk) {  // FP64 corrected diagonal block: y = acc + D_II x_I in FP64
+          const int r = g - node * B;
+          double dsum[KC];
+#pragma unroll
+          for (int w = 0; w < KC; ++w) dsum[w] = static_cast<double>(acc[w]);
+#pragma unroll
+          for (int c = 0; c < B; ++c) {
+            T xi[KC];
+            vload<T, KC>(X + static_cast<size_t>(B * node + c) * ldk + c0, xi);
+            const double dv = dblk[(static_cast<size_t>(node) * B + r) * B + c];
+#pragma unroll
+            for (int w = 0; w < KC; ++w) dsum[w] += dv * static_cast<double>(xi[w]);
+          }
+#pragma unroll
+          for (int w = 0; w < KC; ++w) yo[w] = static_cast<T>(dsum[w]);
+        }
+        vstore<T, KC>(Y + idx, yo);
+#pragma unroll
+        for (int w = 0; w < KC; ++w)
+          dot[w] += static_cast<double>(xo[w]) * static_cast<double>(yo[w]);
+      } else {
+        double f[KC];
+        T y[KC];
+        vload<double, KC>(F + idx, f);
+#pragma unroll
+        for (int w = 0; w < KC; ++w) {
+          const double yy = f[w] - static_cast<double>(acc[w]);
+          y[w] = static_cast<T>(yy);
+          dot[w] += yy * yy;
+        }
+        vstore<T, KC>(Y + idx, y);
+      }
+    }
+  }
+  // per-column sums over the workgroup (every thread holds all KC columns)
+#pragma unroll
+  for (int w = 0; w < KC; ++w) lds[threadIdx.x * KC + w] = dot[w];
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < KC) {
+    const int t = threadIdx.x;
+    double sum = 0.0;
+    for (int th = 0; th < kBT; ++th) sum += lds[th * KC + t];
+    part[static_cast<size_t>(blockIdx.y * KC + t) * kMaxGrid + blockIdx.x] = sum;
+  }
+}
+
 // ---- SpMM dispatch ----------------------------------------------------------------------------
 template <typename T, int B, int MODE>
 void spmm_b(const BlkTune& t, const BlkOp& A, int ldk, const T* X, T* Y, const double* F,
             const int* act, double* part, hipStream_t s) {
   const dim3 grid(t.nrb, ldk / (t.cw * t.lg));
+  if (t.stage == 3) {  // sliced ELL, KC = cw columns per lane (lg = 1)
+    if constexpr (B > 1) {
+      if (A.sl.nslices > 0 && A.sl.nlong == 0 && t.lg == 1) {
+        if (t.cw == 8) {
+          hipLaunchKernelGGL((k_blk_spmm_sell<T, B, 8, MODE>), grid, dim3(kBT), 0, s, A, ldk, t.xcd,
+                             X, Y, F, act, part);
+          return;
+        }
+        if (t.cw == 4) {
+          hipLaunchKernelGGL((k_blk_spmm_sell<T, B, 4, MODE>), grid, dim3(kBT), 0, s, A, ldk, t.xcd,
+                             X, Y, F, act, part);
+          return;
+        }
+      }
+    }
+    throw_error(VBH_ERR_HIP, "block SpMM: no sliced-ELL kernel for the tuned configuration");
+  }
 #define VBH_SPMM_CASE(CW_, LG_)                                                                 \
   if (t.cw == CW_ && t.lg == LG_) {                                                             \
     if (t.stage == 2) {                                                                         \
@@ -1284,7 +1436,7 @@ int log2_of(int p) {
 // kernel whose tile is the full width (or half of it with CW = 4), three grids (512
 // workgroups keep fewer block rows in flight per XCD, so the X rows their
 // neighbours gather stay in that XCD's L2 for wide FP64 blocks).
-std::vector<BlkTune> candidates(int ldk, int B) {
+std::vector<BlkTune> candidates(int ldk, int B, bool sell) {
   const int kt = std::min(ldk, 64);
   std::vector<BlkTune> c;
   auto add = [&](int cw, int lg) {
@@ -1307,6 +1459,18 @@ std::vector<BlkTune> candidates(int ldk, int B) {
     const int lg = kt / cw;
     if (lg >= 1 && lg <= 64 && (cw == 1 ? lg >= 8 : true)) add(cw, lg);
   }
+  if (sell && B > 1)  // sliced ELL: 8 or 4 columns per lane, one lane per row
+    for (int kc : {8, 4})
+      for (int xcd : {1, 0})
+        for (int nrb : {2048, 1024}) {
+          BlkTune t;
+          t.cw = kc;
+          t.lg = 1;
+          t.nrb = nrb;
+          t.xcd = xcd;
+          t.stage = 3;
+          c.push_back(t);
+        }
   return c;
 }
 
@@ -1345,6 +1509,7 @@ static void ensure_block_operator(Ctx& c, BlockState& b) {
     b.op.v64 = b.aos64.p;
     b.op.v32 = b.aos32.p;
     b.op.dblk = c.bsrd.dblk;
+    b.op.sl = c.bsrd;
   } else {
     // scalar CSR: blocks of 1 x 1, values already in array-of-blocks order
     b.aos64.release();
@@ -1356,6 +1521,7 @@ static void ensure_block_operator(Ctx& c, BlockState& b) {
     b.op.v64 = c.csr.v64;
     b.op.v32 = c.csr.v32;
     b.op.dblk = nullptr;
+    b.op.sl = BsrDev{};
   }
   VBH_HIP(hipStreamSynchronize(s));
   b.op_version = c.operator_version;
@@ -1423,7 +1589,8 @@ static BlkTune tune(Ctx& c, BlockState& b, int precision) {
   const auto key = std::make_pair(b.ldk, precision);
   auto it = b.tuned.find(key);
   if (it != b.tuned.end()) return it->second;
-  std::vector<BlkTune> cand = candidates(b.ldk, b.op.B);
+  // (the sliced ELL holds whole rows only when no row overflows kSellCap)
+  std::vector<BlkTune> cand = candidates(b.ldk, b.op.B, b.op.sl.nslices > 0 && b.op.sl.nlong == 0);
   // VBH_BLOCK_TUNE=0 takes the first candidate; "cw,lg,nrb[,xcd[,stage]]" forces one.
   const char* env = std::getenv("VBH_BLOCK_TUNE");
   if (env && env[0] == '0') cand.resize(1);
