@@ -138,7 +138,7 @@ def _variants():
             for stage in ((0, 1, 2) if lg in (4, 8, 16) else (0, 1)):
                 out.append(f"{cw},{lg},1024,{xcd},{stage}")
     # 3: the sliced ELL, one lane per row, 8 or 4 columns per lane
-    for kc in (8, 4):
+    for kc in (8, 4, 2):
         for xcd in (0, 1):
             out.append(f"{kc},1,1024,{xcd},3")
     return out
@@ -179,7 +179,8 @@ def test_block_every_spmm_variant(gpu_backend_factory, monkeypatch, cfg):
                                    (16, "1,16,1024,1,2"), (64, "4,16,1024,0,2"),
                                    (64, "8,8,1024,1,2"), (5, "2,4,1024,0,2"),
                                    (8, "8,1,2048,1,3"), (5, "4,1,1024,0,3"),
-                                   (16, "8,1,1024,0,3"), (64, "4,1,2048,1,3")])
+                                   (16, "8,1,1024,0,3"), (64, "4,1,2048,1,3"),
+                                   (2, "2,1,1024,0,3")])
 def test_block_broadcast_widths(gpu_backend_factory, monkeypatch, k, cfg):
     """The register-broadcast SpMM at the other column widths (groups of 4,
     8 and 16 lanes; k = 5 pads to 8 columns), and the sliced-ELL SpMM (value

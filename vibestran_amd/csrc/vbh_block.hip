@@ -81,7 +81,7 @@ struct BlkTune {
   int xcd = 0;                  // XCD-contiguous row ranges
   int stage = 0;                // block values: 0 direct loads, 1 staged through LDS,
                                 // 2 broadcast across the group's lanes (lg = 4, 8, 16),
-                                // 3 the sliced ELL, one lane per row (lg = 1, cw = 8 or 4)
+                                // 3 the sliced ELL, one lane per row (lg = 1, cw = 8, 4 or 2)
   double ms = 0.0;
 };
 
@@ -1368,6 +1368,11 @@ void spmm_b(const BlkTune& t, const BlkOp& A, int ldk, const T* X, T* Y, const d
                              X, Y, F, act, part);
           return;
         }
+        if (t.cw == 2) {
+          hipLaunchKernelGGL((k_blk_spmm_sell<T, B, 2, MODE>), grid, dim3(kBT), 0, s, A, ldk, t.xcd,
+                             X, Y, F, act, part);
+          return;
+        }
       }
     }
     throw_error(VBH_ERR_HIP, "block SpMM: no sliced-ELL kernel for the tuned configuration");
@@ -1459,8 +1464,8 @@ std::vector<BlkTune> candidates(int ldk, int B, bool sell) {
     const int lg = kt / cw;
     if (lg >= 1 && lg <= 64 && (cw == 1 ? lg >= 8 : true)) add(cw, lg);
   }
-  if (sell && B > 1)  // sliced ELL: 8 or 4 columns per lane, one lane per row
-    for (int kc : {8, 4})
+  if (sell && B > 1)  // sliced ELL: 8, 4 or 2 columns per lane, one lane per row
+    for (int kc : {8, 4, 2})
       for (int xcd : {1, 0})
         for (int nrb : {2048, 1024}) {
           BlkTune t;
@@ -1586,9 +1591,20 @@ static void ensure_block_work(Ctx& c, BlockState& b, int k, int precision) {
 // Autotune: time every candidate SpMM configuration for this operator, ldk
 // and precision on the current work vectors; keep the fastest.
 static BlkTune tune(Ctx& c, BlockState& b, int precision) {
-  const auto key = std::make_pair(b.ldk, precision);
+  // keyed by the width and by the live columns rounded up to a power of two:
+  // a narrow column tile only pays where few of the width's columns run
+  int kq = 1;
+  while (kq < b.k) kq <<= 1;
+  const auto key = std::make_pair(b.ldk * 128 + kq, precision);
   auto it = b.tuned.find(key);
   if (it != b.tuned.end()) return it->second;
+  {  // the candidates run with the solve's live columns (the first k) marked
+    std::vector<int> live(static_cast<size_t>(b.ldk));
+    for (int j = 0; j < b.ldk; ++j) live[static_cast<size_t>(j)] = j < b.k ? 1 : 0;
+    VBH_HIP(hipMemcpyAsync(b.ones.p, live.data(), sizeof(int) * live.size(), hipMemcpyHostToDevice,
+                           c.stream));
+    VBH_HIP(hipStreamSynchronize(c.stream));  // live leaves scope
+  }
   // (the sliced ELL holds whole rows only when no row overflows kSellCap)
   std::vector<BlkTune> cand = candidates(b.ldk, b.op.B, b.op.sl.nslices > 0 && b.op.sl.nlong == 0);
   // VBH_BLOCK_TUNE=0 takes the first candidate; "cw,lg,nrb[,xcd[,stage]]" forces one.
