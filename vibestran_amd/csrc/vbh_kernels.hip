@@ -882,6 +882,8 @@ __device__ __forceinline__ void overflow_rows(const BsrDev& A, const T* __restri
     double acc[B];
 #pragma unroll
     for (int r = 0; r < B; ++r) acc[r] = 0.0;
+    // four blocks' loads in flight per thread (the sums keep their order)
+#pragma unroll 4
     for (int k = kSellCap + threadIdx.x; k < nb; k += blockDim.x) {
       const int J = A.bci[o + k];
       if (split && J == I) continue;
