@@ -596,12 +596,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
   reduce_partB(st, red, par, lds);
 }
 
-// FP64 streams ask for 8 resident waves per SIMD with 3x3 blocks (58 VGPRs
-// instead of 92: C2 83.3 -> 81.5 us per launch) and 7 with 6x6 blocks (66
-// VGPRs; 8 would spill). The FP32 stream keeps its 88 VGPRs: at 8 waves its
-// fewer loads in flight per wave cost more (34.3 -> 40.4 us).
-template <typename T, int B, typename Acc = T>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? (B == 3 ? 8 : 7) : 1) void k_cg_spmv_bsr(
+// FP64 streams of the relative sliced ELL (REL) ask for 8 resident waves per
+// SIMD with 3x3 blocks (58 VGPRs instead of 92: C2 83.3 -> 81.5 us per
+// launch) and 7 with 6x6 blocks (66 VGPRs; 8 would spill). The absolute
+// layout loads a block column per row and slot and needs the registers: under
+// the same bound the FP64 solve of an operator with long rows (MPC tie of a
+// face to its centre grid) took 137.8 ms instead of 91.5. The FP32 stream
+// keeps its 88 VGPRs: at 8 waves its fewer loads in flight per wave cost more
+// (34.3 -> 40.4 us).
+template <typename T, int B, typename Acc = T, bool REL = false>
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 8 && REL) ? (B == 3 ? 8 : 7) : 1) void k_cg_spmv_bsr(
     BsrDev A, CgBufs<T> b, PcgScalars* st, CgRed red, int par) {
   __shared__ double lds[kLds];
   const int l = threadIdx.x & (kSliceRows - 1);
@@ -1043,12 +1047,23 @@ void launch_cg_spmv(int format, const CsrDev& A, const BsrDev& Ab, const CgBufs<
     VBH_LAUNCH((k_cg_spmv_csr<T, kCsrG>), g, s, t0, t1, A, b, st, red, par);
   else if (format == 3 && acc64)
     VBH_LAUNCH((k_cg_spmv_bsr<T, 3, double>), g, s, t0, t1, Ab, b, st, red, par);
-  else if (format == 3)
-    VBH_LAUNCH((k_cg_spmv_bsr<T, 3>), g, s, t0, t1, Ab, b, st, red, par);
-  else if (acc64)
+  else if (format == 3) {
+    if constexpr (sizeof(T) == 8) {
+      if (Ab.rk64) VBH_LAUNCH((k_cg_spmv_bsr<T, 3, T, true>), g, s, t0, t1, Ab, b, st, red, par);
+      else VBH_LAUNCH((k_cg_spmv_bsr<T, 3>), g, s, t0, t1, Ab, b, st, red, par);
+    } else {
+      VBH_LAUNCH((k_cg_spmv_bsr<T, 3>), g, s, t0, t1, Ab, b, st, red, par);
+    }
+  } else if (acc64) {
     VBH_LAUNCH((k_cg_spmv_bsr<T, 6, double>), g, s, t0, t1, Ab, b, st, red, par);
-  else
-    VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red, par);
+  } else {
+    if constexpr (sizeof(T) == 8) {
+      if (Ab.rk64) VBH_LAUNCH((k_cg_spmv_bsr<T, 6, T, true>), g, s, t0, t1, Ab, b, st, red, par);
+      else VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red, par);
+    } else {
+      VBH_LAUNCH((k_cg_spmv_bsr<T, 6>), g, s, t0, t1, Ab, b, st, red, par);
+    }
+  }
   if (format != 0)
     launch_overflow<T, 1>(Ab, b.p, b.Ap, b.r, b.diag, red.partA, red.nA, st, s, b.u64, b.r64);
 }
