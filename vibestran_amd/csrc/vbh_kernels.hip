@@ -934,8 +934,14 @@ __device__ __forceinline__ void overflow_rows(const BsrDev& A, const T* __restri
 // MODE 0: after a plain SpMV. MODE 1: after a PCG SpMV, which either formed
 // Ap = K p (MODE 1 rows) or, in a mixed refresh step, r64 = F - K64 u (MODE 2
 // rows on the FP64 blocks); it reads the same state words the SpMV read.
+// 1,024 threads per long row: a face tied to one grid gives that grid's row
+// thousands of blocks, a few per thread in one batch of loads (605k-DOF hex
+// with its end face tied to the centre grid: mixed solve 58.4-59.4 ms with 256
+// threads, 57.4-57.6 ms with 1,024; FP64 91.7 -> 90.3 ms)
+constexpr int kOvfThreads = 1024;
+
 template <typename T, int B, int MODE>
-__global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __restrict__ x,
+__global__ __launch_bounds__(kOvfThreads) void k_sell_overflow(BsrDev A, const T* __restrict__ x,
                                                           T* __restrict__ y,
                                                           const T* __restrict__ z,  // r (MODE 1)
                                                           const T* __restrict__ diag,
@@ -943,7 +949,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_overflow(BsrDev A, const T* __r
                                                           const PcgScalars* st,
                                                           const double* __restrict__ u64,
                                                           double* __restrict__ r64) {
-  __shared__ double lds[4 * 8];
+  __shared__ double lds[(kOvfThreads / 64) * 8];
   if (MODE == 1) {
     __shared__ int w[2];
     if (threadIdx.x == 0) {
@@ -992,10 +998,10 @@ static void launch_overflow(const BsrDev& A, const T* x, T* y, const T* z, const
   if (A.nlong == 0) return;
   const int g = MODE == 0 ? std::min(A.nlong, kMaxGrid) : std::min(A.nlong, nprod);
   if (A.B == 3)
-    hipLaunchKernelGGL((k_sell_overflow<T, 3, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, z, diag,
+    hipLaunchKernelGGL((k_sell_overflow<T, 3, MODE>), dim3(g), dim3(kOvfThreads), 0, s, A, x, y, z, diag,
                        part, st, u64, r64);
   else
-    hipLaunchKernelGGL((k_sell_overflow<T, 6, MODE>), dim3(g), dim3(kBlock), 0, s, A, x, y, z, diag,
+    hipLaunchKernelGGL((k_sell_overflow<T, 6, MODE>), dim3(g), dim3(kOvfThreads), 0, s, A, x, y, z, diag,
                        part, st, u64, r64);
 }
 
