@@ -602,8 +602,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_csr(CsrDev A, CgBufs<T> b, P
 // layout loads a block column per row and slot and needs the registers: under
 // the same bound the FP64 solve of an operator with long rows (MPC tie of a
 // face to its centre grid) took 137.8 ms instead of 91.5. The FP32 stream
-// keeps its 88 VGPRs: at 8 waves its fewer loads in flight per wave cost more
-// (34.3 -> 40.4 us).
+// keeps its 88 VGPRs: asked for 6, 7 or 8 waves the compiler trims it to 56-58
+// VGPRs, and the fewer loads in flight per wave cost more (34.5 -> 40.1-40.4 us).
 template <typename T, int B, typename Acc = T, bool REL = false>
 __global__ __launch_bounds__(kBlock, (sizeof(T) == 8 && REL) ? (B == 3 ? 8 : 7) : 1) void k_cg_spmv_bsr(
     BsrDev A, CgBufs<T> b, PcgScalars* st, CgRed red, int par) {
